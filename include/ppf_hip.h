@@ -1,0 +1,189 @@
+/*
+ * ppf_hip.h -- C ABI of libppf_hip.so, the MI355X (gfx950) engine hot path of
+ * the pyphasefield drop-in.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a nonzero status (a hipError_t
+ *     value, or PPF_EINVAL for a rejected argument); the message of the last
+ *     failure on the calling thread is returned by ppf_last_error().
+ *   - The caller owns every buffer (device pointers, typically from torch
+ *     tensors).  No entry point allocates device memory or synchronises the
+ *     device: all work is enqueued on the given stream, so a sequence of steps
+ *     can be captured into a HIP graph.
+ *   - Fields are stored halo-padded, +1 cell per axis, exactly as the
+ *     reference Field does (field.py:6-20).  A field pointer addresses padded
+ *     element (x=0, y=0, z=0) of field 0; element (f, z, y, x) lives at
+ *     ptr + f*sf + z*sz + y*sy + x.  Step kernels never read halo cells of
+ *     their input: boundary conditions are folded into the loads; output halos
+ *     are left untouched (ppf_apply_bc materialises them).
+ *   - "step" arguments are the 1-based time-step counter of the step being
+ *     taken (the reference increments it before running the engine,
+ *     simulation.py:263-264); it keys the noise and the temperature ramp.
+ *
+ * No C++ types, exceptions or torch types cross this boundary.
+ */
+#ifndef PPF_HIP_H
+#define PPF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* ppf_stream_t; /* hipStream_t; NULL = default stream */
+
+#define PPF_ABI_VERSION 1
+#define PPF_EINVAL 1000
+
+enum { PPF_F32 = 0, PPF_F64 = 1 };
+/* boundary kinds, per axis; names as in simulation.py:110-117 */
+enum { PPF_PERIODIC = 0, PPF_NEUMANN = 1, PPF_DIRICHLET = 2 };
+
+/* Shape, layout and boundary rules of a stack of halo-padded fields. */
+typedef struct ppf_grid {
+  int32_t ndim;     /* 1, 2 or 3 */
+  int32_t dtype;    /* PPF_F32 or PPF_F64 */
+  int32_t nfields;  /* fields in the stack */
+  int32_t bc[3];    /* per axis in the order x (last array axis), y, z
+                       -- the order of ppf_gpu_utils.py:437-513 */
+  int64_t n[3];     /* interior cells along x, y, z (1 for absent axes) */
+  int64_t sy, sz;   /* element strides between y rows and z planes */
+  int64_t sf;       /* element stride between fields */
+  double dx;        /* cell spacing (Neumann halo = neighbour -/+ dx*bc) */
+} ppf_grid;
+
+/* Temperature seen by an engine (simulation.py:126-177, 303-332). */
+enum { PPF_T_CONST = 0, PPF_T_LINEAR = 1, PPF_T_ARRAY = 2 };
+typedef struct ppf_temperature {
+  int32_t kind;
+  int32_t pad_;
+  /* CONST: T0.  LINEAR: interior cell (i,j,k) at 0-based x,y,z index:
+       T = ((T0 + i*(dx*dTdx)) + j*(dx*dTdy)) + k*(dx*dTdz) + (step-1)*(dTdt*dt)
+     with halos per the temperature rule (periodic wrap, else zero gradient,
+     ppf_gpu_utils.py:197-253). */
+  double T0, dTdx, dTdy, dTdz, dTdt, dt;
+  /* ARRAY: padded array, same dtype as the fields, strides sy/sz; only its
+     interior is read (halos folded by the temperature rule). */
+  const void* data;
+  int64_t sy, sz;
+} ppf_temperature;
+
+/* Warren1995 parameters after Warren1995.py:197-203 derivation. */
+typedef struct ppf_warren_params {
+  double T, T_mA, T_mB, L_A, L_B, D_L, D_S, v_m, R, y_e, alpha;
+  double ebar, W_A, W_B, M_A, M_B, dt;
+} ppf_warren_params;
+
+/* N-component engine parameters: params[15] and c_params[6][N] of
+   Engines/NCGPU_new.py:495-519. */
+#define PPF_MAX_COMP 4
+typedef struct ppf_ncomp_params {
+  int32_t ncomp; /* N components, 2..PPF_MAX_COMP; field stack = N+2 */
+  int32_t pad_;
+  double dx, d, v_m, M_qmax, H, y_e, beta, D_L, D_S, dt, ebar, eqbar;
+  double noise_phi, noise_c, noise_q;
+  double L[PPF_MAX_COMP], T_M[PPF_MAX_COMP], S[PPF_MAX_COMP];
+  double B[PPF_MAX_COMP], W[PPF_MAX_COMP], M[PPF_MAX_COMP];
+} ppf_ncomp_params;
+
+/* Redlich-Kister + ideal-mixing phase (pycalphad redlich_kister_sum +
+   ideal_mixing_energy as used at simulation.py:212-214):
+     G = sum c_i (a+bT+cT lnT+dT^2+e/T) + sum_{i<j} c_i c_j sum_k (A+BT)(c_i-c_j)^k
+         + R T sum c_i ln max(c_i, 1e-10) */
+typedef struct ppf_rk_phase {
+  int32_t ncomp;
+  int32_t pad_;
+  double R;
+  double g0[PPF_MAX_COMP][5];
+  double L[6][3][2]; /* pairs (0,1),(0,2),(0,3),(1,2),(1,3),(2,3) as used */
+} ppf_rk_phase;
+
+/* derivative evaluation mode of the plain thermodynamics */
+enum { PPF_DG_FORWARD_DIFF = 0, /* reference h=1e-7 forward difference */
+       PPF_DG_ANALYTIC = 1 };
+
+typedef struct ppf_thermo {
+  ppf_rk_phase liquid; /* ufunc 1 in NCGPU_new.py:353 */
+  ppf_rk_phase solid;  /* ufunc 0 (FCC_A1 sorts first), NCGPU_new.py:354 */
+  int32_t dg_mode;
+  int32_t pad_;
+} ppf_thermo;
+
+int ppf_abi_version(void);
+const char* ppf_last_error(void);
+
+/* Materialise the halo cells of a field stack: replaces apply_boundary_conditions
+   (ppf_gpu_utils.py:430-514, kernels :11-195).  bcvals: NULL or an array with
+   the same layout whose halo cells hold the bcarray values. */
+int ppf_apply_bc(const ppf_grid* g, void* fields, const void* bcvals, ppf_stream_t s);
+
+/* Temperature halo rule (ppf_gpu_utils.py:197-319): periodic wrap, else
+   zero-gradient.  g->nfields must be 1. */
+int ppf_apply_bc_temperature(const ppf_grid* g, void* T, ppf_stream_t s);
+
+/* Write the temperature of step `step` (interior + halos) into a padded array
+   (replaces init_temperature_field + update_temperature_field for the GPU
+   path: simulation.py:126-156, ppf_gpu_utils.py:516-531). */
+int ppf_fill_temperature(const ppf_grid* g, void* out, const ppf_temperature* t,
+                         int64_t step, ppf_stream_t s);
+
+/* One explicit-Euler step of c <- c + dt*D*lap(c)
+   (Engines/Diffusion.py:198-208; replaces DiffusionGPU.py:18-75). */
+int ppf_diffusion_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                       double D, double dt, ppf_stream_t s);
+
+/* phi <- phi + dt*M*(eps^2 lap(phi) - 16 W (4phi^3-6phi^2+2phi))
+   (Engines/CahnAllen.py:128-135). */
+int ppf_allencahn_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                       double M, double W, double epsilon, double dt, ppf_stream_t s);
+
+/* mu = -eps^2 lap(phi) + 4phi^3-6phi^2+2phi;  phi <- phi + dt*M*lap(mu)
+   (Engines/CahnHilliard.py:134-146; mu on the +1 ring: periodic wrap or
+   zero-flux copy of the adjacent interior cell). */
+int ppf_cahnhilliard_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                          double M, double epsilon, double dt, ppf_stream_t s);
+
+/* Warren1995 binary solidification step, fields (phi, c)
+   (Engines/Warren1995.py:50-92).  Noise: counter RNG keyed by (seed, step). */
+int ppf_warren_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                    const ppf_warren_params* p, uint64_t seed, int64_t step, ppf_stream_t s);
+
+/* N-component thermodynamic pass: replaces NComponent_helper_kernel
+   (Engines/NCGPU_new.py:256-313).  Writes, for every padded cell including the
+   halo ring, transfer[0] = G_S - G_L, transfer[1+l] = M_c,l and
+   transfer[N+l] = dF/dc_l (l < N-1).  `tg` describes transfer: nfields = 2N-1,
+   same n/strides convention as g. */
+int ppf_ncomp_thermo(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                     const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p,
+                     const ppf_thermo* th, uint64_t seed, int64_t step, ppf_stream_t s);
+
+/* N-component main step: replaces NComponent_kernel (Engines/NCGPU_new.py:37-250)
+   reading the transfer arrays ppf_ncomp_thermo wrote. */
+int ppf_ncomp_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                   const ppf_temperature* t, const ppf_grid* tg, const void* transfer,
+                   const ppf_ncomp_params* p, uint64_t seed, int64_t step, ppf_stream_t s);
+
+/* Global FP64 reductions over the interior (conservation checks):
+   kind 0: out[f] = sum of field f (mass), for every field of the stack;
+   kind 1: out[0] = Cahn-Hilliard free energy sum[c^2(1-c)^2 + eps^2/2 |grad c|^2] dx^d
+           (forward differences, halos folded), field 0 only.
+   out: device pointer to FP64 results; work: device scratch of
+   ppf_reduce_work_bytes(g) bytes. */
+int64_t ppf_reduce_work_bytes(const ppf_grid* g);
+int ppf_reduce(const ppf_grid* g, const void* in, const void* bcvals, int kind, double epsilon,
+               double* out, void* work, ppf_stream_t s);
+
+/* Plain thermodynamics at scattered points: for each of n points with
+   temperature T[i] and component fractions c[i*(N-1)+l] (c_N implied),
+   write for each phase (liquid then solid) G and dG/dc_l:
+   out[i*2N + 0] = G_L, out[i*2N + 1..N-1] = dG_L/dc_l,
+   out[i*2N + N] = G_S, out[i*2N + N+1..2N-1] = dG_S/dc_l.  FP64 throughout.
+   Used to generate surrogate training data. */
+int ppf_thermo_eval(const ppf_thermo* th, int64_t n, const double* T, const double* c,
+                    double* out, ppf_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PPF_HIP_H */

@@ -1,0 +1,104 @@
+"""Gibbs energy of a substitutional phase and its derivatives (oracle).
+
+The reference builds G for each phase from a TDB file with pycalphad's
+``Model.redlich_kister_sum`` + ``ideal_mixing_energy`` (``simulation.py:
+212-214``), lambdifies it (``simulation.py:243``) and evaluates it per cell in
+``NComponent_helper_kernel`` (``Engines/NCGPU_new.py:295-296``) with the input
+vector [c_1..c_{N-1}, c_N, T] (``Engines/NCGPU_new.py:286-292``).  For a single
+sublattice that expression is
+
+    G(c, T) = sum_i c_i G_i(T)
+            + sum_{i<j} c_i c_j sum_k L^k_ij(T) (c_i - c_j)^k
+            + R T sum_i c_i ln c_i
+
+with G_i(T) = a + b T + c T lnT + d T^2 + e / T  (one SGTE range) and
+L^k_ij(T) = A + B T.  The chemical-potential derivative is the reference's
+forward difference with h = 1e-7 along e_l - e_N
+(``Engines/NCGPU_new.py:301-313``), including its exact sequence of
+perturb / restore operations on the input vector.
+
+``c ln c`` is evaluated at max(c, CMIN): the reference does not guard
+``log`` at c <= 0 (it returns NaN there); the clamp is applied identically in
+the oracle and on the device.
+
+Coefficient arrays (float64):
+    g0 : [N, 5]         a, b, c, d, e of G_i(T)
+    L  : [N(N-1)/2, 3, 2]  pair (i<j) in lexicographic order, k = 0..2, (A, B)
+"""
+
+import numpy as np
+
+CMIN = 1e-10
+FD_H = 0.0000001
+FD_INV = 10000000.0
+
+
+def pairs(n):
+    return [(i, j) for i in range(n) for j in range(i + 1, n)]
+
+
+def gibbs(cs, T, g0, L, R):
+    """G for component fractions ``cs`` (list of N arrays, c_N last) and T."""
+    n = len(cs)
+    lnT = np.log(T)
+    G = np.zeros(np.broadcast(cs[0], T).shape)
+    for i in range(n):
+        a, b, c, d, e = g0[i]
+        gi = a + b * T + c * T * lnT + d * T * T + e / T
+        G = G + cs[i] * gi
+    for p, (i, j) in enumerate(pairs(n)):
+        L0 = L[p, 0, 0] + L[p, 0, 1] * T
+        L1 = L[p, 1, 0] + L[p, 1, 1] * T
+        L2 = L[p, 2, 0] + L[p, 2, 1] * T
+        dd = cs[i] - cs[j]
+        P = L0 + dd * (L1 + dd * L2)
+        G = G + cs[i] * cs[j] * P
+    ideal = np.zeros_like(G)
+    for i in range(n):
+        ci = np.maximum(cs[i], CMIN)
+        ideal = ideal + ci * np.log(ci)
+    return G + R * T * ideal
+
+
+def fd_derivatives(cs, T, g0, L, R, G=None):
+    """Forward differences of Engines/NCGPU_new.py:301-313, same op order.
+
+    Returns (G, [dG/dc_l for l < N-1]).
+    """
+    n = len(cs)
+    arr = [np.array(c, dtype=np.float64, copy=True) for c in cs]
+    if G is None:
+        G = gibbs(arr, T, g0, L, R)
+    arr[n - 1] = arr[n - 1] - FD_H
+    out = []
+    for l in range(n - 1):
+        arr[l] = arr[l] + FD_H
+        out.append(FD_INV * (gibbs(arr, T, g0, L, R) - G))
+        arr[l] = arr[l] - FD_H
+    return G, out
+
+
+def analytic_derivatives(cs, T, g0, L, R):
+    """dG/dc_l along e_l - e_N (exact), used by the FP32 device path."""
+    n = len(cs)
+    lnT = np.log(T)
+    dG = []
+    part = []
+    for i in range(n):
+        a, b, c, d, e = g0[i]
+        gi = a + b * T + c * T * lnT + d * T * T + e / T
+        ci = np.maximum(cs[i], CMIN)
+        clamp = (cs[i] > CMIN)
+        part.append(gi + R * T * np.where(clamp, np.log(ci) + 1.0, 0.0))
+    for p, (i, j) in enumerate(pairs(n)):
+        L0 = L[p, 0, 0] + L[p, 0, 1] * T
+        L1 = L[p, 1, 0] + L[p, 1, 1] * T
+        L2 = L[p, 2, 0] + L[p, 2, 1] * T
+        dd = cs[i] - cs[j]
+        P = L0 + dd * (L1 + dd * L2)
+        dP = L1 + 2.0 * dd * L2
+        part[i] = part[i] + cs[j] * P + cs[i] * cs[j] * dP
+        part[j] = part[j] + cs[i] * P - cs[i] * cs[j] * dP
+    for l in range(n - 1):
+        dG.append(part[l] - part[n - 1])
+    return dG

@@ -1,0 +1,30 @@
+"""Allen-Cahn engine (class CahnAllen, Engines/CahnAllen.py:285-362 of the
+reference), explicit solver on the GPU:
+phi <- phi + dt*M*(eps^2 lap(phi) - 16 W (4phi^3 - 6phi^2 + 2phi))
+(CahnAllen.py:128-135, D3 resolution: parameters from user_data)."""
+
+import numpy as np
+
+from ..simulation import Simulation
+from . import _gpu
+
+
+class CahnAllen(Simulation):
+    _DEFAULT_USER_DATA = {"M": 0.1, "epsilon": 1., "W": 1., "solver": "explicit", "gmres": False, "adi": False}
+
+    def init_fields(self):
+        """phi = 0.4995 + 0.001 U[0,1) (CahnAllen.py:307-312), seeded."""
+        rng = np.random.default_rng(self.seed)
+        phi = 0.001 * rng.random(self.dimensions) + 0.4995
+        self.add_field(phi, "phi")
+
+    def just_before_simulating(self):
+        if self.dx is None:
+            self.dx = 1.0
+        super().just_before_simulating()
+
+    def simulation_loop(self):
+        _gpu.solver_check(self)
+        u = self.user_data
+        _gpu.stencil_step(self, "ppf_allencahn_step", float(u["M"]), float(u["W"]), float(u["epsilon"]),
+                          float(self.dt), nactive=1)

@@ -1,0 +1,51 @@
+"""Cahn-Hilliard engine (Engines/CahnHilliard.py:347-421 of the reference),
+explicit solver on the GPU:
+mu = -eps^2 lap(phi) + (4phi^3 - 6phi^2 + 2phi);  phi <- phi + dt*M*lap(mu)
+(CahnHilliard.py:134-146 with the D3 resolution: dt applied, mu on the +1
+ring from a consistent 2-cell halo -- periodic wrap, or zero flux on
+non-periodic sides, which conserves mass)."""
+
+import numpy as np
+
+from ..simulation import Simulation
+from . import _gpu
+
+
+class CahnHilliard(Simulation):
+    _DEFAULT_USER_DATA = {"M": 0.1, "epsilon": 1., "W": 1., "solver": "explicit", "gmres": False, "adi": False}
+
+    def init_fields(self):
+        """phi = 0.4995 + 0.001 U[0,1) (CahnHilliard.py:365-370), seeded;
+        user_data "c0" and "noise" override the mean and the half-width."""
+        rng = np.random.default_rng(self.seed)
+        c0 = self.user_data.get("c0", 0.5)
+        amp = self.user_data.get("noise", 0.0005)
+        phi = c0 + amp * (2 * rng.random(self.dimensions) - 1)
+        self.add_field(phi, "phi")
+
+    def just_before_simulating(self):
+        if self.dx is None:
+            self.dx = 1.0
+        super().just_before_simulating()
+
+    def simulation_loop(self):
+        _gpu.solver_check(self)
+        u = self.user_data
+        _gpu.stencil_step(self, "ppf_cahnhilliard_step", float(u["M"]), float(u["epsilon"]), float(self.dt),
+                          nactive=1)
+
+    def free_energy(self):
+        """FP64 F = sum[c^2(1-c)^2 + eps^2/2 |grad c|^2] dx^d on the device."""
+        import torch
+        from .. import _lib
+        if self._fields_gpu_device is None:
+            self.send_fields_to_GPU()
+        L = _lib.lib()
+        dev = self._fields_gpu_device.device
+        work = torch.empty(int(L.ppf_reduce_work_bytes(self._grid)) // 8 + 1, dtype=torch.float64, device=dev)
+        out = torch.zeros(1, dtype=torch.float64, device=dev)
+        _lib.check(L.ppf_reduce(self._grid, _lib.ptr(self._fields_gpu_device),
+                                _lib.ptr(self._boundary_conditions_gpu_device), 1,
+                                float(self.user_data["epsilon"]), _lib.ptr(out), _lib.ptr(work),
+                                _lib.stream_handle()), "reduce")
+        return float(out.item())

@@ -1,0 +1,30 @@
+"""Diffusion engine (Engines/Diffusion.py:635-733 of the reference), explicit
+solver on the GPU: c <- c + dt*D*lap(c) (Diffusion.py:198-208, with the D1
+resolution: D from user_data).  Replaces DiffusionGPU (DiffusionGPU.py:18-117),
+which omitted dt and could not run."""
+
+import numpy as np
+
+from ..simulation import Simulation
+from . import _gpu
+
+
+class Diffusion(Simulation):
+    _DEFAULT_USER_DATA = {"D": 0.1, "solver": "explicit", "gmres": False, "adi": False}
+
+    def init_fields(self):
+        """Square (cube) of c = 1 in the middle half (Diffusion.py:641-659)."""
+        dim = self.dimensions
+        c = np.zeros(dim)
+        sl = tuple(slice(n // 4, 3 * n // 4) for n in dim)
+        c[sl] = 1
+        self.add_field(c, "c")
+
+    def just_before_simulating(self):
+        if self.dx is None:
+            self.dx = 1.0
+        super().just_before_simulating()
+
+    def simulation_loop(self):
+        _gpu.solver_check(self)
+        _gpu.stencil_step(self, "ppf_diffusion_step", float(self.user_data["D"]), float(self.dt), nactive=1)

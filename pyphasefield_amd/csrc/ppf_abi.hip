@@ -1,0 +1,168 @@
+// extern "C" entry points of libppf_hip.so: argument validation, dtype
+// dispatch and thread-local error reporting.  No exception crosses the ABI.
+#include <cstdio>
+#include <string>
+
+#include "ppf_launch.h"
+
+namespace ppf {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(int(e), std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+int check_grid(const ppf_grid* g, int lo, int hi, const char* what) {
+  if (!g) return fail(PPF_EINVAL, std::string(what) + ": null grid");
+  if (g->ndim < lo || g->ndim > hi)
+    return fail(PPF_EINVAL, std::string(what) + ": unsupported ndim " + std::to_string(g->ndim));
+  if (g->dtype != PPF_F32 && g->dtype != PPF_F64) return fail(PPF_EINVAL, std::string(what) + ": bad dtype");
+  for (int a = 0; a < g->ndim; ++a) {
+    if (g->n[a] < 1) return fail(PPF_EINVAL, std::string(what) + ": empty axis");
+    if (g->bc[a] < PPF_PERIODIC || g->bc[a] > PPF_DIRICHLET)
+      return fail(PPF_EINVAL, std::string(what) + ": bad boundary kind");
+  }
+  if (g->ndim >= 2 && g->sy < g->n[0] + 2) return fail(PPF_EINVAL, std::string(what) + ": row stride too small");
+  if (g->ndim >= 3 && g->sz < g->sy * (g->n[1] + 2))
+    return fail(PPF_EINVAL, std::string(what) + ": plane stride too small");
+  if (g->nfields < 1) return fail(PPF_EINVAL, std::string(what) + ": no fields");
+  if (!(g->dx > 0.0)) return fail(PPF_EINVAL, std::string(what) + ": dx must be positive");
+  return 0;
+}
+
+}  // namespace ppf
+
+using namespace ppf;
+
+extern "C" {
+
+int ppf_abi_version(void) { return PPF_ABI_VERSION; }
+
+const char* ppf_last_error(void) { return g_last_error.c_str(); }
+
+int ppf_apply_bc(const ppf_grid* g, void* fields, const void* bcvals, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "apply_bc")) return rc;
+  if (!fields) return fail(PPF_EINVAL, "apply_bc: null fields");
+  return g->dtype == PPF_F32 ? apply_bc<float>(*g, fields, bcvals, stream_of(s))
+                             : apply_bc<double>(*g, fields, bcvals, stream_of(s));
+}
+
+int ppf_apply_bc_temperature(const ppf_grid* g, void* T, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "apply_bc_temperature")) return rc;
+  if (!T) return fail(PPF_EINVAL, "apply_bc_temperature: null T");
+  return g->dtype == PPF_F32 ? apply_bc_temperature<float>(*g, T, stream_of(s))
+                             : apply_bc_temperature<double>(*g, T, stream_of(s));
+}
+
+int ppf_fill_temperature(const ppf_grid* g, void* out, const ppf_temperature* t, int64_t step, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "fill_temperature")) return rc;
+  if (!out || !t) return fail(PPF_EINVAL, "fill_temperature: null argument");
+  if (t->kind == PPF_T_ARRAY && !t->data) return fail(PPF_EINVAL, "fill_temperature: null T array");
+  return g->dtype == PPF_F32 ? fill_temperature<float>(*g, out, *t, step, stream_of(s))
+                             : fill_temperature<double>(*g, out, *t, step, stream_of(s));
+}
+
+int ppf_diffusion_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, double D, double dt,
+                       ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "diffusion_step")) return rc;
+  if (!in || !out || in == out) return fail(PPF_EINVAL, "diffusion_step: need distinct in/out buffers");
+  return g->dtype == PPF_F32 ? diffusion_step<float>(*g, in, out, bcvals, D, dt, stream_of(s))
+                             : diffusion_step<double>(*g, in, out, bcvals, D, dt, stream_of(s));
+}
+
+int ppf_allencahn_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, double M, double W,
+                       double epsilon, double dt, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "allencahn_step")) return rc;
+  if (!in || !out || in == out) return fail(PPF_EINVAL, "allencahn_step: need distinct in/out buffers");
+  return g->dtype == PPF_F32 ? allencahn_step<float>(*g, in, out, bcvals, M, W, epsilon, dt, stream_of(s))
+                             : allencahn_step<double>(*g, in, out, bcvals, M, W, epsilon, dt, stream_of(s));
+}
+
+int ppf_cahnhilliard_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, double M,
+                          double epsilon, double dt, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "cahnhilliard_step")) return rc;
+  if (!in || !out || in == out) return fail(PPF_EINVAL, "cahnhilliard_step: need distinct in/out buffers");
+  return g->dtype == PPF_F32 ? cahnhilliard_step<float>(*g, in, out, bcvals, M, epsilon, dt, stream_of(s))
+                             : cahnhilliard_step<double>(*g, in, out, bcvals, M, epsilon, dt, stream_of(s));
+}
+
+int ppf_warren_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_warren_params* p,
+                    uint64_t seed, int64_t step, ppf_stream_t s) {
+  if (int rc = check_grid(g, 2, 2, "warren_step")) return rc;
+  if (g->nfields < 2) return fail(PPF_EINVAL, "warren_step: needs fields (phi, c)");
+  if (!in || !out || in == out || !p) return fail(PPF_EINVAL, "warren_step: need distinct in/out and params");
+  return g->dtype == PPF_F32 ? warren_step<float>(*g, in, out, bcvals, *p, seed, step, stream_of(s))
+                             : warren_step<double>(*g, in, out, bcvals, *p, seed, step, stream_of(s));
+}
+
+static int check_ncomp(const ppf_grid* g, const ppf_grid* tg, const ppf_ncomp_params* p, const char* what) {
+  if (int rc = check_grid(g, 2, 2, what)) return rc;
+  if (!p || !tg) return fail(PPF_EINVAL, std::string(what) + ": null params");
+  if (p->ncomp < 2 || p->ncomp > PPF_MAX_COMP) return fail(PPF_EINVAL, std::string(what) + ": ncomp out of range");
+  if (g->nfields != p->ncomp + 2) return fail(PPF_EINVAL, std::string(what) + ": field stack must hold N+2 fields");
+  if (tg->nfields != 2 * p->ncomp - 1 || tg->n[0] != g->n[0] || tg->n[1] != g->n[1] || tg->dtype != g->dtype ||
+      tg->sy < g->n[0] + 2)
+    return fail(PPF_EINVAL, std::string(what) + ": transfer grid does not match");
+  return 0;
+}
+
+int ppf_ncomp_thermo(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                     const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                     uint64_t seed, int64_t step, ppf_stream_t s) {
+  if (int rc = check_ncomp(g, tg, p, "ncomp_thermo")) return rc;
+  if (!in || !transfer || !t || !th) return fail(PPF_EINVAL, "ncomp_thermo: null argument");
+  if (th->liquid.ncomp != p->ncomp || th->solid.ncomp != p->ncomp)
+    return fail(PPF_EINVAL, "ncomp_thermo: thermodynamic model has a different component count");
+  if (g->dtype == PPF_F32 && th->dg_mode == PPF_DG_FORWARD_DIFF)
+    return fail(PPF_EINVAL, "ncomp_thermo: the h=1e-7 forward difference needs FP64; use PPF_DG_ANALYTIC");
+  return g->dtype == PPF_F32
+             ? ncomp_thermo<float>(*g, in, bcvals, *t, *tg, transfer, *p, *th, seed, step, stream_of(s))
+             : ncomp_thermo<double>(*g, in, bcvals, *t, *tg, transfer, *p, *th, seed, step, stream_of(s));
+}
+
+int ppf_ncomp_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_temperature* t,
+                   const ppf_grid* tg, const void* transfer, const ppf_ncomp_params* p, uint64_t seed,
+                   int64_t step, ppf_stream_t s) {
+  if (int rc = check_ncomp(g, tg, p, "ncomp_step")) return rc;
+  if (!in || !out || in == out || !transfer || !t) return fail(PPF_EINVAL, "ncomp_step: null argument");
+  return g->dtype == PPF_F32
+             ? ncomp_step<float>(*g, in, out, bcvals, *t, *tg, transfer, *p, seed, step, stream_of(s))
+             : ncomp_step<double>(*g, in, out, bcvals, *t, *tg, transfer, *p, seed, step, stream_of(s));
+}
+
+int64_t ppf_reduce_work_bytes(const ppf_grid* g) {
+  if (!g) return 0;
+  return int64_t(kReduceBlocks) * int64_t(g->nfields > 0 ? g->nfields : 1) * int64_t(sizeof(double));
+}
+
+int ppf_reduce(const ppf_grid* g, const void* in, const void* bcvals, int kind, double epsilon, double* out,
+               void* work, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "reduce")) return rc;
+  if (kind != 0 && kind != 1) return fail(PPF_EINVAL, "reduce: kind must be 0 (sum) or 1 (free energy)");
+  if (!in || !out || !work) return fail(PPF_EINVAL, "reduce: null argument");
+  return g->dtype == PPF_F32 ? reduce<float>(*g, in, bcvals, kind, epsilon, out, work, stream_of(s))
+                             : reduce<double>(*g, in, bcvals, kind, epsilon, out, work, stream_of(s));
+}
+
+int ppf_thermo_eval(const ppf_thermo* th, int64_t n, const double* T, const double* c, double* out,
+                    ppf_stream_t s) {
+  if (!th || !T || !c || !out) return fail(PPF_EINVAL, "thermo_eval: null argument");
+  if (n < 0) return fail(PPF_EINVAL, "thermo_eval: negative count");
+  if (n == 0) return 0;
+  int N = th->liquid.ncomp;
+  if (N != th->solid.ncomp) return fail(PPF_EINVAL, "thermo_eval: phases disagree on component count");
+  if (N < 2 || N > PPF_MAX_COMP) return fail(PPF_EINVAL, "thermo_eval: ncomp must be 2..4");
+  return thermo_eval(*th, n, T, c, out, stream_of(s));
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+// Device-side helpers shared by the engine kernels: error reporting, the
+// boundary-condition fold used by every halo load, and the counter RNG.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/ppf_hip.h"
+
+namespace ppf {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_launch(const char* what);
+int check_grid(const ppf_grid* g, int want_ndim_min, int want_ndim_max, const char* what);
+
+inline hipStream_t stream_of(ppf_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ----------------------------------------------------------- fold rules --
+// Padded coordinate p along an axis with n interior cells (interior 1..n).
+// The value of a halo cell is produced from interior cells by the rules of
+// ppf_gpu_utils.py:11-96: PERIODIC wraps, NEUMANN copies the neighbour and
+// adds -/+ dx*bc, DIRICHLET returns bc.  Corners compose the rules in the
+// reference's kernel order: x innermost, then y, then z.
+
+struct Axis {
+  int64_t n;
+  int64_t stride;
+  int bc;
+};
+
+// Resolve one axis: returns the interior coordinate to read and, for a
+// non-periodic halo, sets *halo to the halo coordinate (0 or n+1) whose bc
+// value applies and *sign to -1 (low side) or +1 (high side).  For
+// coordinates more than one cell outside on a non-periodic axis (second halo
+// ring, values never used by the engines) the nearest halo cell is used.
+__device__ __forceinline__ int64_t resolve(int64_t p, const Axis& a, int64_t* halo, int* sign) {
+  *halo = -1;
+  *sign = 0;
+  if (p >= 1 && p <= a.n) return p;
+  if (a.bc == PPF_PERIODIC) {
+    int64_t q = p;
+    if (q < 1) q += a.n;
+    if (q > a.n) q -= a.n;
+    if (q < 1) q = 1;           // domains thinner than the halo depth
+    if (q > a.n) q = a.n;
+    return q;
+  }
+  if (p < 1) {
+    *halo = 0;
+    *sign = -1;
+    return 1;
+  }
+  *halo = a.n + 1;
+  *sign = 1;
+  return a.n;
+}
+
+// Value of padded cell (x, y, z) of a field with base pointer `f` (field
+// offset applied) and bc-value pointer `b` (may be null = zeros).
+template <typename R>
+__device__ __forceinline__ R fold_x(const R* __restrict__ f, const R* __restrict__ b, const Axis& ax,
+                                    int64_t x, int64_t row, R dx) {
+  int64_t h;
+  int s;
+  int64_t xi = resolve(x, ax, &h, &s);
+  if (h < 0) return f[row + xi];
+  R bv = b ? b[row + h] : R(0);
+  if (ax.bc == PPF_NEUMANN) return f[row + xi] + (s < 0 ? -dx : dx) * bv;
+  return bv;
+}
+
+template <typename R>
+__device__ __forceinline__ R fold_xy(const R* __restrict__ f, const R* __restrict__ b, const Axis& ax,
+                                     const Axis& ay, int64_t x, int64_t y, int64_t plane, R dx) {
+  int64_t h;
+  int s;
+  int64_t yi = resolve(y, ay, &h, &s);
+  if (h < 0) return fold_x(f, b, ax, x, plane + yi * ay.stride, dx);
+  if (ay.bc == PPF_NEUMANN) {
+    // bc value sits at (x clamped into the padded range, halo row)
+    int64_t xb = x < 0 ? 0 : (x > ax.n + 1 ? ax.n + 1 : x);
+    R bv = b ? b[plane + h * ay.stride + xb] : R(0);
+    return fold_x(f, b, ax, x, plane + yi * ay.stride, dx) + (s < 0 ? -dx : dx) * bv;
+  }
+  int64_t xb = x < 0 ? 0 : (x > ax.n + 1 ? ax.n + 1 : x);
+  return b ? b[plane + h * ay.stride + xb] : R(0);
+}
+
+template <typename R>
+__device__ __forceinline__ R fold_xyz(const R* __restrict__ f, const R* __restrict__ b, const Axis& ax,
+                                      const Axis& ay, const Axis& az, int64_t x, int64_t y, int64_t z,
+                                      R dx) {
+  int64_t h;
+  int s;
+  int64_t zi = resolve(z, az, &h, &s);
+  if (h < 0) return fold_xy(f, b, ax, ay, x, y, zi * az.stride, dx);
+  int64_t xb = x < 0 ? 0 : (x > ax.n + 1 ? ax.n + 1 : x);
+  int64_t yb = y < 0 ? 0 : (y > ay.n + 1 ? ay.n + 1 : y);
+  R bv = b ? b[h * az.stride + yb * ay.stride + xb] : R(0);
+  if (az.bc == PPF_NEUMANN) return fold_xy(f, b, ax, ay, x, y, zi * az.stride, dx) + (s < 0 ? -dx : dx) * bv;
+  return bv;
+}
+
+// Temperature rule (ppf_gpu_utils.py:197-319): periodic wrap, else zero
+// gradient; returns the interior coordinate to read.
+__device__ __forceinline__ int64_t resolve_t(int64_t p, int64_t n, int bc) {
+  if (p >= 1 && p <= n) return p;
+  if (bc == PPF_PERIODIC) {
+    int64_t q = p < 1 ? p + n : p - n;
+    return q < 1 ? 1 : (q > n ? n : q);
+  }
+  return p < 1 ? 1 : n;
+}
+
+// Temperature at padded (x, y, z) for the engine step `step`.
+template <typename R>
+__device__ __forceinline__ R temperature_at(const ppf_temperature& t, const int64_t n[3], const int bc[3],
+                                            int64_t x, int64_t y, int64_t z, double dx, int64_t step) {
+  if (t.kind == PPF_T_CONST) return R(t.T0);
+  int64_t xi = resolve_t(x, n[0], bc[0]);
+  int64_t yi = resolve_t(y, n[1], bc[1]);
+  int64_t zi = resolve_t(z, n[2], bc[2]);
+  if (t.kind == PPF_T_ARRAY) {
+    const R* d = static_cast<const R*>(t.data);
+    return d[zi * t.sz + yi * t.sy + xi];
+  }
+  // LINEAR: evaluated in double in the same order as the host/oracle formula
+  double T = t.T0;
+  if (t.dTdx != 0.0) T = T + double(xi - 1) * (dx * t.dTdx);
+  if (t.dTdy != 0.0) T = T + double(yi - 1) * (dx * t.dTdy);
+  if (t.dTdz != 0.0) T = T + double(zi - 1) * (dx * t.dTdz);
+  if (t.dTdt != 0.0) T = T + double(step - 1) * (t.dTdt * t.dt);
+  return R(T);
+}
+
+// ------------------------------------------------------------ counter RNG --
+// Stateless hash of (seed, step, stream, cell); see the oracle's rng module
+// for the definition shared with the CPU side.
+constexpr uint64_t kNStreams = 16;
+enum { S_WARREN_PHI = 0, S_NC_PHI = 1, S_NC_Q1 = 2, S_NC_Q4 = 3, S_NC_C0 = 4 };
+
+__device__ __forceinline__ uint64_t hash64(uint64_t seed, uint64_t counter) {
+  uint64_t z = seed + counter * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t rng_counter(int64_t step, int stream, uint64_t ncells, uint64_t cell) {
+  return (uint64_t(step) * kNStreams + uint64_t(stream)) * ncells + cell;
+}
+
+template <typename R>
+__device__ __forceinline__ R uniform01(uint64_t h);
+template <>
+__device__ __forceinline__ double uniform01<double>(uint64_t h) {
+  return double(h >> 11) * 0x1.0p-53;
+}
+template <>
+__device__ __forceinline__ float uniform01<float>(uint64_t h) {
+  return float(uint32_t(h >> 40)) * 0x1.0p-24f;
+}
+
+template <typename R>
+__device__ __forceinline__ R normal01(uint64_t h);
+template <>
+__device__ __forceinline__ double normal01<double>(uint64_t h) {
+  double u1 = (double(h >> 32) + 1.0) * 0x1.0p-32;
+  double u2 = double(h & 0xffffffffull) * 0x1.0p-32;
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
+}
+template <>
+__device__ __forceinline__ float normal01<float>(uint64_t h) {
+  float u1 = (float(uint32_t(h >> 40)) + 1.0f) * 0x1.0p-24f;
+  float u2 = float(uint32_t(h & 0xffffffffull) >> 8) * 0x1.0p-24f;
+  return sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+}
+
+// XCD-aware block order: the hardware deals workgroups round-robin over the
+// 8 XCDs; remap so that consecutive logical tiles (which share halo rows in
+// L2) are handled by the same XCD.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t nblocks) {
+  constexpr uint32_t kXcd = 8;
+  uint32_t per = nblocks / kXcd;
+  if (per == 0 || b >= per * kXcd) return b;
+  return (b % kXcd) * per + b / kXcd;
+}
+
+}  // namespace ppf

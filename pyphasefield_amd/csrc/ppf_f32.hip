@@ -1,0 +1,24 @@
+// FP32 instantiation of every engine launcher (bulk stencil arithmetic in
+// single precision; FP64 reductions inside the reduce kernels).
+#include "ppf_ncomp.inl"
+#include "ppf_warren.inl"
+
+namespace ppf {
+template int apply_bc<float>(const ppf_grid&, void*, const void*, hipStream_t);
+template int apply_bc_temperature<float>(const ppf_grid&, void*, hipStream_t);
+template int fill_temperature<float>(const ppf_grid&, void*, const ppf_temperature&, int64_t, hipStream_t);
+template int diffusion_step<float>(const ppf_grid&, const void*, void*, const void*, double, double, hipStream_t);
+template int allencahn_step<float>(const ppf_grid&, const void*, void*, const void*, double, double, double, double,
+                                   hipStream_t);
+template int cahnhilliard_step<float>(const ppf_grid&, const void*, void*, const void*, double, double, double,
+                                      hipStream_t);
+template int reduce<float>(const ppf_grid&, const void*, const void*, int, double, double*, void*, hipStream_t);
+template int warren_step<float>(const ppf_grid&, const void*, void*, const void*, const ppf_warren_params&,
+                                uint64_t, int64_t, hipStream_t);
+template int ncomp_thermo<float>(const ppf_grid&, const void*, const void*, const ppf_temperature&,
+                                 const ppf_grid&, void*, const ppf_ncomp_params&, const ppf_thermo&, uint64_t,
+                                 int64_t, hipStream_t);
+template int ncomp_step<float>(const ppf_grid&, const void*, void*, const void*, const ppf_temperature&,
+                               const ppf_grid&, const void*, const ppf_ncomp_params&, uint64_t, int64_t,
+                               hipStream_t);
+}  // namespace ppf

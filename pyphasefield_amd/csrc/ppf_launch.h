@@ -1,0 +1,44 @@
+// Typed launchers (instantiated for float in ppf_f32.hip and for double in
+// ppf_f64.hip) behind the extern "C" entry points of ppf_abi.hip.
+#pragma once
+
+#include "ppf_common.h"
+
+namespace ppf {
+
+template <typename R>
+int apply_bc(const ppf_grid& g, void* fields, const void* bcvals, hipStream_t s);
+template <typename R>
+int apply_bc_temperature(const ppf_grid& g, void* T, hipStream_t s);
+template <typename R>
+int fill_temperature(const ppf_grid& g, void* out, const ppf_temperature& t, int64_t step, hipStream_t s);
+template <typename R>
+int diffusion_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double D, double dt,
+                   hipStream_t s);
+template <typename R>
+int allencahn_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double W,
+                   double eps, double dt, hipStream_t s);
+template <typename R>
+int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M,
+                      double eps, double dt, hipStream_t s);
+template <typename R>
+int reduce(const ppf_grid& g, const void* in, const void* bcvals, int kind, double eps, double* out,
+           void* work, hipStream_t s);
+template <typename R>
+int warren_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_warren_params& p,
+                uint64_t seed, int64_t step, hipStream_t s);
+template <typename R>
+int ncomp_thermo(const ppf_grid& g, const void* in, const void* bcvals, const ppf_temperature& t, const ppf_grid& tg,
+                 void* transfer, const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, int64_t step,
+                 hipStream_t s);
+template <typename R>
+int ncomp_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
+               const ppf_grid& tg, const void* transfer, const ppf_ncomp_params& p, uint64_t seed,
+               int64_t step, hipStream_t s);
+
+int thermo_eval(const ppf_thermo& th, int64_t n, const double* T, const double* c, double* out, hipStream_t s);
+
+// number of partial sums the reduction kernels write (per field)
+constexpr int kReduceBlocks = 1024;
+
+}  // namespace ppf

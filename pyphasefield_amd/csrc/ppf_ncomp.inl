@@ -1,0 +1,484 @@
+// N-component alloy solidification engine (2D): the thermodynamic pass that
+// replaces NComponent_helper_kernel (Engines/NCGPU_new.py:256-313) and the
+// multi-field stencil that replaces NComponent_kernel (NCGPU_new.py:37-250).
+//
+// Field stack: phi, q1, q4, c_1..c_{N-1} (NCGPU_new.py:63-66).
+// Transfer stack written by the thermodynamic pass (2N-1 arrays, every padded
+// cell including the halo ring, like the reference helper):
+//   [0]        G_S - G_L
+//   [1+l]      M_c,l  = v_m c_l (D_L + h (D_S - D_L)) / (8.314 T)      (:308)
+//   [N+l]      dF/dc_l = (dGL_l + h (dGS_l - dGL_l))/v_m + (W_l - W_N) g T + noise  (:311)
+#pragma once
+
+#include "ppf_stencil.inl"
+
+namespace ppf {
+
+constexpr double kCMin = 1e-10;  // c ln c evaluated at max(c, kCMin)
+
+// phase coefficients evaluated at one temperature
+template <typename R, int N>
+struct PhaseAtT {
+  R gi[N];
+  R L[N * (N - 1) / 2][3];
+  R RT;
+};
+
+template <typename R, int N>
+__device__ __forceinline__ void phase_at(const ppf_rk_phase& ph, R T, R lnT, PhaseAtT<R, N>& o) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    R a = R(ph.g0[i][0]), b = R(ph.g0[i][1]), c = R(ph.g0[i][2]), d = R(ph.g0[i][3]), e = R(ph.g0[i][4]);
+    o.gi[i] = a + b * T + c * T * lnT + d * T * T + e / T;
+  }
+#pragma unroll
+  for (int p = 0; p < N * (N - 1) / 2; ++p) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.L[p][k] = R(ph.L[p][k][0]) + R(ph.L[p][k][1]) * T;
+  }
+  o.RT = R(ph.R) * T;
+}
+
+// sum_i c_i ln max(c_i, cmin), summed in component order
+template <typename R, int N>
+__device__ __forceinline__ R ideal_sum(const R (&c)[N]) {
+  R s = R(0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    R ci = c[i] > R(kCMin) ? c[i] : R(kCMin);
+    s = s + ci * log(ci);
+  }
+  return s;
+}
+
+// G of one phase (same operation order as the oracle's thermo.gibbs)
+template <typename R, int N>
+__device__ __forceinline__ R gibbs(const R (&c)[N], const PhaseAtT<R, N>& ph, R ideal) {
+  R G = R(0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) G = G + c[i] * ph.gi[i];
+  int p = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = i + 1; j < N; ++j, ++p) {
+      R dd = c[i] - c[j];
+      R P = ph.L[p][0] + dd * (ph.L[p][1] + dd * ph.L[p][2]);
+      G = G + c[i] * c[j] * P;
+    }
+  }
+  return G + ph.RT * ideal;
+}
+
+// exact dG/dc_l along e_l - e_N
+template <typename R, int N>
+__device__ __forceinline__ void gibbs_grad(const R (&c)[N], const PhaseAtT<R, N>& ph, R (&dG)[N]) {
+  R part[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) part[i] = ph.gi[i] + (c[i] > R(kCMin) ? ph.RT * (log(c[i]) + R(1)) : R(0));
+  int p = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = i + 1; j < N; ++j, ++p) {
+      R dd = c[i] - c[j];
+      R P = ph.L[p][0] + dd * (ph.L[p][1] + dd * ph.L[p][2]);
+      R dP = ph.L[p][1] + R(2) * dd * ph.L[p][2];
+      part[i] = part[i] + c[j] * P + c[i] * c[j] * dP;
+      part[j] = part[j] + c[i] * P - c[i] * c[j] * dP;
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) dG[l] = part[l] - part[N - 1];
+}
+
+struct NcK {
+  ppf_ncomp_params p;
+  ppf_thermo th;
+};
+
+template <typename R, int N>
+__device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, const NcK& k, int64_t step,
+                                            uint64_t seed, uint64_t ncells, uint64_t cell, R* dG, R* Mc,
+                                            R* dF) {
+  const ppf_ncomp_params& p = k.p;
+  R c[N];
+  R cN = R(1);
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) {
+    c[l] = cin[l];
+    cN = cN - cin[l];
+  }
+  c[N - 1] = cN;
+  R lnT = log(T);
+  PhaseAtT<R, N> PL, PS;
+  phase_at<R, N>(k.th.liquid, T, lnT, PL);
+  phase_at<R, N>(k.th.solid, T, lnT, PS);
+  R id0 = ideal_sum<R, N>(c);
+  R G_L = gibbs<R, N>(c, PL, id0);
+  R G_S = gibbs<R, N>(c, PS, id0);
+  *dG = G_S - G_L;
+  R g = (phi * phi) * (R(1) - phi) * (R(1) - phi);
+  R h = (phi * phi * phi) * (R(6) * phi * phi - R(15) * phi + R(10));
+  R dGL[N], dGS[N];
+  if (k.th.dg_mode == PPF_DG_FORWARD_DIFF) {
+    // NCGPU_new.py:301-313, with its in-place perturb / restore sequence
+    const R hstep = R(0.0000001);
+    c[N - 1] = c[N - 1] - hstep;
+#pragma unroll
+    for (int l = 0; l < N - 1; ++l) {
+      c[l] = c[l] + hstep;
+      R id = ideal_sum<R, N>(c);
+      dGL[l] = R(10000000.) * (gibbs<R, N>(c, PL, id) - G_L);
+      dGS[l] = R(10000000.) * (gibbs<R, N>(c, PS, id) - G_S);
+      c[l] = c[l] - hstep;
+    }
+  } else {
+    gibbs_grad<R, N>(c, PL, dGL);
+    gibbs_grad<R, N>(c, PS, dGS);
+  }
+  const R v_m = R(p.v_m), D_L = R(p.D_L), D_S = R(p.D_S);
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) {
+    Mc[l] = v_m * cin[l] * (D_L + h * (D_S - D_L)) / (R(8.314) * T);
+    R noise = R(0);
+    if (p.noise_c != 0.0) {
+      R xi = normal01<R>(hash64(seed, rng_counter(step, S_NC_C0 + l, ncells, cell)));
+      noise = R(p.noise_c) * sqrt(R(2.) * R(8.314) * T / v_m) * xi;
+    }
+    dF[l] = (dGL[l] + h * (dGS[l] - dGL[l])) / v_m + (R(p.W[l]) - R(p.W[N - 1])) * g * T + noise;
+  }
+}
+
+template <typename R, int N>
+__global__ __launch_bounds__(256) void k_ncomp_thermo(const R* __restrict__ in, const R* __restrict__ b, Dom d,
+                                                      R dx, ppf_temperature t, R* __restrict__ tr, int64_t tsy,
+                                                      int64_t tsf, NcK k, uint64_t seed, int64_t step) {
+  int64_t px = d.ax.n + 2, py = d.ay.n + 2;
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= px * py) return;
+  int64_t x = i % px, y = i / px;
+  R phi = fold(in, b, d, x, y, int64_t(0), dx);
+  R cin[N - 1];
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l)
+    cin[l] = fold(in + (3 + l) * d.sf, b ? b + (3 + l) * d.sf : nullptr, d, x, y, int64_t(0), dx);
+  int64_t n3[3] = {d.ax.n, d.ay.n, 1};
+  int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
+  R T = temperature_at<R>(t, n3, bc3, x, y, 1, double(dx), step);
+  R dG, Mc[N - 1], dF[N - 1];
+  thermo_cell<R, N>(cin, phi, T, k, step, seed, uint64_t(px * py), uint64_t(i), &dG, Mc, dF);
+  int64_t o = y * tsy + x;
+  tr[o] = dG;
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) {
+    tr[o + (1 + l) * tsf] = Mc[l];
+    tr[o + (N + l) * tsf] = dF[l];
+  }
+}
+
+template <typename R>
+int ncomp_thermo(const ppf_grid& g, const void* in, const void* bcvals, const ppf_temperature& t, const ppf_grid& tg,
+                 void* transfer,
+                 const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, int64_t step, hipStream_t s) {
+  Dom d = make_dom(g);
+  NcK k{p, th};
+  int64_t total = (g.n[0] + 2) * (g.n[1] + 2);
+  dim3 grid(unsigned((total + 255) / 256));
+  const R* fin = static_cast<const R*>(in);
+  const R* bc = static_cast<const R*>(bcvals);
+  R* tr = static_cast<R*>(transfer);
+  switch (p.ncomp) {
+    case 2:
+      hipLaunchKernelGGL((k_ncomp_thermo<R, 2>), grid, dim3(256), 0, s, fin, bc, d, R(g.dx), t, tr, tg.sy,
+                         tg.sf, k, seed, step);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_ncomp_thermo<R, 3>), grid, dim3(256), 0, s, fin, bc, d, R(g.dx), t, tr, tg.sy,
+                         tg.sf, k, seed, step);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_ncomp_thermo<R, 4>), grid, dim3(256), 0, s, fin, bc, d, R(g.dx), t, tr, tg.sy,
+                         tg.sf, k, seed, step);
+      break;
+    default:
+      return fail(PPF_EINVAL, "ncomp_thermo: ncomp must be 2..4");
+  }
+  return check_launch("ncomp_thermo");
+}
+
+// ------------------------------------------------------------ main step --
+constexpr int kNcTY = 8;  // NC tile: 64 x 8 outputs, 2 rows per thread
+
+template <typename R>
+__device__ __forceinline__ R divagradb(R a, R axp, R axm, R ayp, R aym, R b, R bxp, R bxm, R byp, R bym, R idx) {
+  // NCGPU_new.py:28-30
+  return R(0.5) * idx * idx *
+         ((axp + a) * (bxp - b) - (a + axm) * (b - bxm) + (ayp + a) * (byp - b) - (a + aym) * (b - bym));
+}
+
+template <typename R>
+__device__ __forceinline__ R f_ori_term(R D_q, R D_q_xp, R D_q_xm, R D_q_yp, R D_q_ym, R mgq_xp, R mgq_xm,
+                                        R mgq_yp, R mgq_ym, R q, R q_xp, R q_xm, R q_yp, R q_ym, R idx) {
+  // NCGPU_new.py:33-35
+  return R(0.5) * idx * idx *
+         ((D_q + D_q_xp) * (q_xp - q) / mgq_xp - (D_q + D_q_xm) * (q - q_xm) / mgq_xm +
+          (D_q + D_q_yp) * (q_yp - q) / mgq_yp - (D_q + D_q_ym) * (q - q_ym) / mgq_ym);
+}
+
+template <typename R, int W>
+struct Nb5 {
+  // centre and neighbours of an LDS tile at (ly, lx)
+  R c, xp, xm, yp, ym;
+  __device__ __forceinline__ Nb5(const R (*s)[W], int ly, int lx)
+      : c(s[ly][lx]), xp(s[ly][lx + 1]), xm(s[ly][lx - 1]), yp(s[ly + 1][lx]), ym(s[ly - 1][lx]) {}
+};
+
+template <typename R, int N>
+__global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ in, R* __restrict__ out,
+                                                         const R* __restrict__ b, Dom d, R dx, ppf_temperature t,
+                                                         const R* __restrict__ tr, int64_t tsy, int64_t tsf,
+                                                         NcK k, uint64_t seed, int64_t step) {
+  constexpr int H = 1, W = kTX + 2 * H, HH = kNcTY + 2 * H;
+  constexpr int NF = N + 2;         // phi, q1, q4, c_1..c_{N-1}
+  constexpr int NT = 2 * (N - 1);   // M_c and dF/dc tiles
+  __shared__ R sf[NF][HH][W];
+  __shared__ R sT[HH][W];
+  __shared__ R st[NT][HH][W];
+
+  uint32_t nbx = uint32_t((d.ax.n + kTX - 1) / kTX);
+  uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  int64_t x0 = 1 + int64_t(blk % nbx) * kTX;
+  int64_t y0 = 1 + int64_t(blk / nbx) * kNcTY;
+  bool interior = x0 - 1 >= 1 && x0 + kTX <= d.ax.n && y0 - 1 >= 1 && y0 + kNcTY <= d.ay.n;
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+    load_tile<R, W, HH>(sf[f], in + f * d.sf, b ? b + f * d.sf : nullptr, d, dx, x0 - 1, y0 - 1, interior);
+  int64_t n3[3] = {d.ax.n, d.ay.n, 1};
+  int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
+  for (int i = threadIdx.x; i < W * HH; i += kThreads) {
+    int ly = i / W, lx = i - ly * W;
+    int64_t x = x0 - 1 + lx, y = y0 - 1 + ly;
+    bool ok = x <= d.ax.n + 1 && y <= d.ay.n + 1;
+    sT[ly][lx] = ok ? temperature_at<R>(t, n3, bc3, x, y, 1, double(dx), step) : R(1);
+#pragma unroll
+    for (int f = 0; f < NT; ++f) st[f][ly][lx] = ok ? tr[(1 + f) * tsf + y * tsy + x] : R(0);
+  }
+  __syncthreads();
+
+  const ppf_ncomp_params& p = k.p;
+  const R idx = R(1) / dx;
+  const R v_m = R(p.v_m), M_qmax = R(p.M_qmax), Hh = R(p.H), y_e = R(p.y_e), beta = R(p.beta);
+  const R dt = R(p.dt), ebar = R(p.ebar), eqbar = R(p.eqbar);
+  const R ebar2 = ebar * ebar, eqbar2 = eqbar * eqbar;
+  const uint64_t ncells = uint64_t(d.ax.n + 2) * uint64_t(d.ay.n + 2);
+  int tx = threadIdx.x % kTX, tyb = threadIdx.x / kTX;
+  int64_t x = x0 + tx;
+  if (x > d.ax.n) return;
+  const int lx = tx + 1;
+  for (int kk = 0; kk < kNcTY / 4; ++kk) {
+    int ly = tyb + 4 * kk + 1;
+    int64_t y = y0 + ly - 1;
+    if (y > d.ay.n) break;
+    const R(*phi)[W] = sf[0];
+    const R(*q1)[W] = sf[1];
+    const R(*q4)[W] = sf[2];
+    R pc = phi[ly][lx];
+    // interpolating functions (NCGPU_new.py:84-87)
+    R g = (pc * pc) * (R(1) - pc) * (R(1) - pc);
+    R h = (pc * pc * pc) * (R(6) * pc * pc - R(15) * pc + R(10));
+    R hprime = R(30) * g;
+    R gprime = R(4) * pc * pc * pc - R(6) * pc * pc + R(2) * pc;
+    // gradients (:89-107)
+    R dphidx = R(0.5) * (phi[ly][lx + 1] - phi[ly][lx - 1]) * idx;
+    R dphidx2 = dphidx * dphidx;
+    R dphidx3 = dphidx2 * dphidx;
+    R dphidy = R(0.5) * (phi[ly + 1][lx] - phi[ly - 1][lx]) * idx;
+    R dphidy2 = dphidy * dphidy;
+    R dphidy3 = dphidy2 * dphidy;
+    R Tc = sT[ly][lx];
+    R dTdx = R(0.5) * idx * (sT[ly][lx + 1] - sT[ly][lx - 1]);
+    R dTdy = R(0.5) * idx * (sT[ly + 1][lx] - sT[ly - 1][lx]);
+    R d2phidx2 = (phi[ly][lx + 1] + phi[ly][lx - 1] - R(2) * pc) * idx * idx;
+    R d2phidy2 = (phi[ly + 1][lx] + phi[ly - 1][lx] - R(2) * pc) * idx * idx;
+    R lphi = d2phidx2 + d2phidy2;
+    R d2phidxy = R(0.25) * (phi[ly + 1][lx + 1] - phi[ly + 1][lx - 1] - phi[ly - 1][lx + 1] + phi[ly - 1][lx - 1]) *
+                 idx * idx;
+    R mag_grad_phi2 = dphidx * dphidx + dphidy * dphidy;
+    if (mag_grad_phi2 < R(1e-6)) mag_grad_phi2 = R(1e-6);
+    R mag_grad_phi4 = mag_grad_phi2 * mag_grad_phi2;
+    R mag_grad_phi8 = mag_grad_phi4 * mag_grad_phi4;
+    // orientation gradients (:109-125)
+    Nb5<R, W> Q1(q1, ly, lx), Q4(q4, ly, lx);
+    R dq1dx = R(0.5) * idx * (Q1.xp - Q1.xm);
+    R dq1dy = R(0.5) * idx * (Q1.yp - Q1.ym);
+    R dq4dx = R(0.5) * idx * (Q4.xp - Q4.xm);
+    R dq4dy = R(0.5) * idx * (Q4.yp - Q4.ym);
+    R mgq_xp = idx * sqrt((Q1.xp - Q1.c) * (Q1.xp - Q1.c) + (Q4.xp - Q4.c) * (Q4.xp - Q4.c));
+    R mgq_xm = idx * sqrt((Q1.xm - Q1.c) * (Q1.xm - Q1.c) + (Q4.xm - Q4.c) * (Q4.xm - Q4.c));
+    R mgq_yp = idx * sqrt((Q1.yp - Q1.c) * (Q1.yp - Q1.c) + (Q4.yp - Q4.c) * (Q4.yp - Q4.c));
+    R mgq_ym = idx * sqrt((Q1.ym - Q1.c) * (Q1.ym - Q1.c) + (Q4.ym - Q4.c) * (Q4.ym - Q4.c));
+    R mag_grad_q = R(0.5) * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
+    if (mgq_xp < beta) mgq_xp = beta;
+    if (mgq_xm < beta) mgq_xm = beta;
+    if (mgq_yp < beta) mgq_yp = beta;
+    if (mgq_ym < beta) mgq_ym = beta;
+    // psi terms (:127-146)
+    R q1c = Q1.c, q4c = Q4.c;
+    R q2q2 = (q1c * q1c - q4c * q4c);
+    R qq2 = R(2) * q1c * q4c;
+    R psix = q2q2 * dphidx - qq2 * dphidy;
+    R psiy = qq2 * dphidx + q2q2 * dphidy;
+    R psix2 = psix * psix;
+    R psix3 = psix2 * psix;
+    R psix4 = psix2 * psix2;
+    R psiy2 = psiy * psiy;
+    R psiy3 = psiy2 * psiy;
+    R psiy4 = psiy2 * psiy2;
+    R dq2q2dx = (R(2) * q1c * dq1dx - R(2) * q4c * dq4dx);
+    R dqq2dx = (R(2) * q4c * dq1dx + R(2) * q1c * dq4dx);
+    R dq2q2dy = (R(2) * q1c * dq1dy - R(2) * q4c * dq4dy);
+    R dqq2dy = (R(2) * q4c * dq1dy + R(2) * q1c * dq4dy);
+    R dpsixdx = dq2q2dx * dphidx + q2q2 * d2phidx2 - dqq2dx * dphidy - qq2 * d2phidxy;
+    R dpsixdy = dq2q2dy * dphidx + q2q2 * d2phidxy - dqq2dy * dphidy - qq2 * d2phidy2;
+    R dpsiydx = dq2q2dx * dphidy + q2q2 * d2phidxy + dqq2dx * dphidx + qq2 * d2phidx2;
+    R dpsiydy = dq2q2dy * dphidy + q2q2 * d2phidy2 + dqq2dy * dphidx + qq2 * d2phidxy;
+    R psi4 = psix4 + psiy4;
+    // anisotropic terms (:148-160)
+    R d_term_dx = dTdx * ((R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) / mag_grad_phi2 - dphidx * psi4 / mag_grad_phi4);
+    d_term_dx = d_term_dx + Tc *
+                                (R(6) * psix2 * dpsixdx * q2q2 + R(2) * psix3 * dq2q2dx +
+                                 R(6) * psiy2 * dpsiydx * qq2 + R(2) * psiy3 * dqq2dx) /
+                                mag_grad_phi2;
+    d_term_dx = d_term_dx - Tc * (R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) *
+                                (R(2) * dphidx * d2phidx2 + R(2) * dphidy * d2phidxy) / mag_grad_phi4;
+    d_term_dx = d_term_dx - Tc * (d2phidx2 * psi4 + dphidx * (R(4) * psix3 * dpsixdx + R(4) * psiy3 * dpsiydx)) /
+                                mag_grad_phi4;
+    d_term_dx = d_term_dx + Tc *
+                                (dphidx * psi4 *
+                                 (R(4) * dphidx3 * d2phidx2 + R(4) * dphidx * d2phidx2 * dphidy2 +
+                                  R(4) * dphidy * d2phidxy * dphidx2 + R(4) * dphidy3 * d2phidxy)) /
+                                mag_grad_phi8;
+    d_term_dx = d_term_dx * (R(4) * y_e * ebar2);
+    R d_term_dy =
+        dTdy * ((R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) / mag_grad_phi2 - dphidy * psi4 / mag_grad_phi4);
+    d_term_dy = d_term_dy + Tc *
+                                (R(-6) * psix2 * dpsixdy * qq2 - R(2) * psix3 * dqq2dy +
+                                 R(6) * psiy2 * dpsiydy * q2q2 + R(2) * psiy3 * dq2q2dy) /
+                                mag_grad_phi2;
+    d_term_dy = d_term_dy - Tc * (R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) *
+                                (R(2) * dphidx * d2phidxy + R(2) * dphidy * d2phidy2) / mag_grad_phi4;
+    d_term_dy = d_term_dy - Tc * (d2phidy2 * psi4 + dphidy * (R(4) * psix3 * dpsixdy + R(4) * psiy3 * dpsiydy)) /
+                                mag_grad_phi4;
+    d_term_dy = d_term_dy + Tc *
+                                (dphidy * psi4 *
+                                 (R(4) * dphidx3 * d2phidxy + R(4) * dphidx * d2phidxy * dphidy2 +
+                                  R(4) * dphidy * d2phidy2 * dphidx2 + R(4) * dphidy3 * d2phidy2)) /
+                                mag_grad_phi8;
+    d_term_dy = d_term_dy * (R(4) * y_e * ebar2);
+    // c_N, cW, M_phi (:162-173)
+    R cW = R(0), c_N = R(1), M_phi = R(0);
+#pragma unroll
+    for (int l = 0; l < N - 1; ++l) {
+      R ci = sf[3 + l][ly][lx];
+      cW = cW + ci * R(p.W[l]);
+      c_N = c_N - ci;
+      M_phi = M_phi + ci * R(p.M[l]);
+    }
+    cW = cW + c_N * R(p.W[N - 1]);
+    M_phi = M_phi + c_N * R(p.M[N - 1]);
+    R M_q = M_qmax + (R(1e-6) - M_qmax) * h;
+    // dphidt (:179-189)
+    R dG = tr[y * tsy + x];  // G_S - G_L
+    R dphidt = ebar2 * (R(1) - R(3) * y_e) * (Tc * lphi + dTdx * dphidx + dTdy * dphidy);
+    dphidt = dphidt + d_term_dx + d_term_dy;
+    dphidt = dphidt - hprime * dG / v_m;
+    dphidt = dphidt - gprime * Tc * cW;
+    dphidt = dphidt - R(4) * Hh * Tc * pc * mag_grad_q;
+    dphidt = dphidt * M_phi;
+    uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
+    if (p.noise_phi != 0.0) {
+      R xi = normal01<R>(hash64(seed, rng_counter(step, S_NC_PHI, ncells, cell)));
+      dphidt = dphidt + sqrt(R(2.) * R(8.314) * Tc * M_phi / v_m) * xi * R(p.noise_phi);
+    }
+    int64_t o = y * d.ay.stride + x;
+    // dc_i/dt (:191-203) and update (:243-249)
+#pragma unroll
+    for (int l = 0; l < N - 1; ++l) {
+      Nb5<R, W> Mc(st[l], ly, lx), dF(st[N - 1 + l], ly, lx);
+      R acc = divagradb(Mc.c, Mc.xp, Mc.xm, Mc.yp, Mc.ym, dF.c, dF.xp, dF.xm, dF.yp, dF.ym, idx);
+#pragma unroll
+      for (int m = 0; m < N - 1; ++m) {
+        Nb5<R, W> cj(sf[3 + m], ly, lx), dFj(st[N - 1 + m], ly, lx);
+        acc = acc - divagradb(Mc.c * cj.c, Mc.xp * cj.xp, Mc.xm * cj.xm, Mc.yp * cj.yp, Mc.ym * cj.ym, dFj.c,
+                              dFj.xp, dFj.xm, dFj.yp, dFj.ym, idx);
+      }
+      out[o + (3 + l) * d.sf] = acc * dt + sf[3 + l][ly][lx];
+    }
+    // dq/dt (:205-232)
+    R D_q = R(2) * Hh * Tc * (pc * pc);
+    R D_q_xp = R(2) * Hh * sT[ly][lx + 1] * (phi[ly][lx + 1] * phi[ly][lx + 1]);
+    R D_q_xm = R(2) * Hh * sT[ly][lx - 1] * (phi[ly][lx - 1] * phi[ly][lx - 1]);
+    R D_q_yp = R(2) * Hh * sT[ly + 1][lx] * (phi[ly + 1][lx] * phi[ly + 1][lx]);
+    R D_q_ym = R(2) * Hh * sT[ly - 1][lx] * (phi[ly - 1][lx] * phi[ly - 1][lx]);
+    R f_ori_1 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q1.c, Q1.xp, Q1.xm,
+                           Q1.yp, Q1.ym, idx);
+    R f_ori_4 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q4.c, Q4.xp, Q4.xm,
+                           Q4.yp, Q4.ym, idx);
+    R lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx * idx;
+    R lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx * idx;
+    R noise_q1 = R(0), noise_q4 = R(0);
+    if (p.noise_q != 0.0) {
+      R amp = R(p.noise_q) * sqrt(R(0.0000000001) * R(8.314) * Tc / v_m);
+      noise_q1 = amp * normal01<R>(hash64(seed, rng_counter(step, S_NC_Q1, ncells, cell)));
+      noise_q4 = amp * normal01<R>(hash64(seed, rng_counter(step, S_NC_Q4, ncells, cell)));
+    }
+    R t1 = f_ori_1 + lq1 * eqbar2 + noise_q1;
+    R t4 = f_ori_4 + lq4 * eqbar2 + noise_q4;
+    R dq1dt = M_q * ((R(1) - q1c * q1c) * t1 - q1c * q4c * t4);
+    R dq4dt = M_q * ((R(1) - q4c * q4c) * t4 - q1c * q4c * t1);
+    R phin = pc + dt * dphidt;
+    if (phin < R(0.0001)) phin = R(0.0001);
+    if (phin > R(0.9999)) phin = R(0.9999);
+    R q1n = q1c + dt * dq1dt;
+    R q4n = q4c + dt * dq4dt;
+    R renorm = sqrt(q1n * q1n + q4n * q4n);
+    out[o] = phin;
+    out[o + d.sf] = q1n / renorm;
+    out[o + 2 * d.sf] = q4n / renorm;
+  }
+}
+
+template <typename R>
+int ncomp_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
+               const ppf_grid& tg, const void* transfer, const ppf_ncomp_params& p, uint64_t seed, int64_t step,
+               hipStream_t s) {
+  Dom d = make_dom(g);
+  NcK k;
+  k.p = p;
+  k.th = ppf_thermo{};
+  unsigned blocks = unsigned(((g.n[0] + kTX - 1) / kTX) * ((g.n[1] + kNcTY - 1) / kNcTY));
+  const R* fin = static_cast<const R*>(in);
+  R* fout = static_cast<R*>(out);
+  const R* bc = static_cast<const R*>(bcvals);
+  const R* tr = static_cast<const R*>(transfer);
+  switch (p.ncomp) {
+    case 2:
+      hipLaunchKernelGGL((k_ncomp_step<R, 2>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t,
+                         tr, tg.sy, tg.sf, k, seed, step);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_ncomp_step<R, 3>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t,
+                         tr, tg.sy, tg.sf, k, seed, step);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_ncomp_step<R, 4>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t,
+                         tr, tg.sy, tg.sf, k, seed, step);
+      break;
+    default:
+      return fail(PPF_EINVAL, "ncomp_step: ncomp must be 2..4");
+  }
+  return check_launch("ncomp_step");
+}
+
+}  // namespace ppf

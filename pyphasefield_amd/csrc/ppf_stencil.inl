@@ -1,0 +1,516 @@
+// Stencil kernels of the Diffusion, Allen-Cahn and Cahn-Hilliard engines, the
+// boundary-condition materialisation, the temperature fill and the FP64
+// conservation reductions.  Included by ppf_f32.hip and ppf_f64.hip, which
+// instantiate every launcher for one precision.
+#pragma once
+
+#include "ppf_launch.h"
+
+namespace ppf {
+
+struct Dom {
+  Axis ax, ay, az;
+  int ndim;
+  int64_t sf;
+};
+
+inline Dom make_dom(const ppf_grid& g) {
+  Dom d;
+  d.ndim = g.ndim;
+  d.ax = Axis{g.n[0], 1, g.bc[0]};
+  d.ay = Axis{g.ndim >= 2 ? g.n[1] : 1, g.sy, g.ndim >= 2 ? g.bc[1] : PPF_PERIODIC};
+  d.az = Axis{g.ndim >= 3 ? g.n[2] : 1, g.sz, g.ndim >= 3 ? g.bc[2] : PPF_PERIODIC};
+  d.sf = g.sf;
+  return d;
+}
+
+__device__ __forceinline__ bool inside(const Dom& d, int64_t x, int64_t y, int64_t z) {
+  bool in = x >= 1 && x <= d.ax.n;
+  if (d.ndim >= 2) in = in && y >= 1 && y <= d.ay.n;
+  if (d.ndim >= 3) in = in && z >= 1 && z <= d.az.n;
+  return in;
+}
+
+__device__ __forceinline__ int64_t offset(const Dom& d, int64_t x, int64_t y, int64_t z) {
+  int64_t o = x;
+  if (d.ndim >= 2) o += y * d.ay.stride;
+  if (d.ndim >= 3) o += z * d.az.stride;
+  return o;
+}
+
+// value of padded cell (x,y,z) with the boundary rules folded in
+template <typename R>
+__device__ __forceinline__ R fold(const R* __restrict__ f, const R* __restrict__ b, const Dom& d, int64_t x,
+                                  int64_t y, int64_t z, R dx) {
+  if (inside(d, x, y, z)) return f[offset(d, x, y, z)];
+  if (d.ndim == 1) return fold_x(f, b, d.ax, x, 0, dx);
+  if (d.ndim == 2) return fold_xy(f, b, d.ax, d.ay, x, y, 0, dx);
+  return fold_xyz(f, b, d.ax, d.ay, d.az, x, y, z, dx);
+}
+
+// padded extents and decode of a linear padded index
+__device__ __forceinline__ void decode(const Dom& d, int64_t t, int64_t* x, int64_t* y, int64_t* z) {
+  int64_t px = d.ax.n + 2;
+  int64_t py = d.ndim >= 2 ? d.ay.n + 2 : 1;
+  *x = t % px;
+  int64_t r = t / px;
+  *y = d.ndim >= 2 ? r % py : 0;
+  *z = d.ndim >= 3 ? r / py : 0;
+}
+
+inline int64_t padded_cells(const ppf_grid& g) {
+  int64_t c = g.n[0] + 2;
+  if (g.ndim >= 2) c *= g.n[1] + 2;
+  if (g.ndim >= 3) c *= g.n[2] + 2;
+  return c;
+}
+
+inline int64_t interior_cells(const ppf_grid& g) {
+  int64_t c = g.n[0];
+  if (g.ndim >= 2) c *= g.n[1];
+  if (g.ndim >= 3) c *= g.n[2];
+  return c;
+}
+
+// ------------------------------------------------------------------ halos --
+template <typename R>
+__global__ __launch_bounds__(256) void k_apply_bc(R* __restrict__ f, const R* __restrict__ b, Dom d, R dx,
+                                                  int64_t total) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int fi = blockIdx.y;
+  if (t >= total) return;
+  int64_t x, y, z;
+  decode(d, t, &x, &y, &z);
+  if (inside(d, x, y, z)) return;
+  R* ff = f + fi * d.sf;
+  const R* bb = b ? b + fi * d.sf : nullptr;
+  R v;
+  if (d.ndim == 1) v = fold_x(ff, bb, d.ax, x, 0, dx);
+  else if (d.ndim == 2) v = fold_xy(ff, bb, d.ax, d.ay, x, y, 0, dx);
+  else v = fold_xyz(ff, bb, d.ax, d.ay, d.az, x, y, z, dx);
+  ff[offset(d, x, y, z)] = v;
+}
+
+template <typename R>
+int apply_bc(const ppf_grid& g, void* fields, const void* bcvals, hipStream_t s) {
+  Dom d = make_dom(g);
+  int64_t total = padded_cells(g);
+  dim3 grid(unsigned((total + 255) / 256), unsigned(g.nfields));
+  hipLaunchKernelGGL(k_apply_bc<R>, grid, dim3(256), 0, s, static_cast<R*>(fields),
+                     static_cast<const R*>(bcvals), d, R(g.dx), total);
+  return check_launch("apply_bc");
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_apply_bc_t(R* __restrict__ T, Dom d, int64_t total) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int64_t x, y, z;
+  decode(d, t, &x, &y, &z);
+  if (inside(d, x, y, z)) return;
+  int64_t xi = resolve_t(x, d.ax.n, d.ax.bc);
+  int64_t yi = d.ndim >= 2 ? resolve_t(y, d.ay.n, d.ay.bc) : 0;
+  int64_t zi = d.ndim >= 3 ? resolve_t(z, d.az.n, d.az.bc) : 0;
+  T[offset(d, x, y, z)] = T[offset(d, xi, yi, zi)];
+}
+
+template <typename R>
+int apply_bc_temperature(const ppf_grid& g, void* T, hipStream_t s) {
+  Dom d = make_dom(g);
+  int64_t total = padded_cells(g);
+  hipLaunchKernelGGL(k_apply_bc_t<R>, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, static_cast<R*>(T),
+                     d, total);
+  return check_launch("apply_bc_temperature");
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_fill_t(R* __restrict__ out, Dom d, ppf_temperature t, double dx,
+                                                int64_t step, int64_t total) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int64_t x, y, z;
+  decode(d, i, &x, &y, &z);
+  int64_t n[3] = {d.ax.n, d.ndim >= 2 ? d.ay.n : 1, d.ndim >= 3 ? d.az.n : 1};
+  int bc[3] = {d.ax.bc, d.ay.bc, d.az.bc};
+  // absent axes sit at interior coordinate 1
+  int64_t yy = d.ndim >= 2 ? y : 1, zz = d.ndim >= 3 ? z : 1;
+  out[offset(d, x, y, z)] = temperature_at<R>(t, n, bc, x, yy, zz, dx, step);
+}
+
+template <typename R>
+int fill_temperature(const ppf_grid& g, void* out, const ppf_temperature& t, int64_t step, hipStream_t s) {
+  Dom d = make_dom(g);
+  int64_t total = padded_cells(g);
+  hipLaunchKernelGGL(k_fill_t<R>, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, static_cast<R*>(out), d,
+                     t, g.dx, step, total);
+  return check_launch("fill_temperature");
+}
+
+// ------------------------------------------------------- 2D tile loading --
+// Tile of TX x TY outputs, 256 threads as 64 columns x 4 row groups; each
+// thread owns rows ty, ty+4, ... of column tx.
+constexpr int kTX = 64;
+constexpr int kTY = 16;
+constexpr int kThreads = 256;
+
+template <typename R, int W, int HH>
+__device__ __forceinline__ void load_tile(R (*s)[W], const R* __restrict__ f, const R* __restrict__ b,
+                                          const Dom& d, R dx, int64_t xs, int64_t ys, bool interior) {
+  // s[ly][lx] = padded cell (xs + lx, ys + ly)
+  for (int t = threadIdx.x; t < W * HH; t += kThreads) {
+    int ly = t / W;
+    int lx = t - ly * W;
+    int64_t x = xs + lx, y = ys + ly;
+    R v;
+    if (interior) {
+      v = f[y * d.ay.stride + x];
+    } else if (x > d.ax.n + 2 || y > d.ay.n + 2) {
+      v = R(0);  // beyond a partial tile: never used
+    } else {
+      v = fold(f, b, d, x, y, int64_t(0), dx);
+    }
+    s[ly][lx] = v;
+  }
+}
+
+__device__ __forceinline__ void tile_coords(const Dom& d, int64_t* x0, int64_t* y0) {
+  uint32_t nbx = uint32_t((d.ax.n + kTX - 1) / kTX);
+  uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  *x0 = 1 + int64_t(b % nbx) * kTX;
+  *y0 = 1 + int64_t(b / nbx) * kTY;
+}
+
+__device__ __forceinline__ bool tile_interior(const Dom& d, int64_t x0, int64_t y0, int h) {
+  return x0 - h >= 1 && x0 + kTX - 1 + h <= d.ax.n && y0 - h >= 1 && y0 + kTY - 1 + h <= d.ay.n;
+}
+
+inline unsigned tiles2d(const ppf_grid& g) {
+  return unsigned(((g.n[0] + kTX - 1) / kTX) * ((g.n[1] + kTY - 1) / kTY));
+}
+
+// -------------------------------------------- Diffusion / Allen-Cahn (2D) --
+// kind 0: c + dt*(D*lap c)                          (Diffusion.py:204-207)
+// kind 1: phi + dt*M*(eps^2 lap - 16W(4p^3-6p^2+2p)) (CahnAllen.py:129-135)
+struct PointParams {
+  double a, b, c, dt;  // kind 0: a=D; kind 1: a=M, b=W, c=eps
+};
+
+template <typename R, int KIND>
+__device__ __forceinline__ R point_update(R v, R lap, const PointParams& p) {
+  if (KIND == 0) {
+    return v + R(p.dt) * (R(p.a) * lap);
+  } else {
+    R eps2 = R(p.c) * R(p.c);
+    R well = R(4) * v * v * v - R(6) * v * v + R(2) * v;
+    return v + R(p.dt) * R(p.a) * (eps2 * lap - R(16) * R(p.b) * well);
+  }
+}
+
+template <typename R, int KIND>
+__global__ __launch_bounds__(kThreads) void k_point2d(const R* __restrict__ in, R* __restrict__ out,
+                                                      const R* __restrict__ b, Dom d, R dx, PointParams p) {
+  constexpr int H = 1, W = kTX + 2 * H, HH = kTY + 2 * H;
+  __shared__ R s[HH][W];
+  int64_t x0, y0;
+  tile_coords(d, &x0, &y0);
+  load_tile<R, W, HH>(s, in, b, d, dx, x0 - H, y0 - H, tile_interior(d, x0, y0, H));
+  __syncthreads();
+  const R idx = R(1) / dx;
+  const R idx2 = idx * idx;
+  int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+  int64_t x = x0 + tx;
+  if (x > d.ax.n) return;
+#pragma unroll
+  for (int k = 0; k < kTY / 4; ++k) {
+    int ly = ty + 4 * k;
+    int64_t y = y0 + ly;
+    if (y > d.ay.n) break;
+    R c = s[ly + 1][tx + 1];
+    // field.py:64-66 order: axis 0 (y) then axis 1 (x), each (c[+1]+c[-1]-2c)*idx2
+    R lap = (s[ly + 2][tx + 1] + s[ly][tx + 1] - R(2) * c) * idx2;
+    lap += (s[ly + 1][tx + 2] + s[ly + 1][tx] - R(2) * c) * idx2;
+    out[y * d.ay.stride + x] = point_update<R, KIND>(c, lap, p);
+  }
+}
+
+// generic 1D / 3D version (one cell per thread, folded global loads)
+template <typename R, int KIND>
+__global__ __launch_bounds__(256) void k_point_nd(const R* __restrict__ in, R* __restrict__ out,
+                                                  const R* __restrict__ b, Dom d, R dx, PointParams p) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t nx = d.ax.n, ny = d.ndim >= 2 ? d.ay.n : 1, nz = d.ndim >= 3 ? d.az.n : 1;
+  if (t >= nx * ny * nz) return;
+  int64_t x = 1 + t % nx;
+  int64_t y = d.ndim >= 2 ? 1 + (t / nx) % ny : 0;
+  int64_t z = d.ndim >= 3 ? 1 + t / (nx * ny) : 0;
+  const R idx = R(1) / dx;
+  const R idx2 = idx * idx;
+  R c = in[offset(d, x, y, z)];
+  R lap;
+  if (d.ndim == 3) {
+    lap = (fold(in, b, d, x, y, z + 1, dx) + fold(in, b, d, x, y, z - 1, dx) - R(2) * c) * idx2;
+    lap += (fold(in, b, d, x, y + 1, z, dx) + fold(in, b, d, x, y - 1, z, dx) - R(2) * c) * idx2;
+    lap += (fold(in, b, d, x + 1, y, z, dx) + fold(in, b, d, x - 1, y, z, dx) - R(2) * c) * idx2;
+  } else {
+    lap = (fold(in, b, d, x + 1, y, z, dx) + fold(in, b, d, x - 1, y, z, dx) - R(2) * c) * idx2;
+  }
+  out[offset(d, x, y, z)] = point_update<R, KIND>(c, lap, p);
+}
+
+template <typename R, int KIND>
+int point_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const PointParams& p,
+               hipStream_t s, const char* what) {
+  Dom d = make_dom(g);
+  if (g.ndim == 2) {
+    hipLaunchKernelGGL((k_point2d<R, KIND>), dim3(tiles2d(g)), dim3(kThreads), 0, s,
+                       static_cast<const R*>(in), static_cast<R*>(out), static_cast<const R*>(bcvals), d,
+                       R(g.dx), p);
+  } else {
+    int64_t total = interior_cells(g);
+    hipLaunchKernelGGL((k_point_nd<R, KIND>), dim3(unsigned((total + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const R*>(in), static_cast<R*>(out), static_cast<const R*>(bcvals), d,
+                       R(g.dx), p);
+  }
+  return check_launch(what);
+}
+
+template <typename R>
+int diffusion_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double D, double dt,
+                   hipStream_t s) {
+  PointParams p{D, 0, 0, dt};
+  return point_step<R, 0>(g, in, out, bcvals, p, s, "diffusion_step");
+}
+
+template <typename R>
+int allencahn_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double W,
+                   double eps, double dt, hipStream_t s) {
+  PointParams p{M, W, eps, dt};
+  return point_step<R, 1>(g, in, out, bcvals, p, s, "allencahn_step");
+}
+
+// ------------------------------------------------------ Cahn-Hilliard --
+template <typename R>
+__device__ __forceinline__ R ch_mu(R c, R lap, R eps2) {
+  return -eps2 * lap + (R(4) * c * c * c - R(6) * c * c + R(2) * c);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_ch2d(const R* __restrict__ in, R* __restrict__ out,
+                                                   const R* __restrict__ b, Dom d, R dx, R eps, R dtM) {
+  constexpr int H = 2, W = kTX + 2 * H, HH = kTY + 2 * H;
+  constexpr int MW = kTX + 2, MH = kTY + 2;
+  __shared__ R s[HH][W];
+  __shared__ R mu[MH][MW];
+  int64_t x0, y0;
+  tile_coords(d, &x0, &y0);
+  load_tile<R, W, HH>(s, in, b, d, dx, x0 - H, y0 - H, tile_interior(d, x0, y0, H));
+  __syncthreads();
+  const R idx = R(1) / dx;
+  const R idx2 = idx * idx;
+  const R eps2 = eps * eps;
+  // mu on the tile + 1 ring: mu[my][mx] at padded (x0-1+mx, y0-1+my)
+  for (int t = threadIdx.x; t < MW * MH; t += kThreads) {
+    int my = t / MW, mx = t - my * MW;
+    int sy = my + 1, sx = mx + 1;
+    R c = s[sy][sx];
+    R lap = (s[sy + 1][sx] + s[sy - 1][sx] - R(2) * c) * idx2;
+    lap += (s[sy][sx + 1] + s[sy][sx - 1] - R(2) * c) * idx2;
+    mu[my][mx] = ch_mu(c, lap, eps2);
+  }
+  __syncthreads();
+  // non-periodic sides: the ring outside the domain carries zero flux
+  if (d.ax.bc != PPF_PERIODIC || d.ay.bc != PPF_PERIODIC) {
+    for (int t = threadIdx.x; t < MW * MH; t += kThreads) {
+      int my = t / MW, mx = t - my * MW;
+      int64_t x = x0 - 1 + mx, y = y0 - 1 + my;
+      bool xin = x >= 1 && x <= d.ax.n, yin = y >= 1 && y <= d.ay.n;
+      if (xin == yin) continue;  // interior, or a corner (never read)
+      R v = mu[my][mx];
+      if (!xin && d.ax.bc != PPF_PERIODIC) v = mu[my][x < 1 ? mx + 1 : mx - 1];
+      if (!yin && d.ay.bc != PPF_PERIODIC) v = mu[y < 1 ? my + 1 : my - 1][mx];
+      // the cell read is always interior (never rewritten here): no hazard
+      mu[my][mx] = v;
+    }
+  }
+  __syncthreads();
+  int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+  int64_t x = x0 + tx;
+  if (x > d.ax.n) return;
+#pragma unroll
+  for (int k = 0; k < kTY / 4; ++k) {
+    int ly = ty + 4 * k;
+    int64_t y = y0 + ly;
+    if (y > d.ay.n) break;
+    R m = mu[ly + 1][tx + 1];
+    // CahnHilliard.py:142-146 order: sum over axes of (mu[-1] + mu[+1] - 2mu), then /dx^2
+    R dm = (mu[ly][tx + 1] + mu[ly + 2][tx + 1] - R(2) * m);
+    dm = dm + (mu[ly + 1][tx] + mu[ly + 1][tx + 2] - R(2) * m);
+    dm = dm / (dx * dx);
+    out[y * d.ay.stride + x] = s[ly + 2][tx + 2] + dtM * dm;
+  }
+}
+
+// generic 1D / 3D version: mu at the centre and its 2*ndim neighbours
+template <typename R>
+__device__ __forceinline__ R ch_mu_at(const R* __restrict__ in, const R* __restrict__ b, const Dom& d, int64_t x,
+                                      int64_t y, int64_t z, R dx, R idx2, R eps2) {
+  R c = fold(in, b, d, x, y, z, dx);
+  R lap;
+  if (d.ndim == 3) {
+    lap = (fold(in, b, d, x, y, z + 1, dx) + fold(in, b, d, x, y, z - 1, dx) - R(2) * c) * idx2;
+    lap += (fold(in, b, d, x, y + 1, z, dx) + fold(in, b, d, x, y - 1, z, dx) - R(2) * c) * idx2;
+    lap += (fold(in, b, d, x + 1, y, z, dx) + fold(in, b, d, x - 1, y, z, dx) - R(2) * c) * idx2;
+  } else {
+    lap = (fold(in, b, d, x + 1, y, z, dx) + fold(in, b, d, x - 1, y, z, dx) - R(2) * c) * idx2;
+  }
+  return ch_mu(c, lap, eps2);
+}
+
+// mu at a neighbour position: zero-flux ring on non-periodic sides
+template <typename R>
+__device__ __forceinline__ R ch_mu_nb(const R* __restrict__ in, const R* __restrict__ b, const Dom& d, int64_t x,
+                                      int64_t y, int64_t z, int64_t cx, int64_t cy, int64_t cz, R dx, R idx2,
+                                      R eps2) {
+  bool out_x = x < 1 || x > d.ax.n;
+  bool out_y = d.ndim >= 2 && (y < 1 || y > d.ay.n);
+  bool out_z = d.ndim >= 3 && (z < 1 || z > d.az.n);
+  if ((out_x && d.ax.bc != PPF_PERIODIC) || (out_y && d.ay.bc != PPF_PERIODIC) ||
+      (out_z && d.az.bc != PPF_PERIODIC))
+    return ch_mu_at(in, b, d, cx, cy, cz, dx, idx2, eps2);
+  return ch_mu_at(in, b, d, x, y, z, dx, idx2, eps2);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_ch_nd(const R* __restrict__ in, R* __restrict__ out,
+                                               const R* __restrict__ b, Dom d, R dx, R eps, R dtM) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t nx = d.ax.n, ny = d.ndim >= 2 ? d.ay.n : 1, nz = d.ndim >= 3 ? d.az.n : 1;
+  if (t >= nx * ny * nz) return;
+  int64_t x = 1 + t % nx;
+  int64_t y = d.ndim >= 2 ? 1 + (t / nx) % ny : 0;
+  int64_t z = d.ndim >= 3 ? 1 + t / (nx * ny) : 0;
+  const R idx = R(1) / dx;
+  const R idx2 = idx * idx;
+  const R eps2 = eps * eps;
+  R m = ch_mu_at(in, b, d, x, y, z, dx, idx2, eps2);
+  R dm;
+  if (d.ndim == 3) {
+    dm = ch_mu_nb(in, b, d, x, y, z - 1, x, y, z, dx, idx2, eps2) +
+         ch_mu_nb(in, b, d, x, y, z + 1, x, y, z, dx, idx2, eps2) - R(2) * m;
+    dm = dm + (ch_mu_nb(in, b, d, x, y - 1, z, x, y, z, dx, idx2, eps2) +
+               ch_mu_nb(in, b, d, x, y + 1, z, x, y, z, dx, idx2, eps2) - R(2) * m);
+    dm = dm + (ch_mu_nb(in, b, d, x - 1, y, z, x, y, z, dx, idx2, eps2) +
+               ch_mu_nb(in, b, d, x + 1, y, z, x, y, z, dx, idx2, eps2) - R(2) * m);
+  } else {
+    dm = ch_mu_nb(in, b, d, x - 1, y, z, x, y, z, dx, idx2, eps2) +
+         ch_mu_nb(in, b, d, x + 1, y, z, x, y, z, dx, idx2, eps2) - R(2) * m;
+  }
+  dm = dm / (dx * dx);
+  int64_t o = offset(d, x, y, z);
+  out[o] = in[o] + dtM * dm;
+}
+
+template <typename R>
+int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double eps,
+                      double dt, hipStream_t s) {
+  Dom d = make_dom(g);
+  R dtM = R(dt * M);
+  if (g.ndim == 2) {
+    hipLaunchKernelGGL(k_ch2d<R>, dim3(tiles2d(g)), dim3(kThreads), 0, s, static_cast<const R*>(in),
+                       static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), R(eps), dtM);
+  } else {
+    int64_t total = interior_cells(g);
+    hipLaunchKernelGGL(k_ch_nd<R>, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const R*>(in), static_cast<R*>(out), static_cast<const R*>(bcvals), d,
+                       R(g.dx), R(eps), dtM);
+  }
+  return check_launch("cahnhilliard_step");
+}
+
+// ------------------------------------------------------------ reductions --
+// Two-pass FP64 reduction with a fixed summation order (deterministic):
+// kReduceBlocks blocks write partial sums, one block adds them.
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double part[4];
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = (part[0] + part[1]) + (part[2] + part[3]);
+  __syncthreads();
+  return r;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_reduce_partial(const R* __restrict__ in, const R* __restrict__ b, Dom d,
+                                                        R dx, int kind, double eps, double* __restrict__ work) {
+  int fi = blockIdx.y;
+  const R* f = in + fi * d.sf;
+  const R* bb = b ? b + fi * d.sf : nullptr;
+  int64_t nx = d.ax.n, ny = d.ndim >= 2 ? d.ay.n : 1, nz = d.ndim >= 3 ? d.az.n : 1;
+  int64_t total = nx * ny * nz;
+  double acc = 0.0;
+  const double idx = 1.0 / double(dx);
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    int64_t x = 1 + t % nx;
+    int64_t y = d.ndim >= 2 ? 1 + (t / nx) % ny : 0;
+    int64_t z = d.ndim >= 3 ? 1 + t / (nx * ny) : 0;
+    double c = double(f[offset(d, x, y, z)]);
+    if (kind == 0) {
+      acc += c;
+    } else {
+      double e = c * c * (1.0 - c) * (1.0 - c);
+      double gx = (double(fold(f, bb, d, x + 1, y, z, dx)) - c) * idx;
+      double g2 = gx * gx;
+      if (d.ndim >= 2) {
+        double gy = (double(fold(f, bb, d, x, y + 1, z, dx)) - c) * idx;
+        g2 += gy * gy;
+      }
+      if (d.ndim >= 3) {
+        double gz = (double(fold(f, bb, d, x, y, z + 1, dx)) - c) * idx;
+        g2 += gz * gz;
+      }
+      acc += e + 0.5 * eps * eps * g2;
+    }
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) work[fi * kReduceBlocks + blockIdx.x] = s;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ work, double* __restrict__ out,
+                                                      double scale) {
+  int fi = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kReduceBlocks; i += 256) acc += work[fi * kReduceBlocks + i];
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) out[fi] = s * scale;
+}
+
+template <typename R>
+int reduce(const ppf_grid& g, const void* in, const void* bcvals, int kind, double eps, double* out, void* work,
+           hipStream_t s) {
+  Dom d = make_dom(g);
+  int nf = kind == 0 ? g.nfields : 1;
+  double scale = 1.0;
+  if (kind == 1) {
+    for (int a = 0; a < g.ndim; ++a) scale *= g.dx;
+  }
+  hipLaunchKernelGGL(k_reduce_partial<R>, dim3(kReduceBlocks, unsigned(nf)), dim3(256), 0, s,
+                     static_cast<const R*>(in), static_cast<const R*>(bcvals), d, R(g.dx), kind, eps,
+                     static_cast<double*>(work));
+  int rc = check_launch("reduce_partial");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_reduce_final<R>, dim3(unsigned(nf)), dim3(256), 0, s, static_cast<const double*>(work), out,
+                     scale);
+  return check_launch("reduce_final");
+}
+
+}  // namespace ppf

@@ -1,0 +1,565 @@
+"""Simulation: the reference's runtime core (simulation.py:19-723), GPU-resident.
+
+Same constructor arguments, setters, lifecycle hooks and save/load format as
+the reference; the per-step work runs in HIP kernels (libppf_hip.so) on
+torch-allocated device buffers.  Differences, all deliberate (defect ids of
+the design notes):
+
+* the one-time setup flag works (``simulation.py:260`` compares instead of
+  assigning, so the reference re-uploads host data on every call, D5);
+* ``user_data`` defaults to a fresh dict per instance (D5);
+* boundary conditions follow the reference GPU kernels on every axis
+  (``ppf_gpu_utils.py:430-514``), folded into the step kernels' loads, so no
+  halo kernels run per step; halos are materialised on retrieval;
+* LINEAR_GRADIENT temperature is T0 + x*dx*dTdx + y*dx*dTdy + counter*dTdt*dt
+  (``simulation.py:154`` omits dt, D5), evaluated inside the kernels;
+* noise comes from a counter RNG keyed by (seed, step, cell).
+
+New, optional: ``precision`` ("FP64" default, or "FP32") and ``seed``.
+"""
+
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from .field import Field
+
+
+class Simulation:
+    def __init__(self, dimensions, framework=None, dx=None, dt=None, initial_time_step=0,
+                 temperature_type=None, initial_T=None, dTdx=None, dTdy=None, dTdz=None, dTdt=None,
+                 temperature_path=None, temperature_units="K",
+                 tdb_container=None, tdb_path=None, tdb_components=None, tdb_phases=None,
+                 save_path=None, autosave=False, save_images=False, autosave_rate=None,
+                 boundary_conditions=None, user_data=None, precision="FP64", seed=0):
+        self._framework = framework
+        self._uses_gpu = True  # every framework runs the HIP path
+        self._begun_simulation = False
+        self.fields = []
+        self._fields_gpu_device = None
+        self._fields_out_gpu_device = None
+        self._num_transfer_arrays = None
+        self._fields_transfer_gpu_device = None
+        self.dimensions = list(dimensions) if dimensions is not None else None
+        self.dx = dx
+        self.dt = dt
+        self.time_step_counter = initial_time_step
+        self.temperature = None
+        self._temperature_gpu_device = None
+        self._temperature_type = temperature_type
+        self._temperature_path = temperature_path
+        self._temperature_units = temperature_units
+        self._initial_T = initial_T
+        self._dTdx = dTdx
+        self._dTdy = dTdy
+        self._dTdz = dTdz
+        self._dTdt = dTdt
+        self._tdb_container = tdb_container
+        self._tdb = None
+        self._tdb_path = tdb_path
+        self._tdb_components = tdb_components
+        self._tdb_phases = tdb_phases
+        self._tdb_ufuncs = []
+        self._save_path = save_path
+        self._autosave_flag = autosave
+        self._autosave_rate = autosave_rate
+        self._autosave_save_images_flag = save_images
+        self._boundary_conditions_type = boundary_conditions
+        self._boundary_conditions_array = None
+        self._boundary_conditions_gpu_device = None
+        self._debug_mode_flag = False
+        self.user_data = {} if user_data is None else user_data
+        self._fill_defaults()
+        self._precision = self._check_precision(precision)
+        self.seed = int(seed)
+        # host/device coherence
+        self._device_newer = False
+        self._host_dirty = False
+        self._t_from_array = False
+        self._grid = None
+
+    # ------------------------------------------------------------ config --
+    # per-engine user_data defaults, applied at construction and again
+    # whenever set_user_data replaces the dict (the reference sets them only
+    # in the engine constructors, so set_user_data silently dropped them)
+    _DEFAULT_USER_DATA = {}
+
+    def _fill_defaults(self):
+        for k, v in self._DEFAULT_USER_DATA.items():
+            self.user_data.setdefault(k, v)
+
+    @staticmethod
+    def _check_precision(p):
+        p = str(p).upper()
+        if p not in ("FP32", "FP64"):
+            raise ValueError("precision must be 'FP32' or 'FP64'")
+        return p
+
+    def set_precision(self, precision):
+        self._precision = self._check_precision(precision)
+
+    def get_precision(self):
+        return self._precision
+
+    def _torch_dtype(self):
+        import torch
+        return torch.float32 if self._precision == "FP32" else torch.float64
+
+    # ------------------------------------------------------- lifecycle --
+    def init_fields(self):
+        pass
+
+    def init_boundary_conditions(self):
+        """bcarray [F, *padded] of zeros (simulation.py:109-123)."""
+        dim = [d + 2 for d in self.dimensions]
+        self._boundary_conditions_array = np.zeros([len(self.fields)] + dim)
+
+    def _bc_list(self):
+        bc = self._boundary_conditions_type or "PERIODIC"
+        return _lib.bc_codes(bc, len(self.dimensions))
+
+    def init_temperature_field(self):
+        """simulation.py:126-177 (D5: counter*dTdt*dt)."""
+        t = self._temperature_type
+        if t is None:
+            return
+        t = str(t).upper()
+        if t == "ISOTHERMAL":
+            array = np.zeros(self.dimensions) + self._initial_T
+        elif t == "LINEAR_GRADIENT":
+            array = self._linear_temperature(self.time_step_counter)
+        elif t == "XDMF_FILE":
+            raise NotImplementedError("XDMF_FILE thermal histories are not supported by this engine path")
+        else:
+            raise ValueError("Unknown temperature profile.")
+        self.temperature = Field(data=array, simulation=self, colormap="jet",
+                                 name="Temperature (" + str(self._temperature_units) + ")")
+        self._apply_temperature_rule_host()
+
+    def _linear_temperature(self, counter):
+        dims = self.dimensions
+        array = np.zeros(dims) + self._initial_T
+        dx = self.dx
+        if self._dTdx:
+            array = array + np.arange(dims[-1], dtype=np.float64) * (dx * self._dTdx)
+        if len(dims) > 1 and self._dTdy:
+            shp = [1] * len(dims)
+            shp[-2] = dims[-2]
+            array = array + (np.arange(dims[-2], dtype=np.float64) * (dx * self._dTdy)).reshape(shp)
+        if len(dims) > 2 and self._dTdz:
+            shp = [1] * len(dims)
+            shp[-3] = dims[-3]
+            array = array + (np.arange(dims[-3], dtype=np.float64) * (dx * self._dTdz)).reshape(shp)
+        if self._dTdt:
+            array = array + counter * (self._dTdt * self.dt)
+        return array
+
+    def _apply_temperature_rule_host(self):
+        T = self.temperature._data
+        bcs = self._bc_list()
+        for a in range(T.ndim):
+            ax = T.ndim - 1 - a
+            lo = [slice(None)] * T.ndim
+            lo1, hi, hi1 = list(lo), list(lo), list(lo)
+            lo[ax], lo1[ax], hi[ax], hi1[ax] = 0, 1, -1, -2
+            if bcs[a] == _lib.PPF_PERIODIC:
+                T[tuple(lo)] = T[tuple(hi1)]
+                T[tuple(hi)] = T[tuple(lo1)]
+            else:
+                T[tuple(lo)] = T[tuple(lo1)]
+                T[tuple(hi)] = T[tuple(hi1)]
+
+    def init_tdb_params(self):
+        """Thermodynamic model: a ThermoModel container (tdb_container) or
+        engine-provided; reading TDB files is not part of this engine path."""
+        if self._tdb_container is not None:
+            self._tdb = self._tdb_container
+            self._tdb_phases = list(self._tdb_container.phases)
+            self._tdb_components = list(self._tdb_container.components)
+            return
+        if self._tdb_path is not None:
+            from .thermo import load_tdb_model
+            self._tdb = load_tdb_model(self._tdb_path, self._tdb_components, self._tdb_phases)
+            self._tdb_phases = list(self._tdb.phases)
+            self._tdb_components = list(self._tdb.components)
+
+    def initialize_fields_and_imported_data(self):
+        """simulation.py:273-281."""
+        self._requires_initialization = False
+        self.init_tdb_params()
+        self.init_temperature_field()
+        self.init_fields()
+        self.init_boundary_conditions()
+
+    def just_before_simulating(self):
+        """simulation.py:285-289: upload once (the engine subclass then packs
+        its kernel parameters)."""
+        self.send_fields_to_GPU()
+
+    def simulation_loop(self):
+        pass
+
+    def add_field(self, array, array_name, colormap="GnBu"):
+        field = Field(data=array, name=array_name, simulation=self, colormap=colormap)
+        self.fields.append(field)
+
+    def default_value(self, var, value):
+        if getattr(self, var, None) is None:
+            setattr(self, var, value)
+
+    def simulate(self, number_of_timesteps):
+        """simulation.py:245-271: counter++, engine step, T update, BC, autosave."""
+        if not self._begun_simulation:
+            self._begun_simulation = True
+            self.just_before_simulating()
+        elif self._host_dirty:
+            self.send_fields_to_GPU()
+        self._device_newer = True
+        for _ in range(int(number_of_timesteps)):
+            self.time_step_counter += 1
+            self.simulation_loop()
+            self.update_temperature_field()
+            if self._autosave_flag and self._autosave_rate:
+                if self.time_step_counter % self._autosave_rate == 0:
+                    self.retrieve_fields_from_GPU()
+                    self.save_simulation()
+                    self._host_dirty = False
+
+    # ------------------------------------------------ device state --
+    def _device(self):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("pyphasefield_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        return torch.device("cuda")
+
+    def _allocate_device(self):
+        import torch
+        dev = self._device()
+        dt = self._torch_dtype()
+        F = len(self.fields)
+        shape = list(self.dimensions)
+        self._fields_gpu_device = _lib.alloc_stack(F, shape, dt, dev)
+        self._fields_out_gpu_device = _lib.alloc_stack(F, shape, dt, dev)
+        self._grid = _lib.make_grid(self._fields_gpu_device, len(shape), self._boundary_conditions_type,
+                                    self.dx if self.dx else 1.0)
+        self._grid_out = _lib.make_grid(self._fields_out_gpu_device, len(shape), self._boundary_conditions_type,
+                                        self.dx if self.dx else 1.0)
+        torch.cuda.synchronize()
+
+    def send_fields_to_GPU(self):
+        """ppf_gpu_utils.py:396-420: upload fields, bcarray and (array-mode) T."""
+        import torch
+        if not self.fields:
+            raise RuntimeError("no fields to simulate")
+        if self.dimensions is None:
+            self.dimensions = list(self.fields[0]._data.shape[i] - 2 for i in range(self.fields[0]._data.ndim))
+        need = (self._fields_gpu_device is None or self._fields_gpu_device.shape[0] != len(self.fields)
+                or list(self._fields_gpu_device.shape[1:]) != [d + 2 for d in self.dimensions]
+                or self._fields_gpu_device.dtype != self._torch_dtype())
+        if need:
+            self._allocate_device()
+        else:
+            # boundary kinds / dx may have changed through the setters
+            self._grid = _lib.make_grid(self._fields_gpu_device, len(self.dimensions),
+                                        self._boundary_conditions_type, self.dx if self.dx else 1.0)
+            self._grid_out = _lib.make_grid(self._fields_out_gpu_device, len(self.dimensions),
+                                            self._boundary_conditions_type, self.dx if self.dx else 1.0)
+        dev = self._fields_gpu_device.device
+        dt = self._torch_dtype()
+        for i, f in enumerate(self.fields):
+            self._fields_gpu_device[i].copy_(torch.from_numpy(np.ascontiguousarray(f._data)).to(dev, dt))
+        self._fields_out_gpu_device.copy_(self._fields_gpu_device)
+        bca = self._boundary_conditions_array
+        if bca is not None and np.any(bca != 0):
+            if bca.shape[0] != len(self.fields):
+                raise ValueError("boundary_conditions_array must hold one array per field")
+            bcd = _lib.alloc_stack(len(self.fields), list(self.dimensions), dt, dev)
+            bcd.copy_(torch.from_numpy(np.ascontiguousarray(bca)).to(dev, dt))
+            self._boundary_conditions_gpu_device = bcd
+        else:
+            self._boundary_conditions_gpu_device = None
+        if self.temperature is not None and self._t_from_array:
+            Td = _lib.alloc_stack(1, list(self.dimensions), dt, dev)
+            Td[0].copy_(torch.from_numpy(np.ascontiguousarray(self.temperature._data)).to(dev, dt))
+            self._temperature_gpu_device = Td
+        else:
+            self._temperature_gpu_device = None
+        self._host_dirty = False
+
+    def _swap_buffers(self):
+        self._fields_gpu_device, self._fields_out_gpu_device = self._fields_out_gpu_device, self._fields_gpu_device
+        self._grid, self._grid_out = self._grid_out, self._grid
+
+    def temperature_descriptor(self):
+        """Temperature as the kernels see it (ppf_temperature)."""
+        t = _lib.Temperature()
+        t.kind = _lib.PPF_T_CONST
+        if self.temperature is None or self._temperature_type is None:
+            t.T0 = float(self._initial_T or 0.0)
+            return t
+        kind = str(self._temperature_type).upper()
+        if self._t_from_array:
+            Td = self._temperature_gpu_device
+            t.kind = _lib.PPF_T_ARRAY
+            t.data = Td.data_ptr()
+            nd = len(self.dimensions)
+            t.sy = Td.stride(-2) if nd >= 2 else 0
+            t.sz = Td.stride(-3) if nd >= 3 else 0
+            return t
+        if kind == "ISOTHERMAL":
+            t.T0 = float(self._initial_T)
+            return t
+        t.kind = _lib.PPF_T_LINEAR
+        t.T0 = float(self._initial_T)
+        t.dTdx = float(self._dTdx or 0.0)
+        t.dTdy = float(self._dTdy or 0.0)
+        t.dTdz = float(self._dTdz or 0.0)
+        t.dTdt = float(self._dTdt or 0.0)
+        t.dt = float(self.dt or 0.0)
+        return t
+
+    def update_temperature_field(self, force_cpu=False):
+        """ppf_gpu_utils.py:516-531: analytic modes need no work; an uploaded
+        temperature array advances by dTdt*dt."""
+        if self._t_from_array and self._dTdt and self._temperature_gpu_device is not None:
+            self._temperature_gpu_device.add_(self._dTdt * self.dt)
+
+    def apply_boundary_conditions(self):
+        """Materialise halo cells (ppf_gpu_utils.py:430-514); the step
+        kernels fold the same rules into their loads."""
+        if self._fields_gpu_device is None:
+            return
+        L = _lib.lib()
+        _lib.check(L.ppf_apply_bc(self._grid, _lib.ptr(self._fields_gpu_device),
+                                  _lib.ptr(self._boundary_conditions_gpu_device), _lib.stream_handle()),
+                   "apply_bc")
+
+    def retrieve_fields_from_GPU(self):
+        """ppf_gpu_utils.py:423-428 (+ halos and temperature materialised)."""
+        if self._fields_gpu_device is None:
+            return
+        import torch
+        self.apply_boundary_conditions()
+        host = self._fields_gpu_device.to("cpu", torch.float64).numpy()
+        for i, f in enumerate(self.fields):
+            f._data = np.array(host[i])
+        if self.temperature is not None and not self._t_from_array:
+            L = _lib.lib()
+            Tt = _lib.alloc_stack(1, list(self.dimensions), self._torch_dtype(), self._fields_gpu_device.device)
+            g = _lib.make_grid(Tt, len(self.dimensions), self._boundary_conditions_type, self.dx or 1.0)
+            _lib.check(L.ppf_fill_temperature(g, _lib.ptr(Tt), self.temperature_descriptor(),
+                                              int(self.time_step_counter) + 1, _lib.stream_handle()),
+                       "fill_temperature")
+            self.temperature._data = Tt[0].to("cpu", torch.float64).numpy().copy()
+        elif self.temperature is not None and self._temperature_gpu_device is not None:
+            self.temperature._data = self._temperature_gpu_device[0].to("cpu", torch.float64).numpy().copy()
+        self._device_newer = False
+
+    def _field_host_access(self, field):
+        if self._device_newer:
+            self.retrieve_fields_from_GPU()
+        self._host_dirty = True
+        if field is self.temperature and self.temperature is not None and self._begun_simulation:
+            self._t_from_array = True
+
+    # -------------------------------------------------- reductions --
+    def field_sums(self):
+        """FP64 sum over the interior of every field (mass check)."""
+        import torch
+        if self._fields_gpu_device is None:
+            return [float(np.sum(f.get_cells())) for f in self.fields]
+        L = _lib.lib()
+        dev = self._fields_gpu_device.device
+        work = torch.empty(int(L.ppf_reduce_work_bytes(self._grid)) // 8 + 1, dtype=torch.float64, device=dev)
+        out = torch.zeros(len(self.fields), dtype=torch.float64, device=dev)
+        _lib.check(L.ppf_reduce(self._grid, _lib.ptr(self._fields_gpu_device),
+                                _lib.ptr(self._boundary_conditions_gpu_device), 0, 0.0, _lib.ptr(out),
+                                _lib.ptr(work), _lib.stream_handle()), "reduce")
+        return out.cpu().tolist()
+
+    # ------------------------------------------------- save / load --
+    def load_simulation(self, file_path=None, step=-1):
+        """simulation.py:334-397."""
+        self.fields = []
+        if file_path is None:
+            file_path = self._save_path
+            if self._save_path is None:
+                raise ValueError("Simulation needs a path to load data from!")
+        if Path.cwd().joinpath(file_path).exists():
+            file_path = Path.cwd().joinpath(file_path)
+        elif self._save_path is not None and Path.cwd().joinpath(self._save_path).joinpath(file_path).exists():
+            file_path = Path.cwd().joinpath(self._save_path).joinpath(file_path)
+        else:
+            file_path = Path(file_path)
+        if file_path.is_dir():
+            if step > -1:
+                file_path = file_path.joinpath("step_" + str(step) + ".npz")
+            else:
+                raise ValueError("Given path is a folder, must specify a timestep!")
+        if self._save_path is None:
+            self._save_path = str(file_path.parent)
+        with np.load(file_path, allow_pickle=False) as fields_dict:
+            for key in fields_dict.files:
+                self.add_field(fields_dict[key], key)
+        self.dimensions = list(self.fields[0]._data.shape[i] - 2 for i in range(self.fields[0]._data.ndim))
+        if step < 0:
+            filename = file_path.stem
+            pos = filename.find("step_")
+            if pos == -1:
+                self.time_step_counter = 0
+            else:
+                i = pos + len("step_")
+                j = i
+                while j < len(filename) and filename[j].isdigit():
+                    j += 1
+                self.time_step_counter = int(filename[i:j]) if j > i else 0
+        else:
+            self.time_step_counter = int(step)
+        if self._temperature_type is not None and str(self._temperature_type).upper() == "LINEAR_GRADIENT":
+            self.temperature = Field(data=self._linear_temperature(self.time_step_counter), simulation=self,
+                                     colormap="jet", name="Temperature (" + str(self._temperature_units) + ")")
+            self._apply_temperature_rule_host()
+        if self._boundary_conditions_array is None or self._boundary_conditions_array.shape[0] != len(self.fields):
+            self.init_boundary_conditions()
+        self._begun_simulation = False
+        self._device_newer = False
+        self._t_from_array = False
+        return 0
+
+    def save_simulation(self):
+        """simulation.py:399-420: save_path/step_<N>.npz keyed by field name."""
+        save_dict = dict()
+        for f in self.fields:
+            save_dict[f.name] = f.get_cells()
+        if not self._save_path:
+            print("self._save_path not defined, aborting save!")
+            return
+        save_loc = Path(self._save_path)
+        save_loc.mkdir(parents=True, exist_ok=True)
+        np.savez(str(save_loc) + "/step_" + str(self.time_step_counter), **save_dict)
+        return 0
+
+    def plot_simulation(self, fields=None, interpolation="bicubic", units="cells", save_images=False, size=None,
+                        norm=False):
+        """Matplotlib view of the fields (simulation.py:422-467)."""
+        import matplotlib.pyplot as plt
+        if fields is None:
+            fields = range(len(self.fields))
+        for i in fields:
+            f = self.fields[i]
+            if size is not None:
+                plt.figure(figsize=size)
+            if f.data.ndim == 1:
+                plt.plot(f.get_cells())
+            else:
+                plt.imshow(f.data, interpolation=interpolation, cmap=f.colormap)
+                plt.colorbar()
+            plt.title(f.name)
+            if save_images and self._save_path:
+                plt.savefig(str(self._save_path) + "/" + f.name + "_" + str(self.time_step_counter) + ".png")
+            plt.show()
+
+    # --------------------------------------------------- setters --
+    def set_dimensions(self, dimensions):
+        self.dimensions = list(dimensions)
+
+    def get_dimensions(self):
+        return self.dimensions
+
+    def set_framework(self, framework):
+        self._framework = framework
+
+    def get_framework(self):
+        return self._framework
+
+    def set_dx(self, dx):
+        self.dx = dx
+
+    def set_cell_spacing(self, dx):
+        self.dx = dx
+
+    def get_dx(self):
+        return self.dx
+
+    def get_cell_spacing(self):
+        return self.dx
+
+    def set_dt(self, dt):
+        self.dt = dt
+
+    def set_time_step_length(self, dt):
+        self.dt = dt
+
+    def get_dt(self):
+        return self.dt
+
+    def get_time_step_length(self):
+        return self.dt
+
+    def set_time_step_counter(self, time_step_counter):
+        self.time_step_counter = time_step_counter
+
+    def get_time_step_counter(self):
+        return self.time_step_counter
+
+    def increment_time_step_counter(self):
+        self.time_step_counter += 1
+
+    def set_temperature_type(self, temperature_type):
+        self._temperature_type = temperature_type
+
+    def set_temperature_initial_T(self, initial_T):
+        self._initial_T = initial_T
+
+    def set_temperature_dTdx(self, dTdx):
+        self._dTdx = dTdx
+
+    def set_temperature_dTdy(self, dTdy):
+        self._dTdy = dTdy
+
+    def set_temperature_dTdz(self, dTdz):
+        self._dTdz = dTdz
+
+    def set_temperature_dTdt(self, dTdt):
+        self._dTdt = dTdt
+
+    def set_temperature_path(self, temperature_path):
+        self._temperature_path = temperature_path
+
+    def set_temperature_units(self, temperature_units):
+        self._temperature_units = temperature_units
+
+    def set_tdb_container(self, tdb_container):
+        self._tdb_container = tdb_container
+
+    def set_tdb_path(self, tdb_path):
+        self._tdb_path = tdb_path
+
+    def set_tdb_phases(self, tdb_phases):
+        self._tdb_phases = tdb_phases
+
+    def set_tdb_components(self, tdb_components):
+        self._tdb_components = tdb_components
+
+    def set_save_path(self, save_path):
+        self._save_path = save_path
+
+    def set_autosave_flag(self, autosave_flag):
+        self._autosave_flag = autosave_flag
+
+    def set_autosave_save_images_flag(self, autosave_save_images_flag):
+        self._autosave_save_images_flag = autosave_save_images_flag
+
+    def set_autosave_rate(self, autosave_rate):
+        self._autosave_rate = autosave_rate
+
+    def set_boundary_conditions(self, boundary_conditions_type):
+        self._boundary_conditions_type = boundary_conditions_type
+
+    def set_user_data(self, data):
+        self.user_data = data
+        self._fill_defaults()
+
+    def set_debug_mode_flag(self, debug_mode_flag):
+        self._debug_mode_flag = debug_mode_flag

@@ -1,0 +1,205 @@
+"""Thermodynamic models for the N-component engine.
+
+The reference builds its Gibbs energies from TDB files with pycalphad
+(``simulation.py:179-243``) and reads the engine parameters from TDB
+``FUNCTION`` entries (``Engines/NCGPU_new.py:366-383``).  pycalphad is not
+part of this engine path; instead a :class:`ThermoModel` holds, per phase,
+the coefficients of a single-sublattice Redlich-Kister + ideal-mixing Gibbs
+energy
+
+    G(c, T) = sum_i c_i (a + bT + cT lnT + dT^2 + e/T)
+            + sum_{i<j} c_i c_j sum_{k=0..2} (A_k + B_k T)(c_i - c_j)^k
+            + R T sum_i c_i ln c_i
+
+which the device evaluates per cell, plus the engine parameters the TDB
+FUNCTIONs carry.  Components and phases are kept sorted alphabetically, as
+the reference sorts them (``simulation.py:201-202``), so the solid (FCC_A1)
+is phase 0 and the liquid phase 1 (``Engines/NCGPU_new.py:353-354``).
+"""
+
+import numpy as np
+
+from . import _lib
+
+R_GAS = 8.314462618  # ideal-mixing gas constant of the synthetic models
+
+# Engine parameters of tests/Ni-Cu_Ideal.tdb:36-70 (units cm, J, K, s).
+NI_CU_IDEAL_FUNCTIONS = {
+    "AL_L": 1080.0, "CU_L": 1728.0, "NI_L": 2350.0,        # latent heat, J/cm^3
+    "AL_TM": 933.5, "CU_TM": 1358.0, "NI_TM": 1728.0,      # melting point, K
+    "AL_S": 0.00002, "CU_S": 0.000029, "NI_S": 0.000037,   # surface energy, J/cm^2
+    "AL_B": 0.36, "CU_B": 0.39, "NI_B": 0.33,              # kinetic coefficient, cm/(K s)
+    "V_M": 7.42,                                           # molar volume, cm^3/mol
+    "D_L": 0.00001, "D_S": 0.000000001,                    # diffusivities, cm^2/s
+    "M_Q": 80000000.0,                                     # orientational mobility
+    "H": 0.00000000001,                                    # orientational energy, J/(K cm)
+    "Y_E": 0.12,                                           # anisotropy
+}
+
+
+def pairs(n):
+    return [(i, j) for i in range(n) for j in range(i + 1, n)]
+
+
+class RKPhase:
+    """Coefficients of one phase: g0 [N, 5] and L [N(N-1)/2, 3, 2]."""
+
+    def __init__(self, name, g0, L, R=R_GAS):
+        self.name = name
+        self.g0 = np.asarray(g0, dtype=np.float64)
+        self.L = np.asarray(L, dtype=np.float64)
+        self.R = float(R)
+        n = self.g0.shape[0]
+        if self.g0.shape != (n, 5) or self.L.shape != (n * (n - 1) // 2, 3, 2):
+            raise ValueError("bad coefficient shapes for phase %s" % name)
+
+    @property
+    def ncomp(self):
+        return self.g0.shape[0]
+
+    def to_ctypes(self):
+        p = _lib.RKPhase()
+        p.ncomp = self.ncomp
+        p.R = self.R
+        for i in range(self.ncomp):
+            for k in range(5):
+                p.g0[i][k] = float(self.g0[i, k])
+        for q in range(self.L.shape[0]):
+            for k in range(3):
+                for m in range(2):
+                    p.L[q][k][m] = float(self.L[q, k, m])
+        return p
+
+
+class ThermoModel:
+    """Phases (sorted) + components (sorted) + TDB-FUNCTION-style parameters.
+
+    Plays the role of the reference's TDBContainer (ppf_utils.py:125-186) for
+    the N-component engine: pass it as ``tdb_container``.
+    """
+
+    def __init__(self, components, phases, functions):
+        self.components = sorted(components)
+        if list(components) != self.components:
+            raise ValueError("components must be given in alphabetical order")
+        self.phases = dict(sorted(phases.items()))
+        self.functions = dict(functions)
+        for ph in self.phases.values():
+            if ph.ncomp != len(self.components):
+                raise ValueError("phase %s has the wrong number of components" % ph.name)
+
+    @property
+    def solid(self):
+        return list(self.phases.values())[0]
+
+    @property
+    def liquid(self):
+        return list(self.phases.values())[1]
+
+    def to_ctypes(self, dg_mode=_lib.PPF_DG_FORWARD_DIFF):
+        t = _lib.Thermo()
+        t.liquid = self.liquid.to_ctypes()
+        t.solid = self.solid.to_ctypes()
+        t.dg_mode = dg_mode
+        return t
+
+    def value(self, name):
+        return float(self.functions[name])
+
+
+def _hessian_min_eig(phase, T, n_grid=24):
+    """Smallest eigenvalue of d2G/dc2 (independent c_1..c_{N-1}) over the
+    simplex interior, by central differences on the closed form."""
+    n = phase.ncomp
+    lo = 0.02
+    pts = []
+    g = np.linspace(lo, 1 - lo, n_grid)
+    if n == 2:
+        pts = [[a] for a in g]
+    elif n == 3:
+        pts = [[a, b] for a in g for b in g if a + b < 1 - lo]
+    else:
+        pts = [[a, b, c] for a in g[::3] for b in g[::3] for c in g[::3] if a + b + c < 1 - lo]
+    pts = np.array(pts)
+    h = 1e-4
+
+    def G(cs):
+        full = list(cs.T) + [1 - cs.sum(axis=1)]
+        lnT = np.log(T)
+        out = np.zeros(len(cs))
+        for i in range(n):
+            a, b, c, d, e = phase.g0[i]
+            out += full[i] * (a + b * T + c * T * lnT + d * T * T + e / T)
+        for p, (i, j) in enumerate(pairs(n)):
+            Lk = phase.L[p, :, 0] + phase.L[p, :, 1] * T
+            dd = full[i] - full[j]
+            out += full[i] * full[j] * (Lk[0] + dd * (Lk[1] + dd * Lk[2]))
+        for i in range(n):
+            out += phase.R * T * full[i] * np.log(np.maximum(full[i], 1e-10))
+        return out
+
+    m = n - 1
+    H = np.zeros((len(pts), m, m))
+    for a in range(m):
+        for b in range(m):
+            ea = np.zeros(m)
+            eb = np.zeros(m)
+            ea[a] = h
+            eb[b] = h
+            H[:, a, b] = (G(pts + ea + eb) - G(pts + ea - eb) - G(pts - ea + eb) + G(pts - ea - eb)) / (4 * h * h)
+    return float(np.min(np.linalg.eigvalsh(H)))
+
+
+def synthetic_rk_model(components=("AL", "CU", "NI"), seed=0, T_range=(1000.0, 3047.0), scale=1e4,
+                       functions=None, max_tries=1000):
+    """Seeded synthetic ternary (or N-ary) Redlich-Kister liquid + FCC model.
+
+    Liquid: G_i = a_i + b_i T with a_i ~ N(0, scale), b_i ~ N(0, scale)/1000
+    (J/mol and J/(mol K)); L^k_ij ~ N(0, scale) J/mol, independent of T.
+    FCC:    G_i = liquid G_i - L_i v_m (T_M,i - T)/T_M,i (latent heat from the
+    engine parameters, so each pure element melts at its T_M) and its own
+    L^k_ij drawn like the liquid's.
+
+    Draws are rejected (seed, seed+1, ...) until both phases are convex in
+    composition over ``T_range`` (no spinodal: the N-component engine has no
+    gradient energy in c, so a concave G would make the explicit scheme
+    ill-posed).  Returns (model, accepted_seed).
+    """
+    comps = sorted(components)
+    n = len(comps)
+    fn = dict(NI_CU_IDEAL_FUNCTIONS if functions is None else functions)
+    v_m = fn["V_M"]
+    npair = n * (n - 1) // 2
+    for s in range(seed, seed + max_tries):
+        rng = np.random.default_rng(s)
+        g0L = np.zeros((n, 5))
+        g0L[:, 0] = rng.normal(0.0, scale, n)
+        g0L[:, 1] = rng.normal(0.0, scale, n) / 1000.0
+        LL = np.zeros((npair, 3, 2))
+        LL[:, :, 0] = rng.normal(0.0, scale, (npair, 3))
+        LS = np.zeros((npair, 3, 2))
+        LS[:, :, 0] = rng.normal(0.0, scale, (npair, 3))
+        g0S = g0L.copy()
+        for i, cname in enumerate(comps):
+            Lh = fn[cname + "_L"] * v_m   # J/mol
+            Tm = fn[cname + "_TM"]
+            # -Lh (Tm - T)/Tm = -Lh + (Lh/Tm) T
+            g0S[i, 0] -= Lh
+            g0S[i, 1] += Lh / Tm
+        liquid = RKPhase("LIQUID", g0L, LL)
+        solid = RKPhase("FCC_A1", g0S, LS)
+        ok = True
+        for T in np.linspace(T_range[0], T_range[1], 5):
+            if _hessian_min_eig(liquid, T) <= 0 or _hessian_min_eig(solid, T) <= 0:
+                ok = False
+                break
+        if ok:
+            model = ThermoModel(comps, {"FCC_A1": solid, "LIQUID": liquid}, fn)
+            return model, s
+    raise RuntimeError("no convex synthetic model found")
+
+
+def load_tdb_model(path, components=None, phases=None):
+    raise NotImplementedError(
+        "reading TDB files is not part of this engine path; build a ThermoModel "
+        "(e.g. synthetic_rk_model) and pass it as tdb_container")

@@ -1,0 +1,357 @@
+"""GPU parity: every engine step through libppf_hip.so against the FP64 oracle.
+
+Bars: FP64 <= 1e-10 relative L2 (and the same for L-infinity where stated);
+FP32 within the tolerance written in each test.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import bc as obc
+from oracle import engines as oe
+from tests.oracle_runner import padded_T, rel_l2, run_ncomp, run_point, run_warren
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def _engines():
+    import pyphasefield_amd.Engines as engines
+    return engines
+
+
+def _setup(sim, bc, dx, dt, precision):
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(dx)
+    sim.set_dt(dt)
+    sim.set_boundary_conditions(bc)
+    sim.set_precision(precision)
+    sim.initialize_fields_and_imported_data()
+
+
+# ------------------------------------------------------------ boundary rules --
+@pytest.mark.parametrize("bcx", ["PERIODIC", "NEUMANN", "DIRCHLET"])
+@pytest.mark.parametrize("bcy", ["PERIODIC", "NEUMANN", "DIRCHLET"])
+def test_apply_bc_2d_matches_reference_rules(gpu, bcx, bcy):
+    import torch
+    from pyphasefield_amd import _lib
+    rng = np.random.default_rng(3)
+    ny, nx, F = 7, 9, 2
+    host = rng.random((F, ny + 2, nx + 2))
+    bca = rng.random((F, ny + 2, nx + 2))
+    dev = _lib.alloc_stack(F, [ny, nx], torch.float64, gpu)
+    dev.copy_(torch.from_numpy(host))
+    bcd = _lib.alloc_stack(F, [ny, nx], torch.float64, gpu)
+    bcd.copy_(torch.from_numpy(bca))
+    g = _lib.make_grid(dev, 2, [bcx, bcy], 0.37)
+    _lib.check(_lib.lib().ppf_apply_bc(g, _lib.ptr(dev), _lib.ptr(bcd), _lib.stream_handle()), "apply_bc")
+    want = host.copy()
+    obc.apply_bc(want, [bcx, bcy], bca, 0.37)
+    np.testing.assert_array_equal(dev.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("bc", ["PERIODIC", "NEUMANN", ["NEUMANN", "PERIODIC", "DIRCHLET"]])
+def test_apply_bc_3d_and_1d(gpu, bc):
+    import torch
+    from pyphasefield_amd import _lib
+    rng = np.random.default_rng(4)
+    for shape in ([5, 4, 6], [11]):
+        nd = len(shape)
+        host = rng.random([1] + [s + 2 for s in shape])
+        bca = rng.random(host.shape)
+        dev = _lib.alloc_stack(1, shape, torch.float64, gpu)
+        dev.copy_(torch.from_numpy(host))
+        bcd = _lib.alloc_stack(1, shape, torch.float64, gpu)
+        bcd.copy_(torch.from_numpy(bca))
+        b = bc if isinstance(bc, str) else bc[:nd]
+        g = _lib.make_grid(dev, nd, b, 0.5)
+        _lib.check(_lib.lib().ppf_apply_bc(g, _lib.ptr(dev), _lib.ptr(bcd), _lib.stream_handle()), "apply_bc")
+        want = host.copy()
+        obc.apply_bc(want, b, bca, 0.5)
+        np.testing.assert_array_equal(dev.cpu().numpy(), want)
+
+
+# ------------------------------------------------------------- Diffusion --
+def test_diffusion_config1_fp64_parity(gpu):
+    """Diffusion 256^2, FP64, D=1, dx=1, dt=0.1, periodic, c ~ U[0,1) seeded, 1000 steps."""
+    E = _engines()
+    sim = E.Diffusion(dimensions=[256, 256])
+    sim.set_user_data({"D": 1.0})
+    _setup(sim, "PERIODIC", 1.0, 0.1, "FP64")
+    c0 = np.random.default_rng(0).random((256, 256))
+    sim.fields[0].data[1:-1, 1:-1] = c0
+    sim.simulate(1000)
+    got = sim.fields[0].get_cells()
+    want = run_point("diffusion", c0, 1000, "PERIODIC", 1.0, 0.1, D=1.0)[1:-1, 1:-1]
+    assert rel_l2(got, want) <= FP64_TOL
+    assert abs(got.sum() - c0.sum()) <= 1e-9 * c0.sum()
+
+
+@pytest.mark.parametrize("bc", ["NEUMANN", "DIRCHLET", ["PERIODIC", "NEUMANN"], ["DIRCHLET", "PERIODIC"]])
+def test_diffusion_2d_bc_values_fp64(gpu, bc):
+    E = _engines()
+    ny, nx = 37, 70
+    sim = E.Diffusion(dimensions=[ny, nx])
+    sim.set_user_data({"D": 0.7})
+    _setup(sim, bc, 0.5, 0.05, "FP64")
+    rng = np.random.default_rng(1)
+    c0 = rng.random((ny, nx))
+    bca = rng.random((1, ny + 2, nx + 2))
+    sim.fields[0].data[1:-1, 1:-1] = c0
+    sim._boundary_conditions_array = bca.copy()
+    sim.simulate(40)
+    got = sim.fields[0].data
+    want = run_point("diffusion", c0, 40, bc, 0.5, 0.05, bcarray=bca, D=0.7)
+    assert rel_l2(got[1:-1, 1:-1], want[1:-1, 1:-1]) <= FP64_TOL
+    assert rel_l2(got, want) <= FP64_TOL  # halos materialised on retrieval
+
+
+@pytest.mark.parametrize("shape", [[9, 10, 11], [50]])
+def test_diffusion_3d_1d_fp64(gpu, shape):
+    E = _engines()
+    sim = E.Diffusion(dimensions=shape)
+    sim.set_user_data({"D": 1.0})
+    _setup(sim, "NEUMANN" if len(shape) == 3 else "PERIODIC", 1.0, 0.1, "FP64")
+    c0 = np.random.default_rng(2).random(shape)
+    sim.fields[0].data[tuple(slice(1, -1) for _ in shape)] = c0
+    sim.simulate(25)
+    want = run_point("diffusion", c0, 25, "NEUMANN" if len(shape) == 3 else "PERIODIC", 1.0, 0.1, D=1.0)
+    assert rel_l2(sim.fields[0].data, want) <= FP64_TOL
+
+
+def test_diffusion_fp32(gpu):
+    E = _engines()
+    sim = E.Diffusion(dimensions=[130, 200])
+    sim.set_user_data({"D": 1.0})
+    _setup(sim, "PERIODIC", 1.0, 0.2, "FP32")
+    c0 = np.random.default_rng(5).random((130, 200))
+    sim.fields[0].data[1:-1, 1:-1] = c0
+    sim.simulate(100)
+    want = run_point("diffusion", c0, 100, "PERIODIC", 1.0, 0.2, D=1.0)[1:-1, 1:-1]
+    got = sim.fields[0].get_cells()
+    assert rel_l2(got, want) <= 2e-6
+    assert np.max(np.abs(got - want)) <= 5e-6
+
+
+# ------------------------------------------------------------- Allen-Cahn --
+@pytest.mark.parametrize("bc", ["PERIODIC", "NEUMANN", ["NEUMANN", "DIRCHLET"]])
+def test_allencahn_fp64(gpu, bc):
+    E = _engines()
+    sim = E.CahnAllen(dimensions=[33, 47], seed=11)
+    sim.set_user_data({"M": 0.5, "W": 0.25, "epsilon": 1.3})
+    _setup(sim, bc, 1.0, 0.05, "FP64")
+    rng = np.random.default_rng(6)
+    phi0 = rng.random((33, 47))
+    bca = rng.random((1, 35, 49))
+    sim.fields[0].data[1:-1, 1:-1] = phi0
+    sim._boundary_conditions_array = bca.copy()
+    sim.simulate(60)
+    want = run_point("allencahn", phi0, 60, bc, 1.0, 0.05, bcarray=bca, M=0.5, W=0.25, epsilon=1.3)
+    assert rel_l2(sim.fields[0].data, want) <= FP64_TOL
+
+
+def test_allencahn_3d_fp32(gpu):
+    E = _engines()
+    sim = E.CahnAllen(dimensions=[12, 14, 16], seed=12)
+    sim.set_user_data({"M": 0.5, "W": 0.25, "epsilon": 1.0})
+    _setup(sim, "PERIODIC", 1.0, 0.02, "FP32")
+    phi0 = np.random.default_rng(7).random((12, 14, 16))
+    sim.fields[0].data[1:-1, 1:-1, 1:-1] = phi0
+    sim.simulate(50)
+    want = run_point("allencahn", phi0, 50, "PERIODIC", 1.0, 0.02, M=0.5, W=0.25, epsilon=1.0)
+    assert rel_l2(sim.fields[0].get_cells(), want[1:-1, 1:-1, 1:-1]) <= 3e-6
+
+
+# ---------------------------------------------------------- Cahn-Hilliard --
+@pytest.mark.parametrize("bc", ["PERIODIC", "NEUMANN", ["PERIODIC", "NEUMANN"], ["DIRCHLET", "PERIODIC"]])
+def test_cahnhilliard_2d_fp64(gpu, bc):
+    E = _engines()
+    ny, nx = 40, 70
+    sim = E.CahnHilliard(dimensions=[ny, nx], seed=1)
+    sim.set_user_data({"M": 1.0, "epsilon": 1.0})
+    _setup(sim, bc, 1.0, 0.01, "FP64")
+    rng = np.random.default_rng(8)
+    c0 = 0.5 + 0.01 * (2 * rng.random((ny, nx)) - 1)
+    sim.fields[0].data[1:-1, 1:-1] = c0
+    sim.simulate(300)
+    want = run_point("cahnhilliard", c0, 300, bc, 1.0, 0.01, M=1.0, epsilon=1.0)
+    got = sim.fields[0].get_cells()
+    assert rel_l2(got, want[1:-1, 1:-1]) <= FP64_TOL
+    if bc in ("PERIODIC", "NEUMANN"):
+        assert abs(sim.field_sums()[0] - c0.sum()) <= 1e-10 * c0.sum()
+
+
+def test_cahnhilliard_3d_fp64_and_reductions(gpu):
+    E = _engines()
+    shape = [10, 12, 14]
+    sim = E.CahnHilliard(dimensions=shape, seed=2)
+    sim.set_user_data({"M": 1.0, "epsilon": 1.0})
+    _setup(sim, "PERIODIC", 1.0, 0.01, "FP64")
+    rng = np.random.default_rng(9)
+    c0 = 0.5 + 0.01 * (2 * rng.random(shape) - 1)
+    sim.fields[0].data[1:-1, 1:-1, 1:-1] = c0
+    sim.simulate(100)
+    want = run_point("cahnhilliard", c0, 100, "PERIODIC", 1.0, 0.01, M=1.0, epsilon=1.0)
+    assert rel_l2(sim.fields[0].data, want) <= FP64_TOL
+    mass = sim.field_sums()[0]
+    assert abs(mass - c0.sum()) <= 1e-11 * c0.sum()
+    F = sim.free_energy()
+    F_or = oe.cahnhilliard_free_energy(want, 1.0, 1.0)
+    assert abs(F - F_or) <= 1e-10 * abs(F_or)
+
+
+def test_cahnhilliard_fp32_mass_fp64_reduction(gpu):
+    E = _engines()
+    sim = E.CahnHilliard(dimensions=[128, 128], seed=3)
+    sim.set_user_data({"M": 1.0, "epsilon": 1.0})
+    _setup(sim, "PERIODIC", 1.0, 0.01, "FP32")
+    c0 = 0.5 + 0.01 * (2 * np.random.default_rng(10).random((128, 128)) - 1)
+    sim.fields[0].data[1:-1, 1:-1] = c0
+    m0 = sim.field_sums()[0]
+    sim.simulate(200)
+    want = run_point("cahnhilliard", c0, 200, "PERIODIC", 1.0, 0.01, M=1.0, epsilon=1.0)[1:-1, 1:-1]
+    got = sim.fields[0].get_cells()
+    assert rel_l2(got, want) <= 1e-5
+    # FP32 stencil, FP64 sum: conserved to the FP32 rounding of the update
+    assert abs(sim.field_sums()[0] - m0) <= 1e-6 * m0
+
+
+# --------------------------------------------------------------- Warren --
+def _warren_sim(E, n, precision, alpha, seed=5, shape="circle"):
+    sim = E.Warren1995(dimensions=[n, n], seed=seed)
+    sim.set_user_data({"alpha": alpha, "y_e": 0.04, "seed_shape": shape, "seed_radius": 8,
+                       "c_solid": 0.40, "c_liquid": 0.41})
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(4.6e-8)
+    sim.set_boundary_conditions("NEUMANN")
+    sim.set_precision(precision)
+    sim.initialize_fields_and_imported_data()
+    return sim
+
+
+@pytest.mark.parametrize("alpha", [0.0, 0.3])
+def test_warren_fp64_parity(gpu, alpha):
+    from pyphasefield_amd.Engines.Warren1995 import warren_derived
+    E = _engines()
+    sim = _warren_sim(E, 48, "FP64", alpha)
+    phi0 = sim.fields[0].get_cells().copy()
+    c0 = sim.fields[1].get_cells().copy()
+    sim.simulate(200)
+    p = warren_derived(sim.user_data, sim.dx)
+    want = run_warren(phi0, c0, 200, "NEUMANN", sim.dx, p, seed=5)
+    assert rel_l2(sim.fields[0].data, want[0]) <= FP64_TOL
+    assert rel_l2(sim.fields[1].data, want[1]) <= FP64_TOL
+
+
+def test_warren_fp32_vs_fp64_oracle(gpu):
+    """FP32 Warren: phi and c within stated tolerances, interface within 1 cell."""
+    from pyphasefield_amd.Engines.Warren1995 import warren_derived
+    E = _engines()
+    sim = _warren_sim(E, 64, "FP32", 0.01)
+    phi0 = sim.fields[0].get_cells().copy()
+    c0 = sim.fields[1].get_cells().copy()
+    sim.simulate(400)
+    p = warren_derived(sim.user_data, sim.dx)
+    want = run_warren(phi0, c0, 400, "NEUMANN", sim.dx, p, seed=5)
+    gphi = sim.fields[0].get_cells()
+    gc = sim.fields[1].get_cells()
+    assert np.max(np.abs(gphi - want[0][1:-1, 1:-1])) <= 1e-3
+    assert np.max(np.abs(gc - want[1][1:-1, 1:-1])) <= 1e-4
+    # interface (phi = 0.5 contour) position along the centre row within 1 cell
+    row = 32
+    a = np.argmax(gphi[row] < 0.5)
+    b = np.argmax(want[0][1 + row, 1:-1] < 0.5)
+    assert abs(int(a) - int(b)) <= 1
+
+
+# -------------------------------------------------------------- N-component --
+def _nc_sim(E, ny, nx, precision, noise, bc="NEUMANN", temp="LINEAR_GRADIENT", dg=None, seeds=3):
+    from pyphasefield_amd import thermo
+    model, _ = thermo.synthetic_rk_model(seed=0)
+    sim = E.NCGPU_new(dimensions=[ny, nx], seed=7)
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(0.0000046)
+    sim.set_temperature_type(temp)
+    sim.set_temperature_initial_T(1574.)
+    if temp == "LINEAR_GRADIENT":
+        sim.set_temperature_dTdx(1.0 / 0.0000046)   # 1 K per cell
+        sim.set_temperature_dTdy(0.0)
+        sim.set_temperature_dTdt(-2.0e5)
+    sim.set_tdb_container(model)
+    sim.set_boundary_conditions(bc)
+    sim.set_precision(precision)
+    ud = {"d_ratio": 1.1, "sim_type": "seeds", "number_of_seeds": seeds, "seed_radius": 4,
+          "initial_concentration_array": [0.30, 0.30], "noise_phi": noise, "noise_c": noise, "noise_q": noise}
+    if dg:
+        ud["dg_mode"] = dg
+    sim.set_user_data(ud)
+    sim.initialize_fields_and_imported_data()
+    return sim, model
+
+
+def _nc_oracle(sim, model, steps, bc, step0=0):
+    u = sim.user_data
+    fields0 = np.stack([f.get_cells().copy() for f in sim.fields])
+    ny, nx = sim.dimensions
+
+    def T_of(s):
+        Ti = oe.temperature_linear((ny, nx), sim._initial_T, sim._dTdx or 0.0, sim._dTdy or 0.0, sim.dx,
+                                   sim._dTdt or 0.0, sim.dt, s - 1)
+        return padded_T(Ti, bc)
+
+    liq = (model.liquid.g0, model.liquid.L, model.liquid.R)
+    sol = (model.solid.g0, model.solid.L, model.solid.R)
+    return fields0, lambda: run_ncomp(fields0, steps, bc, T_of, u["params"], u["c_params"], liq, sol,
+                                      seed=sim.seed, step0=step0)
+
+
+@pytest.mark.parametrize("noise,bc", [(0.0, "NEUMANN"), (1.0, "NEUMANN"), (0.0, "PERIODIC")])
+def test_ncomp_fp64_parity(gpu, noise, bc):
+    E = _engines()
+    sim, model = _nc_sim(E, 40, 52, "FP64", noise, bc=bc)
+    sim.just_before_simulating()  # packs params for the oracle
+    sim._begun_simulation = True
+    _, run = _nc_oracle(sim, model, 30, bc)
+    sim.simulate(30)
+    want = run()
+    for i in range(len(sim.fields)):
+        assert rel_l2(sim.fields[i].data, want[i]) <= FP64_TOL, sim.fields[i].name
+
+
+def test_ncomp_fp32_analytic_vs_fp64_oracle(gpu):
+    E = _engines()
+    sim, model = _nc_sim(E, 48, 48, "FP32", 0.0)
+    sim.just_before_simulating()
+    sim._begun_simulation = True
+    _, run = _nc_oracle(sim, model, 100, "NEUMANN")
+    sim.simulate(100)
+    want = run()
+    for i in range(3, len(sim.fields)):
+        assert np.max(np.abs(sim.fields[i].get_cells() - want[i][1:-1, 1:-1])) <= 1e-4
+    assert np.max(np.abs(sim.fields[0].get_cells() - want[0][1:-1, 1:-1])) <= 1e-3
+
+
+def test_ncomp_thermo_eval_matches_oracle(gpu):
+    import torch
+    from oracle import thermo as oth
+    from pyphasefield_amd import _lib, thermo
+    model, _ = thermo.synthetic_rk_model(seed=0)
+    rng = np.random.default_rng(0)
+    n = 1000
+    T = rng.uniform(1000, 3000, n)
+    c = rng.dirichlet([2, 2, 2], n)[:, :2]
+    Td = torch.from_numpy(T).to(gpu)
+    cd = torch.from_numpy(np.ascontiguousarray(c)).to(gpu)
+    out = torch.empty(n * 6, dtype=torch.float64, device=gpu)
+    th = model.to_ctypes()
+    _lib.check(_lib.lib().ppf_thermo_eval(th, n, _lib.ptr(Td), _lib.ptr(cd), _lib.ptr(out),
+                                          _lib.stream_handle()), "thermo_eval")
+    o = out.cpu().numpy().reshape(n, 6)
+    cs = [c[:, 0], c[:, 1], 1 - c[:, 0] - c[:, 1]]
+    for k, ph in ((0, model.liquid), (3, model.solid)):
+        G = oth.gibbs(cs, T, ph.g0, ph.L, ph.R)
+        d = oth.analytic_derivatives(cs, T, ph.g0, ph.L, ph.R)
+        assert np.max(np.abs(o[:, k] - G) / np.abs(G).max()) <= 1e-13
+        for l in range(2):
+            assert np.max(np.abs(o[:, k + 1 + l] - d[l])) <= 1e-9 * np.abs(d[l]).max()
