@@ -191,6 +191,16 @@ def warren_step(phi, c, p, dt, dx, seed=0, step=0, vectorize=False):
     ``vectorize=True`` evaluates the interpolants with np.vectorize exactly as
     the reference does (``Warren1995.py:26-28``), for the CPU baseline.
     """
+    deltaphi, deltac = warren_rates(phi, c, p, dx, seed, step, vectorize)
+    phi_o = phi.copy()
+    c_o = c.copy()
+    _set_interior(phi_o, phi + deltaphi * dt)
+    _set_interior(c_o, c + deltac * dt)
+    return phi_o, c_o
+
+
+def warren_rates(phi, c, p, dx, seed=0, step=0, vectorize=False):
+    """(dphi/dt, dc/dt) of Engines/Warren1995.py:56-86 on the padded arrays."""
     if vectorize:
         g, pp, gprime = _g_vec(phi), _p_vec(phi), _gprime_vec(phi)
     else:
@@ -228,12 +238,7 @@ def warren_step(phi, c, p, dt, dx, seed=0, step=0, vectorize=False):
         u = _rng.uniform_f64(seed, step, _rng.S_WARREN_PHI, phi.size, cells)
         randArray = 2 * u - 1
         deltaphi = deltaphi + M_phi * p["alpha"] * randArray * (16 * g) * ((1 - c) * H_A + c * H_B)
-
-    phi_o = phi.copy()
-    c_o = c.copy()
-    _set_interior(phi_o, phi + deltaphi * dt)
-    _set_interior(c_o, c + deltac * dt)
-    return phi_o, c_o
+    return deltaphi, deltac
 
 
 # ---------------------------------------------------------------------------
@@ -395,6 +400,28 @@ def ncomp_main(fields, T, transfer, params, c_params, seed=0, step=0):
 
     Returns fields_out (padded; interior updated, halos copied from input).
     """
+    dt = dict(zip(PARAM_NAMES, params))["dt"]
+    dphidt, dcdt, dq1dt, dq4dt = ncomp_rates(fields, T, transfer, params, c_params, seed, step)
+    F = fields.shape[0]
+    out = fields.copy()
+    p = fields[0][1:-1, 1:-1]
+    for l in range(3, F):
+        out[l][1:-1, 1:-1] = dcdt[l - 3] * dt + fields[l][1:-1, 1:-1]
+    phi_new = p + dt * dphidt
+    phi_new = np.where(phi_new < 0.0001, 0.0001, phi_new)
+    phi_new = np.where(phi_new > 0.9999, 0.9999, phi_new)
+    q1c, q4c = fields[1][1:-1, 1:-1], fields[2][1:-1, 1:-1]
+    q1n = q1c + dt * dq1dt
+    q4n = q4c + dt * dq4dt
+    renorm = np.sqrt(q1n ** 2 + q4n ** 2)
+    out[0][1:-1, 1:-1] = phi_new
+    out[1][1:-1, 1:-1] = q1n / renorm
+    out[2][1:-1, 1:-1] = q4n / renorm
+    return out
+
+
+def ncomp_rates(fields, T, transfer, params, c_params, seed=0, step=0):
+    """(dphi/dt, [dc_l/dt], dq1/dt, dq4/dt) on the interior (NCGPU_new.py:84-232)."""
     pr = dict(zip(PARAM_NAMES, params))
     dx, v_m, M_qmax, H, y_e, beta = pr["dx"], pr["v_m"], pr["M_qmax"], pr["H"], pr["y_e"], pr["beta"]
     dt, ebar, eqbar = pr["dt"], pr["ebar"], pr["eqbar"]
@@ -460,7 +487,7 @@ def ncomp_main(fields, T, transfer, params, c_params, seed=0, step=0):
         xi = _rng.normal_f64(seed, step, _rng.S_NC_PHI, phi.size, cells)
         dphidt = dphidt + np.sqrt(2. * 8.314 * Tc * M_phi / v_m) * xi * noise_amp_phi
 
-    out = fields.copy()
+    dcdt = []
     for l in range(3, F):
         M_c = _Nb(transfer[l - 1])
         dF = _Nb(transfer[l - 1 + F - 3])
@@ -472,7 +499,7 @@ def ncomp_main(fields, T, transfer, params, c_params, seed=0, step=0):
             acc = acc - _divagradb(M_c.c * cj.c, M_c.xp * cj.xp, M_c.xm * cj.xm,
                                    M_c.yp * cj.yp, M_c.ym * cj.ym,
                                    dFj.c, dFj.xp, dFj.xm, dFj.yp, dFj.ym, idx)
-        out[l][1:-1, 1:-1] = acc * dt + fields[l][1:-1, 1:-1]
+        dcdt.append(acc)
 
     D_q = 2. * H * Tc * (p ** 2)
     D_q_xp = 2. * H * TT.xp * (P.xp ** 2)
@@ -498,16 +525,7 @@ def ncomp_main(fields, T, transfer, params, c_params, seed=0, step=0):
     t4 = f_ori_4 + lq4 * eqbar2 + noise_q4
     dq1dt = M_q * ((1 - q1c ** 2) * t1 - q1c * q4c * t4)
     dq4dt = M_q * ((1 - q4c ** 2) * t4 - q1c * q4c * t1)
-    phi_new = p + dt * dphidt
-    phi_new = np.where(phi_new < 0.0001, 0.0001, phi_new)
-    phi_new = np.where(phi_new > 0.9999, 0.9999, phi_new)
-    q1n = q1c + dt * dq1dt
-    q4n = q4c + dt * dq4dt
-    renorm = np.sqrt(q1n ** 2 + q4n ** 2)
-    out[0][1:-1, 1:-1] = phi_new
-    out[1][1:-1, 1:-1] = q1n / renorm
-    out[2][1:-1, 1:-1] = q4n / renorm
-    return out
+    return dphidt, dcdt, dq1dt, dq4dt
 
 
 # ---------------------------------------------------------------------------
