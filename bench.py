@@ -1,0 +1,245 @@
+"""Benchmark of the engine hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--no-cpu]
+
+Default workload (``--config warren``): Warren1995 binary solidification,
+2D 2048x2048, FP32 (phi, c), circular solid seed of radius 8 at the centre,
+c = 0.40 in the solid and 0.41 in the liquid, isothermal 1574 K, 4-fold
+anisotropy 0.04, interface noise amplitude 0.01, Neumann boundaries.  A step
+is one explicit-Euler time step of the whole grid; inputs are resident in HBM
+before the timed region.  Prints ONE JSON line with the metric
+(cell-updates/s), the roofline of the dominant kernel and the CPU baseline
+(the FP64 numpy restatement of the reference engine timed on this host).
+
+Other configs (one JSON line each): ``diffusion`` (2D 256^2 FP64),
+``ch3d`` (Cahn-Hilliard 3D 512^3 FP32), ``ncomp`` (N-component 2048^2, add
+``--variant fp64|fp32|surrogate``), ``ensemble`` (256 batched AC/CH
+instances), ``copy`` (device copy of 1 GiB, the HBM calibration kernel for
+rocprofv3 counters).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def _traffic(kernel_key):
+    """HBM bytes per launch from the committed rocprofv3 counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get(kernel_key)
+    except (OSError, ValueError):
+        return None
+
+
+def _events():
+    import torch
+    s = torch.cuda.current_stream()
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True), s
+
+
+def _timed(fn, steps):
+    """Run fn(steps) between a barrier/sync pair; return seconds (HIP events on
+    the stream the kernels are launched on)."""
+    import torch
+    torch.cuda.synchronize()
+    e0, e1, s = _events()
+    e0.record(s)
+    fn(steps)
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3
+
+
+def _line(metric, value, unit, steps, warmup, t, dtype, config, roofline, cpu, higher=True, scaling="weak"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": t / steps * 1e3, "higher_is_better": higher, "scaling": scaling, "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
+
+
+def _roofline(bytes_per_launch, seconds_per_launch, kernel_key):
+    achieved = bytes_per_launch / seconds_per_launch / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic(kernel_key), "kernel": kernel_key,
+            "bytes_per_launch_algorithmic": bytes_per_launch}
+
+
+# ------------------------------------------------------------------ Warren --
+def warren_sim(n, precision="FP32", alpha=0.01):
+    import pyphasefield_amd.Engines as engines
+    sim = engines.Warren1995(dimensions=[n, n], seed=1234)
+    sim.set_user_data({"alpha": alpha, "y_e": 0.04, "T": 1574., "seed_shape": "circle", "seed_radius": 8,
+                       "c_solid": 0.40, "c_liquid": 0.41})
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(4.6e-8)
+    sim.set_boundary_conditions("NEUMANN")
+    sim.set_precision(precision)
+    sim.initialize_fields_and_imported_data()
+    return sim
+
+
+def cpu_warren(n, k=3):
+    """FP64 numpy restatement of engine_Warren1995 as the reference writes it
+    (np.roll, np.vectorize interpolants), 1 warm-up + k timed steps."""
+    from oracle import bc as obc
+    from oracle import engines as oe
+    from pyphasefield_amd.Engines.Warren1995 import _DEFAULTS, warren_derived
+    p = warren_derived(dict(_DEFAULTS, alpha=0.01), 4.6e-8)
+    yy, xx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    solid = (yy - n / 2) ** 2 + (xx - n / 2) ** 2 < 64
+    f = np.stack([obc.pad(np.where(solid, 0.0, 1.0)), obc.pad(np.where(solid, 0.40, 0.41))])
+    obc.apply_bc(f, "NEUMANN", None, 4.6e-8)
+    t0 = 0.0
+    for s in range(k + 1):
+        if s == 1:
+            t0 = time.perf_counter()
+        a, b = oe.warren_step(f[0], f[1], p, p["dt"], 4.6e-8, seed=1, step=s + 1, vectorize=True)
+        f = np.stack([a, b])
+        obc.apply_bc(f, "NEUMANN", None, 4.6e-8)
+    dt = time.perf_counter() - t0
+    return {"value": n * n * k / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d timed steps (+1 warm-up) of the %dx%d FP64 numpy restatement with np.vectorize "
+                      "interpolants (Warren1995.py:26-28), single-threaded numpy" % (k, n, n)}
+
+
+def bench_warren(args):
+    n = 2048
+    sim = warren_sim(n, "FP32")
+    sim.simulate(args.warmup)
+    t = _timed(sim.simulate, args.steps)
+    cells = n * n
+    value = cells * args.steps / t
+    roof = _roofline(16.0 * cells, t / args.steps, "k_warren<float>")
+    cpu = None if args.no_cpu else cpu_warren(n)
+    return _line("cell-updates/s", value, "cell-updates/s", args.steps, args.warmup, t, "f32",
+                 {"workload": "Warren1995 2D 2048x2048 FP32, Neumann, seed r=8, noise 0.01",
+                  "engine": "Warren1995", "grid": [n, n], "bytes_per_cell_update": 16}, roof, cpu)
+
+
+# --------------------------------------------------------------- Diffusion --
+def cpu_diffusion(n, k):
+    from oracle import bc as obc
+    from oracle import engines as oe
+    a = obc.pad(np.random.default_rng(0).random((n, n)))
+    obc.apply_bc(a[None], "PERIODIC")
+    t0 = time.perf_counter()
+    for _ in range(k):
+        a = oe.diffusion_step(a, 1.0, 0.1, 1.0)
+        obc.apply_bc(a[None], "PERIODIC")
+    dt = time.perf_counter() - t0
+    return {"value": n * n * k / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d steps of the %dx%d FP64 numpy restatement (np.roll laplacian)" % (k, n, n)}
+
+
+def bench_diffusion(args):
+    import pyphasefield_amd.Engines as engines
+    n = 256
+    sim = engines.Diffusion(dimensions=[n, n])
+    sim.set_user_data({"D": 1.0})
+    sim.set_dx(1.0)
+    sim.set_dt(0.1)
+    sim.set_boundary_conditions("PERIODIC")
+    sim.set_precision("FP64")
+    sim.initialize_fields_and_imported_data()
+    sim.fields[0].data[1:-1, 1:-1] = np.random.default_rng(0).random((n, n))
+    sim.simulate(args.warmup)
+    t = _timed(sim.simulate, args.steps)
+    cells = n * n
+    roof = _roofline(16.0 * cells, t / args.steps, "k_point2d<double,0>")
+    cpu = None if args.no_cpu else cpu_diffusion(n, 1000)
+    return _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f64",
+                 {"workload": "Diffusion 2D 256x256 FP64 periodic, D=1 dx=1 dt=0.1", "engine": "Diffusion",
+                  "grid": [n, n], "bytes_per_cell_update": 16}, roof, cpu)
+
+
+# ------------------------------------------------------------ CH 3D 512^3 --
+def bench_ch3d(args):
+    import pyphasefield_amd.Engines as engines
+    n = args.size or 512
+    sim = engines.CahnHilliard(dimensions=[n, n, n], seed=7)
+    sim.set_user_data({"M": 1.0, "epsilon": 1.0, "c0": 0.5, "noise": 0.01})
+    sim.set_dx(1.0)
+    sim.set_dt(0.01)
+    sim.set_boundary_conditions("PERIODIC")
+    sim.set_precision("FP32")
+    sim.initialize_fields_and_imported_data()
+    sim.simulate(args.warmup)
+    t = _timed(sim.simulate, args.steps)
+    cells = n ** 3
+    roof = _roofline(8.0 * cells, t / args.steps, "k_ch3d<float>")
+    mass = sim.field_sums()[0]
+    line = _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f32",
+                 {"workload": "Cahn-Hilliard 3D %d^3 FP32 periodic, M=1 kappa=1 dt=0.01" % n,
+                  "engine": "CahnHilliard", "grid": [n, n, n], "bytes_per_cell_update": 8}, roof, None)
+    line["fp64_mass"] = mass
+    line["fp64_free_energy"] = sim.free_energy()
+    return line
+
+
+# -------------------------------------------------------------- copy ref --
+def bench_copy(args):
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+
+    def run(k):
+        for _ in range(k):
+            b.copy_(a)
+    run(args.warmup)
+    t = _timed(run, args.steps)
+    gbs = 2 * nbytes * args.steps / t / 1e9
+    return {"metric": "device copy GB/s (read + write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": t / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "copy 1 GiB fp32 -> fp32"},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": 2 * nbytes}, "cpu_baseline": None}
+
+
+CONFIGS = {"warren": bench_warren, "diffusion": bench_diffusion, "ch3d": bench_ch3d, "copy": bench_copy}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--config", default="warren", choices=sorted(CONFIGS) + ["ncomp", "ensemble"])
+    ap.add_argument("--variant", default="fp32")
+    ap.add_argument("--size", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this engine path has no multi-GPU mode "
+              "(the domain would shard by slabs with a 1-2 cell halo exchange; not built)", file=sys.stderr)
+        sys.exit(2)
+    defaults = {"warren": (2000, 100), "diffusion": (1000, 50), "ch3d": (50, 5), "copy": (50, 5),
+                "ncomp": (500, 20), "ensemble": (500, 20)}
+    st, wu = defaults[args.config]
+    args.steps = args.steps if args.steps is not None else st
+    args.warmup = args.warmup if args.warmup is not None else wu
+    if args.config == "ncomp":
+        from pyphasefield_amd.bench_ncomp import bench_ncomp
+        line = bench_ncomp(args, _timed, _line, _roofline)
+    elif args.config == "ensemble":
+        from pyphasefield_amd.ensemble import bench_ensemble
+        line = bench_ensemble(args, _timed, _line, _roofline)
+    else:
+        line = CONFIGS[args.config](args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -271,7 +271,7 @@ def ncomp_helper(fields, T, params, c_params, thermo_l, thermo_s, seed=0, step=0
     G_L = _thermo.gibbs(arr, T, *thermo_l)
     G_S = _thermo.gibbs(arr, T, *thermo_s)
     g = (phi ** 2) * (1 - phi) ** 2
-    h = (phi ** 3) * (6. * phi ** 2 - 15. * phi + 10.)
+    h = (phi * phi * phi) * (6. * phi ** 2 - 15. * phi + 10.)  # numba int power: products
     transfer = np.zeros((2 * ncomp,) + phi.shape)
     transfer[0] = G_L
     transfer[1] = G_S
@@ -437,9 +437,9 @@ def ncomp_rates(fields, T, transfer, params, c_params, seed=0, step=0):
 
     p = P.c
     g = (p ** 2) * (1 - p) ** 2
-    h = (p ** 3) * (6. * p ** 2 - 15. * p + 10.)
+    h = (p * p * p) * (6. * p ** 2 - 15. * p + 10.)  # numba int power: products
     hprime = 30. * g
-    gprime = 4. * p ** 3 - 6. * p ** 2 + 2 * p
+    gprime = 4. * (p * p * p) - 6. * p ** 2 + 2 * p
     dphidx = 0.5 * (P.xp - P.xm) * idx
     dphidy = 0.5 * (P.yp - P.ym) * idx
     dTdx = 0.5 * idx * (TT.xp - TT.xm)

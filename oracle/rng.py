@@ -12,8 +12,12 @@ and across checkpoint/resume.
     h       = splitmix64_finalize(seed + counter * GOLDEN)
     uniform_f64 = (h >> 11) * 2**-53          in [0, 1)
     uniform_f32 = (h >> 40) * 2**-24          in [0, 1)
-    normal      = sqrt(-2 ln u1) * cos(2 pi u2)  (Box-Muller),
-                  u1 = ((h >> 32) + 1) * 2**-32 in (0, 1],  u2 = (h & 0xffffffff) * 2**-32
+    normal      = Marsaglia's polar method: attempt k = 0, 1, ... uses
+                  h_0 = h, h_k = hash64(h_{k-1}, k); u = 2((h_k >> 32) + 1/2) 2**-32 - 1,
+                  v = 2((h_k & 0xffffffff) + 1/2) 2**-32 - 1, s = u^2 + v^2; the first
+                  attempt with 0 < s < 1 gives u * sqrt(-2 ln(s) / s), with ln the
+                  shared FP64 logarithm of oracle.thermo.plog (no libm call whose
+                  last bit could differ between host and device).
 
 ``cell`` is the linear index in the canonical padded layout
 (k*(ny+2) + j)*(nx+2) + i, independent of the device memory pitch.
@@ -59,11 +63,28 @@ def uniform_f32(seed, step, stream, ncells, cells):
     return ((h >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)).astype(np.float32)
 
 
+MAX_POLAR_ATTEMPTS = 64
+
+
 def normal_f64(seed, step, stream, ncells, cells):
+    from .thermo import plog
     h = hash64(seed, counters(step, stream, ncells, cells))
-    u1 = ((h >> np.uint64(32)).astype(np.float64) + 1.0) * (2.0 ** -32)
-    u2 = (h & np.uint64(0xFFFFFFFF)).astype(np.float64) * (2.0 ** -32)
-    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    out = np.zeros(h.shape, dtype=np.float64)
+    todo = np.ones(h.shape, dtype=bool)
+    for k in range(MAX_POLAR_ATTEMPTS):
+        if k > 0:
+            h = np.where(todo, hash64(h, np.uint64(k)), h)
+        u = 2.0 * (((h >> np.uint64(32)).astype(np.float64) + 0.5) * (2.0 ** -32)) - 1.0
+        v = 2.0 * (((h & np.uint64(0xFFFFFFFF)).astype(np.float64) + 0.5) * (2.0 ** -32)) - 1.0
+        s = u * u + v * v
+        ok = todo & (s < 1.0) & (s > 0.0)
+        ss = np.where(ok, s, 0.5)
+        val = u * np.sqrt(-2.0 * plog(ss) / ss)
+        out = np.where(ok, val, out)
+        todo = todo & ~ok
+        if not todo.any():
+            break
+    return out
 
 
 def padded_cells(shape_padded):

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_thermo_eval(ppf_thermo th, int64_t n, c
   }
   cc[N - 1] = cN;
   double t = T[i];
-  double lnT = log(t);
+  double lnT = plog(t);
   PhaseAtT<double, N> PL, PS;
   phase_at<double, N>(th.liquid, t, lnT, PL);
   phase_at<double, N>(th.solid, t, lnT, PS);

@@ -16,6 +16,13 @@ namespace ppf {
 
 constexpr double kCMin = 1e-10;  // c ln c evaluated at max(c, kCMin)
 
+template <typename R>
+__device__ __forceinline__ R rlog(R x);
+template <>
+__device__ __forceinline__ double rlog<double>(double x) { return plog(x); }
+template <>
+__device__ __forceinline__ float rlog<float>(float x) { return logf(x); }
+
 // phase coefficients evaluated at one temperature
 template <typename R, int N>
 struct PhaseAtT {
@@ -46,7 +53,7 @@ __device__ __forceinline__ R ideal_sum(const R (&c)[N]) {
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     R ci = c[i] > R(kCMin) ? c[i] : R(kCMin);
-    s = s + ci * log(ci);
+    s = s + ci * rlog<R>(ci);
   }
   return s;
 }
@@ -75,7 +82,7 @@ template <typename R, int N>
 __device__ __forceinline__ void gibbs_grad(const R (&c)[N], const PhaseAtT<R, N>& ph, R (&dG)[N]) {
   R part[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) part[i] = ph.gi[i] + (c[i] > R(kCMin) ? ph.RT * (log(c[i]) + R(1)) : R(0));
+  for (int i = 0; i < N; ++i) part[i] = ph.gi[i] + (c[i] > R(kCMin) ? ph.RT * (rlog<R>(c[i]) + R(1)) : R(0));
   int p = 0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -110,7 +117,7 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
     cN = cN - cin[l];
   }
   c[N - 1] = cN;
-  R lnT = log(T);
+  R lnT = rlog<R>(T);
   PhaseAtT<R, N> PL, PS;
   phase_at<R, N>(k.th.liquid, T, lnT, PL);
   phase_at<R, N>(k.th.solid, T, lnT, PS);
@@ -118,8 +125,10 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
   R G_L = gibbs<R, N>(c, PL, id0);
   R G_S = gibbs<R, N>(c, PS, id0);
   *dG = G_S - G_L;
-  R g = (phi * phi) * (R(1) - phi) * (R(1) - phi);
-  R h = (phi * phi * phi) * (R(6) * phi * phi - R(15) * phi + R(10));
+  // grouping of NCGPU_new.py:298-299 (numba integer powers are products)
+  R phi3 = phi * phi * phi;
+  R g = (phi * phi) * ((R(1) - phi) * (R(1) - phi));
+  R h = phi3 * (R(6) * (phi * phi) - R(15) * phi + R(10));
   R dGL[N], dGS[N];
   if (k.th.dg_mode == PPF_DG_FORWARD_DIFF) {
     // NCGPU_new.py:301-313, with its in-place perturb / restore sequence
@@ -285,10 +294,11 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
     const R(*q4)[W] = sf[2];
     R pc = phi[ly][lx];
     // interpolating functions (NCGPU_new.py:84-87)
-    R g = (pc * pc) * (R(1) - pc) * (R(1) - pc);
-    R h = (pc * pc * pc) * (R(6) * pc * pc - R(15) * pc + R(10));
+    R pc3 = pc * pc * pc;
+    R g = (pc * pc) * ((R(1) - pc) * (R(1) - pc));
+    R h = pc3 * (R(6) * (pc * pc) - R(15) * pc + R(10));
     R hprime = R(30) * g;
-    R gprime = R(4) * pc * pc * pc - R(6) * pc * pc + R(2) * pc;
+    R gprime = R(4) * pc3 - R(6) * (pc * pc) + R(2) * pc;
     // gradients (:89-107)
     R dphidx = R(0.5) * (phi[ly][lx + 1] - phi[ly][lx - 1]) * idx;
     R dphidx2 = dphidx * dphidx;
