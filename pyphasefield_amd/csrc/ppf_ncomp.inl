@@ -353,39 +353,73 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
     R dpsixdy = dq2q2dy * dphidx + q2q2 * d2phidxy - dqq2dy * dphidy - qq2 * d2phidy2;
     R dpsiydx = dq2q2dx * dphidy + q2q2 * d2phidxy + dqq2dx * dphidx + qq2 * d2phidx2;
     R dpsiydy = dq2q2dy * dphidy + q2q2 * d2phidy2 + dqq2dy * dphidx + qq2 * d2phidxy;
-    R psi4 = psix4 + psiy4;
-    // anisotropic terms (:148-160)
-    R d_term_dx = dTdx * ((R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) / mag_grad_phi2 - dphidx * psi4 / mag_grad_phi4);
-    d_term_dx = d_term_dx + Tc *
-                                (R(6) * psix2 * dpsixdx * q2q2 + R(2) * psix3 * dq2q2dx +
-                                 R(6) * psiy2 * dpsiydx * qq2 + R(2) * psiy3 * dqq2dx) /
-                                mag_grad_phi2;
-    d_term_dx = d_term_dx - Tc * (R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) *
-                                (R(2) * dphidx * d2phidx2 + R(2) * dphidy * d2phidxy) / mag_grad_phi4;
-    d_term_dx = d_term_dx - Tc * (d2phidx2 * psi4 + dphidx * (R(4) * psix3 * dpsixdx + R(4) * psiy3 * dpsiydx)) /
-                                mag_grad_phi4;
-    d_term_dx = d_term_dx + Tc *
-                                (dphidx * psi4 *
-                                 (R(4) * dphidx3 * d2phidx2 + R(4) * dphidx * d2phidx2 * dphidy2 +
-                                  R(4) * dphidy * d2phidxy * dphidx2 + R(4) * dphidy3 * d2phidxy)) /
-                                mag_grad_phi8;
-    d_term_dx = d_term_dx * (R(4) * y_e * ebar2);
-    R d_term_dy =
-        dTdy * ((R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) / mag_grad_phi2 - dphidy * psi4 / mag_grad_phi4);
-    d_term_dy = d_term_dy + Tc *
-                                (R(-6) * psix2 * dpsixdy * qq2 - R(2) * psix3 * dqq2dy +
-                                 R(6) * psiy2 * dpsiydy * q2q2 + R(2) * psiy3 * dq2q2dy) /
-                                mag_grad_phi2;
-    d_term_dy = d_term_dy - Tc * (R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) *
-                                (R(2) * dphidx * d2phidxy + R(2) * dphidy * d2phidy2) / mag_grad_phi4;
-    d_term_dy = d_term_dy - Tc * (d2phidy2 * psi4 + dphidy * (R(4) * psix3 * dpsixdy + R(4) * psiy3 * dpsiydy)) /
-                                mag_grad_phi4;
-    d_term_dy = d_term_dy + Tc *
-                                (dphidy * psi4 *
-                                 (R(4) * dphidx3 * d2phidxy + R(4) * dphidx * d2phidxy * dphidy2 +
-                                  R(4) * dphidy * d2phidy2 * dphidx2 + R(4) * dphidy3 * d2phidy2)) /
-                                mag_grad_phi8;
-    d_term_dy = d_term_dy * (R(4) * y_e * ebar2);
+    R d_term_dx, d_term_dy;
+    if constexpr (sizeof(R) == 8) {
+      // reference form (oracle parity)
+      R psi4 = psix4 + psiy4;
+      // anisotropic terms (:148-160)
+      d_term_dx = dTdx * ((R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) / mag_grad_phi2 - dphidx * psi4 / mag_grad_phi4);
+      d_term_dx = d_term_dx + Tc *
+                                  (R(6) * psix2 * dpsixdx * q2q2 + R(2) * psix3 * dq2q2dx +
+                                   R(6) * psiy2 * dpsiydx * qq2 + R(2) * psiy3 * dqq2dx) /
+                                  mag_grad_phi2;
+      d_term_dx = d_term_dx - Tc * (R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) *
+                                  (R(2) * dphidx * d2phidx2 + R(2) * dphidy * d2phidxy) / mag_grad_phi4;
+      d_term_dx = d_term_dx - Tc * (d2phidx2 * psi4 + dphidx * (R(4) * psix3 * dpsixdx + R(4) * psiy3 * dpsiydx)) /
+                                  mag_grad_phi4;
+      d_term_dx = d_term_dx + Tc *
+                                  (dphidx * psi4 *
+                                   (R(4) * dphidx3 * d2phidx2 + R(4) * dphidx * d2phidx2 * dphidy2 +
+                                    R(4) * dphidy * d2phidxy * dphidx2 + R(4) * dphidy3 * d2phidxy)) /
+                                  mag_grad_phi8;
+      d_term_dx = d_term_dx * (R(4) * y_e * ebar2);
+      d_term_dy =
+          dTdy * ((R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) / mag_grad_phi2 - dphidy * psi4 / mag_grad_phi4);
+      d_term_dy = d_term_dy + Tc *
+                                  (R(-6) * psix2 * dpsixdy * qq2 - R(2) * psix3 * dqq2dy +
+                                   R(6) * psiy2 * dpsiydy * q2q2 + R(2) * psiy3 * dq2q2dy) /
+                                  mag_grad_phi2;
+      d_term_dy = d_term_dy - Tc * (R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) *
+                                  (R(2) * dphidx * d2phidxy + R(2) * dphidy * d2phidy2) / mag_grad_phi4;
+      d_term_dy = d_term_dy - Tc * (d2phidy2 * psi4 + dphidy * (R(4) * psix3 * dpsixdy + R(4) * psiy3 * dpsiydy)) /
+                                  mag_grad_phi4;
+      d_term_dy = d_term_dy + Tc *
+                                  (dphidy * psi4 *
+                                   (R(4) * dphidx3 * d2phidxy + R(4) * dphidx * d2phidxy * dphidy2 +
+                                    R(4) * dphidy * d2phidy2 * dphidx2 + R(4) * dphidy3 * d2phidy2)) /
+                                  mag_grad_phi8;
+      d_term_dy = d_term_dy * (R(4) * y_e * ebar2);
+    } else {
+      // FP32: the same algebra with gradients and psi divided by m = sqrt of the
+      // clamped |grad phi|^2 (every term is homogeneous in that scale), so no
+      // |grad phi|^8 is formed; with dx ~ 5e-6 cm the reference form exceeds
+      // the FP32 range.
+      R m = sqrt(mag_grad_phi2);
+      R im = R(1) / m;
+      R X = dphidx * im, Yg = dphidy * im, Px = psix * im, Py = psiy * im;
+      R Px2 = Px * Px, Py2 = Py * Py, Px3 = Px2 * Px, Py3 = Py2 * Py;
+      R P4 = Px2 * Px2 + Py2 * Py2;
+      R Ax = R(2) * Px3 * q2q2 + R(2) * Py3 * qq2;
+      R Ay = R(-2) * Px3 * qq2 + R(2) * Py3 * q2q2;
+      R tx = dTdx * m * (Ax - X * P4);
+      tx += Tc * (R(6) * Px2 * dpsixdx * q2q2 + R(2) * Px3 * m * dq2q2dx + R(6) * Py2 * dpsiydx * qq2 +
+                  R(2) * Py3 * m * dqq2dx);
+      tx -= Tc * Ax * (R(2) * X * d2phidx2 + R(2) * Yg * d2phidxy);
+      tx -= Tc * (d2phidx2 * P4 + X * (R(4) * Px3 * dpsixdx + R(4) * Py3 * dpsiydx));
+      tx += Tc * (X * P4 *
+                  (R(4) * X * X * X * d2phidx2 + R(4) * X * d2phidx2 * Yg * Yg + R(4) * Yg * d2phidxy * X * X +
+                   R(4) * Yg * Yg * Yg * d2phidxy));
+      R ty = dTdy * m * (Ay - Yg * P4);
+      ty += Tc * (R(-6) * Px2 * dpsixdy * qq2 - R(2) * Px3 * m * dqq2dy + R(6) * Py2 * dpsiydy * q2q2 +
+                  R(2) * Py3 * m * dq2q2dy);
+      ty -= Tc * Ay * (R(2) * X * d2phidxy + R(2) * Yg * d2phidy2);
+      ty -= Tc * (d2phidy2 * P4 + Yg * (R(4) * Px3 * dpsixdy + R(4) * Py3 * dpsiydy));
+      ty += Tc * (Yg * P4 *
+                  (R(4) * X * X * X * d2phidxy + R(4) * X * d2phidxy * Yg * Yg + R(4) * Yg * d2phidy2 * X * X +
+                   R(4) * Yg * Yg * Yg * d2phidy2));
+      d_term_dx = tx * (R(4) * y_e * ebar2);
+      d_term_dy = ty * (R(4) * y_e * ebar2);
+    }
     // c_N, cW, M_phi (:162-173)
     R cW = R(0), c_N = R(1), M_phi = R(0);
 #pragma unroll
