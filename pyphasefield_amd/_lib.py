@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libppf_hip.so")
+LIB_PATH = os.environ.get("PPF_LIB") or os.path.join(HERE, "libppf_hip.so")
 
 PPF_F32, PPF_F64 = 0, 1
 PPF_PERIODIC, PPF_NEUMANN, PPF_DIRICHLET = 0, 1, 2
