@@ -186,6 +186,69 @@ def bench_ch3d(args):
     return line
 
 
+# ---------------------------------------------------------- N-component --
+NC_DX = 0.0000046
+NC_T0 = 1000.0
+
+
+def ncomp_sim(n, precision, dg=None, seeds=16, noise=0.0, thermo_eval=None):
+    """N-component 2048^2: AL-CU-NI synthetic Redlich-Kister model (seed 0),
+    16 random-orientation seeds of radius 6, T = 1000 K + 1 K/cell along x,
+    Neumann, engine parameters of the Ni-Cu test database."""
+    import pyphasefield_amd.Engines as engines
+    from pyphasefield_amd import thermo
+    model, _ = thermo.synthetic_rk_model(seed=0)
+    sim = engines.NCGPU_new(dimensions=[n, n], seed=2024)
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(NC_DX)
+    sim.set_temperature_type("LINEAR_GRADIENT")
+    sim.set_temperature_initial_T(NC_T0)
+    sim.set_temperature_dTdx(1.0 / NC_DX)
+    sim.set_temperature_dTdy(0.0)
+    sim.set_temperature_dTdt(0.0)
+    sim.set_tdb_container(model)
+    sim.set_boundary_conditions("NEUMANN")
+    sim.set_precision(precision)
+    ud = {"d_ratio": 1.1, "sim_type": "seeds", "number_of_seeds": seeds, "seed_radius": 6,
+          "initial_concentration_array": [0.30, 0.30], "noise_phi": noise, "noise_c": noise, "noise_q": noise}
+    if dg:
+        ud["dg_mode"] = dg
+    if thermo_eval:
+        ud["thermo_eval"] = thermo_eval
+    sim.set_user_data(ud)
+    sim.initialize_fields_and_imported_data()
+    return sim
+
+
+def bench_ncomp(args):
+    import torch
+    n = args.size or 2048
+    variant = args.variant
+    precision = "FP64" if variant == "fp64" else "FP32"
+    dg = "forward_difference" if variant == "fp64" else "analytic"
+    sim = ncomp_sim(n, precision, dg, thermo_eval="surrogate" if variant == "surrogate" else None)
+    sim.simulate(args.warmup)
+    t = _timed(sim.simulate, args.steps)
+    cells = n * n
+    # per-kernel averages from separate timed loops on the same stream
+    k = max(10, args.steps // 10)
+    t_thermo = _timed(lambda m: [sim.thermo_pass() for _ in range(m)], k) / k
+    t_total = t / args.steps
+    esize = 8 if precision == "FP64" else 4
+    nf = len(sim.fields)
+    bytes_cell = 2 * nf * esize
+    kernels = sim.kernel_names()
+    roof = _roofline(bytes_cell * cells, t_total, "+".join(kernels))
+    roof["seconds_per_step_thermo"] = t_thermo
+    roof["seconds_per_step_stencil"] = t_total - t_thermo
+    line = _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t,
+                 "f64" if precision == "FP64" else "f32",
+                 {"workload": "N-component 2D %dx%d, 3 components (AL, CU, NI), %s" % (n, n, variant),
+                  "engine": "NCGPU_new", "grid": [n, n], "fields": nf, "bytes_per_cell_update": bytes_cell,
+                  "thermo": sim.user_data.get("thermo_eval", "plain") + "/" + dg}, roof, None)
+    return line
+
+
 # -------------------------------------------------------------- copy ref --
 def bench_copy(args):
     import torch
@@ -208,7 +271,8 @@ def bench_copy(args):
                          "frac": gbs / HBM_PEAK_GBS, "traffic": 2 * nbytes}, "cpu_baseline": None}
 
 
-CONFIGS = {"warren": bench_warren, "diffusion": bench_diffusion, "ch3d": bench_ch3d, "copy": bench_copy}
+CONFIGS = {"warren": bench_warren, "diffusion": bench_diffusion, "ch3d": bench_ch3d, "copy": bench_copy,
+           "ncomp": bench_ncomp}
 
 
 def main():
@@ -216,7 +280,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--config", default="warren", choices=sorted(CONFIGS) + ["ncomp", "ensemble"])
+    ap.add_argument("--config", default="warren", choices=sorted(CONFIGS) + ["ensemble"])
     ap.add_argument("--variant", default="fp32")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -230,10 +294,7 @@ def main():
     st, wu = defaults[args.config]
     args.steps = args.steps if args.steps is not None else st
     args.warmup = args.warmup if args.warmup is not None else wu
-    if args.config == "ncomp":
-        from pyphasefield_amd.bench_ncomp import bench_ncomp
-        line = bench_ncomp(args, _timed, _line, _roofline)
-    elif args.config == "ensemble":
+    if args.config == "ensemble":
         from pyphasefield_amd.ensemble import bench_ensemble
         line = bench_ensemble(args, _timed, _line, _roofline)
     else:

@@ -133,6 +133,11 @@ class NCGPU_new(Simulation):
                                                             self._torch_dtype(), dev)
         self._tgrid = _lib.make_grid(self._fields_transfer_gpu_device, 2, self._boundary_conditions_type, self.dx)
 
+    def kernel_names(self):
+        t = "float" if self._precision == "FP32" else "double"
+        return ["k_ncomp_thermo<%s,%d>" % (t, len(self._tdb_components)),
+                "k_ncomp_step<%s,%d>" % (t, len(self._tdb_components))]
+
     def thermo_pass(self):
         L = _lib.lib()
         rc = L.ppf_ncomp_thermo(self._grid, _lib.ptr(self._fields_gpu_device),

@@ -36,7 +36,11 @@ __device__ __forceinline__ void phase_at(const ppf_rk_phase& ph, R T, R lnT, Pha
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     R a = R(ph.g0[i][0]), b = R(ph.g0[i][1]), c = R(ph.g0[i][2]), d = R(ph.g0[i][3]), e = R(ph.g0[i][4]);
-    o.gi[i] = a + b * T + c * T * lnT + d * T * T + e / T;
+    if (ph.g0[i][2] == 0.0 && ph.g0[i][3] == 0.0 && ph.g0[i][4] == 0.0) {
+      o.gi[i] = a + b * T;  // uniform branch; same value as the full form
+    } else {
+      o.gi[i] = a + b * T + c * T * lnT + d * T * T + e / T;
+    }
   }
 #pragma unroll
   for (int p = 0; p < N * (N - 1) / 2; ++p) {
@@ -44,6 +48,13 @@ __device__ __forceinline__ void phase_at(const ppf_rk_phase& ph, R T, R lnT, Pha
     for (int k = 0; k < 3; ++k) o.L[p][k] = R(ph.L[p][k][0]) + R(ph.L[p][k][1]) * T;
   }
   o.RT = R(ph.R) * T;
+}
+
+// whether any coefficient multiplies T ln T (lnT is only needed then)
+__device__ __forceinline__ bool needs_lnT(const ppf_thermo& th, int N) {
+  for (int i = 0; i < N; ++i)
+    if (th.liquid.g0[i][2] != 0.0 || th.solid.g0[i][2] != 0.0) return true;
+  return false;
 }
 
 // sum_i c_i ln max(c_i, cmin), summed in component order
@@ -117,7 +128,7 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
     cN = cN - cin[l];
   }
   c[N - 1] = cN;
-  R lnT = rlog<R>(T);
+  R lnT = needs_lnT(k.th, N) ? rlog<R>(T) : R(0);
   PhaseAtT<R, N> PL, PS;
   phase_at<R, N>(k.th.liquid, T, lnT, PL);
   phase_at<R, N>(k.th.solid, T, lnT, PS);
@@ -159,25 +170,40 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
   }
 }
 
+// One thread per padded cell, 64 x 4 cells per block.  Blocks whose cells
+// are all interior read the fields directly; the others fold the boundary
+// rules in (the halo ring of the transfer arrays is computed from folded
+// values, as the reference helper does from its BC-filled halos).
 template <typename R, int N>
 __global__ __launch_bounds__(256) void k_ncomp_thermo(const R* __restrict__ in, const R* __restrict__ b, Dom d,
                                                       R dx, ppf_temperature t, R* __restrict__ tr, int64_t tsy,
                                                       int64_t tsf, NcK k, uint64_t seed, int64_t step) {
-  int64_t px = d.ax.n + 2, py = d.ay.n + 2;
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= px * py) return;
-  int64_t x = i % px, y = i / px;
-  R phi = fold(in, b, d, x, y, int64_t(0), dx);
-  R cin[N - 1];
+  const int64_t px = d.ax.n + 2, py = d.ay.n + 2;
+  const uint32_t nbx = uint32_t((px + 63) / 64);
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int64_t xb = int64_t(blk % nbx) * 64, yb = int64_t(blk / nbx) * 4;
+  const int64_t x = xb + (threadIdx.x & 63), y = yb + (threadIdx.x >> 6);
+  if (x >= px || y >= py) return;
+  const bool interior = xb >= 1 && xb + 63 <= d.ax.n && yb >= 1 && yb + 3 <= d.ay.n;
+  R phi, cin[N - 1];
+  if (interior) {
+    const int64_t o = y * d.ay.stride + x;
+    phi = in[o];
 #pragma unroll
-  for (int l = 0; l < N - 1; ++l)
-    cin[l] = fold(in + (3 + l) * d.sf, b ? b + (3 + l) * d.sf : nullptr, d, x, y, int64_t(0), dx);
+    for (int l = 0; l < N - 1; ++l) cin[l] = in[o + (3 + l) * d.sf];
+  } else {
+    phi = fold(in, b, d, x, y, int64_t(0), dx);
+#pragma unroll
+    for (int l = 0; l < N - 1; ++l)
+      cin[l] = fold(in + (3 + l) * d.sf, b ? b + (3 + l) * d.sf : nullptr, d, x, y, int64_t(0), dx);
+  }
   int64_t n3[3] = {d.ax.n, d.ay.n, 1};
   int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
   R T = temperature_at<R>(t, n3, bc3, x, y, 1, double(dx), step);
   R dG, Mc[N - 1], dF[N - 1];
-  thermo_cell<R, N>(cin, phi, T, k, step, seed, uint64_t(px * py), uint64_t(i), &dG, Mc, dF);
-  int64_t o = y * tsy + x;
+  const uint64_t cell = uint64_t(y) * uint64_t(px) + uint64_t(x);
+  thermo_cell<R, N>(cin, phi, T, k, step, seed, uint64_t(px * py), cell, &dG, Mc, dF);
+  const int64_t o = y * tsy + x;
   tr[o] = dG;
 #pragma unroll
   for (int l = 0; l < N - 1; ++l) {
@@ -192,8 +218,7 @@ int ncomp_thermo(const ppf_grid& g, const void* in, const void* bcvals, const pp
                  const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, int64_t step, hipStream_t s) {
   Dom d = make_dom(g);
   NcK k{p, th};
-  int64_t total = (g.n[0] + 2) * (g.n[1] + 2);
-  dim3 grid(unsigned((total + 255) / 256));
+  dim3 grid(unsigned(((g.n[0] + 2 + 63) / 64) * ((g.n[1] + 2 + 3) / 4)));
   const R* fin = static_cast<const R*>(in);
   const R* bc = static_cast<const R*>(bcvals);
   R* tr = static_cast<R*>(transfer);
@@ -233,6 +258,15 @@ __device__ __forceinline__ R f_ori_term(R D_q, R D_q_xp, R D_q_xm, R D_q_yp, R D
   return R(0.5) * idx * idx *
          ((D_q + D_q_xp) * (q_xp - q) / mgq_xp - (D_q + D_q_xm) * (q - q_xm) / mgq_xm +
           (D_q + D_q_yp) * (q_yp - q) / mgq_yp - (D_q + D_q_ym) * (q - q_ym) / mgq_ym);
+}
+
+template <typename R>
+__device__ __forceinline__ R fsqrt(R x) {
+  if constexpr (sizeof(R) == 4) {
+    return __builtin_amdgcn_sqrtf(x);
+  } else {
+    return sqrt(x);
+  }
 }
 
 template <typename R, int W>
@@ -324,10 +358,10 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
     R dq1dy = R(0.5) * idx * (Q1.yp - Q1.ym);
     R dq4dx = R(0.5) * idx * (Q4.xp - Q4.xm);
     R dq4dy = R(0.5) * idx * (Q4.yp - Q4.ym);
-    R mgq_xp = idx * sqrt((Q1.xp - Q1.c) * (Q1.xp - Q1.c) + (Q4.xp - Q4.c) * (Q4.xp - Q4.c));
-    R mgq_xm = idx * sqrt((Q1.xm - Q1.c) * (Q1.xm - Q1.c) + (Q4.xm - Q4.c) * (Q4.xm - Q4.c));
-    R mgq_yp = idx * sqrt((Q1.yp - Q1.c) * (Q1.yp - Q1.c) + (Q4.yp - Q4.c) * (Q4.yp - Q4.c));
-    R mgq_ym = idx * sqrt((Q1.ym - Q1.c) * (Q1.ym - Q1.c) + (Q4.ym - Q4.c) * (Q4.ym - Q4.c));
+    R mgq_xp = idx * fsqrt((Q1.xp - Q1.c) * (Q1.xp - Q1.c) + (Q4.xp - Q4.c) * (Q4.xp - Q4.c));
+    R mgq_xm = idx * fsqrt((Q1.xm - Q1.c) * (Q1.xm - Q1.c) + (Q4.xm - Q4.c) * (Q4.xm - Q4.c));
+    R mgq_yp = idx * fsqrt((Q1.yp - Q1.c) * (Q1.yp - Q1.c) + (Q4.yp - Q4.c) * (Q4.yp - Q4.c));
+    R mgq_ym = idx * fsqrt((Q1.ym - Q1.c) * (Q1.ym - Q1.c) + (Q4.ym - Q4.c) * (Q4.ym - Q4.c));
     R mag_grad_q = R(0.5) * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
     if (mgq_xp < beta) mgq_xp = beta;
     if (mgq_xm < beta) mgq_xm = beta;
@@ -394,7 +428,7 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
       // clamped |grad phi|^2 (every term is homogeneous in that scale), so no
       // |grad phi|^8 is formed; with dx ~ 5e-6 cm the reference form exceeds
       // the FP32 range.
-      R m = sqrt(mag_grad_phi2);
+      R m = fsqrt(mag_grad_phi2);
       R im = R(1) / m;
       R X = dphidx * im, Yg = dphidy * im, Px = psix * im, Py = psiy * im;
       R Px2 = Px * Px, Py2 = Py * Py, Px3 = Px2 * Px, Py3 = Py2 * Py;
@@ -486,7 +520,7 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
     if (phin > R(0.9999)) phin = R(0.9999);
     R q1n = q1c + dt * dq1dt;
     R q4n = q4c + dt * dq4dt;
-    R renorm = sqrt(q1n * q1n + q4n * q4n);
+    R renorm = fsqrt(q1n * q1n + q4n * q4n);
     out[o] = phin;
     out[o + d.sf] = q1n / renorm;
     out[o + 2 * d.sf] = q4n / renorm;
