@@ -183,6 +183,54 @@ int ppf_reduce(const ppf_grid* g, const void* in, const void* bcvals, int kind, 
 int ppf_thermo_eval(const ppf_thermo* th, int64_t n, const double* T, const double* c,
                     double* out, ppf_stream_t s);
 
+/* ---- surrogate thermodynamics (BF16 MFMA, FP32 accumulate) -------------
+   No counterpart in the reference: a learned replacement for the per-cell
+   Gibbs-energy evaluation of NComponent_helper_kernel (NCGPU_new.py:295-305),
+   trained by the kernels below on samples of the plain evaluation. */
+
+/* C[M x N] = alpha A B + beta C with A(i,k) = A[i*sai + k*sak],
+   B(k,j) = B[k*sbk + j*sbj], C row-major (ldc); FP32 in memory, BF16 MFMA
+   products with FP32 accumulation.  A is rounded to BF16; split_b = 1 feeds B
+   as the BF16 pair hi + lo (hi = bf16(b), lo = bf16(b - hi)).  ksplit > 1
+   splits K across blocks with atomic accumulation (beta must be 0 and C
+   zeroed by the caller). */
+int ppf_mlp_gemm(int M, int N, int K, const float* A, int64_t sai, int64_t sak, const float* B,
+                 int64_t sbk, int64_t sbj, float* C, int64_t ldc, float alpha, float beta, int ksplit,
+                 int split_b, ppf_stream_t s);
+/* H[i,j] = act(Z[i,j] + b[i]) on [M x N] row-major; act 0 = identity, 1 = ReLU; b may be NULL */
+int ppf_mlp_bias_act(const float* Z, const float* b, float* H, int M, int N, int act, ppf_stream_t s);
+/* dZ = dH * (H > 0) */
+int ppf_mlp_relu_bwd(const float* dH, const float* H, float* dZ, int64_t n, ppf_stream_t s);
+/* out[i] = sum_j X[i,j] on [M x N] row-major */
+int ppf_mlp_rowsum(const float* X, float* out, int M, int N, ppf_stream_t s);
+/* number of per-block partial sums ppf_mlp_mse writes */
+int64_t ppf_mlp_mse_partials(int64_t n);
+/* dY = 2 (Y - T) / n; partial[b] = sum of squared errors of block b */
+int ppf_mlp_mse(const float* Y, const float* T, float* dY, float* partial, int64_t n, ppf_stream_t s);
+/* Adam update of n parameters at step t (>= 1) */
+int ppf_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                 float eps, int64_t t, ppf_stream_t s);
+
+/* Two trained 3-64-64-3 ReLU networks (liquid, solid), inputs (T, c1, c2),
+   BF16 weights, activations as BF16 hi + lo pairs, FP32 accumulation.
+   Outputs (normalised): liquid (G_ni, dG_ni/dc1, dG_ni/dc2); solid
+   (G_S,ni - G_L,ni, dG_S,ni/dc1, dG_S,ni/dc2), where "ni" is G minus the ideal
+   mixing term R T sum c ln c (evaluated analytically in the kernel). */
+typedef struct ppf_mlp_net {
+  const void* weights[2];  /* per phase: packed BF16 MFMA operands (see pyphasefield_amd.surrogate) */
+  const float* bias[2];    /* per phase: b1[64], b2[64], b3[4] (FP32) */
+  float in_lo[3], in_scale[3];          /* x_hat = (x - lo) * scale - 1 */
+  float out_mu[2][3], out_sigma[2][3];  /* y = y_hat * sigma + mu */
+  float T_lo, T_hi, c_lo;  /* range gate: cells outside use the plain FP64 evaluation */
+  int32_t pad_;
+} ppf_mlp_net;
+
+/* N-component thermodynamic pass with the surrogate (FP32, N = 3): same
+   outputs as ppf_ncomp_thermo. */
+int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                         const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                         const ppf_mlp_net* net, uint64_t seed, int64_t step, ppf_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

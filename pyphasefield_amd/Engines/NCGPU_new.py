@@ -132,14 +132,53 @@ class NCGPU_new(Simulation):
         self._fields_transfer_gpu_device = _lib.alloc_stack(2 * ncomp - 1, list(self.dimensions),
                                                             self._torch_dtype(), dev)
         self._tgrid = _lib.make_grid(self._fields_transfer_gpu_device, 2, self._boundary_conditions_type, self.dx)
+        self._surrogate = None
+        if u.get("thermo_eval", "plain") == "surrogate":
+            self._setup_surrogate()
+
+    def _temperature_range(self):
+        t = self.temperature_descriptor()
+        if t.kind == _lib.PPF_T_CONST:
+            return t.T0, t.T0
+        T = self.temperature._data[1:-1, 1:-1]
+        return float(T.min()), float(T.max())
+
+    def _setup_surrogate(self):
+        """Use user_data["surrogate"] (a trained SurrogateThermo) or train one
+        on 1e6 samples of the plain evaluation over this run's (T, c) box."""
+        if self._precision != "FP32":
+            raise ValueError("the surrogate thermodynamics run in FP32")
+        if len(self._tdb_components) != 3:
+            raise ValueError("the surrogate kernel is built for 3 components")
+        from ..surrogate import SurrogateThermo
+        sur = self.user_data.get("surrogate")
+        if sur is None:
+            lo, hi = self._temperature_range()
+            pad = max(5.0, 0.02 * (hi - lo))
+            sur = SurrogateThermo(self._tdb, (lo - pad, hi + pad), c_lo=0.02, seed=self.seed)
+            sur.train(n_samples=int(self.user_data.get("surrogate_samples", 1_000_000)),
+                      epochs=int(self.user_data.get("surrogate_epochs", 30)))
+            self.user_data["surrogate"] = sur
+        self._surrogate = sur
 
     def kernel_names(self):
         t = "float" if self._precision == "FP32" else "double"
+        if getattr(self, "_surrogate", None) is not None:
+            return ["k_ncomp_thermo_mlp", "k_ncomp_step<float,3>"]
         return ["k_ncomp_thermo<%s,%d>" % (t, len(self._tdb_components)),
                 "k_ncomp_step<%s,%d>" % (t, len(self._tdb_components))]
 
     def thermo_pass(self):
         L = _lib.lib()
+        if self._surrogate is not None:
+            net, _keep = self._surrogate.packed()
+            rc = L.ppf_ncomp_thermo_mlp(self._grid, _lib.ptr(self._fields_gpu_device),
+                                        _lib.ptr(self._boundary_conditions_gpu_device),
+                                        self.temperature_descriptor(), self._tgrid,
+                                        _lib.ptr(self._fields_transfer_gpu_device), self._ncparams, self._thermo,
+                                        net, self.seed, int(self.time_step_counter), _lib.stream_handle())
+            _lib.check(rc, "ncomp_thermo_mlp")
+            return
         rc = L.ppf_ncomp_thermo(self._grid, _lib.ptr(self._fields_gpu_device),
                                 _lib.ptr(self._boundary_conditions_gpu_device), self.temperature_descriptor(),
                                 self._tgrid, _lib.ptr(self._fields_transfer_gpu_device), self._ncparams,

@@ -1,0 +1,200 @@
+"""Surrogate thermodynamics on the GPU: MFMA GEMM, training gradients,
+packed inference kernel, accuracy, and the N-component engine driven by it.
+
+Checkers: numpy FP64 (and BF16-emulating) forward/backward of the same
+network; the plain evaluation (ppf_thermo_eval / oracle.thermo); the FP64
+oracle of the N-component engine."""
+
+import numpy as np
+import pytest
+
+from tests.oracle_runner import padded_T, run_ncomp
+from tests.test_gpu_parity import interface_error_cells
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(a):
+    from pyphasefield_amd.surrogate import _f2bf_bits
+    return (_f2bf_bits(np.asarray(a, np.float32)).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N,K,ks", [(64, 96, 3, 1), (3, 64, 1000, 4), (64, 64, 64, 1), (37, 50, 17, 1),
+                                      (64, 3, 4096, 8)])
+def test_mfma_gemm_matches_bf16_numpy(gpu, M, N, K, ks):
+    import torch
+    from pyphasefield_amd import _lib
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    Ad, Bd = torch.from_numpy(A).to(gpu), torch.from_numpy(B).to(gpu)
+    C = torch.zeros(M, N, device=gpu)
+    _lib.check(_lib.lib().ppf_mlp_gemm(M, N, K, _lib.ptr(Ad), K, 1, _lib.ptr(Bd), N, 1, _lib.ptr(C), N, 1.0, 0.0, ks,
+                                       0, _lib.stream_handle()), "gemm")
+    want = _bf(A) @ _bf(B)
+    got = C.cpu().numpy()
+    assert np.max(np.abs(got - want)) <= 1e-4 * max(1.0, np.abs(want).max()) * np.sqrt(K)
+    # transposed operands through strides: C2 = A^T(as K x M)... use B^T A^T = (AB)^T
+    C2 = torch.zeros(N, M, device=gpu)
+    _lib.check(_lib.lib().ppf_mlp_gemm(N, M, K, _lib.ptr(Bd), 1, N, _lib.ptr(Ad), 1, K, _lib.ptr(C2), M, 1.0, 0.0, 1,
+                                       0, _lib.stream_handle()), "gemm")
+    assert np.max(np.abs(C2.cpu().numpy() - want.T)) <= 1e-4 * max(1.0, np.abs(want).max()) * np.sqrt(K)
+    # split B: BF16 A times (hi + lo) B
+    from pyphasefield_amd.surrogate import _split
+    C3 = torch.zeros(M, N, device=gpu)
+    _lib.check(_lib.lib().ppf_mlp_gemm(M, N, K, _lib.ptr(Ad), K, 1, _lib.ptr(Bd), N, 1, _lib.ptr(C3), N, 1.0, 0.0, ks,
+                                       1, _lib.stream_handle()), "gemm")
+    want3 = _bf(A) @ _split(B)
+    assert np.max(np.abs(C3.cpu().numpy() - want3)) <= 1e-4 * max(1.0, np.abs(want3).max()) * np.sqrt(K)
+
+
+def _small_surrogate(gpu, seed=0):
+    from pyphasefield_amd import thermo
+    from pyphasefield_amd.surrogate import SurrogateThermo
+    model, _ = thermo.synthetic_rk_model(seed=0)
+    return SurrogateThermo(model, (1000.0, 3047.0), c_lo=0.02, seed=seed)
+
+
+def test_training_gradients_match_numpy(gpu):
+    import torch
+    sur = _small_surrogate(gpu)
+    net = sur.nets[0]
+    rng = np.random.default_rng(1)
+    B = 512
+    X = rng.uniform(-1, 1, (3, B)).astype(np.float32)
+    Yt = rng.standard_normal((3, B)).astype(np.float32)
+    keep = {}
+    Xd = torch.from_numpy(X).to(gpu)
+    Y = sur.forward(net, Xd, keep)
+    part, dY = sur.loss_and_grad(Y, torch.from_numpy(Yt).to(gpu))
+    sur.backward(net, dY, keep)
+    params = net.numpy()
+    Yb, lb, gb = __import__("pyphasefield_amd.surrogate", fromlist=["x"]).numpy_forward_backward(
+        params, X.astype(np.float64), Yt.astype(np.float64), bf16=True)
+    Yf, lf, gf = __import__("pyphasefield_amd.surrogate", fromlist=["x"]).numpy_forward_backward(
+        params, X.astype(np.float64), Yt.astype(np.float64), bf16=False)
+    np.testing.assert_allclose(Y.cpu().numpy(), Yb, rtol=0, atol=2e-4 * np.abs(Yb).max())
+    assert abs(float(part.sum()) / Y.numel() - lb) <= 1e-4 * lb
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        g = net.gview[k].cpu().numpy()
+        scale = np.abs(gb[k]).max()
+        # against the BF16-emulating reference: FP32 accumulation-order level
+        assert np.max(np.abs(g - gb[k])) <= 2e-3 * scale, k
+        # against exact FP64 arithmetic: BF16 operand rounding level
+        assert np.max(np.abs(g - gf[k])) <= 3e-2 * np.abs(gf[k]).max(), k
+
+
+@pytest.fixture(scope="module")
+def trained(gpu):
+    sur = _small_surrogate(gpu, seed=3)
+    sur.train(n_samples=1_000_000, epochs=12)
+    return sur
+
+
+def test_training_converges_and_beats_untrained(gpu, trained):
+    first = [h[2] for h in trained.history if h[1] == 0]
+    last = [h[2] for h in trained.history if h[1] == max(x[1] for x in trained.history)]
+    assert all(l < 1e-3 for l in last), trained.history
+    assert all(l < f * 1e-2 for f, l in zip(first, last))
+    # accuracy against the plain evaluation on fresh samples
+    T, c = trained.sample_inputs(20000)
+    yL, yS = trained.plain_targets(T, c)
+    pL, pS = trained.predict(T, c)
+    untrained = _small_surrogate(gpu, seed=4)
+    untrained.out_mu, untrained.out_sigma = trained.out_mu, trained.out_sigma
+    uL, uS = untrained.predict(T, c)
+    for want, got, bad in ((yL, pL, uL), (yS, pS, uS)):
+        err = np.abs(got - want).max(axis=0)
+        err_bad = np.abs(bad - want).max(axis=0)
+        assert np.all(err < 0.05 * err_bad)
+    # driving force and chemical-potential errors in J/mol
+    assert np.abs(pS[:, 0] - yS[:, 0]).max() < 60.0
+    assert np.abs(pL[:, 1:] - yL[:, 1:]).max() < 200.0
+
+
+def test_packed_inference_kernel_matches_training_forward(gpu, trained):
+    """The fused MFMA kernel (transposed lane layout) reproduces the GEMM
+    forward of the same networks, through the engine's transfer arrays."""
+    import pyphasefield_amd.Engines as engines
+    from pyphasefield_amd import thermo
+    model = trained.model
+    sim = engines.NCGPU_new(dimensions=[40, 70], seed=5)
+    sim.set_dx(0.0000046)
+    sim.set_temperature_type("LINEAR_GRADIENT")
+    sim.set_temperature_initial_T(1200.)
+    sim.set_temperature_dTdx(10.0 / 0.0000046)
+    sim.set_temperature_dTdy(0.0)
+    sim.set_temperature_dTdt(0.0)
+    sim.set_tdb_container(model)
+    sim.set_boundary_conditions("NEUMANN")
+    sim.set_precision("FP32")
+    sim.set_user_data({"d_ratio": 1.1, "sim_type": "seeds", "number_of_seeds": 4, "seed_radius": 5,
+                       "initial_concentration_array": [0.3, 0.3], "noise_phi": 0., "noise_c": 0., "noise_q": 0.,
+                       "thermo_eval": "surrogate", "surrogate": trained})
+    sim.initialize_fields_and_imported_data()
+    rng = np.random.default_rng(6)
+    sim.fields[3].data[1:-1, 1:-1] = 0.1 + 0.4 * rng.random((40, 70))
+    sim.fields[4].data[1:-1, 1:-1] = 0.1 + 0.4 * rng.random((40, 70))
+    sim.simulate(0)
+    sim.just_before_simulating()
+    sim.thermo_pass()
+    tr = sim._fields_transfer_gpu_device.cpu().numpy().astype(np.float64)
+    Tpad = sim.temperature._data
+    c1 = sim.fields[3].data
+    c2 = sim.fields[4].data
+    inner = (slice(1, -1), slice(1, -1))
+    T = Tpad[inner].ravel()
+    cc = np.stack([c1[inner].ravel(), c2[inner].ravel()], axis=1)
+    pL, pS = trained.predict(T, cc)
+    assert np.max(np.abs(tr[0][inner].ravel() - pS[:, 0])) <= 1e-3 * np.abs(pS[:, 0]).max() + 0.5
+    # mu_l (phi = 0 in the liquid cells: h = g = 0) = (dG_L,l)/v_m with the ideal term added
+    phi = sim.fields[0].data[inner].ravel()
+    liquid = phi < 1e-3
+    R = model.liquid.R
+    cs = [cc[:, 0], cc[:, 1], 1 - cc.sum(1)]
+    v_m = sim.user_data["v_m"]
+    for l in range(2):
+        dGL = pL[:, 1 + l] + R * T * (np.log(cs[l]) - np.log(cs[2]))
+        W = sim.user_data["W"]
+        mu = dGL / v_m
+        got = tr[3 + l][inner].ravel()
+        assert np.max(np.abs(got[liquid] - mu[liquid])) <= 1e-3 * np.abs(mu).max()
+
+
+def test_ncomp_surrogate_vs_fp64_oracle(gpu, trained):
+    """Surrogate engine (FP32 stencil + BF16 networks) against the FP64
+    forward-difference oracle: composition L-inf <= 1e-3, interface within 1 cell."""
+    import pyphasefield_amd.Engines as engines
+    from oracle import engines as oe
+    model = trained.model
+    n = 48
+    sim = engines.NCGPU_new(dimensions=[n, n], seed=9)
+    sim.set_dx(0.0000046)
+    sim.set_temperature_type("LINEAR_GRADIENT")
+    sim.set_temperature_initial_T(1500.)
+    sim.set_temperature_dTdx(1.0 / 0.0000046)
+    sim.set_temperature_dTdy(0.0)
+    sim.set_temperature_dTdt(0.0)
+    sim.set_tdb_container(model)
+    sim.set_boundary_conditions("NEUMANN")
+    sim.set_precision("FP32")
+    sim.set_user_data({"d_ratio": 1.1, "sim_type": "seeds", "number_of_seeds": 3, "seed_radius": 5,
+                       "initial_concentration_array": [0.3, 0.3], "noise_phi": 0., "noise_c": 0., "noise_q": 0.,
+                       "thermo_eval": "surrogate", "surrogate": trained})
+    sim.initialize_fields_and_imported_data()
+    sim.just_before_simulating()
+    sim._begun_simulation = True
+    u = sim.user_data
+    f0 = np.stack([f.get_cells().copy() for f in sim.fields])
+    steps = 200
+    sim.simulate(steps)
+
+    def T_of(s):
+        return padded_T(oe.temperature_linear((n, n), 1500., 1.0 / 0.0000046, 0.0, sim.dx, 0.0, sim.dt, s - 1),
+                        "NEUMANN")
+    liq = (model.liquid.g0, model.liquid.L, model.liquid.R)
+    sol = (model.solid.g0, model.solid.L, model.solid.R)
+    want = run_ncomp(f0, steps, "NEUMANN", T_of, u["params"], u["c_params"], liq, sol, seed=sim.seed)
+    for i in (3, 4):
+        assert np.max(np.abs(sim.fields[i].get_cells() - want[i][1:-1, 1:-1])) <= 1e-3
+    assert interface_error_cells(sim.fields[0].get_cells(), want[0][1:-1, 1:-1]) <= 1
