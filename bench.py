@@ -191,7 +191,7 @@ NC_DX = 0.0000046
 NC_T0 = 1000.0
 
 
-def ncomp_sim(n, precision, dg=None, seeds=16, noise=0.0, thermo_eval=None):
+def ncomp_sim(n, precision, dg=None, seeds=16, noise=0.0, thermo_eval=None, surrogate=None):
     """N-component 2048^2: AL-CU-NI synthetic Redlich-Kister model (seed 0),
     16 random-orientation seeds of radius 6, T = 1000 K + 1 K/cell along x,
     Neumann, engine parameters of the Ni-Cu test database."""
@@ -215,6 +215,8 @@ def ncomp_sim(n, precision, dg=None, seeds=16, noise=0.0, thermo_eval=None):
         ud["dg_mode"] = dg
     if thermo_eval:
         ud["thermo_eval"] = thermo_eval
+    if surrogate is not None:
+        ud["surrogate"] = surrogate
     sim.set_user_data(ud)
     sim.initialize_fields_and_imported_data()
     return sim
@@ -226,8 +228,16 @@ def bench_ncomp(args):
     variant = args.variant
     precision = "FP64" if variant == "fp64" else "FP32"
     dg = "forward_difference" if variant == "fp64" else "analytic"
-    sim = ncomp_sim(n, precision, dg, thermo_eval="surrogate" if variant == "surrogate" else None)
+    sim = ncomp_sim(n, precision, dg, thermo_eval="surrogate" if variant == "surrogate" else None,
+                    surrogate=args.surrogate if variant == "surrogate" and args.surrogate and
+                    os.path.exists(args.surrogate) else None)
+    if variant == "surrogate":
+        sim._setup_surrogate()
+        sim._surrogate.split3 = not args.no_split3
+        sim._surrogate._packed = None
     sim.simulate(args.warmup)
+    if variant == "surrogate" and args.surrogate and not os.path.exists(args.surrogate):
+        sim.user_data["surrogate"].save(args.surrogate)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     # per-kernel averages from separate timed loops on the same stream
@@ -284,6 +294,10 @@ def main():
     ap.add_argument("--variant", default="fp32")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-split3", action="store_true",
+                    help="ncomp surrogate: feed the last layer BF16 activations only")
+    ap.add_argument("--surrogate", default=None,
+                    help="ncomp surrogate: load the trained networks from this .npz, or train and save there")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this engine path has no multi-GPU mode "

@@ -222,8 +222,9 @@ typedef struct ppf_mlp_net {
   float in_lo[3], in_scale[3];          /* x_hat = (x - lo) * scale - 1 */
   float out_mu[2][3], out_sigma[2][3];  /* y = y_hat * sigma + mu */
   float T_lo, T_hi, c_lo;  /* range gate: cells outside use the plain FP64 evaluation */
-  int32_t pad_;
+  int32_t flags;           /* PPF_MLP_SPLIT3: layer-3 inputs as BF16 hi + lo (else BF16 only) */
 } ppf_mlp_net;
+#define PPF_MLP_SPLIT3 1
 
 /* N-component thermodynamic pass with the surrogate (FP32, N = 3): same
    outputs as ppf_ncomp_thermo. */

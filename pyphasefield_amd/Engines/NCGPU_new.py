@@ -144,20 +144,24 @@ class NCGPU_new(Simulation):
         return float(T.min()), float(T.max())
 
     def _setup_surrogate(self):
-        """Use user_data["surrogate"] (a trained SurrogateThermo) or train one
-        on 1e6 samples of the plain evaluation over this run's (T, c) box."""
+        """Use user_data["surrogate"] (a trained SurrogateThermo, or the path of
+        one saved with SurrogateThermo.save) or train one on 1e6 samples of the
+        plain evaluation over this run's (T, c) box."""
         if self._precision != "FP32":
             raise ValueError("the surrogate thermodynamics run in FP32")
         if len(self._tdb_components) != 3:
             raise ValueError("the surrogate kernel is built for 3 components")
         from ..surrogate import SurrogateThermo
         sur = self.user_data.get("surrogate")
+        if isinstance(sur, str):
+            sur = SurrogateThermo.load(sur, self._tdb)
+            self.user_data["surrogate"] = sur
         if sur is None:
             lo, hi = self._temperature_range()
             pad = max(5.0, 0.02 * (hi - lo))
             sur = SurrogateThermo(self._tdb, (lo - pad, hi + pad), c_lo=0.02, seed=self.seed)
             sur.train(n_samples=int(self.user_data.get("surrogate_samples", 1_000_000)),
-                      epochs=int(self.user_data.get("surrogate_epochs", 30)))
+                      epochs=int(self.user_data.get("surrogate_epochs", 20)))
             self.user_data["surrogate"] = sur
         self._surrogate = sur
 

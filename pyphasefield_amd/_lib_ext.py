@@ -7,10 +7,13 @@ from . import _lib as L
 c_i32, c_i64, c_u64, c_f32, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float, ctypes.c_void_p
 
 
+PPF_MLP_SPLIT3 = 1
+
+
 class MlpNet(ctypes.Structure):
     _fields_ = [("weights", c_vp * 2), ("bias", c_vp * 2), ("in_lo", c_f32 * 3), ("in_scale", c_f32 * 3),
                 ("out_mu", (c_f32 * 3) * 2), ("out_sigma", (c_f32 * 3) * 2), ("T_lo", c_f32), ("T_hi", c_f32),
-                ("c_lo", c_f32), ("pad_", c_i32)]
+                ("c_lo", c_f32), ("flags", c_i32)]
 
 
 P = ctypes.POINTER

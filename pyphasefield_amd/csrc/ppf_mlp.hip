@@ -13,6 +13,8 @@
 // 8s + 4(l/32) + t, exactly items 4j..4j+3 of the h-th 32-row D tile.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ppf_ncomp.inl"
 
 namespace ppf {
@@ -145,32 +147,47 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float
 }
 
 // ----------------------------------------------- N-component inference --
-// Packed weights of one phase network (host prepares the lane layout):
-//   w1[2][64] bf16x4   A operand of layer 1 (rows = hidden half h; k = 0..7 hold
-//                      the weights of T, c1, c2, T, c1, c2, -, - matching the
-//                      hi/lo input pairs)
-//   w2[2][8][64]       A operand of layer 2 (output half m, k-block s)
-//   w3[8][64]          A operand of layer 3 (rows = outputs 0..2, zero padded)
-// followed by FP32 b1[64], b2[64], b3[4] (b3[3] unused).
-constexpr int kMlpW1 = 2 * 64, kMlpW2 = 16 * 64, kMlpW3 = 8 * 64;
-constexpr int kMlpWords = kMlpW1 + kMlpW2 + kMlpW3;  // bf16x4 words per phase
+// Inference uses v_mfma_f32_32x32x16_bf16: lane l, element t (0..7) of the A
+// and B operands pair up along K in the same (l / 32, t) slot, so any slot ->
+// k assignment shared by A and B is valid.  The slot of hidden row k used here
+// lets the D tile of one layer feed the next layer without data movement:
+// D item v of lane l holds row 32h + 8(v/4) + 4(l/32) + v%4 of hidden half h,
+// so items 8p..8p+7 form the B operand of K block kb = 2h + p with
+//   slot (l/32, t) -> row 32h + 16p + 8(t/4) + 4(l/32) + t%4.
+// Packed weights of one phase network (bf16x8 words, 64 lanes each; host
+// packing in pyphasefield_amd.surrogate.pack_phase):
+//   set 0..1    layer 1, hidden half h: slot (0, t) = input t of
+//               (T_hi, c1_hi, c2_hi, T_lo, c1_lo, c2_lo, 0, 0), lanes >= 32 zero
+//   set 2..9    layer 2, output half m = (set-2)/4, K block kb = (set-2)%4
+//   set 10..13  layer 3, K block kb, output rows 0..2 (rows 3..31 zero)
+// followed by FP32 b1[64], b2[64], b3[4] (b3[3] unused).  Activations enter
+// each layer as BF16 pairs hi + lo (two MFMAs with the same weights), so the
+// only BF16 rounding is that of the weights, which training sees.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kMlpSets = 14;
+constexpr int kMlpWords = kMlpSets * 64;  // bf16x8 words per phase
+
+__device__ __forceinline__ f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
 
 struct MlpK {
-  const bf16x4* w[2];   // liquid, solid packed weights (device)
+  const bf16x8* w[2];   // liquid, solid packed weights (device)
   const float* bias[2]; // b1[64], b2[64], b3[4] per phase (device)
   float in_lo[3], in_scale[3];        // x_hat = (x - lo) * scale - 1 over (T, c1, c2)
   float out_mu[2][3], out_sigma[2][3];
   float T_lo, T_hi, c_lo;             // range gate: outside, the plain FP64 evaluation is used
+  float R;                            // gas constant of the ideal-mixing term
 };
 
-// D tile of a 32-row layer as B operands of the next layer: ReLU, then the
-// BF16 pair hi + lo of each activation
-__device__ __forceinline__ void relu_pack(const f32x16& d, int j, bf16x4& hi, bf16x4& lo) {
+// ReLU of items 8p..8p+7 of a D tile as the BF16 pair (hi, lo)
+__device__ __forceinline__ void relu_split(const f32x16& d, int p, bf16x8& hi, bf16x8& lo) {
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const float v = fmaxf(d[4 * j + t], 0.f);
-    hi[t] = f2bf(v);
-    lo[t] = f2bf(v - bf2f(hi[t]));
+  for (int t = 0; t < 8; ++t) {
+    const float v = fmaxf(d[8 * p + t], 0.f);
+    const __bf16 h = static_cast<__bf16>(v);
+    hi[t] = h;
+    lo[t] = static_cast<__bf16>(v - static_cast<float>(h));
   }
 }
 
@@ -185,212 +202,260 @@ __device__ __forceinline__ f32x16 bias_tile(const float* b, int row0, int u) {
   return acc;
 }
 
-// Evaluates one phase network for the 32 cells whose inputs the lanes hold
-// (lanes l and l+32 hold cell l%32; lanes >= 32 pass zeros for k = 4..7).
-// Returns, in lanes 0..31, items 0..2 = normalised outputs of cell l.
-__device__ __forceinline__ f32x16 mlp_phase(const bf16x4* __restrict__ sw, const float* __restrict__ sb,
-                                            bf16x4 x, int lane) {
+// One phase network on G groups of 32 cells; the weight operands read from
+// LDS are shared by the groups.  x[g]: layer-1 B operand of group g.
+// out[g][q] = normalised output q of cell (lane % 32) of group g, valid in
+// lanes 0..31.  split3 = false feeds the layer-3 inputs as BF16 only.
+template <int G>
+__device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const float* __restrict__ b,
+                                         const bf16x8 (&x)[G], int lane, bool split3, float (&out)[G][3]) {
   const int u = lane >> 5;
-  f32x16 h1[2];
+  f32x16 h[G][2];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) h1[h] = mfma32(sw[h * 64 + lane], x, bias_tile(sb, 32 * h, u));
-  f32x16 h2[2];
+  for (int hh = 0; hh < 2; ++hh) {
+    const f32x16 bt = bias_tile(b, 32 * hh, u);
+    const bf16x8 a = w[hh * 64 + lane];
+#pragma unroll
+    for (int g = 0; g < G; ++g) h[g][hh] = mfma16(a, x[g], bt);
+  }
+  bf16x8 bh[G][4], bl[G][4];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) relu_split(h[g][kb >> 1], kb & 1, bh[g][kb], bl[g][kb]);
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
-    f32x16 acc = bias_tile(sb + 64, 32 * m, u);
+    const f32x16 bt = bias_tile(b + 64, 32 * m, u);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      bf16x4 hi, lo;
-      relu_pack(h1[s >> 2], s & 3, hi, lo);
-      const bf16x4 w = sw[kMlpW1 + (m * 8 + s) * 64 + lane];
-      acc = mfma32(w, hi, acc);
-      acc = mfma32(w, lo, acc);
+    for (int g = 0; g < G; ++g) h[g][m] = bt;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      const bf16x8 a = w[(2 + 4 * m + kb) * 64 + lane];
+#pragma unroll
+      for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bh[g][kb], h[g][m]);
+#pragma unroll
+      for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bl[g][kb], h[g][m]);
     }
-    h2[m] = acc;
   }
-  f32x16 y;
+  f32x16 y[G];
 #pragma unroll
-  for (int v = 0; v < 16; ++v) y[v] = 0.f;
-  if (u == 0) {
-    y[0] = sb[128];
-    y[1] = sb[129];
-    y[2] = sb[130];
+  for (int g = 0; g < G; ++g) {
+#pragma unroll
+    for (int v = 0; v < 16; ++v) y[g][v] = 0.f;
+    if (u == 0) {
+      y[g][0] = b[128];
+      y[g][1] = b[129];
+      y[g][2] = b[130];
+    }
+  }
+  if (split3) {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) relu_split(h[g][kb >> 1], kb & 1, bh[g][kb], bl[g][kb]);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      const bf16x8 a = w[(10 + kb) * 64 + lane];
+#pragma unroll
+      for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], y[g]);
+#pragma unroll
+      for (int g = 0; g < G; ++g) y[g] = mfma16(a, bl[g][kb], y[g]);
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) bh[g][kb][t] = static_cast<__bf16>(fmaxf(h[g][kb >> 1][8 * (kb & 1) + t], 0.f));
+      }
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      const bf16x8 a = w[(10 + kb) * 64 + lane];
+#pragma unroll
+      for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], y[g]);
+    }
   }
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    bf16x4 hi, lo;
-    relu_pack(h2[s >> 2], s & 3, hi, lo);
-    const bf16x4 w = sw[kMlpW1 + kMlpW2 + s * 64 + lane];
-    y = mfma32(w, hi, y);
-    y = mfma32(w, lo, y);
-  }
-  return y;
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) out[g][q] = y[g][q];
 }
 
-// plain FP64 evaluation for cells outside the trained range (same outputs)
-template <int N>
-__device__ void plain_outputs(const float (&cin)[N - 1], float T, const ppf_thermo& th, double* yL, double* yS) {
-  double c[N];
-  double cN = 1.0;
-#pragma unroll
-  for (int l = 0; l < N - 1; ++l) {
-    c[l] = double(cin[l]);
-    cN -= c[l];
-  }
-  c[N - 1] = cN;
+struct Plain5 {
+  float dG, dGL[2], dGS[2];
+};
+
+// plain FP64 evaluation for cells outside the trained range: G_S - G_L and
+// both phases' derivatives (kept out of line: rare, and heavy in registers)
+__device__ __noinline__ Plain5 plain_outputs(float c1, float c2, float T, const ppf_thermo* th) {
+  constexpr int N = 3;
+  double c[N] = {double(c1), double(c2), 1.0 - double(c1) - double(c2)};
   double t = double(T);
   double lnT = plog(t);
   PhaseAtT<double, N> PL, PS;
-  phase_at<double, N>(th.liquid, t, lnT, PL);
-  phase_at<double, N>(th.solid, t, lnT, PS);
+  phase_at<double, N>(th->liquid, t, lnT, PL);
+  phase_at<double, N>(th->solid, t, lnT, PS);
   double id = ideal_sum<double, N>(c);
   double gl = gibbs<double, N>(c, PL, id), gs = gibbs<double, N>(c, PS, id);
   double dl[N], ds[N];
   gibbs_grad<double, N>(c, PL, dl);
   gibbs_grad<double, N>(c, PS, ds);
-  yL[0] = gl;
-  yS[0] = gs - gl;
-#pragma unroll
+  Plain5 r;
+  r.dG = float(gs - gl);
   for (int l = 0; l < N - 1; ++l) {
-    yL[1 + l] = dl[l];
-    yS[1 + l] = ds[l];
+    r.dGL[l] = float(dl[l]);
+    r.dGS[l] = float(ds[l]);
   }
+  return r;
 }
 
-// One wave per 64 padded cells of a row; 4 rows per 256-thread block.
-__global__ __launch_bounds__(256) void k_ncomp_thermo_mlp(const float* __restrict__ in, const float* __restrict__ b,
-                                                          Dom d, float dx, ppf_temperature t,
-                                                          float* __restrict__ tr, int64_t tsy, int64_t tsf, NcK k,
-                                                          MlpK net, uint64_t seed, int64_t step) {
+__device__ __forceinline__ float fast_ln(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+
+// Scalars of the N-component pass the surrogate kernel needs (the plain
+// evaluation's tables stay in global memory for the fallback).
+struct NcMlpK {
+  float v_m, inv_v_m, D_L, dD, noise_c, W0, W1;  // W0, W1: W_l - W_N
+};
+
+// Persistent: each block stages both networks in LDS once and its waves walk
+// 64-cell tiles of the padded domain in row-major order (wave w of block b
+// takes tiles 4b + w, 4b + w + 4 gridDim.x, ...).  Lane l owns cell 64 tile + l;
+// MFMA group 0 is the tile's first 32 cells, group 1 the next 32 (their inputs
+// moved to lanes 0..31 by a shuffle), so the weight operands are read once per
+// 64 cells.
+__global__ __launch_bounds__(256, 2) void k_ncomp_thermo_mlp(const float* __restrict__ in,
+                                                             const float* __restrict__ b, Dom d, float dx,
+                                                             ppf_temperature t, float* __restrict__ tr, int64_t tsy,
+                                                             int64_t tsf, NcMlpK k, MlpK net, ppf_thermo th,
+                                                             uint64_t seed, int64_t step, uint32_t ntiles,
+                                                             int split3) {
   constexpr int N = 3;
-  __shared__ bf16x4 sw[2][kMlpWords];
+  __shared__ bf16x8 sw[2][kMlpWords];
   __shared__ float sb[2][132];
+  __shared__ ppf_thermo sth;  // plain tables for the out-of-range fallback
   for (int i = threadIdx.x; i < 2 * kMlpWords; i += 256) sw[i / kMlpWords][i % kMlpWords] = net.w[i / kMlpWords][i % kMlpWords];
   for (int i = threadIdx.x; i < 2 * 132; i += 256) sb[i / 132][i % 132] = net.bias[i / 132][i % 132];
+  for (int i = threadIdx.x; i < int(sizeof(ppf_thermo) / 4); i += 256)
+    reinterpret_cast<uint32_t*>(&sth)[i] = reinterpret_cast<const uint32_t*>(&th)[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, u = lane >> 5;
-  const int64_t px = d.ax.n + 2, py = d.ay.n + 2;
-  const uint32_t nbx = uint32_t((px + 63) / 64);
-  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int64_t xb = int64_t(blk % nbx) * 64, y = int64_t(blk / nbx) * 4 + (threadIdx.x >> 6);
-  if (y >= py) return;  // whole wave leaves together
-  const bool interior = xb >= 1 && xb + 63 <= d.ax.n && y >= 1 && y <= d.ay.n;
+  const uint32_t px = uint32_t(d.ax.n + 2), py = uint32_t(d.ay.n + 2), ncell = px * py;
+  const uint32_t nx = uint32_t(d.ax.n), ny = uint32_t(d.ay.n);
+  const uint32_t sy = uint32_t(d.ay.stride), sf = uint32_t(d.sf), tsy32 = uint32_t(tsy), tsf32 = uint32_t(tsf);
   const int64_t n3[3] = {d.ax.n, d.ay.n, 1};
   const int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
-  // two groups of 32 cells: group g covers columns xb + 32g + (lane % 32)
-  f32x16 yL[2], yS[2];
-  float phig[2], Tg[2], cg[2][N - 1];
-  bool gate[2];
+  for (uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += gridDim.x * 4) {
+    const uint32_t cell = tile * 64 + lane;
+    const bool valid = cell < ncell;
+    const uint32_t y = valid ? cell / px : 1, x = valid ? cell - y * px : 1;
+    float phi, c1, c2;
+    if (x >= 1 && x <= nx && y >= 1 && y <= ny) {
+      const uint32_t o = y * sy + x;
+      phi = in[o];
+      c1 = in[o + 3 * sf];
+      c2 = in[o + 4 * sf];
+    } else {
+      phi = fold(in, b, d, int64_t(x), int64_t(y), int64_t(0), dx);
+      c1 = fold(in + 3 * d.sf, b ? b + 3 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
+      c2 = fold(in + 4 * d.sf, b ? b + 4 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
+    }
+    const float T = temperature_at<float>(t, n3, bc3, int64_t(x), int64_t(y), 1, double(dx), step);
+    const float cN = 1.f - c1 - c2;
+    const bool gate = T < net.T_lo || T > net.T_hi || c1 < net.c_lo || c2 < net.c_lo || cN < net.c_lo;
+    const float xv[3] = {(T - net.in_lo[0]) * net.in_scale[0] - 1.f, (c1 - net.in_lo[1]) * net.in_scale[1] - 1.f,
+                         (c2 - net.in_lo[2]) * net.in_scale[2] - 1.f};
+    bf16x8 xin[2];
 #pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    const int64_t x = xb + 32 * g + (lane & 31);
-    const bool valid = x < px;
-    float phi = 0.f, T = float(net.T_lo), cin[N - 1] = {0.f, 0.f};
-    if (valid) {
-      if (interior) {
-        const int64_t o = y * d.ay.stride + x;
-        phi = in[o];
+    for (int g = 0; g < 2; ++g) {
 #pragma unroll
-        for (int l = 0; l < N - 1; ++l) cin[l] = in[o + (3 + l) * d.sf];
-      } else {
-        phi = fold(in, b, d, x, y, int64_t(0), dx);
-#pragma unroll
-        for (int l = 0; l < N - 1; ++l)
-          cin[l] = fold(in + (3 + l) * d.sf, b ? b + (3 + l) * d.sf : nullptr, d, x, y, int64_t(0), dx);
+      for (int q = 0; q < 3; ++q) {
+        const float other = __shfl(xv[q], (lane & 31) + 32, 64);
+        const float xs = g == 0 ? xv[q] : other;
+        const __bf16 hq = static_cast<__bf16>(xs);
+        const __bf16 lq = static_cast<__bf16>(xs - static_cast<float>(hq));
+        xin[g][q] = u == 0 ? hq : static_cast<__bf16>(0.f);
+        xin[g][3 + q] = u == 0 ? lq : static_cast<__bf16>(0.f);
       }
-      T = temperature_at<float>(t, n3, bc3, x, y, 1, double(dx), step);
+      xin[g][6] = static_cast<__bf16>(0.f);
+      xin[g][7] = static_cast<__bf16>(0.f);
     }
-    phig[g] = phi;
-    Tg[g] = T;
+    // the two networks one after the other (not unrolled: one network's
+    // operands and activations live at a time)
+    float GL[3], GS[3];
+#pragma unroll 1
+    for (int ph = 0; ph < 2; ++ph) {
+      float o[2][3];
+      mlp_eval<2>(sw[ph], sb[ph], xin, lane, split3 != 0, o);
+      // own cell: lanes 0..31 -> group 0 (own lane), lanes 32..63 -> group 1 (lane - 32)
 #pragma unroll
-    for (int l = 0; l < N - 1; ++l) cg[g][l] = cin[l];
-    const float cN = 1.f - cin[0] - cin[1];
-    gate[g] = T < net.T_lo || T > net.T_hi || cin[0] < net.c_lo || cin[1] < net.c_lo || cN < net.c_lo;
-    // inputs as BF16 pairs in the 8 k slots of layer 1:
-    // lanes < 32 (k 0..3): T_hi, c1_hi, c2_hi, T_lo; lanes >= 32 (k 4..7): c1_lo, c2_lo, 0, 0
-    const float xv[3] = {(T - net.in_lo[0]) * net.in_scale[0] - 1.f, (cin[0] - net.in_lo[1]) * net.in_scale[1] - 1.f,
-                         (cin[1] - net.in_lo[2]) * net.in_scale[2] - 1.f};
-    short xh[3], xl[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      xh[q] = f2bf(xv[q]);
-      xl[q] = f2bf(xv[q] - bf2f(xh[q]));
+      for (int q = 0; q < 3; ++q) {
+        const float o1 = __shfl(o[1][q], lane & 31, 64);
+        const float v = (u == 0 ? o[0][q] : o1) * net.out_sigma[ph][q] + net.out_mu[ph][q];
+        GL[q] = ph == 0 ? v : GL[q];
+        GS[q] = ph == 1 ? v : GS[q];
+      }
     }
-    bf16x4 xin;
-    xin[0] = u == 0 ? xh[0] : xl[1];
-    xin[1] = u == 0 ? xh[1] : xl[2];
-    xin[2] = u == 0 ? xh[2] : short(0);
-    xin[3] = u == 0 ? xl[0] : short(0);
-    yL[g] = mlp_phase(sw[0], sb[0], xin, lane);
-    yS[g] = mlp_phase(sw[1], sb[1], xin, lane);
-  }
-  // lanes 0..31 finish group 0, lanes 32..63 group 1 (its outputs moved over)
-  float oL[3], oS[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const float l1 = __shfl(yL[1][q], lane & 31, 64);
-    const float s1 = __shfl(yS[1][q], lane & 31, 64);
-    oL[q] = u == 0 ? yL[0][q] : l1;
-    oS[q] = u == 0 ? yS[0][q] : s1;
-  }
-  const int g = u;
-  const int64_t x = xb + 32 * g + (lane & 31);
-  if (x >= px) return;
-  const float phi = phig[g], T = Tg[g];
-  const float cin[N - 1] = {cg[g][0], cg[g][1]};
-  float GLn[N], GSn[N];  // non-ideal G and derivatives (liquid), G_S - G_L and solid derivatives
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    GLn[q] = oL[q] * net.out_sigma[0][q] + net.out_mu[0][q];
-    GSn[q] = oS[q] * net.out_sigma[1][q] + net.out_mu[1][q];
-  }
-  const ppf_ncomp_params& p = k.p;
-  const float cN = 1.f - cin[0] - cin[1];
-  float dG, dGL[N - 1], dGS[N - 1];
-  if (gate[g]) {
-    double yl[3], ys[3];
-    plain_outputs<N>(cin, T, k.th, yl, ys);
-    dG = float(ys[0]);
+    if (!valid) continue;
+    float dG, dGL[N - 1], dGS[N - 1];
+    // ideal mixing R T sum c ln c is analytic: it cancels in G_S - G_L and
+    // adds R T (ln c_l - ln c_N) to each derivative
+    const float RT = net.R * T;
+    const float lnN = fast_ln(fmaxf(cN, float(kCMin)));
+    const float cl[N - 1] = {c1, c2};
+    dG = GS[0];
 #pragma unroll
     for (int l = 0; l < N - 1; ++l) {
-      dGL[l] = float(yl[1 + l]);
-      dGS[l] = float(ys[1 + l]);
+      const float idl = RT * (fast_ln(fmaxf(cl[l], float(kCMin))) - lnN);
+      dGL[l] = GL[1 + l] + idl;
+      dGS[l] = GS[1 + l] + idl;
     }
-  } else {
-    // ideal mixing R T sum c ln c is analytic: it cancels in G_S - G_L and adds
-    // R T (ln c_l - ln c_N) to each derivative
-    const float RT = float(k.th.liquid.R) * T;
-    const float lnN = logf(fmaxf(cN, float(kCMin)));
-    dG = GSn[0];
+    if (gate) {
+      const Plain5 r = plain_outputs(c1, c2, T, &sth);
+      dG = r.dG;
+#pragma unroll
+      for (int l = 0; l < N - 1; ++l) {
+        dGL[l] = r.dGL[l];
+        dGS[l] = r.dGS[l];
+      }
+    }
+    const float g_ = (phi * phi) * ((1.f - phi) * (1.f - phi));
+    const float h_ = (phi * phi * phi) * (6.f * (phi * phi) - 15.f * phi + 10.f);
+    const uint32_t o = y * tsy32 + x;
+    tr[o] = dG;
+    const float mob = k.v_m * (k.D_L + h_ * k.dD) * __builtin_amdgcn_rcpf(8.314f * T);
+    const float Wd[N - 1] = {k.W0, k.W1};
 #pragma unroll
     for (int l = 0; l < N - 1; ++l) {
-      const float idl = RT * (logf(fmaxf(cin[l], float(kCMin))) - lnN);
-      dGL[l] = GLn[1 + l] + idl;
-      dGS[l] = GSn[1 + l] + idl;
+      tr[o + (1 + l) * tsf32] = mob * cl[l];
+      float noise = 0.f;
+      if (k.noise_c != 0.f) {
+        float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + l, uint64_t(ncell), uint64_t(cell))));
+        noise = k.noise_c * sqrtf(2.f * 8.314f * T * k.inv_v_m) * xi;
+      }
+      tr[o + (N + l) * tsf32] = (dGL[l] + h_ * (dGS[l] - dGL[l])) * k.inv_v_m + Wd[l] * g_ * T + noise;
     }
-  }
-  const float g_ = (phi * phi) * ((1.f - phi) * (1.f - phi));
-  const float h_ = (phi * phi * phi) * (6.f * (phi * phi) - 15.f * phi + 10.f);
-  const float v_m = float(p.v_m), D_L = float(p.D_L), D_S = float(p.D_S);
-  const uint64_t ncells = uint64_t(px * py), cell = uint64_t(y) * uint64_t(px) + uint64_t(x);
-  const int64_t o = y * tsy + x;
-  tr[o] = dG;
-#pragma unroll
-  for (int l = 0; l < N - 1; ++l) {
-    tr[o + (1 + l) * tsf] = v_m * cin[l] * (D_L + h_ * (D_S - D_L)) / (8.314f * T);
-    float noise = 0.f;
-    if (p.noise_c != 0.0) {
-      float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + l, ncells, cell)));
-      noise = float(p.noise_c) * sqrtf(2.f * 8.314f * T / v_m) * xi;
-    }
-    tr[o + (N + l) * tsf] =
-        (dGL[l] + h_ * (dGS[l] - dGL[l])) / v_m + (float(p.W[l]) - float(p.W[N - 1])) * g_ * T + noise;
   }
 }
 
 }  // namespace ppf
 
 using namespace ppf;
+
+namespace {
+// compute units of the current device (cached per device id)
+int device_cus() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cached[dev] = n;
+  }
+  return cached[dev];
+}
+}  // namespace
 
 extern "C" {
 
@@ -457,10 +522,17 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
   for (int q = 0; q < 2; ++q)
     if (!net->weights[q] || !net->bias[q]) return fail(PPF_EINVAL, "ncomp_thermo_mlp: missing network");
   Dom d = make_dom(*g);
-  NcK k{*p, *th};
+  NcMlpK k;
+  k.v_m = float(p->v_m);
+  k.inv_v_m = float(1.0 / p->v_m);
+  k.D_L = float(p->D_L);
+  k.dD = float(p->D_S - p->D_L);
+  k.noise_c = float(p->noise_c);
+  k.W0 = float(p->W[0]) - float(p->W[2]);
+  k.W1 = float(p->W[1]) - float(p->W[2]);
   MlpK m;
   for (int q = 0; q < 2; ++q) {
-    m.w[q] = static_cast<const bf16x4*>(net->weights[q]);
+    m.w[q] = static_cast<const bf16x8*>(net->weights[q]);
     m.bias[q] = net->bias[q];
     for (int j = 0; j < 3; ++j) {
       m.out_mu[q][j] = net->out_mu[q][j];
@@ -474,10 +546,15 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
   m.T_lo = net->T_lo;
   m.T_hi = net->T_hi;
   m.c_lo = net->c_lo;
-  dim3 grid(unsigned(((g->n[0] + 2 + 63) / 64) * ((g->n[1] + 2 + 3) / 4)));
-  hipLaunchKernelGGL(k_ncomp_thermo_mlp, grid, dim3(256), 0, stream_of(s), static_cast<const float*>(in),
+  m.R = float(th->liquid.R);
+  const int64_t ncell = (g->n[0] + 2) * (g->n[1] + 2);
+  if (ncell >= (int64_t(1) << 31) || tg->sf * 5 >= (int64_t(1) << 32) || g->sf * 5 >= (int64_t(1) << 32))
+    return fail(PPF_EINVAL, "ncomp_thermo_mlp: grid too large for 32-bit offsets");
+  const uint32_t ntiles = uint32_t((ncell + 63) / 64);
+  const unsigned blocks = unsigned(std::min<int64_t>((ntiles + 3) / 4, int64_t(2) * device_cus()));
+  hipLaunchKernelGGL(k_ncomp_thermo_mlp, dim3(blocks), dim3(256), 0, stream_of(s), static_cast<const float*>(in),
                      static_cast<const float*>(bcvals), d, float(g->dx), *t, static_cast<float*>(transfer), tg->sy,
-                     tg->sf, k, m, seed, step);
+                     tg->sf, k, m, *th, seed, step, ntiles, (net->flags & PPF_MLP_SPLIT3) ? 1 : 0);
   return check_launch("ncomp_thermo_mlp");
 }
 

@@ -33,40 +33,45 @@ def _f2bf_bits(a):
     return u.astype(np.uint16)
 
 
+def _slot_rows(h, p):
+    """Hidden row held by slot (u, t) of K block kb = 2h + p: an array [2, 8]."""
+    u = np.arange(2)[:, None]
+    t = np.arange(8)[None, :]
+    return 32 * h + 16 * p + 8 * (t // 4) + 4 * u + t % 4
+
+
 def pack_phase(W1, W2, W3):
     """BF16 A-operand lane layout of ``ppf_ncomp_thermo_mlp`` for one network.
 
-    32x32x8 MFMA A operand: lane l, element t holds A[l % 32][4 (l // 32) + t].
-    Words: layer 1 (2 row halves), layer 2 (2 row halves x 8 k-blocks),
-    layer 3 (8 k-blocks, output rows 0..2 padded to 32).
+    32x32x16 MFMA A operand: lane l, element t holds A[l % 32][slot (l // 32, t)];
+    slot (u, t) of K block kb = 2h + p is hidden row 32h + 16p + 8(t//4) + 4u + t%4
+    (the row the previous layer's D tile leaves in that lane and element).
+    Sets of 64 lanes x 8: layer 1 (2 hidden halves; slots (0, 0..5) carry the
+    inputs T, c1, c2, T, c1, c2 for the hi and lo parts), layer 2 (2 output
+    halves x 4 K blocks), layer 3 (4 K blocks, output rows 0..2, rest zero).
     """
     lanes = np.arange(64)
     r = lanes % 32
     u = lanes // 32
-    words = []
-    # layer-1 k slots 0..7 carry the inputs (T_hi, c1_hi, c2_hi, T_lo, c1_lo, c2_lo, 0, 0)
-    col_of_k = [0, 1, 2, 0, 1, 2, -1, -1]
+    sets = []
+    col_of_t = [0, 1, 2, 0, 1, 2]
     for h in range(2):
-        blk = np.zeros((64, 4), np.float32)
-        for t in range(4):
-            for lane in range(64):
-                col = col_of_k[4 * u[lane] + t]
-                if col >= 0:
-                    blk[lane, t] = W1[32 * h + r[lane], col]
-        words.append(blk)
+        blk = np.zeros((64, 8), np.float32)
+        for t, col in enumerate(col_of_t):
+            blk[u == 0, t] = W1[32 * h + r[u == 0], col]
+        sets.append(blk)
     for m in range(2):
-        for s in range(8):
-            blk = np.zeros((64, 4), np.float32)
-            for t in range(4):
-                blk[:, t] = W2[32 * m + r, 8 * s + 4 * u + t]
-            words.append(blk)
-    for s in range(8):
-        blk = np.zeros((64, 4), np.float32)
-        for t in range(4):
-            ok = r < 3
-            blk[ok, t] = W3[r[ok], 8 * s + 4 * u[ok] + t]
-        words.append(blk)
-    return _f2bf_bits(np.concatenate(words, axis=0))  # [26*64, 4] uint16
+        for kb in range(4):
+            rows = _slot_rows(kb // 2, kb % 2)      # [2, 8]
+            blk = W2[32 * m + r[:, None], rows[u]]  # [64, 8]
+            sets.append(blk.astype(np.float32))
+    for kb in range(4):
+        rows = _slot_rows(kb // 2, kb % 2)
+        blk = np.zeros((64, 8), np.float32)
+        ok = r < 3
+        blk[ok] = W3[r[ok][:, None], rows[u[ok]]]
+        sets.append(blk)
+    return _f2bf_bits(np.concatenate(sets, axis=0))  # [14*64, 8] uint16
 
 
 class PhaseNet:
@@ -106,7 +111,7 @@ class PhaseNet:
 class SurrogateThermo:
     """Trainable surrogate of a ThermoModel (3 components) on a (T, c) box."""
 
-    def __init__(self, model, T_range, c_lo=0.02, seed=0, device="cuda"):
+    def __init__(self, model, T_range, c_lo=0.02, seed=0, device="cuda", split3=True):
         import torch
         if len(model.components) != 3:
             raise ValueError("the surrogate kernel is built for 3 components")
@@ -124,6 +129,8 @@ class SurrogateThermo:
         self.out_sigma = np.ones((2, 3))
         self.history = []
         self._packed = None
+        # inference: layer-3 inputs as BF16 hi + lo pairs (True) or BF16 only
+        self.split3 = bool(split3)
 
     # --------------------------------------------------------- samples --
     def sample_inputs(self, n):
@@ -222,7 +229,7 @@ class SurrogateThermo:
         return part, dY
 
     # ------------------------------------------------------------ train --
-    def train(self, n_samples=1_000_000, epochs=30, batch=8192, lr=2e-3, lr_final=5e-5, verbose=False):
+    def train(self, n_samples=1_000_000, epochs=20, batch=1024, lr=2e-3, lr_final=5e-5, verbose=False):
         """Adam on normalised MSE; the learning rate decays geometrically."""
         import torch
         L = _lib.lib()
@@ -276,6 +283,33 @@ class SurrogateThermo:
             out.append(Y * self.out_sigma[p] + self.out_mu[p])
         return out
 
+    # ------------------------------------------------------ save / load --
+    def save(self, path):
+        """Trained parameters and normalisation to an .npz file (no pickles)."""
+        np.savez(path, T_range=np.array([self.T_lo, self.T_hi]), c_lo=np.array(self.c_lo), in_lo=self.in_lo,
+                 in_scale=self.in_scale, out_mu=self.out_mu, out_sigma=self.out_sigma,
+                 liquid=self.nets[0].flat.cpu().numpy(), solid=self.nets[1].flat.cpu().numpy(),
+                 components=np.array(list(self.model.components)))
+
+    @classmethod
+    def load(cls, path, model, device="cuda"):
+        """Surrogate of ``model`` from :meth:`save` output; the components must match."""
+        import torch
+        z = np.load(path, allow_pickle=False)
+        if [str(c) for c in z["components"]] != list(model.components):
+            raise ValueError("surrogate %s was trained for components %s, not %s"
+                             % (path, list(z["components"]), list(model.components)))
+        sur = cls(model, tuple(z["T_range"]), c_lo=float(z["c_lo"]), device=device)
+        sur.in_lo, sur.in_scale = z["in_lo"], z["in_scale"]
+        sur.out_mu, sur.out_sigma = z["out_mu"], z["out_sigma"]
+        for net, key in zip(sur.nets, ("liquid", "solid")):
+            flat = z[key]
+            if flat.shape != tuple(net.flat.shape):
+                raise ValueError("surrogate %s: %s network has %d parameters, expected %d"
+                                 % (path, key, flat.size, net.flat.numel()))
+            net.flat.copy_(torch.from_numpy(flat.astype(np.float32)))
+        return sur
+
     # ------------------------------------------------------- inference --
     def packed(self):
         """Device buffers + ppf_mlp_net descriptor for ppf_ncomp_thermo_mlp."""
@@ -303,6 +337,7 @@ class SurrogateThermo:
             net.in_lo[q] = float(self.in_lo[q])
             net.in_scale[q] = float(self.in_scale[q])
         net.T_lo, net.T_hi, net.c_lo = self.T_lo, self.T_hi, self.c_lo
+        net.flags = _lib_ext.PPF_MLP_SPLIT3 if self.split3 else 0
         self._packed = (net, keep)
         return self._packed
 

@@ -60,3 +60,25 @@ def padded_T(T_int, bc):
     T = obc.pad(T_int)
     obc.apply_bc_single(T, bc)
     return T
+
+
+def interface_band(phi_a, phi_b, lo=0.01, hi=0.99, width=1):
+    """Cells that are not bulk liquid (phi < lo) or bulk solid (phi > hi) in
+    both fields, dilated by ``width`` cells (Chebyshev)."""
+    bulk = ((phi_a < lo) & (phi_b < lo)) | ((phi_a > hi) & (phi_b > hi))
+    band = ~bulk
+    out = band.copy()
+    for dy in range(-width, width + 1):
+        for dx in range(-width, width + 1):
+            out |= np.roll(np.roll(band, dy, 0), dx, 1)
+    return out
+
+
+def composition_errors(c_got, c_want, phi_got, phi_want):
+    """(full-domain L-inf, bulk L-inf outside the interface band, relative L2)
+    over the composition fields (lists of interior arrays)."""
+    band = interface_band(phi_got, phi_want)
+    full = max(float(np.abs(a - b).max()) for a, b in zip(c_got, c_want))
+    bulk = max(float(np.abs(a - b)[~band].max()) for a, b in zip(c_got, c_want))
+    l2 = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(c_got, c_want))
+    return full, bulk, l2
