@@ -233,7 +233,8 @@ def bench_ncomp(args):
                     os.path.exists(args.surrogate) else None)
     if variant == "surrogate":
         sim._setup_surrogate()
-        sim._surrogate.split3 = not args.no_split3
+        sim._surrogate.split2 = "2" in args.mlp_split
+        sim._surrogate.split3 = "3" in args.mlp_split
         sim._surrogate._packed = None
     sim.simulate(args.warmup)
     if variant == "surrogate" and args.surrogate and not os.path.exists(args.surrogate):
@@ -294,8 +295,8 @@ def main():
     ap.add_argument("--variant", default="fp32")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-split3", action="store_true",
-                    help="ncomp surrogate: feed the last layer BF16 activations only")
+    ap.add_argument("--mlp-split", default="none",
+                    help="ncomp surrogate: layers (2, 3) whose inputs enter as BF16 hi + lo pairs")
     ap.add_argument("--surrogate", default=None,
                     help="ncomp surrogate: load the trained networks from this .npz, or train and save there")
     args = ap.parse_args()

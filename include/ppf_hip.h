@@ -212,7 +212,8 @@ int ppf_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float 
                  float eps, int64_t t, ppf_stream_t s);
 
 /* Two trained 3-64-64-3 ReLU networks (liquid, solid), inputs (T, c1, c2),
-   BF16 weights, activations as BF16 hi + lo pairs, FP32 accumulation.
+   BF16 weights, FP32 accumulation; inputs as BF16 hi + lo pairs, hidden
+   activations as pairs or BF16 (flags).
    Outputs (normalised): liquid (G_ni, dG_ni/dc1, dG_ni/dc2); solid
    (G_S,ni - G_L,ni, dG_S,ni/dc1, dG_S,ni/dc2), where "ni" is G minus the ideal
    mixing term R T sum c ln c (evaluated analytically in the kernel). */
@@ -222,9 +223,11 @@ typedef struct ppf_mlp_net {
   float in_lo[3], in_scale[3];          /* x_hat = (x - lo) * scale - 1 */
   float out_mu[2][3], out_sigma[2][3];  /* y = y_hat * sigma + mu */
   float T_lo, T_hi, c_lo;  /* range gate: cells outside use the plain FP64 evaluation */
-  int32_t flags;           /* PPF_MLP_SPLIT3: layer-3 inputs as BF16 hi + lo (else BF16 only) */
+  int32_t flags;           /* PPF_MLP_SPLIT2 / PPF_MLP_SPLIT3: the inputs of layer 2 / 3 enter as
+                              BF16 hi + lo pairs (else BF16 only) */
 } ppf_mlp_net;
 #define PPF_MLP_SPLIT3 1
+#define PPF_MLP_SPLIT2 2
 
 /* N-component thermodynamic pass with the surrogate (FP32, N = 3): same
    outputs as ppf_ncomp_thermo. */

@@ -8,6 +8,7 @@ c_i32, c_i64, c_u64, c_f32, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint
 
 
 PPF_MLP_SPLIT3 = 1
+PPF_MLP_SPLIT2 = 2
 
 
 class MlpNet(ctypes.Structure):

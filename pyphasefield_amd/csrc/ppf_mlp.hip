@@ -160,9 +160,10 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float
 //               (T_hi, c1_hi, c2_hi, T_lo, c1_lo, c2_lo, 0, 0), lanes >= 32 zero
 //   set 2..9    layer 2, output half m = (set-2)/4, K block kb = (set-2)%4
 //   set 10..13  layer 3, K block kb, output rows 0..2 (rows 3..31 zero)
-// followed by FP32 b1[64], b2[64], b3[4] (b3[3] unused).  Activations enter
-// each layer as BF16 pairs hi + lo (two MFMAs with the same weights), so the
-// only BF16 rounding is that of the weights, which training sees.
+// followed by FP32 b1[64], b2[64], b3[4] (b3[3] unused).  The inputs enter
+// layer 1 as BF16 pairs hi + lo; the activations entering layers 2 and 3 are
+// either such pairs (two MFMAs with the same weights: the only BF16 rounding
+// is then that of the weights, which training sees) or BF16 only.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int kMlpSets = 14;
 constexpr int kMlpWords = kMlpSets * 64;  // bf16x8 words per phase
@@ -180,14 +181,59 @@ struct MlpK {
   float R;                            // gas constant of the ideal-mixing term
 };
 
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// two FP32 values to one register of BF16 (round to nearest even): low half
+// from a, high half from b (a single v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
+
+// max(x, 0) as a signed-integer max of the bits (one instruction, no NaN
+// canonicalisation; negative floats are negative integers)
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
 // ReLU of items 8p..8p+7 of a D tile as the BF16 pair (hi, lo)
 __device__ __forceinline__ void relu_split(const f32x16& d, int p, bf16x8& hi, bf16x8& lo) {
+  u32x4 wh, wl;
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const float v = fmaxf(d[8 * p + t], 0.f);
-    const __bf16 h = static_cast<__bf16>(v);
-    hi[t] = h;
-    lo[t] = static_cast<__bf16>(v - static_cast<float>(h));
+  for (int t = 0; t < 4; ++t) {
+    const float v0 = relu(d[8 * p + 2 * t]), v1 = relu(d[8 * p + 2 * t + 1]);
+    const uint32_t h = cvt_pk_bf16(v0, v1);
+    wh[t] = h;
+    wl[t] = cvt_pk_bf16(v0 - __uint_as_float(h << 16), v1 - __uint_as_float(h & 0xffff0000u));
+  }
+  hi = __builtin_bit_cast(bf16x8, wh);
+  lo = __builtin_bit_cast(bf16x8, wl);
+}
+
+// ReLU of items 8p..8p+7 of a D tile rounded to BF16: pairs are converted
+// first and the ReLU is taken on the BF16 bits as signed 16-bit integers
+// (every negative BF16 value, -0 included, is a negative integer)
+__device__ __forceinline__ void relu_bf16(const f32x16& d, int p, bf16x8& hi) {
+  u32x4 w;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const uint32_t pr = cvt_pk_bf16(d[8 * p + 2 * t], d[8 * p + 2 * t + 1]);
+    w[t] = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, pr), s16x2{0, 0}));
+  }
+  hi = __builtin_bit_cast(bf16x8, w);
+}
+
+// next-layer B operands (K blocks kb = 0..3) from the two D tiles of a layer
+template <bool SPLIT>
+__device__ __forceinline__ void pack_layer(const f32x16 (&h)[2], bf16x8 (&bh)[4], bf16x8 (&bl)[4]) {
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    if (SPLIT) {
+      relu_split(h[kb >> 1], kb & 1, bh[kb], bl[kb]);
+    } else {
+      relu_bf16(h[kb >> 1], kb & 1, bh[kb]);
+    }
   }
 }
 
@@ -205,10 +251,11 @@ __device__ __forceinline__ f32x16 bias_tile(const float* b, int row0, int u) {
 // One phase network on G groups of 32 cells; the weight operands read from
 // LDS are shared by the groups.  x[g]: layer-1 B operand of group g.
 // out[g][q] = normalised output q of cell (lane % 32) of group g, valid in
-// lanes 0..31.  split3 = false feeds the layer-3 inputs as BF16 only.
-template <int G>
+// lanes 0..31.  S2 / S3: the inputs of layer 2 / layer 3 enter as BF16 hi + lo
+// pairs (two MFMAs per K block) instead of BF16 only.
+template <int G, bool S2, bool S3>
 __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const float* __restrict__ b,
-                                         const bf16x8 (&x)[G], int lane, bool split3, float (&out)[G][3]) {
+                                         const bf16x8 (&x)[G], int lane, float (&out)[G][3]) {
   const int u = lane >> 5;
   f32x16 h[G][2];
 #pragma unroll
@@ -220,9 +267,7 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
   }
   bf16x8 bh[G][4], bl[G][4];
 #pragma unroll
-  for (int g = 0; g < G; ++g)
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) relu_split(h[g][kb >> 1], kb & 1, bh[g][kb], bl[g][kb]);
+  for (int g = 0; g < G; ++g) pack_layer<S2>(h[g], bh[g], bl[g]);
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
     const f32x16 bt = bias_tile(b + 64, 32 * m, u);
@@ -233,10 +278,14 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
       const bf16x8 a = w[(2 + 4 * m + kb) * 64 + lane];
 #pragma unroll
       for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bh[g][kb], h[g][m]);
+      if (S2) {
 #pragma unroll
-      for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bl[g][kb], h[g][m]);
+        for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bl[g][kb], h[g][m]);
+      }
     }
   }
+#pragma unroll
+  for (int g = 0; g < G; ++g) pack_layer<S3>(h[g], bh[g], bl[g]);
   f32x16 y[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -248,32 +297,14 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
       y[g][2] = b[130];
     }
   }
-  if (split3) {
 #pragma unroll
-    for (int g = 0; g < G; ++g)
+  for (int kb = 0; kb < 4; ++kb) {
+    const bf16x8 a = w[(10 + kb) * 64 + lane];
 #pragma unroll
-      for (int kb = 0; kb < 4; ++kb) relu_split(h[g][kb >> 1], kb & 1, bh[g][kb], bl[g][kb]);
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-      const bf16x8 a = w[(10 + kb) * 64 + lane];
-#pragma unroll
-      for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], y[g]);
+    for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], y[g]);
+    if (S3) {
 #pragma unroll
       for (int g = 0; g < G; ++g) y[g] = mfma16(a, bl[g][kb], y[g]);
-    }
-  } else {
-#pragma unroll
-    for (int g = 0; g < G; ++g)
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb) {
-#pragma unroll
-        for (int t = 0; t < 8; ++t) bh[g][kb][t] = static_cast<__bf16>(fmaxf(h[g][kb >> 1][8 * (kb & 1) + t], 0.f));
-      }
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-      const bf16x8 a = w[(10 + kb) * 64 + lane];
-#pragma unroll
-      for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], y[g]);
     }
   }
 #pragma unroll
@@ -286,16 +317,15 @@ struct Plain5 {
   float dG, dGL[2], dGS[2];
 };
 
-// plain FP64 evaluation for cells outside the trained range: G_S - G_L and
-// both phases' derivatives (kept out of line: rare, and heavy in registers)
-__device__ __noinline__ Plain5 plain_outputs(float c1, float c2, float T, const ppf_thermo* th) {
+// plain FP64 evaluation of G_S - G_L and both phases' derivatives
+__device__ __forceinline__ Plain5 plain_outputs(float c1, float c2, float T, const ppf_thermo& th) {
   constexpr int N = 3;
   double c[N] = {double(c1), double(c2), 1.0 - double(c1) - double(c2)};
   double t = double(T);
   double lnT = plog(t);
   PhaseAtT<double, N> PL, PS;
-  phase_at<double, N>(th->liquid, t, lnT, PL);
-  phase_at<double, N>(th->solid, t, lnT, PS);
+  phase_at<double, N>(th.liquid, t, lnT, PL);
+  phase_at<double, N>(th.solid, t, lnT, PS);
   double id = ideal_sum<double, N>(c);
   double gl = gibbs<double, N>(c, PL, id), gs = gibbs<double, N>(c, PS, id);
   double dl[N], ds[N];
@@ -318,26 +348,47 @@ struct NcMlpK {
   float v_m, inv_v_m, D_L, dD, noise_c, W0, W1;  // W0, W1: W_l - W_N
 };
 
+// transfer outputs of one cell (layout of ppf_ncomp_thermo): G_S - G_L,
+// mobilities M_c,l, chemical potentials dF/dc_l (+ noise)
+__device__ __forceinline__ void write_transfer(float* __restrict__ tr, uint32_t o, uint32_t tsf, const NcMlpK& k,
+                                               float phi, float T, const float (&cl)[2], float dG,
+                                               const float (&dGL)[2], const float (&dGS)[2], uint64_t seed,
+                                               int64_t step, uint32_t ncell, uint32_t cell) {
+  constexpr int N = 3;
+  const float g_ = (phi * phi) * ((1.f - phi) * (1.f - phi));
+  const float h_ = (phi * phi * phi) * (6.f * (phi * phi) - 15.f * phi + 10.f);
+  tr[o] = dG;
+  const float mob = k.v_m * (k.D_L + h_ * k.dD) * __builtin_amdgcn_rcpf(8.314f * T);
+  const float Wd[N - 1] = {k.W0, k.W1};
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) {
+    tr[o + (1 + l) * tsf] = mob * cl[l];
+    float noise = 0.f;
+    if (k.noise_c != 0.f) {
+      float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + l, uint64_t(ncell), uint64_t(cell))));
+      noise = k.noise_c * sqrtf(2.f * 8.314f * T * k.inv_v_m) * xi;
+    }
+    tr[o + (N + l) * tsf] = (dGL[l] + h_ * (dGS[l] - dGL[l])) * k.inv_v_m + Wd[l] * g_ * T + noise;
+  }
+}
+
 // Persistent: each block stages both networks in LDS once and its waves walk
 // 64-cell tiles of the padded domain in row-major order (wave w of block b
 // takes tiles 4b + w, 4b + w + 4 gridDim.x, ...).  Lane l owns cell 64 tile + l;
 // MFMA group 0 is the tile's first 32 cells, group 1 the next 32 (their inputs
 // moved to lanes 0..31 by a shuffle), so the weight operands are read once per
 // 64 cells.
-__global__ __launch_bounds__(256, 2) void k_ncomp_thermo_mlp(const float* __restrict__ in,
+template <bool S2, bool S3>
+__global__ __launch_bounds__(256, 3) void k_ncomp_thermo_mlp(const float* __restrict__ in,
                                                              const float* __restrict__ b, Dom d, float dx,
                                                              ppf_temperature t, float* __restrict__ tr, int64_t tsy,
-                                                             int64_t tsf, NcMlpK k, MlpK net, ppf_thermo th,
-                                                             uint64_t seed, int64_t step, uint32_t ntiles,
-                                                             int split3) {
+                                                             int64_t tsf, NcMlpK k, MlpK net, uint64_t seed,
+                                                             int64_t step, uint32_t ntiles) {
   constexpr int N = 3;
   __shared__ bf16x8 sw[2][kMlpWords];
   __shared__ float sb[2][132];
-  __shared__ ppf_thermo sth;  // plain tables for the out-of-range fallback
   for (int i = threadIdx.x; i < 2 * kMlpWords; i += 256) sw[i / kMlpWords][i % kMlpWords] = net.w[i / kMlpWords][i % kMlpWords];
   for (int i = threadIdx.x; i < 2 * 132; i += 256) sb[i / 132][i % 132] = net.bias[i / 132][i % 132];
-  for (int i = threadIdx.x; i < int(sizeof(ppf_thermo) / 4); i += 256)
-    reinterpret_cast<uint32_t*>(&sth)[i] = reinterpret_cast<const uint32_t*>(&th)[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, u = lane >> 5;
   const uint32_t px = uint32_t(d.ax.n + 2), py = uint32_t(d.ay.n + 2), ncell = px * py;
@@ -386,7 +437,7 @@ __global__ __launch_bounds__(256, 2) void k_ncomp_thermo_mlp(const float* __rest
 #pragma unroll 1
     for (int ph = 0; ph < 2; ++ph) {
       float o[2][3];
-      mlp_eval<2>(sw[ph], sb[ph], xin, lane, split3 != 0, o);
+      mlp_eval<2, S2, S3>(sw[ph], sb[ph], xin, lane, o);
       // own cell: lanes 0..31 -> group 0 (own lane), lanes 32..63 -> group 1 (lane - 32)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -397,6 +448,11 @@ __global__ __launch_bounds__(256, 2) void k_ncomp_thermo_mlp(const float* __rest
       }
     }
     if (!valid) continue;
+    const uint32_t o = y * tsy32 + x;
+    if (gate) {  // outside the trained box: marked for k_ncomp_thermo_fallback
+      tr[o] = __builtin_nanf("");
+      continue;
+    }
     float dG, dGL[N - 1], dGS[N - 1];
     // ideal mixing R T sum c ln c is analytic: it cancels in G_S - G_L and
     // adds R T (ln c_l - ln c_N) to each derivative
@@ -410,34 +466,33 @@ __global__ __launch_bounds__(256, 2) void k_ncomp_thermo_mlp(const float* __rest
       dGL[l] = GL[1 + l] + idl;
       dGS[l] = GS[1 + l] + idl;
     }
-    if (gate) {
-      const Plain5 r = plain_outputs(c1, c2, T, &sth);
-      dG = r.dG;
-#pragma unroll
-      for (int l = 0; l < N - 1; ++l) {
-        dGL[l] = r.dGL[l];
-        dGS[l] = r.dGS[l];
-      }
-    }
-    const float g_ = (phi * phi) * ((1.f - phi) * (1.f - phi));
-    const float h_ = (phi * phi * phi) * (6.f * (phi * phi) - 15.f * phi + 10.f);
-    const uint32_t o = y * tsy32 + x;
-    tr[o] = dG;
-    const float mob = k.v_m * (k.D_L + h_ * k.dD) * __builtin_amdgcn_rcpf(8.314f * T);
-    const float Wd[N - 1] = {k.W0, k.W1};
-#pragma unroll
-    for (int l = 0; l < N - 1; ++l) {
-      tr[o + (1 + l) * tsf32] = mob * cl[l];
-      float noise = 0.f;
-      if (k.noise_c != 0.f) {
-        float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + l, uint64_t(ncell), uint64_t(cell))));
-        noise = k.noise_c * sqrtf(2.f * 8.314f * T * k.inv_v_m) * xi;
-      }
-      tr[o + (N + l) * tsf32] = (dGL[l] + h_ * (dGS[l] - dGL[l])) * k.inv_v_m + Wd[l] * g_ * T + noise;
-    }
+    write_transfer(tr, o, tsf32, k, phi, T, cl, dG, dGL, dGS, seed, step, ncell, cell);
   }
 }
 
+// Cells the surrogate kernel marked (NaN in G_S - G_L): the plain FP64
+// evaluation, same outputs.  One thread per padded cell.
+__global__ __launch_bounds__(256) void k_ncomp_thermo_fallback(const float* __restrict__ in,
+                                                               const float* __restrict__ b, Dom d, float dx,
+                                                               ppf_temperature t, float* __restrict__ tr,
+                                                               int64_t tsy, int64_t tsf, NcMlpK k, ppf_thermo th,
+                                                               uint64_t seed, int64_t step) {
+  const uint32_t px = uint32_t(d.ax.n + 2), py = uint32_t(d.ay.n + 2), ncell = px * py;
+  const int64_t n3[3] = {d.ax.n, d.ay.n, 1};
+  const int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
+  for (uint32_t cell = blockIdx.x * 256 + threadIdx.x; cell < ncell; cell += gridDim.x * 256) {
+    const uint32_t y = cell / px, x = cell - y * px;
+    const uint32_t o = y * uint32_t(tsy) + x;
+    if (!__builtin_isnan(tr[o])) continue;
+    const float phi = fold(in, b, d, int64_t(x), int64_t(y), int64_t(0), dx);
+    const float c1 = fold(in + 3 * d.sf, b ? b + 3 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
+    const float c2 = fold(in + 4 * d.sf, b ? b + 4 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
+    const float T = temperature_at<float>(t, n3, bc3, int64_t(x), int64_t(y), 1, double(dx), step);
+    const Plain5 r = plain_outputs(c1, c2, T, th);
+    const float cl[2] = {c1, c2};
+    write_transfer(tr, o, uint32_t(tsf), k, phi, T, cl, r.dG, r.dGL, r.dGS, seed, step, ncell, cell);
+  }
+}
 }  // namespace ppf
 
 using namespace ppf;
@@ -551,10 +606,18 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
   if (ncell >= (int64_t(1) << 31) || tg->sf * 5 >= (int64_t(1) << 32) || g->sf * 5 >= (int64_t(1) << 32))
     return fail(PPF_EINVAL, "ncomp_thermo_mlp: grid too large for 32-bit offsets");
   const uint32_t ntiles = uint32_t((ncell + 63) / 64);
-  const unsigned blocks = unsigned(std::min<int64_t>((ntiles + 3) / 4, int64_t(2) * device_cus()));
-  hipLaunchKernelGGL(k_ncomp_thermo_mlp, dim3(blocks), dim3(256), 0, stream_of(s), static_cast<const float*>(in),
+  const unsigned blocks = unsigned(std::min<int64_t>((ntiles + 3) / 4, int64_t(3) * device_cus()));
+  const bool s2 = (net->flags & PPF_MLP_SPLIT2) != 0, s3 = (net->flags & PPF_MLP_SPLIT3) != 0;
+  auto kern = s2 ? (s3 ? k_ncomp_thermo_mlp<true, true> : k_ncomp_thermo_mlp<true, false>)
+                 : (s3 ? k_ncomp_thermo_mlp<false, true> : k_ncomp_thermo_mlp<false, false>);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, stream_of(s), static_cast<const float*>(in),
                      static_cast<const float*>(bcvals), d, float(g->dx), *t, static_cast<float*>(transfer), tg->sy,
-                     tg->sf, k, m, *th, seed, step, ntiles, (net->flags & PPF_MLP_SPLIT3) ? 1 : 0);
+                     tg->sf, k, m, seed, step, ntiles);
+  if (int rc = check_launch("ncomp_thermo_mlp")) return rc;
+  const unsigned fblocks = unsigned(std::min<int64_t>((ncell + 255) / 256, int64_t(8) * device_cus()));
+  hipLaunchKernelGGL(k_ncomp_thermo_fallback, dim3(fblocks), dim3(256), 0, stream_of(s),
+                     static_cast<const float*>(in), static_cast<const float*>(bcvals), d, float(g->dx), *t,
+                     static_cast<float*>(transfer), tg->sy, tg->sf, k, *th, seed, step);
   return check_launch("ncomp_thermo_mlp");
 }
 

@@ -111,7 +111,7 @@ class PhaseNet:
 class SurrogateThermo:
     """Trainable surrogate of a ThermoModel (3 components) on a (T, c) box."""
 
-    def __init__(self, model, T_range, c_lo=0.02, seed=0, device="cuda", split3=True):
+    def __init__(self, model, T_range, c_lo=0.02, seed=0, device="cuda", split2=False, split3=False):
         import torch
         if len(model.components) != 3:
             raise ValueError("the surrogate kernel is built for 3 components")
@@ -129,7 +129,11 @@ class SurrogateThermo:
         self.out_sigma = np.ones((2, 3))
         self.history = []
         self._packed = None
-        # inference: layer-3 inputs as BF16 hi + lo pairs (True) or BF16 only
+        # inference: the inputs of layers 2 / 3 as BF16 hi + lo pairs (True)
+        # or BF16 only (default: BF16 activations add ~10 J/mol of rounding
+        # scatter and halve the MFMA and conversion work; the training GEMMs
+        # always use pairs)
+        self.split2 = bool(split2)
         self.split3 = bool(split3)
 
     # --------------------------------------------------------- samples --
@@ -337,7 +341,8 @@ class SurrogateThermo:
             net.in_lo[q] = float(self.in_lo[q])
             net.in_scale[q] = float(self.in_scale[q])
         net.T_lo, net.T_hi, net.c_lo = self.T_lo, self.T_hi, self.c_lo
-        net.flags = _lib_ext.PPF_MLP_SPLIT3 if self.split3 else 0
+        net.flags = ((_lib_ext.PPF_MLP_SPLIT2 if self.split2 else 0) |
+                     (_lib_ext.PPF_MLP_SPLIT3 if self.split3 else 0))
         self._packed = (net, keep)
         return self._packed
 
