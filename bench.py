@@ -305,7 +305,7 @@ def main():
               "(the domain would shard by slabs with a 1-2 cell halo exchange; not built)", file=sys.stderr)
         sys.exit(2)
     defaults = {"warren": (2000, 100), "diffusion": (1000, 50), "ch3d": (50, 5), "copy": (50, 5),
-                "ncomp": (500, 20), "ensemble": (500, 20)}
+                "ncomp": (500, 20), "ensemble": (5000, 100)}
     st, wu = defaults[args.config]
     args.steps = args.steps if args.steps is not None else st
     args.warmup = args.warmup if args.warmup is not None else wu

@@ -235,6 +235,40 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
                          const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
                          const ppf_mlp_net* net, uint64_t seed, int64_t step, ppf_stream_t s);
 
+/* ---- Batched ensemble (no reference counterpart: the reference runs one
+   Simulation per instance, each stepping with engine_CahnAllenExplicit,
+   Engines/CahnAllen.py:128-135, or engine_CahnHilliardExplicit,
+   Engines/CahnHilliard.py:134-146).  Independent 2D FP32 instances, one
+   persistent launch per step over a host-built tile work-list. ---- */
+#define PPF_ENS_AC 0
+#define PPF_ENS_CH 1
+#define PPF_ENS_XCDS 8
+typedef struct ppf_ens_instance {
+  int32_t kind;            /* PPF_ENS_AC or PPF_ENS_CH */
+  int32_t bc[2];           /* x, y: PPF_PERIODIC or PPF_NEUMANN (zero gradient) */
+  int32_t pad_;
+  int64_t n[2];            /* interior cells along x, y */
+  int64_t pitch;           /* elements per padded row */
+  int64_t offset;          /* element offset of padded cell (0, 0) in both arenas */
+  double M, W, eps, dt, dx; /* Allen-Cahn: M, W, eps; Cahn-Hilliard: M, eps (W unused) */
+} ppf_ens_instance;        /* 88 bytes */
+
+/* number of tile configurations, and the tile shapes of one:
+   shape = {Allen-Cahn TX, TY, Cahn-Hilliard TX, TY} */
+int ppf_ensemble_num_configs(void);
+int ppf_ensemble_config(int config, int32_t* shape);
+/* One explicit step of every instance: out = step(in).  instances: device
+   array of n_instances descriptors; tiles: device array of n_tiles x 4 int32
+   (instance, x0, y0, 0) with x0, y0 the padded coordinates of the tile's
+   first output cell (1 + multiples of the configuration's tile shape for the
+   instance's kind); segments: host array of PPF_ENS_XCDS + 1 tile indices,
+   0 = s[0] <= s[1] <= ... <= s[8] = n_tiles (tiles of XCD k are
+   [s[k], s[k+1])); blocks: persistent grid size.  Halo cells are never read
+   (boundary rules folded into the loads); in and out must not alias. */
+int ppf_ensemble_step(const ppf_ens_instance* instances, int32_t n_instances, const int32_t* tiles,
+                      int32_t n_tiles, const int32_t* segments, int32_t config, int32_t blocks,
+                      const float* in, float* out, ppf_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

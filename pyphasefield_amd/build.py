@@ -33,7 +33,8 @@ UNITS = {
     "ppf_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "ppf_f64.hip": ["-ffp-contract=off"],
     "ppf_mlp.hip": [],
-    "ppf_batched.hip": [],
+    # same arithmetic flags as the single-instance FP32 kernels it must match
+    "ppf_batched.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 
@@ -55,7 +56,10 @@ def _compile(src, extra):
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    units = {k: v for k, v in UNITS.items() if os.path.exists(os.path.join(CSRC, k))}
+    missing = [k for k in UNITS if not os.path.exists(os.path.join(CSRC, k))]
+    if missing:
+        raise FileNotFoundError("missing HIP sources: %s" % ", ".join(missing))
+    units = dict(UNITS)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(units))) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), units.items()))
     tmp = LIB + ".tmp"

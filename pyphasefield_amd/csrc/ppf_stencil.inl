@@ -180,8 +180,9 @@ __device__ __forceinline__ void tile_coords(const Dom& d, int64_t* x0, int64_t* 
   *y0 = 1 + int64_t(b / nbx) * kTY;
 }
 
+template <int TX = kTX, int TY = kTY>
 __device__ __forceinline__ bool tile_interior(const Dom& d, int64_t x0, int64_t y0, int h) {
-  return x0 - h >= 1 && x0 + kTX - 1 + h <= d.ax.n && y0 - h >= 1 && y0 + kTY - 1 + h <= d.ay.n;
+  return x0 - h >= 1 && x0 + TX - 1 + h <= d.ax.n && y0 - h >= 1 && y0 + TY - 1 + h <= d.ay.n;
 }
 
 inline unsigned tiles2d(const ppf_grid& g) {
@@ -206,23 +207,26 @@ __device__ __forceinline__ R point_update(R v, R lap, const PointParams& p) {
   }
 }
 
-template <typename R, int KIND>
-__global__ __launch_bounds__(kThreads) void k_point2d(const R* __restrict__ in, R* __restrict__ out,
-                                                      const R* __restrict__ b, Dom d, R dx, PointParams p) {
-  constexpr int H = 1, W = kTX + 2 * H, HH = kTY + 2 * H;
-  __shared__ R s[HH][W];
-  int64_t x0, y0;
-  tile_coords(d, &x0, &y0);
-  load_tile<R, W, HH>(s, in, b, d, dx, x0 - H, y0 - H, tile_interior(d, x0, y0, H));
+// One TX x TY output tile at padded (x0, y0) of a 2D field: 1-cell halo
+// staged in LDS (s holds (TY + 2) x (TX + 2) values), 256 threads as TX
+// columns x (256 / TX) row groups.
+template <typename R, int KIND, int TX, int TY>
+__device__ __forceinline__ void point_tile(R* smem, const R* __restrict__ in, R* __restrict__ out,
+                                           const R* __restrict__ b, const Dom& d, R dx, const PointParams& p,
+                                           int64_t x0, int64_t y0) {
+  constexpr int H = 1, W = TX + 2 * H, HH = TY + 2 * H, RP = kThreads / TX;
+  static_assert(kThreads % TX == 0 && TY % RP == 0, "tile shape");
+  R(*s)[W] = reinterpret_cast<R(*)[W]>(smem);
+  load_tile<R, W, HH>(s, in, b, d, dx, x0 - H, y0 - H, tile_interior<TX, TY>(d, x0, y0, H));
   __syncthreads();
   const R idx = R(1) / dx;
   const R idx2 = idx * idx;
-  int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+  int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   int64_t x = x0 + tx;
   if (x > d.ax.n) return;
 #pragma unroll
-  for (int k = 0; k < kTY / 4; ++k) {
-    int ly = ty + 4 * k;
+  for (int k = 0; k < TY / RP; ++k) {
+    int ly = ty + RP * k;
     int64_t y = y0 + ly;
     if (y > d.ay.n) break;
     R c = s[ly + 1][tx + 1];
@@ -231,6 +235,15 @@ __global__ __launch_bounds__(kThreads) void k_point2d(const R* __restrict__ in, 
     lap += (s[ly + 1][tx + 2] + s[ly + 1][tx] - R(2) * c) * idx2;
     out[y * d.ay.stride + x] = point_update<R, KIND>(c, lap, p);
   }
+}
+
+template <typename R, int KIND>
+__global__ __launch_bounds__(kThreads) void k_point2d(const R* __restrict__ in, R* __restrict__ out,
+                                                      const R* __restrict__ b, Dom d, R dx, PointParams p) {
+  __shared__ R s[(kTY + 2) * (kTX + 2)];
+  int64_t x0, y0;
+  tile_coords(d, &x0, &y0);
+  point_tile<R, KIND, kTX, kTY>(s, in, out, b, d, dx, p, x0, y0);
 }
 
 // generic 1D / 3D version (one cell per thread, folded global loads)
@@ -294,16 +307,18 @@ __device__ __forceinline__ R ch_mu(R c, R lap, R eps2) {
   return -eps2 * lap + (R(4) * c * c * c - R(6) * c * c + R(2) * c);
 }
 
-template <typename R>
-__global__ __launch_bounds__(kThreads) void k_ch2d(const R* __restrict__ in, R* __restrict__ out,
-                                                   const R* __restrict__ b, Dom d, R dx, R eps, R dtM) {
-  constexpr int H = 2, W = kTX + 2 * H, HH = kTY + 2 * H;
-  constexpr int MW = kTX + 2, MH = kTY + 2;
-  __shared__ R s[HH][W];
-  __shared__ R mu[MH][MW];
-  int64_t x0, y0;
-  tile_coords(d, &x0, &y0);
-  load_tile<R, W, HH>(s, in, b, d, dx, x0 - H, y0 - H, tile_interior(d, x0, y0, H));
+// One TX x TY Cahn-Hilliard output tile: 2-cell halo of c in LDS, mu on the
+// tile + 1 ring in LDS (smem holds (TY + 4)(TX + 4) + (TY + 2)(TX + 2) values).
+template <typename R, int TX, int TY>
+__device__ __forceinline__ void ch_tile(R* smem, const R* __restrict__ in, R* __restrict__ out,
+                                        const R* __restrict__ b, const Dom& d, R dx, R eps, R dtM, int64_t x0,
+                                        int64_t y0) {
+  constexpr int H = 2, W = TX + 2 * H, HH = TY + 2 * H, RP = kThreads / TX;
+  constexpr int MW = TX + 2, MH = TY + 2;
+  static_assert(kThreads % TX == 0 && TY % RP == 0, "tile shape");
+  R(*s)[W] = reinterpret_cast<R(*)[W]>(smem);
+  R(*mu)[MW] = reinterpret_cast<R(*)[MW]>(smem + W * HH);
+  load_tile<R, W, HH>(s, in, b, d, dx, x0 - H, y0 - H, tile_interior<TX, TY>(d, x0, y0, H));
   __syncthreads();
   const R idx = R(1) / dx;
   const R idx2 = idx * idx;
@@ -333,12 +348,12 @@ __global__ __launch_bounds__(kThreads) void k_ch2d(const R* __restrict__ in, R* 
     }
   }
   __syncthreads();
-  int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+  int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   int64_t x = x0 + tx;
   if (x > d.ax.n) return;
 #pragma unroll
-  for (int k = 0; k < kTY / 4; ++k) {
-    int ly = ty + 4 * k;
+  for (int k = 0; k < TY / RP; ++k) {
+    int ly = ty + RP * k;
     int64_t y = y0 + ly;
     if (y > d.ay.n) break;
     R m = mu[ly + 1][tx + 1];
@@ -348,6 +363,15 @@ __global__ __launch_bounds__(kThreads) void k_ch2d(const R* __restrict__ in, R* 
     dm = dm / (dx * dx);
     out[y * d.ay.stride + x] = s[ly + 2][tx + 2] + dtM * dm;
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_ch2d(const R* __restrict__ in, R* __restrict__ out,
+                                                   const R* __restrict__ b, Dom d, R dx, R eps, R dtM) {
+  __shared__ R s[(kTY + 4) * (kTX + 4) + (kTY + 2) * (kTX + 2)];
+  int64_t x0, y0;
+  tile_coords(d, &x0, &y0);
+  ch_tile<R, kTX, kTY>(s, in, out, b, d, dx, eps, dtM, x0, y0);
 }
 
 // generic 1D / 3D version: mu at the centre and its 2*ndim neighbours

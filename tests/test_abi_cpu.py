@@ -45,6 +45,10 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.NcompParams) == 8 + 15 * 8 + 6 * 4 * 8
     assert ctypes.sizeof(_lib.RKPhase) == 8 + 8 + 4 * 5 * 8 + 6 * 3 * 2 * 8
     assert ctypes.sizeof(_lib.Thermo) == 2 * ctypes.sizeof(_lib.RKPhase) + 8
+    from pyphasefield_amd.ensemble import EnsInstance
+    assert ctypes.sizeof(EnsInstance) == 88
+    from pyphasefield_amd._lib_ext import MlpNet
+    assert ctypes.sizeof(MlpNet) == 2 * 8 + 2 * 8 + 6 * 4 + 12 * 4 + 4 * 4
 
 
 def test_invalid_arguments_are_rejected_without_gpu_work():
