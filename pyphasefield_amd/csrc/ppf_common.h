@@ -14,6 +14,8 @@ namespace ppf {
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_launch(const char* what);
+// compute units of the current device (cached per device id)
+int device_cus();
 int check_grid(const ppf_grid* g, int want_ndim_min, int want_ndim_max, const char* what);
 
 inline hipStream_t stream_of(ppf_stream_t s) { return reinterpret_cast<hipStream_t>(s); }

@@ -497,21 +497,6 @@ __global__ __launch_bounds__(256) void k_ncomp_thermo_fallback(const float* __re
 
 using namespace ppf;
 
-namespace {
-// compute units of the current device (cached per device id)
-int device_cus() {
-  static int cached[64] = {};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cached[dev] == 0) {
-    int n = 0;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-    cached[dev] = n;
-  }
-  return cached[dev];
-}
-}  // namespace
-
 extern "C" {
 
 int ppf_mlp_gemm(int M, int N, int K, const float* A, int64_t sai, int64_t sak, const float* B, int64_t sbk,

@@ -434,6 +434,218 @@ __global__ __launch_bounds__(256) void k_ch_nd(const R* __restrict__ in, R* __re
   out[o] = in[o] + dtM * dm;
 }
 
+// 3D Cahn-Hilliard, marching in z.  A block owns a C3X x C3Y column of
+// outputs and a chunk of planes; planes of c (2-cell xy halo) and of mu (1-cell
+// xy ring) rotate through LDS rings, so every c value is read from memory once
+// per block (plus the xy halo and a few extra planes per chunk).  Iteration k
+// of the march: store c(k+1) (fetched into registers during iteration k-1;
+// halo cells outside the domain were written to LDS directly) and start
+// fetching c(k+2); barrier; mu(k) on the ring from c(k-1..k+1);
+// barrier; output plane k-1 from mu(k-2..k) and c(k-1).  Five c slots and four
+// mu slots keep every slot written in an iteration clear of every slot read
+// in it or the one before, so two barriers per plane suffice.  Zero flux on
+// non-periodic sides: a mu neighbour outside the domain is replaced by the
+// centre value (as k_ch_nd and the oracle).
+#ifndef PPF_CH3D_TY
+#define PPF_CH3D_TY 8
+#endif
+constexpr int C3X = 64;
+template <int C3Y>
+struct Ch3dShape {
+  static constexpr int W = C3X + 4, H = C3Y + 4, N = W * H;      // c plane, 2-cell halo
+  static constexpr int MW = C3X + 2, MH = C3Y + 2, MN = MW * MH;  // mu plane, tile + 1 ring
+  static constexpr int LOADS = (N + kThreads - 1) / kThreads;
+  static constexpr int MUS = (MN + kThreads - 1) / kThreads;
+  static constexpr size_t lds(size_t r) { return r * (5 * N + 4 * MN); }
+};
+
+// Per-thread constants of the march: the c-plane elements this thread
+// fetches (in-plane offset, whether the (x, y) position is inside the domain)
+// and the mu ring positions it evaluates.
+template <int C3Y>
+struct Ch3dLanes {
+  int32_t off[Ch3dShape<C3Y>::LOADS];  // y * sy + x of fetched element j
+  uint32_t xy_in;        // bit j: element j inside the domain in x and y
+  uint32_t valid;        // bit j: element j exists (t < C3N)
+};
+
+// this thread's share of c plane z: elements inside the domain into
+// registers (bit j of the returned mask), the halo cells outside it straight
+// into the plane's LDS slot through the boundary rules (one rolled loop, so
+// the fold code is not replicated)
+template <typename R, int C3Y>
+__device__ __forceinline__ uint32_t ch3d_fetch(R (&v)[Ch3dShape<C3Y>::LOADS], R* __restrict__ slot,
+                                               const R* __restrict__ in, const R* __restrict__ b, const Dom& d,
+                                               R dx, int64_t x0, int64_t y0, int64_t z,
+                                               const Ch3dLanes<C3Y>& L) {
+  constexpr int C3LOADS = Ch3dShape<C3Y>::LOADS, C3N = Ch3dShape<C3Y>::N, C3W = Ch3dShape<C3Y>::W;
+  const uint32_t mask = (z >= 1 && z <= d.az.n) ? L.xy_in : 0u;
+  const R* plane = in + z * d.az.stride;
+#pragma unroll
+  for (int j = 0; j < C3LOADS; ++j)
+    if ((mask >> j) & 1u) v[j] = plane[L.off[j]];
+  if (mask != L.valid) {
+#pragma unroll 1
+    for (int j = 0; j < C3LOADS; ++j) {
+      const int t = threadIdx.x + j * kThreads;
+      if (t >= C3N || (mask >> j) & 1u) continue;
+      const int ly = t / C3W, lx = t - ly * C3W;
+      const int64_t x = x0 - 2 + lx, y = y0 - 2 + ly;
+      // beyond a partial tile: never used
+      slot[t] = (x > d.ax.n + 2 || y > d.ay.n + 2) ? R(0) : fold(in, b, d, x, y, z, dx);
+    }
+  }
+  return mask;
+}
+
+template <typename R, int C3Y>
+__device__ __forceinline__ void ch3d_store(R* __restrict__ dst, const R (&v)[Ch3dShape<C3Y>::LOADS],
+                                           uint32_t mask) {
+  constexpr int C3LOADS = Ch3dShape<C3Y>::LOADS;
+#pragma unroll
+  for (int j = 0; j < C3LOADS; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    if ((mask >> j) & 1u) dst[t] = v[j];
+  }
+}
+
+template <typename R, int C3Y>
+__global__ __launch_bounds__(kThreads) void k_ch3d(const R* __restrict__ in, R* __restrict__ out,
+                                                   const R* __restrict__ b, Dom d, R dx, R eps, R dtM,
+                                                   int32_t zchunk) {
+  using S = Ch3dShape<C3Y>;
+  constexpr int C3W = S::W, C3N = S::N, C3MW = S::MW, C3MN = S::MN, C3LOADS = S::LOADS, C3MUS = S::MUS;
+  extern __shared__ __align__(16) unsigned char ch3d_lds[];
+  R(*sc)[C3N] = reinterpret_cast<R(*)[C3N]>(ch3d_lds);
+  R(*sm)[C3MN] = reinterpret_cast<R(*)[C3MN]>(ch3d_lds + 5 * C3N * sizeof(R));
+  const uint32_t nbx = uint32_t((d.ax.n + C3X - 1) / C3X), nby = uint32_t((d.ay.n + C3Y - 1) / C3Y);
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t bxy = blk % (nbx * nby), bz = blk / (nbx * nby);
+  const int64_t x0 = 1 + int64_t(bxy % nbx) * C3X, y0 = 1 + int64_t(bxy / nbx) * C3Y;
+  const int64_t zb = 1 + int64_t(bz) * zchunk;
+  const int64_t ze = zb + zchunk - 1 < d.az.n ? zb + zchunk - 1 : d.az.n;
+  const R idx = R(1) / dx;
+  const R idx2 = idx * idx;
+  const R eps2 = eps * eps;
+  const bool zflux = d.az.bc != PPF_PERIODIC, xflux = d.ax.bc != PPF_PERIODIC, yflux = d.ay.bc != PPF_PERIODIC;
+  Ch3dLanes<C3Y> L;
+  L.xy_in = 0;
+  L.valid = 0;
+#pragma unroll
+  for (int j = 0; j < C3LOADS; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    const int ly = t / C3W, lx = t - ly * C3W;
+    const int64_t x = x0 - 2 + lx, y = y0 - 2 + ly;
+    const bool in_xy = t < C3N && x >= 1 && x <= d.ax.n && y >= 1 && y <= d.ay.n;
+    L.off[j] = in_xy ? int32_t(y * d.ay.stride + x) : 0;
+    L.xy_in |= in_xy ? 1u << j : 0u;
+    L.valid |= t < C3N ? 1u << j : 0u;
+  }
+  int32_t mu_c[C3MUS];  // c-plane index of each mu ring position of this thread (-1: none)
+#pragma unroll
+  for (int j = 0; j < C3MUS; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    const int my = t / C3MW, mx = t - my * C3MW;
+    mu_c[j] = t < C3MN ? (my + 1) * C3W + (mx + 1) : -1;
+  }
+  // LDS ring positions (rotated once per plane; plain 32-bit scalars)
+  int c_m = 0, c_0 = 1, c_p = 2, c_n = 3, c_f = 4;  // c(k-1), c(k), c(k+1), c(k+2), free
+  int m_mm = 0, m_m = 1, m_0 = 2, m_f = 3;           // mu(k-2), mu(k-1), mu(k), free
+  const int32_t zb32 = int32_t(zb), ze32 = int32_t(ze), nz32 = int32_t(d.az.n);
+  R v[C3LOADS];
+  // c(zb-2) -> c_m, c(zb-1) -> c_0, c(zb) in flight towards c_p
+  uint32_t mask = ch3d_fetch<R, C3Y>(v, sc[c_m], in, b, d, dx, x0, y0, zb - 2, L);
+  ch3d_store<R, C3Y>(sc[c_m], v, mask);
+  mask = ch3d_fetch<R, C3Y>(v, sc[c_0], in, b, d, dx, x0, y0, zb - 1, L);
+  ch3d_store<R, C3Y>(sc[c_0], v, mask);
+  mask = ch3d_fetch<R, C3Y>(v, sc[c_p], in, b, d, dx, x0, y0, zb, L);
+  const int tx = threadIdx.x % C3X, ty = threadIdx.x / C3X;
+  const int64_t x = x0 + tx;
+  const bool x_in = x <= d.ax.n;
+  const bool xlo = xflux && x == 1, xhi = xflux && x == d.ax.n;
+  for (int32_t k = zb32 - 1; k <= ze32 + 1; ++k) {
+    // iteration k: mu(k) and output plane k - 1.  c(k+1) -> c_p from
+    // registers; c(k+2) is fetched into registers (its halo cells straight into
+    // slot c_n, which last held c(k-3), read for the last time in iteration k-2)
+    ch3d_store<R, C3Y>(sc[c_p], v, mask);
+    if (k + 2 <= ze32 + 2) mask = ch3d_fetch<R, C3Y>(v, sc[c_n], in, b, d, dx, x0, y0, int64_t(k) + 2, L);
+    __syncthreads();
+    // mu(k) into slot m_0 on the tile + 1 ring (corners are never read)
+    {
+      const R* cm = sc[c_m];
+      const R* c0 = sc[c_0];
+      const R* cp = sc[c_p];
+      R* mu = sm[m_0];
+#pragma unroll
+      for (int j = 0; j < C3MUS; ++j) {
+        const int i = mu_c[j];
+        if (i < 0) break;
+        const R c = c0[i];
+        // axis 0 (z), then y, then x
+        R lap = (cp[i] + cm[i] - R(2) * c) * idx2;
+        lap += (c0[i + C3W] + c0[i - C3W] - R(2) * c) * idx2;
+        lap += (c0[i + 1] + c0[i - 1] - R(2) * c) * idx2;
+        mu[threadIdx.x + j * kThreads] = ch_mu(c, lap, eps2);
+      }
+    }
+    __syncthreads();
+    const int32_t z = k - 1;  // output plane, from mu(k-2) = m_mm, mu(k-1) = m_m, mu(k) = m_0, c(k-1) = c_m
+    if (z >= zb32 && x_in) {
+      const R* mm = sm[m_mm];
+      const R* m0 = sm[m_m];
+      const R* mp = sm[m_0];
+      const R* c0 = sc[c_m];
+      const bool zlo = zflux && z == 1, zhi = zflux && z == nz32;
+      R* oz = out + int64_t(z) * d.az.stride;
+#pragma unroll
+      for (int r = 0; r < C3Y / 4; ++r) {
+        const int ly = ty + 4 * r;
+        const int64_t y = y0 + ly;
+        if (y > d.ay.n) break;
+        const int i = (ly + 1) * C3MW + (tx + 1);
+        const R m = m0[i];
+        const R mzm = zlo ? m : mm[i];
+        const R mzp = zhi ? m : mp[i];
+        const R mym = (yflux && y == 1) ? m : m0[i - C3MW];
+        const R myp = (yflux && y == d.ay.n) ? m : m0[i + C3MW];
+        const R mxm = xlo ? m : m0[i - 1];
+        const R mxp = xhi ? m : m0[i + 1];
+        R dm = mzm + mzp - R(2) * m;
+        dm = dm + (mym + myp - R(2) * m);
+        dm = dm + (mxm + mxp - R(2) * m);
+        dm = dm / (dx * dx);
+        oz[y * d.ay.stride + x] = c0[(ly + 2) * C3W + (tx + 2)] + dtM * dm;
+      }
+    }
+    // rotate: c(k) becomes c(k-1) of the next iteration, ...
+    const int c_old = c_m;
+    c_m = c_0;
+    c_0 = c_p;
+    c_p = c_n;
+    c_n = c_f;
+    c_f = c_old;
+    const int m_old = m_mm;
+    m_mm = m_m;
+    m_m = m_0;
+    m_0 = m_f;
+    m_f = m_old;
+  }
+}
+
+// z chunk: about PPF_CH3D_BLOCKS blocks (several rounds of resident blocks, so
+// that blocks finishing early are replaced), at least 16 planes per chunk
+#ifndef PPF_CH3D_BLOCKS
+#define PPF_CH3D_BLOCKS 8192
+#endif
+inline int32_t ch3d_zchunk(const ppf_grid& g, int c3y) {
+  const int64_t xy = ((g.n[0] + C3X - 1) / C3X) * ((g.n[1] + c3y - 1) / c3y);
+  int64_t chunks = (PPF_CH3D_BLOCKS + xy - 1) / xy;
+  int64_t zc = (g.n[2] + chunks - 1) / chunks;
+  if (zc < 16) zc = 16;
+  if (zc > g.n[2]) zc = g.n[2];
+  return int32_t(zc);
+}
+
 template <typename R>
 int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double eps,
                       double dt, hipStream_t s) {
@@ -442,6 +654,13 @@ int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* 
   if (g.ndim == 2) {
     hipLaunchKernelGGL(k_ch2d<R>, dim3(tiles2d(g)), dim3(kThreads), 0, s, static_cast<const R*>(in),
                        static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), R(eps), dtM);
+  } else if (g.ndim == 3 && g.sz < (int64_t(1) << 31)) {  // in-plane offsets are 32-bit
+    constexpr int Y = sizeof(R) == 8 ? 8 : PPF_CH3D_TY;  // FP64 tiles stay under 64 KB of LDS
+    const size_t lds = Ch3dShape<Y>::lds(sizeof(R));
+    const int32_t zc = ch3d_zchunk(g, Y);
+    const int64_t blocks = ((g.n[0] + C3X - 1) / C3X) * ((g.n[1] + Y - 1) / Y) * ((g.n[2] + zc - 1) / zc);
+    hipLaunchKernelGGL((k_ch3d<R, Y>), dim3(unsigned(blocks)), dim3(kThreads), lds, s, static_cast<const R*>(in),
+                       static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), R(eps), dtM, zc);
   } else {
     int64_t total = interior_cells(g);
     hipLaunchKernelGGL(k_ch_nd<R>, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s,

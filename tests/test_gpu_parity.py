@@ -182,6 +182,26 @@ def test_cahnhilliard_2d_fp64(gpu, bc):
         assert abs(sim.field_sums()[0] - c0.sum()) <= 1e-10 * c0.sum()
 
 
+@pytest.mark.parametrize("bc", ["PERIODIC", "NEUMANN", ["NEUMANN", "PERIODIC", "NEUMANN"],
+                                ["PERIODIC", "DIRCHLET", "NEUMANN"]])
+def test_cahnhilliard_3d_marching_kernel_fp64(gpu, bc):
+    """z-marching 3D kernel: partial xy tiles, several z chunks (the last one
+    partial), every boundary kind on some axis."""
+    E = _engines()
+    shape = [40, 20, 70]
+    sim = E.CahnHilliard(dimensions=shape, seed=3)
+    sim.set_user_data({"M": 0.8, "epsilon": 1.2})
+    _setup(sim, bc, 1.0, 0.004, "FP64")
+    rng = np.random.default_rng(10)
+    c0 = 0.5 + 0.05 * (2 * rng.random(shape) - 1)
+    sim.fields[0].data[1:-1, 1:-1, 1:-1] = c0
+    sim.simulate(25)
+    want = run_point("cahnhilliard", c0, 25, bc, 1.0, 0.004, M=0.8, epsilon=1.2)
+    assert rel_l2(sim.fields[0].get_cells(), want[1:-1, 1:-1, 1:-1]) <= FP64_TOL
+    if bc in ("PERIODIC", "NEUMANN"):
+        assert abs(sim.field_sums()[0] - c0.sum()) <= 1e-10 * c0.sum()
+
+
 def test_cahnhilliard_3d_fp64_and_reductions(gpu):
     E = _engines()
     shape = [10, 12, 14]
