@@ -295,6 +295,11 @@ def main():
     ap.add_argument("--variant", default="fp32")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ens-config", type=int, default=None,
+                    help="ensemble: tile configuration (skips the autotuner)")
+    ap.add_argument("--ens-blocks", type=int, default=None, help="ensemble: persistent grid size")
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="ensemble: steps per captured HIP graph (0: eager launches)")
     ap.add_argument("--mlp-split", default="none",
                     help="ncomp surrogate: layers (2, 3) whose inputs enter as BF16 hi + lo pairs")
     ap.add_argument("--surrogate", default=None,

@@ -387,8 +387,15 @@ def bench_ensemble(args, _timed, _line, _roofline):
     n = args.size or 256
     specs = random_specs(n, seed=0)
     ens = Ensemble(specs, random_initial(specs, seed=0))
-    best = ens.autotune(steps=20, cache=False)
-    gs = 50 if args.steps % 50 == 0 else 2
+    if getattr(args, "ens_config", None) is not None:
+        ens.config = int(args.ens_config)
+        ens.blocks = int(args.ens_blocks or 8 * ens.cus)
+        best = {"config": ens.config, "blocks": ens.blocks, "shape": list(ens.shape(ens.config))}
+    else:
+        best = ens.autotune(steps=20, cache=False)
+    gs = getattr(args, "graph_steps", None)
+    if gs is None:
+        gs = 50 if args.steps % 50 == 0 else 2
     ens.simulate(args.warmup - args.warmup % 2 if args.warmup > 1 else 0, graph_steps=gs)
     t = _timed(lambda k: ens.simulate(k, graph_steps=gs), args.steps)
     roof = _roofline(8.0 * ens.cells, t / args.steps, "k_ensemble")
