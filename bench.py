@@ -163,6 +163,26 @@ def bench_diffusion(args):
 
 
 # ------------------------------------------------------------ CH 3D 512^3 --
+def cpu_ch3d(n=256, k=2):
+    """FP64 numpy restatement of the Cahn-Hilliard step (mu with its periodic
+    2-cell ring, then the Laplacian of mu), n^3 sample, 1 warm-up + k steps."""
+    from oracle import bc as obc
+    from oracle import engines as oe
+    rng = np.random.default_rng(7)
+    a = obc.pad(0.5 + 0.01 * (2 * rng.random((n, n, n)) - 1))
+    obc.apply_bc(a[None], "PERIODIC")
+    t0 = 0.0
+    for s in range(k + 1):
+        if s == 1:
+            t0 = time.perf_counter()
+        a = oe.cahnhilliard_step(a, 1.0, 1.0, 0.01, 1.0, "PERIODIC")
+        obc.apply_bc(a[None], "PERIODIC")
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * k / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d timed steps (+1 warm-up) of the FP64 numpy Cahn-Hilliard restatement at %d^3 "
+                      "(the benchmark grid is 512^3), single-threaded numpy" % (k, n)}
+
+
 def bench_ch3d(args):
     import pyphasefield_amd.Engines as engines
     n = args.size or 512
@@ -180,7 +200,8 @@ def bench_ch3d(args):
     mass = sim.field_sums()[0]
     line = _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f32",
                  {"workload": "Cahn-Hilliard 3D %d^3 FP32 periodic, M=1 kappa=1 dt=0.01" % n,
-                  "engine": "CahnHilliard", "grid": [n, n, n], "bytes_per_cell_update": 8}, roof, None)
+                  "engine": "CahnHilliard", "grid": [n, n, n], "bytes_per_cell_update": 8}, roof,
+                 None if args.no_cpu else cpu_ch3d())
     line["fp64_mass"] = mass
     line["fp64_free_energy"] = sim.free_energy()
     return line
@@ -222,6 +243,35 @@ def ncomp_sim(n, precision, dg=None, seeds=16, noise=0.0, thermo_eval=None, surr
     return sim
 
 
+def cpu_ncomp(n=512, k=3):
+    """FP64 numpy restatement of NComponent_helper_kernel + NComponent_kernel
+    on an n x n sample of the same workload, 1 warm-up + k steps."""
+    from oracle import bc as obc
+    from oracle import engines as oe
+    sim = ncomp_sim(n, "FP64", "forward_difference")
+    sim._pack_params()
+    u = sim.user_data
+    model = sim._tdb
+    liq = (model.liquid.g0, model.liquid.L, model.liquid.R)
+    sol = (model.solid.g0, model.solid.L, model.solid.R)
+    f = np.stack([obc.pad(fl.get_cells()) for fl in sim.fields])
+    obc.apply_bc(f, "NEUMANN", None, sim.dx)
+    T = obc.pad(oe.temperature_linear((n, n), NC_T0, 1.0 / NC_DX, 0.0, sim.dx, 0.0, sim.dt, 0))
+    obc.apply_bc_single(T, "NEUMANN")
+    t0 = 0.0
+    for s in range(k + 1):
+        if s == 1:
+            t0 = time.perf_counter()
+        tr = oe.ncomp_helper(f, T, u["params"], u["c_params"], liq, sol, seed=sim.seed, step=s + 1)
+        f = oe.ncomp_main(f, T, tr, u["params"], u["c_params"], seed=sim.seed, step=s + 1)
+        obc.apply_bc(f, "NEUMANN", None, sim.dx)
+    dt = time.perf_counter() - t0
+    return {"value": n * n * k / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d timed steps (+1 warm-up) of the FP64 numpy restatement of NComponent_helper_kernel + "
+                      "NComponent_kernel (forward-difference thermodynamics) on a %dx%d sample of the same "
+                      "workload (the benchmark grid is 2048x2048), single-threaded numpy" % (k, n, n)}
+
+
 def bench_ncomp(args):
     import torch
     n = args.size or 2048
@@ -256,7 +306,8 @@ def bench_ncomp(args):
                  "f64" if precision == "FP64" else "f32",
                  {"workload": "N-component 2D %dx%d, 3 components (AL, CU, NI), %s" % (n, n, variant),
                   "engine": "NCGPU_new", "grid": [n, n], "fields": nf, "bytes_per_cell_update": bytes_cell,
-                  "thermo": sim.user_data.get("thermo_eval", "plain") + "/" + dg}, roof, None)
+                  "thermo": sim.user_data.get("thermo_eval", "plain") + "/" + dg}, roof,
+                 None if args.no_cpu else cpu_ncomp())
     return line
 
 

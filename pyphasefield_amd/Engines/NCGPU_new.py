@@ -97,8 +97,9 @@ class NCGPU_new(Simulation):
             for i in range(ncomp - 1):
                 self.add_field(np.zeros(dim) + 1. / ncomp, "c_" + self._tdb_components[i])
 
-    def just_before_simulating(self):
-        super().just_before_simulating()
+    def _pack_params(self):
+        """user_data["params"] / ["c_params"] in the order of NCGPU_new.py:497-517
+        and the matching ppf_ncomp_params (host only)."""
         u = self.user_data
         for k in ("noise_phi", "noise_c", "noise_q"):
             u.setdefault(k, 1.)
@@ -123,6 +124,12 @@ class NCGPU_new(Simulation):
                                 u["noise_q"]])
         u["c_params"] = np.array([u["L"], u["T_M"], u["S"], u["B"], u["W"], u["M"]])
         self._ncparams = p
+
+    def just_before_simulating(self):
+        super().just_before_simulating()
+        self._pack_params()
+        u = self.user_data
+        ncomp = len(self._tdb_components)
         mode = u.get("dg_mode", "forward_difference" if self._precision == "FP64" else "analytic")
         if mode not in ("forward_difference", "analytic"):
             raise ValueError("dg_mode must be 'forward_difference' or 'analytic'")
