@@ -225,6 +225,9 @@ typedef struct ppf_mlp_net {
   float T_lo, T_hi, c_lo;  /* range gate: cells outside use the plain FP64 evaluation */
   int32_t flags;           /* PPF_MLP_SPLIT2 / PPF_MLP_SPLIT3: the inputs of layer 2 / 3 enter as
                               BF16 hi + lo pairs (else BF16 only) */
+  int32_t* marks;          /* optional device int32[2], zero-initialised once: counts of cells
+                              handed to the FP64 fallback (lets it skip its scan when none are);
+                              NULL: the fallback always scans */
 } ppf_mlp_net;
 #define PPF_MLP_SPLIT3 1
 #define PPF_MLP_SPLIT2 2

@@ -179,6 +179,7 @@ struct MlpK {
   float out_mu[2][3], out_sigma[2][3];
   float T_lo, T_hi, c_lo;             // range gate: outside, the plain FP64 evaluation is used
   float R;                            // gas constant of the ideal-mixing term
+  int32_t* marks;                     // [2] cells marked for the fallback, by step parity (may be null)
 };
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
@@ -272,12 +273,11 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
   for (int m = 0; m < 2; ++m) {
     const f32x16 bt = bias_tile(b + 64, 32 * m, u);
 #pragma unroll
-    for (int g = 0; g < G; ++g) h[g][m] = bt;
-#pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
       const bf16x8 a = w[(2 + 4 * m + kb) * 64 + lane];
+      // the bias tile enters as the first accumulator operand (no copy)
 #pragma unroll
-      for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bh[g][kb], h[g][m]);
+      for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bh[g][kb], kb == 0 ? bt : h[g][m]);
       if (S2) {
 #pragma unroll
         for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bl[g][kb], h[g][m]);
@@ -287,30 +287,22 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
 #pragma unroll
   for (int g = 0; g < G; ++g) pack_layer<S3>(h[g], bh[g], bl[g]);
   f32x16 y[G];
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-#pragma unroll
-    for (int v = 0; v < 16; ++v) y[g][v] = 0.f;
-    if (u == 0) {
-      y[g][0] = b[128];
-      y[g][1] = b[129];
-      y[g][2] = b[130];
-    }
-  }
+  const f32x16 zero = {};
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
     const bf16x8 a = w[(10 + kb) * 64 + lane];
 #pragma unroll
-    for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], y[g]);
+    for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], kb == 0 ? zero : y[g]);
     if (S3) {
 #pragma unroll
       for (int g = 0; g < G; ++g) y[g] = mfma16(a, bl[g][kb], y[g]);
     }
   }
+  // output bias on the three used rows (valid in lanes 0..31)
 #pragma unroll
   for (int g = 0; g < G; ++g)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) out[g][q] = y[g][q];
+    for (int q = 0; q < 3; ++q) out[g][q] = y[g][q] + b[128 + q];
 }
 
 struct Plain5 {
@@ -378,8 +370,12 @@ __device__ __forceinline__ void write_transfer(float* __restrict__ tr, uint32_t 
 // MFMA group 0 is the tile's first 32 cells, group 1 the next 32 (their inputs
 // moved to lanes 0..31 by a shuffle), so the weight operands are read once per
 // 64 cells.
+// waves per SIMD the register budget is set for (3: no spills at 153 VGPRs)
+#ifndef PPF_MLP_WAVES
+#define PPF_MLP_WAVES 3
+#endif
 template <bool S2, bool S3>
-__global__ __launch_bounds__(256, 3) void k_ncomp_thermo_mlp(const float* __restrict__ in,
+__global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(const float* __restrict__ in,
                                                              const float* __restrict__ b, Dom d, float dx,
                                                              ppf_temperature t, float* __restrict__ tr, int64_t tsy,
                                                              int64_t tsf, NcMlpK k, MlpK net, uint64_t seed,
@@ -451,6 +447,7 @@ __global__ __launch_bounds__(256, 3) void k_ncomp_thermo_mlp(const float* __rest
     const uint32_t o = y * tsy32 + x;
     if (gate) {  // outside the trained box: marked for k_ncomp_thermo_fallback
       tr[o] = __builtin_nanf("");
+      if (net.marks) atomicAdd(&net.marks[step & 1], 1);
       continue;
     }
     float dG, dGL[N - 1], dGS[N - 1];
@@ -477,10 +474,18 @@ __global__ __launch_bounds__(256) void k_ncomp_thermo_fallback(const float* __re
                                                                const float* __restrict__ b, Dom d, float dx,
                                                                ppf_temperature t, float* __restrict__ tr,
                                                                int64_t tsy, int64_t tsf, NcMlpK k, ppf_thermo th,
-                                                               uint64_t seed, int64_t step) {
+                                                               const int32_t* marks_in, uint64_t seed,
+                                                               int64_t step) {
+  int32_t* marks = const_cast<int32_t*>(marks_in);
   const uint32_t px = uint32_t(d.ax.n + 2), py = uint32_t(d.ay.n + 2), ncell = px * py;
   const int64_t n3[3] = {d.ax.n, d.ay.n, 1};
   const int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
+  if (marks) {
+    // this step's count (zero: nothing to recompute); the next step's
+    // counter is cleared here, before the next surrogate pass runs
+    if (blockIdx.x == 0 && threadIdx.x == 0) marks[(step + 1) & 1] = 0;
+    if (marks[step & 1] == 0) return;
+  }
   const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
   if (cell < ncell) {
     const uint32_t y = cell / px, x = cell - y * px;
@@ -589,11 +594,12 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
   m.T_hi = net->T_hi;
   m.c_lo = net->c_lo;
   m.R = float(th->liquid.R);
+  m.marks = net->marks;
   const int64_t ncell = (g->n[0] + 2) * (g->n[1] + 2);
   if (ncell >= (int64_t(1) << 31) || tg->sf * 5 >= (int64_t(1) << 32) || g->sf * 5 >= (int64_t(1) << 32))
     return fail(PPF_EINVAL, "ncomp_thermo_mlp: grid too large for 32-bit offsets");
   const uint32_t ntiles = uint32_t((ncell + 63) / 64);
-  const unsigned blocks = unsigned(std::min<int64_t>((ntiles + 3) / 4, int64_t(3) * device_cus()));
+  const unsigned blocks = unsigned(std::min<int64_t>((ntiles + 3) / 4, int64_t(PPF_MLP_WAVES) * device_cus()));
   const bool s2 = (net->flags & PPF_MLP_SPLIT2) != 0, s3 = (net->flags & PPF_MLP_SPLIT3) != 0;
   auto kern = s2 ? (s3 ? k_ncomp_thermo_mlp<true, true> : k_ncomp_thermo_mlp<true, false>)
                  : (s3 ? k_ncomp_thermo_mlp<false, true> : k_ncomp_thermo_mlp<false, false>);
@@ -604,7 +610,7 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
   const unsigned fblocks = unsigned((ncell + 255) / 256);
   hipLaunchKernelGGL(k_ncomp_thermo_fallback, dim3(fblocks), dim3(256), 0, stream_of(s),
                      static_cast<const float*>(in), static_cast<const float*>(bcvals), d, float(g->dx), *t,
-                     static_cast<float*>(transfer), tg->sy, tg->sf, k, *th, seed, step);
+                     static_cast<float*>(transfer), tg->sy, tg->sf, k, *th, net->marks, seed, step);
   return check_launch("ncomp_thermo_mlp");
 }
 

@@ -294,18 +294,58 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
   int64_t x0 = 1 + int64_t(blk % nbx) * kTX;
   int64_t y0 = 1 + int64_t(blk / nbx) * kNcTY;
   bool interior = x0 - 1 >= 1 && x0 + kTX <= d.ax.n && y0 - 1 >= 1 && y0 + kNcTY <= d.ay.n;
-#pragma unroll
-  for (int f = 0; f < NF; ++f)
-    load_tile<R, W, HH>(sf[f], in + f * d.sf, b ? b + f * d.sf : nullptr, d, dx, x0 - 1, y0 - 1, interior);
   int64_t n3[3] = {d.ax.n, d.ay.n, 1};
   int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
-  for (int i = threadIdx.x; i < W * HH; i += kThreads) {
-    int ly = i / W, lx = i - ly * W;
-    int64_t x = x0 - 1 + lx, y = y0 - 1 + ly;
-    bool ok = x <= d.ax.n + 1 && y <= d.ay.n + 1;
-    sT[ly][lx] = ok ? temperature_at<R>(t, n3, bc3, x, y, 1, double(dx), step) : R(1);
+  if (interior) {
+    // every tile element of every array: one 32-bit offset per thread and slot,
+    // computed once (the rows of the field and transfer stacks may differ in pitch)
+    constexpr int NE = (W * HH + kThreads - 1) / kThreads;
+    int32_t eo[NE], et[NE];
 #pragma unroll
-    for (int f = 0; f < NT; ++f) st[f][ly][lx] = ok ? tr[(1 + f) * tsf + y * tsy + x] : R(0);
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * kThreads;
+      const int ly = e / W, lx = e - ly * W;
+      eo[j] = ly * int32_t(d.ay.stride) + lx;
+      et[j] = ly * int32_t(tsy) + lx;
+    }
+    const R* fb = in + (y0 - 1) * d.ay.stride + (x0 - 1);
+    const R* tb = tr + (y0 - 1) * tsy + (x0 - 1);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      R* dst = &sf[f][0][0];
+#pragma unroll
+      for (int j = 0; j < NE; ++j) {
+        const int e = threadIdx.x + j * kThreads;
+        if (e < W * HH) dst[e] = fb[f * d.sf + eo[j]];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NT; ++f) {
+      R* dst = &st[f][0][0];
+#pragma unroll
+      for (int j = 0; j < NE; ++j) {
+        const int e = threadIdx.x + j * kThreads;
+        if (e < W * HH) dst[e] = tb[(1 + f) * tsf + et[j]];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * kThreads;
+      const int ly = e / W, lx = e - ly * W;
+      if (e < W * HH) sT[ly][lx] = temperature_at<R>(t, n3, bc3, x0 - 1 + lx, y0 - 1 + ly, 1, double(dx), step);
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      load_tile<R, W, HH>(sf[f], in + f * d.sf, b ? b + f * d.sf : nullptr, d, dx, x0 - 1, y0 - 1, false);
+    for (int i = threadIdx.x; i < W * HH; i += kThreads) {
+      int ly = i / W, lx = i - ly * W;
+      int64_t x = x0 - 1 + lx, y = y0 - 1 + ly;
+      bool ok = x <= d.ax.n + 1 && y <= d.ay.n + 1;
+      sT[ly][lx] = ok ? temperature_at<R>(t, n3, bc3, x, y, 1, double(dx), step) : R(1);
+#pragma unroll
+      for (int f = 0; f < NT; ++f) st[f][ly][lx] = ok ? tr[(1 + f) * tsf + y * tsy + x] : R(0);
+    }
   }
   __syncthreads();
 
@@ -499,10 +539,20 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
     R D_q_xm = R(2) * Hh * sT[ly][lx - 1] * (phi[ly][lx - 1] * phi[ly][lx - 1]);
     R D_q_yp = R(2) * Hh * sT[ly + 1][lx] * (phi[ly + 1][lx] * phi[ly + 1][lx]);
     R D_q_ym = R(2) * Hh * sT[ly - 1][lx] * (phi[ly - 1][lx] * phi[ly - 1][lx]);
-    R f_ori_1 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q1.c, Q1.xp, Q1.xm,
+    R f_ori_1, f_ori_4;
+    if constexpr (sizeof(R) == 8) {
+      f_ori_1 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q1.c, Q1.xp, Q1.xm,
                            Q1.yp, Q1.ym, idx);
-    R f_ori_4 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q4.c, Q4.xp, Q4.xm,
+      f_ori_4 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q4.c, Q4.xp, Q4.xm,
                            Q4.yp, Q4.ym, idx);
+    } else {
+      // FP32: the four face weights (D_q + D_q,nb) / |grad q|_face once, shared by q1 and q4
+      const R wxp = (D_q + D_q_xp) * __builtin_amdgcn_rcpf(mgq_xp), wxm = (D_q + D_q_xm) * __builtin_amdgcn_rcpf(mgq_xm);
+      const R wyp = (D_q + D_q_yp) * __builtin_amdgcn_rcpf(mgq_yp), wym = (D_q + D_q_ym) * __builtin_amdgcn_rcpf(mgq_ym);
+      const R hi2 = R(0.5) * idx * idx;
+      f_ori_1 = hi2 * (wxp * (Q1.xp - Q1.c) - wxm * (Q1.c - Q1.xm) + wyp * (Q1.yp - Q1.c) - wym * (Q1.c - Q1.ym));
+      f_ori_4 = hi2 * (wxp * (Q4.xp - Q4.c) - wxm * (Q4.c - Q4.xm) + wyp * (Q4.yp - Q4.c) - wym * (Q4.c - Q4.ym));
+    }
     R lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx * idx;
     R lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx * idx;
     R noise_q1 = R(0), noise_q4 = R(0);
@@ -520,10 +570,16 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
     if (phin > R(0.9999)) phin = R(0.9999);
     R q1n = q1c + dt * dq1dt;
     R q4n = q4c + dt * dq4dt;
-    R renorm = fsqrt(q1n * q1n + q4n * q4n);
     out[o] = phin;
-    out[o + d.sf] = q1n / renorm;
-    out[o + 2 * d.sf] = q4n / renorm;
+    if constexpr (sizeof(R) == 8) {
+      R renorm = fsqrt(q1n * q1n + q4n * q4n);
+      out[o + d.sf] = q1n / renorm;
+      out[o + 2 * d.sf] = q4n / renorm;
+    } else {
+      const R irn = __builtin_amdgcn_rsqf(q1n * q1n + q4n * q4n);
+      out[o + d.sf] = q1n * irn;
+      out[o + 2 * d.sf] = q4n * irn;
+    }
   }
 }
 

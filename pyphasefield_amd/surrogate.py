@@ -341,6 +341,10 @@ class SurrogateThermo:
             net.in_lo[q] = float(self.in_lo[q])
             net.in_scale[q] = float(self.in_scale[q])
         net.T_lo, net.T_hi, net.c_lo = self.T_lo, self.T_hi, self.c_lo
+        # per-step-parity counts of cells handed to the FP64 fallback
+        marks = torch.zeros(2, dtype=torch.int32, device=self.device)
+        keep.append(marks)
+        net.marks = marks.data_ptr()
         net.flags = ((_lib_ext.PPF_MLP_SPLIT2 if self.split2 else 0) |
                      (_lib_ext.PPF_MLP_SPLIT3 if self.split3 else 0))
         self._packed = (net, keep)
