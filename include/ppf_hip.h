@@ -103,11 +103,40 @@ typedef struct ppf_rk_phase {
 enum { PPF_DG_FORWARD_DIFF = 0, /* reference h=1e-7 forward difference */
        PPF_DG_ANALYTIC = 1 };
 
+/* Piecewise temperature function read from a TDB file (FUNCTION /
+   PARAMETER with temperature ranges): range r covers (T_hi[r-1], T_hi[r]]
+   (below the first, the first; above the last, the last) with
+     f = c0 + c1 T + c2 T ln T + c3 T^2 + c4 T^3 + c5/T + c6 T^7 + c7 T^-9
+   summed in that order (T^7 = (T^3 T^3) T, T^-9 = (u^3 u^3) u^3, u = 1/T,
+   u^3 = (u u) u).  nranges = 0: identically zero. */
+#define PPF_TPOLY_TERMS 8
+#define PPF_TPOLY_RANGES 6
+typedef struct ppf_tpoly {
+  int32_t nranges;
+  int32_t pad_;
+  double T_hi[PPF_TPOLY_RANGES];
+  double c[PPF_TPOLY_RANGES][PPF_TPOLY_TERMS];
+} ppf_tpoly; /* 440 bytes */
+
+/* A TDB phase of one sublattice: unary G of each component (PARAMETER
+   G(phase, i; 0)) and Redlich-Kister terms L^k of each pair (i < j in
+   component order, pairs as in ppf_rk_phase.L, orders 0..2). */
+typedef struct ppf_tdb_phase {
+  ppf_tpoly g0[PPF_MAX_COMP];
+  ppf_tpoly L[6][3];
+} ppf_tdb_phase;
+typedef struct ppf_tdb_tables {
+  ppf_tdb_phase liquid, solid;
+} ppf_tdb_tables;
+
 typedef struct ppf_thermo {
   ppf_rk_phase liquid; /* ufunc 1 in NCGPU_new.py:353 */
   ppf_rk_phase solid;  /* ufunc 0 (FCC_A1 sorts first), NCGPU_new.py:354 */
   int32_t dg_mode;
   int32_t pad_;
+  /* device pointer or NULL: when set, the unary G and L^k of both phases come
+     from these tables (the g0 and L fields above are ignored; R is used) */
+  const ppf_tdb_tables* tables;
 } ppf_thermo;
 
 int ppf_abi_version(void);

@@ -68,19 +68,65 @@ def pairs(n):
     return [(i, j) for i in range(n) for j in range(i + 1, n)]
 
 
+def tpoly(tab, T, lnT):
+    """Piecewise TDB temperature function in the device table format
+    (T_hi[r], c[r][8], nranges): range r covers (T_hi[r-1], T_hi[r]], the
+    first below and the last above; terms 1, T, T lnT, T^2, T^3, 1/T, T^7,
+    T^-9 summed in that order (the device's ppf_tpoly order, so FP64 values
+    agree bit for bit)."""
+    T_hi, c, n = tab
+    T = np.asarray(T, dtype=np.float64)
+    if n <= 0:
+        return np.zeros_like(T)
+    r = np.zeros(T.shape, dtype=np.int64)
+    for k in range(n - 1):
+        r = np.where((r == k) & (T > T_hi[k]), k + 1, r)
+    C = np.asarray(c)[r]  # [..., 8]
+    u = 1.0 / T
+    T2 = T * T
+    T3 = T2 * T
+    T7 = (T3 * T3) * T
+    u3 = (u * u) * u
+    u9 = (u3 * u3) * u3
+    v = C[..., 0] + C[..., 1] * T
+    v = v + C[..., 2] * (T * lnT)
+    v = v + C[..., 3] * T2
+    v = v + C[..., 4] * T3
+    v = v + C[..., 5] * u
+    v = v + C[..., 6] * T7
+    v = v + C[..., 7] * u9
+    return v
+
+
+def _is_table(g0):
+    return isinstance(g0, (list, tuple)) and len(g0) > 0 and isinstance(g0[0], tuple)
+
+
+def _unary(g0, i, T, lnT):
+    if _is_table(g0):
+        return tpoly(g0[i], T, lnT)
+    a, b, c, d, e = g0[i]
+    return a + b * T + c * T * lnT + d * T * T + e / T
+
+
+def _rk(L, p, T, lnT):
+    if isinstance(L, (list, tuple)):
+        return [tpoly(L[p][k], T, lnT) for k in range(3)]
+    return [L[p, k, 0] + L[p, k, 1] * T for k in range(3)]
+
+
 def gibbs(cs, T, g0, L, R):
-    """G for component fractions ``cs`` (list of N arrays, c_N last) and T."""
+    """G for component fractions ``cs`` (list of N arrays, c_N last) and T.
+    g0 / L: the Redlich-Kister coefficient arrays, or TDB tables (lists of
+    (T_hi, c, nranges), see :func:`tpoly`)."""
     n = len(cs)
     lnT = plog(T)
     G = np.zeros(np.broadcast(cs[0], T).shape)
     for i in range(n):
-        a, b, c, d, e = g0[i]
-        gi = a + b * T + c * T * lnT + d * T * T + e / T
+        gi = _unary(g0, i, T, lnT)
         G = G + cs[i] * gi
     for p, (i, j) in enumerate(pairs(n)):
-        L0 = L[p, 0, 0] + L[p, 0, 1] * T
-        L1 = L[p, 1, 0] + L[p, 1, 1] * T
-        L2 = L[p, 2, 0] + L[p, 2, 1] * T
+        L0, L1, L2 = _rk(L, p, T, lnT)
         dd = cs[i] - cs[j]
         P = L0 + dd * (L1 + dd * L2)
         G = G + cs[i] * cs[j] * P
@@ -116,15 +162,12 @@ def analytic_derivatives(cs, T, g0, L, R):
     dG = []
     part = []
     for i in range(n):
-        a, b, c, d, e = g0[i]
-        gi = a + b * T + c * T * lnT + d * T * T + e / T
+        gi = _unary(g0, i, T, lnT)
         ci = np.maximum(cs[i], CMIN)
         clamp = (cs[i] > CMIN)
         part.append(gi + R * T * np.where(clamp, plog(ci) + 1.0, 0.0))
     for p, (i, j) in enumerate(pairs(n)):
-        L0 = L[p, 0, 0] + L[p, 0, 1] * T
-        L1 = L[p, 1, 0] + L[p, 1, 1] * T
-        L2 = L[p, 2, 0] + L[p, 2, 1] * T
+        L0, L1, L2 = _rk(L, p, T, lnT)
         dd = cs[i] - cs[j]
         P = L0 + dd * (L1 + dd * L2)
         dP = L1 + 2.0 * dd * L2

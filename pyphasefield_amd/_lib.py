@@ -52,8 +52,26 @@ class RKPhase(ctypes.Structure):
                 ("L", ((c_f64 * 2) * 3) * 6)]
 
 
+PPF_TPOLY_TERMS = 8
+PPF_TPOLY_RANGES = 6
+
+
+class TPoly(ctypes.Structure):
+    _fields_ = [("nranges", c_i32), ("pad_", c_i32), ("T_hi", c_f64 * PPF_TPOLY_RANGES),
+                ("c", (c_f64 * PPF_TPOLY_TERMS) * PPF_TPOLY_RANGES)]
+
+
+class TdbPhase(ctypes.Structure):
+    _fields_ = [("g0", TPoly * PPF_MAX_COMP), ("L", (TPoly * 3) * 6)]
+
+
+class TdbTables(ctypes.Structure):
+    _fields_ = [("liquid", TdbPhase), ("solid", TdbPhase)]
+
+
 class Thermo(ctypes.Structure):
-    _fields_ = [("liquid", RKPhase), ("solid", RKPhase), ("dg_mode", c_i32), ("pad_", c_i32)]
+    _fields_ = [("liquid", RKPhase), ("solid", RKPhase), ("dg_mode", c_i32), ("pad_", c_i32),
+                ("tables", ctypes.c_void_p)]
 
 
 _lib = None

@@ -39,8 +39,7 @@ __global__ __launch_bounds__(256) void k_thermo_eval(ppf_thermo th, int64_t n, c
   double t = T[i];
   double lnT = plog(t);
   PhaseAtT<double, N> PL, PS;
-  phase_at<double, N>(th.liquid, t, lnT, PL);
-  phase_at<double, N>(th.solid, t, lnT, PS);
+  phases_at<double, N>(th, t, lnT, PL, PS);
   double id = ideal_sum<double, N>(cc);
   double dL[N], dS[N];
   gibbs_grad<double, N>(cc, PL, dL);

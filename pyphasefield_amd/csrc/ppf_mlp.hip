@@ -316,8 +316,7 @@ __device__ __forceinline__ Plain5 plain_outputs(float c1, float c2, float T, con
   double t = double(T);
   double lnT = plog(t);
   PhaseAtT<double, N> PL, PS;
-  phase_at<double, N>(th.liquid, t, lnT, PL);
-  phase_at<double, N>(th.solid, t, lnT, PS);
+  phases_at<double, N>(th, t, lnT, PL, PS);
   double id = ideal_sum<double, N>(c);
   double gl = gibbs<double, N>(c, PL, id), gs = gibbs<double, N>(c, PS, id);
   double dl[N], ds[N];

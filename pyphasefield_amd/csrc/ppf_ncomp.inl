@@ -50,8 +50,56 @@ __device__ __forceinline__ void phase_at(const ppf_rk_phase& ph, R T, R lnT, Pha
   o.RT = R(ph.R) * T;
 }
 
+// piecewise TDB temperature function (operation order of ppf_tpoly)
+template <typename R>
+__device__ __forceinline__ R tpoly_eval(const ppf_tpoly* __restrict__ f, R T, R lnT) {
+  const int n = f->nranges;
+  if (n <= 0) return R(0);
+  int r = 0;
+  while (r < n - 1 && double(T) > f->T_hi[r]) ++r;
+  const double* c = f->c[r];
+  const R u = R(1) / T;
+  const R T2 = T * T, T3 = T2 * T, T7 = (T3 * T3) * T;
+  const R u3 = (u * u) * u, u9 = (u3 * u3) * u3;
+  R v = R(c[0]) + R(c[1]) * T;
+  v = v + R(c[2]) * (T * lnT);
+  v = v + R(c[3]) * T2;
+  v = v + R(c[4]) * T3;
+  v = v + R(c[5]) * u;
+  v = v + R(c[6]) * T7;
+  v = v + R(c[7]) * u9;
+  return v;
+}
+
+template <typename R, int N>
+__device__ __forceinline__ void phase_at_tables(const ppf_tdb_phase* __restrict__ ph, R Rg, R T, R lnT,
+                                                PhaseAtT<R, N>& o) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) o.gi[i] = tpoly_eval<R>(&ph->g0[i], T, lnT);
+#pragma unroll
+  for (int p = 0; p < N * (N - 1) / 2; ++p) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.L[p][k] = tpoly_eval<R>(&ph->L[p][k], T, lnT);
+  }
+  o.RT = Rg * T;
+}
+
+// both phases at one temperature: TDB tables when present, else the inline
+// Redlich-Kister coefficients
+template <typename R, int N>
+__device__ __forceinline__ void phases_at(const ppf_thermo& th, R T, R lnT, PhaseAtT<R, N>& PL, PhaseAtT<R, N>& PS) {
+  if (th.tables) {
+    phase_at_tables<R, N>(&th.tables->liquid, R(th.liquid.R), T, lnT, PL);
+    phase_at_tables<R, N>(&th.tables->solid, R(th.solid.R), T, lnT, PS);
+  } else {
+    phase_at<R, N>(th.liquid, T, lnT, PL);
+    phase_at<R, N>(th.solid, T, lnT, PS);
+  }
+}
+
 // whether any coefficient multiplies T ln T (lnT is only needed then)
 __device__ __forceinline__ bool needs_lnT(const ppf_thermo& th, int N) {
+  if (th.tables) return true;
   for (int i = 0; i < N; ++i)
     if (th.liquid.g0[i][2] != 0.0 || th.solid.g0[i][2] != 0.0) return true;
   return false;
@@ -130,8 +178,7 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
   c[N - 1] = cN;
   R lnT = needs_lnT(k.th, N) ? rlog<R>(T) : R(0);
   PhaseAtT<R, N> PL, PS;
-  phase_at<R, N>(k.th.liquid, T, lnT, PL);
-  phase_at<R, N>(k.th.solid, T, lnT, PS);
+  phases_at<R, N>(k.th, T, lnT, PL, PS);
   R id0 = ideal_sum<R, N>(c);
   R G_L = gibbs<R, N>(c, PL, id0);
   R G_S = gibbs<R, N>(c, PS, id0);

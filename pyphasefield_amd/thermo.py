@@ -71,6 +71,65 @@ class RKPhase:
         return p
 
 
+# gas constant pycalphad's ideal_mixing_energy uses for TDB phases
+R_PYCALPHAD = 8.3145
+
+
+class TablePhase:
+    """A TDB phase of one sublattice: unary G (Piecewise per component) and
+    Redlich-Kister terms L^k (Piecewise per component pair i < j and order k),
+    evaluated on the device from ppf_tdb_tables."""
+
+    def __init__(self, name, g0, L, R=R_PYCALPHAD):
+        self.name = name
+        self.g0 = list(g0)
+        self.L = dict(L)
+        self.R = float(R)
+
+    @property
+    def ncomp(self):
+        return len(self.g0)
+
+    def tables(self):
+        """(g0, L) in the oracle's table format: [(T_hi, c, n)] per component and
+        [[(T_hi, c, n)] * 3] per pair"""
+        from .tdb import Piecewise
+        n = self.ncomp
+        zero = (np.full(_lib.PPF_TPOLY_RANGES, np.inf), np.zeros((_lib.PPF_TPOLY_RANGES, _lib.PPF_TPOLY_TERMS)), 0)
+        g0 = [f.device_table(_lib.PPF_TPOLY_RANGES) for f in self.g0]
+        L = []
+        for p in range(n * (n - 1) // 2):
+            row = []
+            for k in range(3):
+                f = self.L.get((p, k))
+                row.append(f.device_table(_lib.PPF_TPOLY_RANGES) if f is not None else zero)
+            L.append(row)
+        return g0, L
+
+    def fill(self, st):
+        """write this phase into a _lib.TdbPhase"""
+        g0, L = self.tables()
+
+        def put(dst, tab):
+            T_hi, c, n = tab
+            dst.nranges = int(n)
+            for r in range(_lib.PPF_TPOLY_RANGES):
+                dst.T_hi[r] = float(T_hi[r])
+                for k in range(_lib.PPF_TPOLY_TERMS):
+                    dst.c[r][k] = float(c[r, k])
+        for i, tab in enumerate(g0):
+            put(st.g0[i], tab)
+        for p, row in enumerate(L):
+            for k, tab in enumerate(row):
+                put(st.L[p][k], tab)
+
+    def to_ctypes(self):
+        p = _lib.RKPhase()
+        p.ncomp = self.ncomp
+        p.R = self.R
+        return p
+
+
 class ThermoModel:
     """Phases (sorted) + components (sorted) + TDB-FUNCTION-style parameters.
 
@@ -101,7 +160,33 @@ class ThermoModel:
         t.liquid = self.liquid.to_ctypes()
         t.solid = self.solid.to_ctypes()
         t.dg_mode = dg_mode
+        t.tables = self.device_tables()
         return t
+
+    def device_tables(self):
+        """Device copy of the TDB tables (uploaded once), or None for
+        Redlich-Kister coefficient phases."""
+        kinds = {isinstance(self.liquid, TablePhase), isinstance(self.solid, TablePhase)}
+        if kinds == {False}:
+            return None
+        if kinds != {True}:
+            raise ValueError("both phases must come from the same kind of model")
+        if getattr(self, "_tables_dev", None) is None:
+            import torch
+            tab = _lib.TdbTables()
+            self.liquid.fill(tab.liquid)
+            self.solid.fill(tab.solid)
+            raw = np.frombuffer(bytes(tab), dtype=np.uint8).copy()
+            self._tables_dev = torch.from_numpy(raw).to("cuda")
+        return self._tables_dev.data_ptr()
+
+    def oracle_phase(self, which):
+        """(g0, L, R) of the liquid or solid phase as the oracle takes them."""
+        ph = self.liquid if which == "liquid" else self.solid
+        if isinstance(ph, TablePhase):
+            g0, L = ph.tables()
+            return g0, L, ph.R
+        return ph.g0, ph.L, ph.R
 
     def value(self, name):
         return float(self.functions[name])
@@ -200,6 +285,54 @@ def synthetic_rk_model(components=("AL", "CU", "NI"), seed=0, T_range=(1000.0, 3
 
 
 def load_tdb_model(path, components=None, phases=None):
-    raise NotImplementedError(
-        "reading TDB files is not part of this engine path; build a ThermoModel "
-        "(e.g. synthetic_rk_model) and pass it as tdb_container")
+    """ThermoModel from a TDB file (one-sublattice phases, unary G and binary
+    Redlich-Kister L^0..L^2), replacing pycalphad's Database + Model
+    (simulation.py:196-214).  Components default to the file's elements and
+    phases to its phases, both sorted alphabetically as the reference sorts
+    them (simulation.py:201-202): the first phase is the solid (FCC_A1), the
+    second the liquid (NCGPU_new.py:353-354).  Engine parameters are the
+    file's constant FUNCTIONs (NCGPU_new.py:358-383)."""
+    from . import tdb
+    db = tdb.read(path)
+    comps = sorted(c.upper() for c in (components or db.elements))
+    names = sorted(p.upper() for p in (phases or db.phases))
+    if len(names) != 2:
+        raise ValueError("the N-component engine needs two phases, %s has %s" % (path, names))
+    built = {}
+    for name in names:
+        cons = db.phases.get(name)
+        if cons is None:
+            raise ValueError("phase %s not in %s" % (name, path))
+        missing = [c for c in comps if c not in cons]
+        if missing:
+            raise ValueError("phase %s has no constituent %s" % (name, missing))
+        g0 = []
+        for c in comps:
+            f = db.parameters.get(("G", name, (c,), 0))
+            if f is None:
+                raise ValueError("no G(%s,%s;0) in %s" % (name, c, path))
+            g0.append(f)
+        L = {}
+        for p, (i, j) in enumerate(pairs(len(comps))):
+            a, b = comps[i], comps[j]
+            for k in range(3):
+                f = db.parameters.get(("L", name, (a, b), k))
+                g = db.parameters.get(("L", name, (b, a), k))
+                if f is not None and g is not None:
+                    raise ValueError("L(%s,%s,%s;%d) given twice" % (name, a, b, k))
+                if g is not None:  # L^k_ba = (-1)^k L^k_ab
+                    f = g if k % 2 == 0 else -g
+                if f is not None:
+                    L[(p, k)] = f
+        for key in db.parameters:
+            if key[0] == "L" and key[1] == name and key[3] > 2:
+                raise ValueError("Redlich-Kister order %d of %s is beyond the supported 0..2" % (key[3], name))
+        built[name] = TablePhase(name, g0, L)
+    functions = {}
+    for fname in db.functions:
+        f = db.functions[fname]
+        if f is not None and f.is_const():
+            functions[fname] = f.value_const()
+    model = ThermoModel(comps, built, functions)
+    model.path = str(path)
+    return model
