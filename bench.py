@@ -308,6 +308,21 @@ def bench_ncomp(args):
                   "engine": "NCGPU_new", "grid": [n, n], "fields": nf, "bytes_per_cell_update": bytes_cell,
                   "thermo": sim.user_data.get("thermo_eval", "plain") + "/" + dg}, roof,
                  None if args.no_cpu else cpu_ncomp())
+    if variant == "surrogate":
+        # the plain GPU engines on the same workload, timed in this run
+        del sim
+        torch.cuda.empty_cache()
+        ks = max(20, args.steps // 5)
+        same = {}
+        for name, prec, mode in (("plain_fp64_forward_difference", "FP64", "forward_difference"),
+                                 ("plain_fp32_analytic", "FP32", "analytic")):
+            other = ncomp_sim(n, prec, mode)
+            other.simulate(max(2, args.warmup // 2))
+            tt = _timed(other.simulate, ks) / ks
+            same[name] = {"ms_per_step": tt * 1e3, "surrogate_speedup": tt / t_total, "steps": ks}
+            del other
+            torch.cuda.empty_cache()
+        line["config"]["same_run_plain_engines"] = same
     return line
 
 

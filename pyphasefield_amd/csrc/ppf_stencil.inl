@@ -509,6 +509,17 @@ __device__ __forceinline__ void ch3d_store(R* __restrict__ dst, const R (&v)[Ch3
   }
 }
 
+// (sum of mu differences) / dx^2: FP64 keeps the reference's division, FP32
+// multiplies by 1/dx^2
+template <typename R>
+__device__ __forceinline__ R ch3d_scale(R dm, R dx, R idx2) {
+  if constexpr (sizeof(R) == 8) {
+    return dm / (dx * dx);
+  } else {
+    return dm * idx2;
+  }
+}
+
 template <typename R, int C3Y>
 __global__ __launch_bounds__(kThreads) void k_ch3d(const R* __restrict__ in, R* __restrict__ out,
                                                    const R* __restrict__ b, Dom d, R dx, R eps, R dtM,
@@ -563,6 +574,8 @@ __global__ __launch_bounds__(kThreads) void k_ch3d(const R* __restrict__ in, R* 
   const int64_t x = x0 + tx;
   const bool x_in = x <= d.ax.n;
   const bool xlo = xflux && x == 1, xhi = xflux && x == d.ax.n;
+  const bool any_flux = xflux || yflux || zflux;
+  const int64_t orow = (y0 + ty) * d.ay.stride + x;  // in-plane offset of this thread's first output
   for (int32_t k = zb32 - 1; k <= ze32 + 1; ++k) {
     // iteration k: mu(k) and output plane k - 1.  c(k+1) -> c_p from
     // registers; c(k+2) is fetched into registers (its halo cells straight into
@@ -595,26 +608,41 @@ __global__ __launch_bounds__(kThreads) void k_ch3d(const R* __restrict__ in, R* 
       const R* m0 = sm[m_m];
       const R* mp = sm[m_0];
       const R* c0 = sc[c_m];
-      const bool zlo = zflux && z == 1, zhi = zflux && z == nz32;
-      R* oz = out + int64_t(z) * d.az.stride;
+      R* oz = out + int64_t(z) * d.az.stride + orow;
+      if (!any_flux) {  // periodic on every axis (uniform branch): no boundary selects
 #pragma unroll
-      for (int r = 0; r < C3Y / 4; ++r) {
-        const int ly = ty + 4 * r;
-        const int64_t y = y0 + ly;
-        if (y > d.ay.n) break;
-        const int i = (ly + 1) * C3MW + (tx + 1);
-        const R m = m0[i];
-        const R mzm = zlo ? m : mm[i];
-        const R mzp = zhi ? m : mp[i];
-        const R mym = (yflux && y == 1) ? m : m0[i - C3MW];
-        const R myp = (yflux && y == d.ay.n) ? m : m0[i + C3MW];
-        const R mxm = xlo ? m : m0[i - 1];
-        const R mxp = xhi ? m : m0[i + 1];
-        R dm = mzm + mzp - R(2) * m;
-        dm = dm + (mym + myp - R(2) * m);
-        dm = dm + (mxm + mxp - R(2) * m);
-        dm = dm / (dx * dx);
-        oz[y * d.ay.stride + x] = c0[(ly + 2) * C3W + (tx + 2)] + dtM * dm;
+        for (int r = 0; r < C3Y / 4; ++r) {
+          const int ly = ty + 4 * r;
+          if (y0 + ly > d.ay.n) break;
+          const int i = (ly + 1) * C3MW + (tx + 1);
+          const R m = m0[i];
+          R dm = mm[i] + mp[i] - R(2) * m;
+          dm = dm + (m0[i - C3MW] + m0[i + C3MW] - R(2) * m);
+          dm = dm + (m0[i - 1] + m0[i + 1] - R(2) * m);
+          dm = ch3d_scale(dm, dx, idx2);
+          oz[int64_t(4 * r) * d.ay.stride] = c0[(ly + 2) * C3W + (tx + 2)] + dtM * dm;
+        }
+      } else {
+        const bool zlo = zflux && z == 1, zhi = zflux && z == nz32;
+#pragma unroll
+        for (int r = 0; r < C3Y / 4; ++r) {
+          const int ly = ty + 4 * r;
+          const int64_t y = y0 + ly;
+          if (y > d.ay.n) break;
+          const int i = (ly + 1) * C3MW + (tx + 1);
+          const R m = m0[i];
+          const R mzm = zlo ? m : mm[i];
+          const R mzp = zhi ? m : mp[i];
+          const R mym = (yflux && y == 1) ? m : m0[i - C3MW];
+          const R myp = (yflux && y == d.ay.n) ? m : m0[i + C3MW];
+          const R mxm = xlo ? m : m0[i - 1];
+          const R mxp = xhi ? m : m0[i + 1];
+          R dm = mzm + mzp - R(2) * m;
+          dm = dm + (mym + myp - R(2) * m);
+          dm = dm + (mxm + mxp - R(2) * m);
+          dm = ch3d_scale(dm, dx, idx2);
+          oz[int64_t(4 * r) * d.ay.stride] = c0[(ly + 2) * C3W + (tx + 2)] + dtM * dm;
+        }
       }
     }
     // rotate: c(k) becomes c(k-1) of the next iteration, ...
