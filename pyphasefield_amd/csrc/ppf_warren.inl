@@ -28,9 +28,9 @@ struct WarrenK {
 // (phi[i-1]-phi[i+1])/2dx.  Below, "r" is array axis 0 (rows, our y) and "k"
 // is axis 1 (columns, our x), kept with the reference's sign convention.
 //
-// Layout of the work: a 64 x 16 tile per 256-thread block.  Each thread loads
-// its share of the (64+2) x (16+2) phi/c tile, derives D_C and temp there
-// (pointwise, Warren1995.py:69-70) and stores all four in LDS; then each
+// Layout of the work: a 64 x 16 tile per 256-thread block.  The threads load
+// the (64+2) x (16+2) phi/c tile linearly, derive D_C and temp there
+// (pointwise, Warren1995.py:69-70) and store all four in LDS; then each
 // thread walks 4 consecutive rows of one column keeping a 3-row register
 // window, so a cell costs 14 LDS reads.
 template <typename R>
@@ -65,25 +65,59 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
   const R W_A = R(p.W_A), W_B = R(p.W_B), kA = R(p.kA), kB = R(p.kB);
   const R vmR = v_m / Rg;
   const int64_t xs = x0 - 1, ys = y0 - 1;
-  // 1) phi and c of the (64+2) x (16+2) tile into LDS.  Interior tiles read
-  //    directly; tiles touching the boundary fold the rules in, in a loop
-  //    kept out of line so the interior path stays short.
+  // 1) phi and c of the (64+2) x (16+2) tile, and the pointwise D_C and temp
+  //    (Warren1995.py:69-70) derived from them, into LDS.  The tile is walked
+  //    linearly (LDS index t = ly * W + lx), so every lane of every pass does
+  //    useful work.  Interior tiles read directly; tiles touching the boundary
+  //    fold the rules in, in a loop kept out of line so the interior path
+  //    stays short.
+  R* const fphi = &sphi[0][0];
+  R* const fc = &sc[0][0];
+  R* const fD = &sD[0][0];
+  R* const ftemp = &stemp[0][0];
+  auto pointwise = [&](int t, R ph, R c) {
+    const R pp = ph * ph * ph * (R(10) - R(15) * ph + R(6) * ph * ph);
+    const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
+    const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
+    const R H_A = W_A * gp + kA * g;
+    const R H_B = W_B * gp + kB * g;
+    const R D_C = D_S + pp * (D_L - D_S);
+    R temp;
+    if constexpr (sizeof(R) == 8) {
+      temp = D_C * v_m * c * (R(1) - c) * (H_B - H_A) / Rg;
+    } else {
+      temp = D_C * vmR * c * (R(1) - c) * (H_B - H_A);
+    }
+    fphi[t] = ph;
+    fc[t] = c;
+    fD[t] = D_C;
+    ftemp[t] = temp;
+#if PPF_WARREN_STORE_GH
+    (&sg[0][0])[t] = g;
+    (&shm[0][0])[t] = (R(1) - c) * H_A + c * H_B;
+#endif
+  };
   if (interior) {
     const int32_t sy32 = int32_t(d.ay.stride);
     const R* __restrict__ gphi = in + (ys * d.ay.stride + xs);
     const R* __restrict__ gc = gphi + d.sf;
+    // all loads first (only the last pass is partial), then the arithmetic
+    constexpr int NP = (W * HH + kThreads - 1) / kThreads, NFULL = (W * HH) / kThreads;
+    R vph[NP], vc[NP];
 #pragma unroll
-    for (int k = 0; k < (HH + 3) / 4; ++k) {
-      const int ly = ty + 4 * k;
-      if (ly < HH) {
-        const int32_t o = ly * sy32 + tx;
-        sphi[ly][tx] = gphi[o];
-        sc[ly][tx] = gc[o];
-        if (tx < 2) {
-          sphi[ly][kTX + tx] = gphi[o + kTX];
-          sc[ly][kTX + tx] = gc[o + kTX];
-        }
+    for (int j = 0; j < NP; ++j) {
+      const uint32_t t = threadIdx.x + uint32_t(j * kThreads);
+      if (j < NFULL || t < uint32_t(W * HH)) {
+        const uint32_t ly = t / uint32_t(W), lx = t - ly * uint32_t(W);
+        const int32_t o = int32_t(ly) * sy32 + int32_t(lx);
+        vph[j] = gphi[o];
+        vc[j] = gc[o];
       }
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const uint32_t t = threadIdx.x + uint32_t(j * kThreads);
+      if (j < NFULL || t < uint32_t(W * HH)) pointwise(int(t), vph[j], vc[j]);
     }
   } else {
 #pragma unroll 1
@@ -95,41 +129,7 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
         ph = fold(in, b, d, x, y, int64_t(0), dx);
         c = fold(in + d.sf, b ? b + d.sf : nullptr, d, x, y, int64_t(0), dx);
       }
-      sphi[ly][lx] = ph;
-      sc[ly][lx] = c;
-    }
-  }
-  __syncthreads();
-  // 2) pointwise D_C and temp (Warren1995.py:69-70) on the whole tile
-#pragma unroll
-  for (int k = 0; k < (HH + 3) / 4; ++k) {
-    const int ly = ty + 4 * k;
-    if (ly < HH) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int lx = tx + kTX * h;
-        if (lx < W) {
-          const R ph = sphi[ly][lx], c = sc[ly][lx];
-          const R pp = ph * ph * ph * (R(10) - R(15) * ph + R(6) * ph * ph);
-          const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
-          const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
-          const R H_A = W_A * gp + kA * g;
-          const R H_B = W_B * gp + kB * g;
-          const R D_C = D_S + pp * (D_L - D_S);
-          R temp;
-          if constexpr (sizeof(R) == 8) {
-            temp = D_C * v_m * c * (R(1) - c) * (H_B - H_A) / Rg;
-          } else {
-            temp = D_C * vmR * c * (R(1) - c) * (H_B - H_A);
-          }
-          sD[ly][lx] = D_C;
-          stemp[ly][lx] = temp;
-#if PPF_WARREN_STORE_GH
-          sg[ly][lx] = g;
-          shm[ly][lx] = (R(1) - c) * H_A + c * H_B;
-#endif
-        }
-      }
+      pointwise(t, ph, c);
     }
   }
   __syncthreads();
