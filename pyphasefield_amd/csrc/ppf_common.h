@@ -60,6 +60,15 @@ __device__ __forceinline__ int64_t resolve(int64_t p, const Axis& a, int64_t* ha
   return a.n;
 }
 
+// Coordinate p with a periodic axis wrapped into [1, n] exactly as resolve()
+// does; other axes return p unchanged.
+__device__ __forceinline__ int64_t wrap_periodic(int64_t p, const Axis& a) {
+  if (a.bc != PPF_PERIODIC || (p >= 1 && p <= a.n)) return p;
+  int64_t h;
+  int s;
+  return resolve(p, a, &h, &s);
+}
+
 // Value of padded cell (x, y, z) of a field with base pointer `f` (field
 // offset applied) and bc-value pointer `b` (may be null = zeros).
 template <typename R>

@@ -438,12 +438,13 @@ __global__ __launch_bounds__(256) void k_ch_nd(const R* __restrict__ in, R* __re
 // outputs and a chunk of planes; planes of c (2-cell xy halo) and of mu (1-cell
 // xy ring) rotate through LDS rings, so every c value is read from memory once
 // per block (plus the xy halo and a few extra planes per chunk).  Iteration k
-// of the march: store c(k+1) (fetched into registers during iteration k-1;
+// of the march: store c(k+1) (fetched into registers during iteration k-2;
 // halo cells outside the domain were written to LDS directly) and start
-// fetching c(k+2); barrier; mu(k) on the ring from c(k-1..k+1);
-// barrier; output plane k-1 from mu(k-2..k) and c(k-1).  Five c slots and four
-// mu slots keep every slot written in an iteration clear of every slot read
-// in it or the one before, so two barriers per plane suffice.  Zero flux on
+// fetching c(k+3) into the same registers, so two planes are in flight;
+// barrier; mu(k) on the ring from c(k-1..k+1); barrier; output plane k-1 from
+// mu(k-2..k) and c(k-1).  Six c slots and four mu slots keep every slot
+// written in an iteration clear of every slot read in it or the one before,
+// so two barriers per plane suffice.  Zero flux on
 // non-periodic sides: a mu neighbour outside the domain is replaced by the
 // centre value (as k_ch_nd and the oracle).
 #ifndef PPF_CH3D_TY
@@ -456,7 +457,7 @@ struct Ch3dShape {
   static constexpr int MW = C3X + 2, MH = C3Y + 2, MN = MW * MH;  // mu plane, tile + 1 ring
   static constexpr int LOADS = (N + kThreads - 1) / kThreads;
   static constexpr int MUS = (MN + kThreads - 1) / kThreads;
-  static constexpr size_t lds(size_t r) { return r * (5 * N + 4 * MN); }
+  static constexpr size_t lds(size_t r) { return r * (6 * N + 4 * MN); }
 };
 
 // Per-thread constants of the march: the c-plane elements this thread
@@ -469,6 +470,25 @@ struct Ch3dLanes {
   uint32_t valid;        // bit j: element j exists (t < C3N)
 };
 
+// the halo cells of this thread's share of c plane z that lie outside the
+// domain (bit j of `mask` clear), through the boundary rules straight into
+// LDS
+template <typename R, int C3Y>
+__device__ __forceinline__ void ch3d_fold_halo(R* __restrict__ slot, const R* __restrict__ in,
+                                            const R* __restrict__ b, const Dom& d, R dx, int64_t x0,
+                                            int64_t y0, int64_t z, uint32_t mask) {
+  constexpr int C3LOADS = Ch3dShape<C3Y>::LOADS, C3N = Ch3dShape<C3Y>::N, C3W = Ch3dShape<C3Y>::W;
+#pragma unroll 1
+  for (int j = 0; j < C3LOADS; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    if (t >= C3N || (mask >> j) & 1u) continue;
+    const int ly = t / C3W, lx = t - ly * C3W;
+    const int64_t x = x0 - 2 + lx, y = y0 - 2 + ly;
+    // beyond a partial tile: never used
+    slot[t] = (x > d.ax.n + 2 || y > d.ay.n + 2) ? R(0) : fold(in, b, d, x, y, z, dx);
+  }
+}
+
 // this thread's share of c plane z: elements inside the domain into
 // registers (bit j of the returned mask), the halo cells outside it straight
 // into the plane's LDS slot through the boundary rules (one rolled loop, so
@@ -479,22 +499,13 @@ __device__ __forceinline__ uint32_t ch3d_fetch(R (&v)[Ch3dShape<C3Y>::LOADS], R*
                                                R dx, int64_t x0, int64_t y0, int64_t z,
                                                const Ch3dLanes<C3Y>& L) {
   constexpr int C3LOADS = Ch3dShape<C3Y>::LOADS, C3N = Ch3dShape<C3Y>::N, C3W = Ch3dShape<C3Y>::W;
-  const uint32_t mask = (z >= 1 && z <= d.az.n) ? L.xy_in : 0u;
-  const R* plane = in + z * d.az.stride;
+  const int64_t zw = wrap_periodic(z, d.az);
+  const uint32_t mask = (zw >= 1 && zw <= d.az.n) ? L.xy_in : 0u;
+  const R* plane = in + zw * d.az.stride;
 #pragma unroll
   for (int j = 0; j < C3LOADS; ++j)
     if ((mask >> j) & 1u) v[j] = plane[L.off[j]];
-  if (mask != L.valid) {
-#pragma unroll 1
-    for (int j = 0; j < C3LOADS; ++j) {
-      const int t = threadIdx.x + j * kThreads;
-      if (t >= C3N || (mask >> j) & 1u) continue;
-      const int ly = t / C3W, lx = t - ly * C3W;
-      const int64_t x = x0 - 2 + lx, y = y0 - 2 + ly;
-      // beyond a partial tile: never used
-      slot[t] = (x > d.ax.n + 2 || y > d.ay.n + 2) ? R(0) : fold(in, b, d, x, y, z, dx);
-    }
-  }
+  if (mask != L.valid) ch3d_fold_halo<R, C3Y>(slot, in, b, d, dx, x0, y0, z, mask);
   return mask;
 }
 
@@ -528,7 +539,7 @@ __global__ __launch_bounds__(kThreads) void k_ch3d(const R* __restrict__ in, R* 
   constexpr int C3W = S::W, C3N = S::N, C3MW = S::MW, C3MN = S::MN, C3LOADS = S::LOADS, C3MUS = S::MUS;
   extern __shared__ __align__(16) unsigned char ch3d_lds[];
   R(*sc)[C3N] = reinterpret_cast<R(*)[C3N]>(ch3d_lds);
-  R(*sm)[C3MN] = reinterpret_cast<R(*)[C3MN]>(ch3d_lds + 5 * C3N * sizeof(R));
+  R(*sm)[C3MN] = reinterpret_cast<R(*)[C3MN]>(ch3d_lds + 6 * C3N * sizeof(R));
   const uint32_t nbx = uint32_t((d.ax.n + C3X - 1) / C3X), nby = uint32_t((d.ay.n + C3Y - 1) / C3Y);
   const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
   const uint32_t bxy = blk % (nbx * nby), bz = blk / (nbx * nby);
@@ -546,7 +557,9 @@ __global__ __launch_bounds__(kThreads) void k_ch3d(const R* __restrict__ in, R* 
   for (int j = 0; j < C3LOADS; ++j) {
     const int t = threadIdx.x + j * kThreads;
     const int ly = t / C3W, lx = t - ly * C3W;
-    const int64_t x = x0 - 2 + lx, y = y0 - 2 + ly;
+    // periodic axes wrap here (as resolve()), so only non-periodic halo
+    // cells take the fold path
+    const int64_t x = wrap_periodic(x0 - 2 + lx, d.ax), y = wrap_periodic(y0 - 2 + ly, d.ay);
     const bool in_xy = t < C3N && x >= 1 && x <= d.ax.n && y >= 1 && y <= d.ay.n;
     L.off[j] = in_xy ? int32_t(y * d.ay.stride + x) : 0;
     L.xy_in |= in_xy ? 1u << j : 0u;
@@ -560,28 +573,32 @@ __global__ __launch_bounds__(kThreads) void k_ch3d(const R* __restrict__ in, R* 
     mu_c[j] = t < C3MN ? (my + 1) * C3W + (mx + 1) : -1;
   }
   // LDS ring positions (rotated once per plane; plain 32-bit scalars)
-  int c_m = 0, c_0 = 1, c_p = 2, c_n = 3, c_f = 4;  // c(k-1), c(k), c(k+1), c(k+2), free
-  int m_mm = 0, m_m = 1, m_0 = 2, m_f = 3;           // mu(k-2), mu(k-1), mu(k), free
+  int c_mm = 5, c_m = 0, c_0 = 1, c_p = 2, c_n = 3, c_nn = 4;  // c(k-2) .. c(k+3)
+  int m_mm = 0, m_m = 1, m_0 = 2, m_f = 3;                      // mu(k-2), mu(k-1), mu(k), free
   const int32_t zb32 = int32_t(zb), ze32 = int32_t(ze), nz32 = int32_t(d.az.n);
-  R v[C3LOADS];
-  // c(zb-2) -> c_m, c(zb-1) -> c_0, c(zb) in flight towards c_p
-  uint32_t mask = ch3d_fetch<R, C3Y>(v, sc[c_m], in, b, d, dx, x0, y0, zb - 2, L);
-  ch3d_store<R, C3Y>(sc[c_m], v, mask);
-  mask = ch3d_fetch<R, C3Y>(v, sc[c_0], in, b, d, dx, x0, y0, zb - 1, L);
-  ch3d_store<R, C3Y>(sc[c_0], v, mask);
-  mask = ch3d_fetch<R, C3Y>(v, sc[c_p], in, b, d, dx, x0, y0, zb, L);
+  // two register sets alternate between iterations, so two planes are in
+  // flight from memory while one is computed
+  R va[C3LOADS], vb[C3LOADS];
+  // c(zb-2) -> c_m, c(zb-1) -> c_0; c(zb) (set a) and c(zb+1) (set b) in flight
+  uint32_t ma = ch3d_fetch<R, C3Y>(va, sc[c_m], in, b, d, dx, x0, y0, zb - 2, L);
+  ch3d_store<R, C3Y>(sc[c_m], va, ma);
+  ma = ch3d_fetch<R, C3Y>(va, sc[c_0], in, b, d, dx, x0, y0, zb - 1, L);
+  ch3d_store<R, C3Y>(sc[c_0], va, ma);
+  ma = ch3d_fetch<R, C3Y>(va, sc[c_p], in, b, d, dx, x0, y0, zb, L);
+  uint32_t mb = ch3d_fetch<R, C3Y>(vb, sc[c_n], in, b, d, dx, x0, y0, zb + 1, L);
   const int tx = threadIdx.x % C3X, ty = threadIdx.x / C3X;
   const int64_t x = x0 + tx;
   const bool x_in = x <= d.ax.n;
   const bool xlo = xflux && x == 1, xhi = xflux && x == d.ax.n;
   const bool any_flux = xflux || yflux || zflux;
   const int64_t orow = (y0 + ty) * d.ay.stride + x;  // in-plane offset of this thread's first output
-  for (int32_t k = zb32 - 1; k <= ze32 + 1; ++k) {
+  auto iteration = [&](int32_t k, R(&v)[C3LOADS], uint32_t& mask) {
     // iteration k: mu(k) and output plane k - 1.  c(k+1) -> c_p from
-    // registers; c(k+2) is fetched into registers (its halo cells straight into
-    // slot c_n, which last held c(k-3), read for the last time in iteration k-2)
+    // registers (fetched two iterations ago); c(k+3) is fetched into the same
+    // registers (its halo cells straight into slot c_nn, which last held
+    // c(k-3), read for the last time in iteration k-2)
     ch3d_store<R, C3Y>(sc[c_p], v, mask);
-    if (k + 2 <= ze32 + 2) mask = ch3d_fetch<R, C3Y>(v, sc[c_n], in, b, d, dx, x0, y0, int64_t(k) + 2, L);
+    if (k + 3 <= ze32 + 2) mask = ch3d_fetch<R, C3Y>(v, sc[c_nn], in, b, d, dx, x0, y0, int64_t(k) + 3, L);
     __syncthreads();
     // mu(k) into slot m_0 on the tile + 1 ring (corners are never read)
     {
@@ -645,19 +662,237 @@ __global__ __launch_bounds__(kThreads) void k_ch3d(const R* __restrict__ in, R* 
         }
       }
     }
-    // rotate: c(k) becomes c(k-1) of the next iteration, ...
-    const int c_old = c_m;
+    // rotate: c(k-1) becomes c(k-2) of the next iteration, ...
+    const int c_old = c_mm;
+    c_mm = c_m;
     c_m = c_0;
     c_0 = c_p;
     c_p = c_n;
-    c_n = c_f;
-    c_f = c_old;
+    c_n = c_nn;
+    c_nn = c_old;
     const int m_old = m_mm;
     m_mm = m_m;
     m_m = m_0;
     m_0 = m_f;
     m_f = m_old;
+  };
+  for (int32_t k = zb32 - 1; k <= ze32 + 1; k += 2) {
+    iteration(k, va, ma);
+    if (k + 1 <= ze32 + 1) iteration(k + 1, vb, mb);
   }
+}
+
+// Full-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane l receives the
+// value of lane l-1 (resp. l+1); the end lane receives 0.  All 64 lanes must be
+// active where these are used.
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveShl1 = 0x130;
+template <int CTRL, typename R>
+__device__ __forceinline__ R lane_shift(R v) {
+  if constexpr (sizeof(R) == 4) {
+    return __builtin_bit_cast(R, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+  } else {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = uint32_t(__builtin_amdgcn_mov_dpp(int(uint32_t(u)), CTRL, 0xF, 0xF, true));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_mov_dpp(int(uint32_t(u >> 32)), CTRL, 0xF, 0xF, true));
+    return __builtin_bit_cast(R, (uint64_t(hi) << 32) | lo);
+  }
+}
+
+// 3D Cahn-Hilliard on an all-periodic box, one wave per block, marching in z
+// with the cells in registers.  Lane l owns column x = x0 - 2 + l of a 64-wide
+// window (60 outputs and a 2-cell halo on each side) and the RPW rows
+// y0 - 2 .. y0 + RPW - 3 (RPW - 4 output rows).  x neighbours come from the
+// adjacent lanes (lane shifts), y neighbours from the lane's own rows, z
+// neighbours from the register sets of the neighbouring planes, so the march
+// reads no LDS and needs no barrier; halo coordinates are wrapped once into
+// the row offsets and the lane's column, and planes are read and written
+// with buffer instructions (plane base in the descriptor, row offset scalar,
+// lane offset in one VGPR).  Iteration k: mu(k) on rows 1..RPW-2 from
+// c(k-1..k+1); output plane k-1 on rows 2..RPW-3 from mu(k-2..k) and c(k-1);
+// then c(k+3) is fetched into the registers c(k-1) held.  Four c sets (one
+// plane always in flight) and three mu sets rotate by name: the march is
+// unrolled twelve times so every set index is a compile-time constant.  FP64
+// keeps k_ch3d's operation order; FP32 sums the six neighbours first and
+// evaluates the well in Horner form.
+constexpr int kCh3wTx = 60;
+
+template <typename R>
+__device__ __forceinline__ R buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, int32_t soff) {
+  if constexpr (sizeof(R) == 4) {
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+  } else {
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void buf_store(R v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, int32_t soff) {
+  if constexpr (sizeof(R) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, voff, soff, 0);
+  } else {
+    typedef uint32_t u32x2 __attribute__((__vector_size__(8)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, voff, soff, 0);
+  }
+}
+
+// raw buffer descriptor over [p, p + 4 GiB): dword 3 = 0x00020000 selects the
+// 32-bit data format the untyped buffer instructions need on CDNA
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), int16_t(0), int32_t(-1), 0x00020000);
+}
+
+template <typename R, int RPW>
+__device__ __forceinline__ void ch3w_fetch(R (&v)[RPW], const R* __restrict__ in, const Dom& d,
+                                           const int32_t (&yoff)[RPW], uint32_t xoff, int64_t z) {
+  const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in + wrap_periodic(z, d.az) * d.az.stride);
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) v[r] = buf_load<R>(rs, xoff, yoff[r]);
+}
+
+template <typename R, int RPW>
+__global__ __launch_bounds__(64) void k_ch3w(const R* __restrict__ in, R* __restrict__ out, Dom d, R dx,
+                                             R eps, R dtM, int32_t zchunk) {
+  constexpr int TYO = RPW - 4;
+  const uint32_t nbx = uint32_t((d.ax.n + kCh3wTx - 1) / kCh3wTx), nby = uint32_t((d.ay.n + TYO - 1) / TYO);
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t bxy = blk % (nbx * nby), bz = blk / (nbx * nby);
+  const int64_t x0 = 1 + int64_t(bxy % nbx) * kCh3wTx, y0 = 1 + int64_t(bxy / nbx) * TYO;
+  const int64_t zb = 1 + int64_t(bz) * zchunk;
+  const int64_t ze = zb + zchunk - 1 < d.az.n ? zb + zchunk - 1 : d.az.n;
+  const R idx = R(1) / dx;
+  const R idx2 = idx * idx;
+  const R eps2 = eps * eps;
+  const R k_mu = -eps2 * idx2;  // FP32 form: mu = well(c) + k_mu * (sum of neighbours - 6c)
+  const R k_out = dtM * idx2;   // FP32 form: out = c + k_out * (sum of mu neighbours - 6 mu)
+  const int lane = threadIdx.x;
+  const int64_t xr = x0 - 2 + lane;
+  const uint32_t xoff = uint32_t(wrap_periodic(xr, d.ax)) * uint32_t(sizeof(R));
+  const uint32_t ooff = uint32_t(xr) * uint32_t(sizeof(R));  // used on output lanes only (xr >= 1 there)
+  int32_t yoff[RPW];  // byte offset of row r in its plane (wrapped)
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) yoff[r] = int32_t(wrap_periodic(y0 - 2 + r, d.ay) * d.ay.stride * int64_t(sizeof(R)));
+  const bool out_lane = lane >= 2 && lane < 2 + kCh3wTx && xr <= d.ax.n;
+  const int32_t zb32 = int32_t(zb), ze32 = int32_t(ze);
+  const int32_t ny_out = int32_t(d.ay.n - (y0 - 2));  // rows r <= ny_out are inside the domain
+  const int32_t orow_bytes = int32_t(d.ay.stride * int64_t(sizeof(R)));
+  R C[4][RPW];  // c(k) lives in set (k - (zb - 1)) mod 4
+  R M[3][RPW];  // mu(k) lives in set (k - (zb - 1)) mod 3
+  ch3w_fetch<R, RPW>(C[3], in, d, yoff, xoff, zb - 2);
+  ch3w_fetch<R, RPW>(C[0], in, d, yoff, xoff, zb - 1);
+  ch3w_fetch<R, RPW>(C[1], in, d, yoff, xoff, zb);
+  ch3w_fetch<R, RPW>(C[2], in, d, yoff, xoff, zb + 1);
+  auto iteration = [&](auto phase, int32_t k) {
+    constexpr int p = decltype(phase)::value;
+    __builtin_amdgcn_sched_barrier(0);  // one plane at a time (register pressure)
+    R(&cm)[RPW] = C[(p + 3) % 4];
+    R(&c0)[RPW] = C[p % 4];
+    R(&cp)[RPW] = C[(p + 1) % 4];
+    R(&mm)[RPW] = M[(p + 1) % 3];
+    R(&m0)[RPW] = M[(p + 2) % 3];
+    R(&mp)[RPW] = M[p % 3];
+    // mu(k); axis 0 (z), then y, then x
+#pragma unroll
+    for (int r = 1; r < RPW - 1; ++r) {
+      const R c = c0[r];
+      const R cxp = lane_shift<kDppWaveShl1>(c), cxm = lane_shift<kDppWaveShr1>(c);
+      if constexpr (sizeof(R) == 8) {
+        R lap = (cp[r] + cm[r] - R(2) * c) * idx2;
+        lap += (c0[r + 1] + c0[r - 1] - R(2) * c) * idx2;
+        lap += (cxp + cxm - R(2) * c) * idx2;
+        mp[r] = ch_mu(c, lap, eps2);
+      } else {
+        const R sum = ((cp[r] + cm[r]) + (c0[r + 1] + c0[r - 1])) + (cxp + cxm);
+        const R well = c * __builtin_fmaf(c, __builtin_fmaf(c, R(4), R(-6)), R(2));
+        mp[r] = __builtin_fmaf(k_mu, __builtin_fmaf(R(-6), c, sum), well);
+      }
+    }
+    // output plane k - 1 from mu(k-2) = mm, mu(k-1) = m0, mu(k) = mp, c(k-1) = cm
+    const int32_t z = k - 1;
+    if (z >= zb32) {
+      const __amdgpu_buffer_rsrc_t ws = plane_rsrc(out + int64_t(z) * d.az.stride + (y0 - 2) * d.ay.stride);
+      R o[RPW];
+#pragma unroll
+      for (int r = 2; r < RPW - 2; ++r) {
+        const R m = m0[r];
+        const R mxm = lane_shift<kDppWaveShr1>(m), mxp = lane_shift<kDppWaveShl1>(m);
+        if constexpr (sizeof(R) == 8) {
+          R dm = mm[r] + mp[r] - R(2) * m;
+          dm = dm + (m0[r - 1] + m0[r + 1] - R(2) * m);
+          dm = dm + (mxm + mxp - R(2) * m);
+          o[r] = cm[r] + dtM * ch3d_scale(dm, dx, idx2);
+        } else {
+          const R sum = ((mm[r] + mp[r]) + (m0[r - 1] + m0[r + 1])) + (mxm + mxp);
+          o[r] = __builtin_fmaf(k_out, __builtin_fmaf(R(-6), m, sum), cm[r]);
+        }
+      }
+      // one divergent region for the whole plane's stores
+      if (out_lane) {
+#pragma unroll
+        for (int r = 2; r < RPW - 2; ++r)
+          if (r <= ny_out) buf_store<R>(o[r], ws, ooff, r * orow_bytes);
+      }
+    }
+    // unconditional (the last iterations re-read plane ze + 2, whose values
+    // are never used): a conditional fetch makes the compiler wait for it
+    __builtin_amdgcn_sched_barrier(0);
+    ch3w_fetch<R, RPW>(cm, in, d, yoff, xoff, int64_t(k + 3 <= ze32 + 2 ? k + 3 : ze32 + 2));
+  };
+  for (int32_t k = zb32 - 1;; k += 12) {
+    if (k > ze32 + 1) break;
+    iteration(std::integral_constant<int, 0>{}, k);
+    if (k + 1 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 1>{}, k + 1);
+    if (k + 2 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 2>{}, k + 2);
+    if (k + 3 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 3>{}, k + 3);
+    if (k + 4 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 4>{}, k + 4);
+    if (k + 5 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 5>{}, k + 5);
+    if (k + 6 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 6>{}, k + 6);
+    if (k + 7 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 7>{}, k + 7);
+    if (k + 8 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 8>{}, k + 8);
+    if (k + 9 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 9>{}, k + 9);
+    if (k + 10 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 10>{}, k + 10);
+    if (k + 11 > ze32 + 1) break;
+    iteration(std::integral_constant<int, 11>{}, k + 11);
+  }
+}
+
+#ifndef PPF_CH3D_WAVE
+#define PPF_CH3D_WAVE 1
+#endif
+#ifndef PPF_CH3W_RPW
+#define PPF_CH3W_RPW 16
+#endif
+// z chunks of the register march: as many blocks as PPF_CH3W_ROUNDS rounds of
+// resident waves hold (so the last round is not a sliver), at least 8 planes
+// per chunk
+#ifndef PPF_CH3W_ROUNDS
+#define PPF_CH3W_ROUNDS 4
+#endif
+template <typename R, int RPW>
+int32_t ch3w_zchunk(const ppf_grid& g, int64_t xy_tiles) {
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_ch3w<R, RPW>, 64, 0) != hipSuccess || v < 1) v = 1;
+    per_cu = v;
+  }
+  const int64_t resident = int64_t(device_cus()) * per_cu;
+  int64_t chunks = PPF_CH3W_ROUNDS * resident / xy_tiles;
+  if (chunks < 1) chunks = 1;
+  int64_t zc = (g.n[2] + chunks - 1) / chunks;
+  if (zc < 8) zc = 8;
+  if (zc > g.n[2]) zc = g.n[2];
+  return int32_t(zc);
 }
 
 // z chunk: about PPF_CH3D_BLOCKS blocks (several rounds of resident blocks, so
@@ -682,6 +917,14 @@ int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* 
   if (g.ndim == 2) {
     hipLaunchKernelGGL(k_ch2d<R>, dim3(tiles2d(g)), dim3(kThreads), 0, s, static_cast<const R*>(in),
                        static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), R(eps), dtM);
+  } else if (g.ndim == 3 && g.sz < (int64_t(1) << 31) && PPF_CH3D_WAVE && g.bc[0] == PPF_PERIODIC &&
+             g.bc[1] == PPF_PERIODIC && g.bc[2] == PPF_PERIODIC) {  // in-plane offsets are 32-bit
+    constexpr int RPW = sizeof(R) == 8 ? 8 : PPF_CH3W_RPW;
+    const int64_t xy = ((g.n[0] + kCh3wTx - 1) / kCh3wTx) * ((g.n[1] + RPW - 5) / (RPW - 4));
+    const int32_t zc = ch3w_zchunk<R, RPW>(g, xy);
+    const int64_t blocks = xy * ((g.n[2] + zc - 1) / zc);
+    hipLaunchKernelGGL((k_ch3w<R, RPW>), dim3(unsigned(blocks)), dim3(64), 0, s, static_cast<const R*>(in),
+                       static_cast<R*>(out), d, R(g.dx), R(eps), dtM, zc);
   } else if (g.ndim == 3 && g.sz < (int64_t(1) << 31)) {  // in-plane offsets are 32-bit
     constexpr int Y = sizeof(R) == 8 ? 8 : PPF_CH3D_TY;  // FP64 tiles stay under 64 KB of LDS
     const size_t lds = Ch3dShape<Y>::lds(sizeof(R));
