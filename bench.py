@@ -196,7 +196,7 @@ def bench_ch3d(args):
     sim.simulate(args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n ** 3
-    roof = _roofline(8.0 * cells, t / args.steps, "k_ch3d<float>")
+    roof = _roofline(8.0 * cells, t / args.steps, "k_ch3w<float,16>")
     mass = sim.field_sums()[0]
     line = _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f32",
                  {"workload": "Cahn-Hilliard 3D %d^3 FP32 periodic, M=1 kappa=1 dt=0.01" % n,

@@ -716,6 +716,10 @@ __device__ __forceinline__ R lane_shift(R v) {
 // keeps k_ch3d's operation order; FP32 sums the six neighbours first and
 // evaluates the well in Horner form.
 constexpr int kCh3wTx = 60;
+// planes fetched ahead of c(k+1) (each costs RPW registers)
+#ifndef PPF_CH3W_AHEAD
+#define PPF_CH3W_AHEAD 1
+#endif
 
 template <typename R>
 __device__ __forceinline__ R buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, int32_t soff) {
@@ -750,6 +754,12 @@ __device__ __forceinline__ void ch3w_fetch(R (&v)[RPW], const R* __restrict__ in
   for (int r = 0; r < RPW; ++r) v[r] = buf_load<R>(rs, xoff, yoff[r]);
 }
 
+// f(0), f(1), ... in order while they return true (compile-time phases)
+template <typename F, int... P>
+__device__ __forceinline__ bool ch3w_unrolled(F& f, std::integer_sequence<int, P...>) {
+  return (f(std::integral_constant<int, P>{}) && ...);
+}
+
 template <typename R, int RPW>
 __global__ __launch_bounds__(64) void k_ch3w(const R* __restrict__ in, R* __restrict__ out, Dom d, R dx,
                                              R eps, R dtM, int32_t zchunk) {
@@ -776,18 +786,18 @@ __global__ __launch_bounds__(64) void k_ch3w(const R* __restrict__ in, R* __rest
   const int32_t zb32 = int32_t(zb), ze32 = int32_t(ze);
   const int32_t ny_out = int32_t(d.ay.n - (y0 - 2));  // rows r <= ny_out are inside the domain
   const int32_t orow_bytes = int32_t(d.ay.stride * int64_t(sizeof(R)));
-  R C[4][RPW];  // c(k) lives in set (k - (zb - 1)) mod 4
-  R M[3][RPW];  // mu(k) lives in set (k - (zb - 1)) mod 3
-  ch3w_fetch<R, RPW>(C[3], in, d, yoff, xoff, zb - 2);
-  ch3w_fetch<R, RPW>(C[0], in, d, yoff, xoff, zb - 1);
-  ch3w_fetch<R, RPW>(C[1], in, d, yoff, xoff, zb);
-  ch3w_fetch<R, RPW>(C[2], in, d, yoff, xoff, zb + 1);
+  constexpr int NC = 2 + PPF_CH3W_AHEAD;  // c sets: c(k-1), c(k), then AHEAD planes
+  R C[NC + 1][RPW];  // c(k) lives in set (k - (zb - 1)) mod (NC + 1)
+  R M[3][RPW];       // mu(k) lives in set (k - (zb - 1)) mod 3
+  ch3w_fetch<R, RPW>(C[NC], in, d, yoff, xoff, zb - 2);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) ch3w_fetch<R, RPW>(C[j], in, d, yoff, xoff, zb - 1 + j);
   auto iteration = [&](auto phase, int32_t k) {
     constexpr int p = decltype(phase)::value;
     __builtin_amdgcn_sched_barrier(0);  // one plane at a time (register pressure)
-    R(&cm)[RPW] = C[(p + 3) % 4];
-    R(&c0)[RPW] = C[p % 4];
-    R(&cp)[RPW] = C[(p + 1) % 4];
+    R(&cm)[RPW] = C[(p + NC) % (NC + 1)];
+    R(&c0)[RPW] = C[p % (NC + 1)];
+    R(&cp)[RPW] = C[(p + 1) % (NC + 1)];
     R(&mm)[RPW] = M[(p + 1) % 3];
     R(&m0)[RPW] = M[(p + 2) % 3];
     R(&mp)[RPW] = M[p % 3];
@@ -836,33 +846,18 @@ __global__ __launch_bounds__(64) void k_ch3w(const R* __restrict__ in, R* __rest
     // unconditional (the last iterations re-read plane ze + 2, whose values
     // are never used): a conditional fetch makes the compiler wait for it
     __builtin_amdgcn_sched_barrier(0);
-    ch3w_fetch<R, RPW>(cm, in, d, yoff, xoff, int64_t(k + 3 <= ze32 + 2 ? k + 3 : ze32 + 2));
+    ch3w_fetch<R, RPW>(cm, in, d, yoff, xoff, int64_t(k + NC <= ze32 + 2 ? k + NC : ze32 + 2));
   };
-  for (int32_t k = zb32 - 1;; k += 12) {
-    if (k > ze32 + 1) break;
-    iteration(std::integral_constant<int, 0>{}, k);
-    if (k + 1 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 1>{}, k + 1);
-    if (k + 2 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 2>{}, k + 2);
-    if (k + 3 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 3>{}, k + 3);
-    if (k + 4 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 4>{}, k + 4);
-    if (k + 5 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 5>{}, k + 5);
-    if (k + 6 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 6>{}, k + 6);
-    if (k + 7 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 7>{}, k + 7);
-    if (k + 8 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 8>{}, k + 8);
-    if (k + 9 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 9>{}, k + 9);
-    if (k + 10 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 10>{}, k + 10);
-    if (k + 11 > ze32 + 1) break;
-    iteration(std::integral_constant<int, 11>{}, k + 11);
+  // the march, unrolled over one period of both rings
+  constexpr int kPeriod = (NC + 1) % 3 == 0 ? NC + 1 : 3 * (NC + 1);
+  for (int32_t k = zb32 - 1;; k += kPeriod) {
+    auto step = [&](auto phase) {
+      constexpr int q = decltype(phase)::value;
+      if (k + q > ze32 + 1) return false;
+      iteration(phase, k + q);
+      return true;
+    };
+    if (!ch3w_unrolled(step, std::make_integer_sequence<int, kPeriod>{})) break;
   }
 }
 
