@@ -7,13 +7,10 @@
 
 namespace ppf {
 
-// Tile shape knobs: rows per thread (the tile is 64 x 4*RPT) and whether g
-// and the H mix are kept in LDS (more LDS, fewer VALU) -- chosen by sweep.
+// Tile shape knob of the LDS-tile kernel: rows per thread (the tile is
+// 64 x 4*RPT) -- chosen by sweep.
 #ifndef PPF_WARREN_RPT
 #define PPF_WARREN_RPT 4
-#endif
-#ifndef PPF_WARREN_STORE_GH
-#define PPF_WARREN_STORE_GH 0
 #endif
 constexpr int kWRpt = PPF_WARREN_RPT;
 constexpr int kWTY = 4 * kWRpt;
@@ -42,6 +39,96 @@ __device__ __forceinline__ R recip(R x) {
   }
 }
 
+// Per-launch constants of the cell update, in the arithmetic type.
+template <typename R>
+struct WarrenC {
+  R D_L, D_S, v_m, Rg, vmR, W_A, W_B, kA, kB;
+  R y_e, M_A, M_B, dt, A, dtc, alpha;
+  bool noise;
+  __device__ WarrenC(const WarrenK& p, R dx) {
+    D_L = R(p.D_L); D_S = R(p.D_S); v_m = R(p.v_m); Rg = R(p.R); vmR = v_m / Rg;
+    W_A = R(p.W_A); W_B = R(p.W_B); kA = R(p.kA); kB = R(p.kB);
+    const R idx2 = R(1) / (dx * dx);
+    y_e = R(p.y_e); M_A = R(p.M_A); M_B = R(p.M_B); dt = R(p.dt);
+    A = R(p.ebar2) * idx2;  // ebar^2 / dx^2
+    dtc = R(p.dt) * idx2;   // dt / dx^2 for the c update
+    alpha = R(p.alpha);
+    noise = p.alpha != 0.0;
+  }
+};
+
+// pointwise D_C and temp (Warren1995.py:69-70)
+template <typename R>
+__device__ __forceinline__ void warren_point(const WarrenC<R>& K, R ph, R c, R& D_C, R& temp) {
+  const R pp = ph * ph * ph * (R(10) - R(15) * ph + R(6) * ph * ph);
+  const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
+  const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
+  const R H_A = K.W_A * gp + K.kA * g;
+  const R H_B = K.W_B * gp + K.kB * g;
+  D_C = K.D_S + pp * (K.D_L - K.D_S);
+  if constexpr (sizeof(R) == 8) {
+    temp = D_C * K.v_m * c * (R(1) - c) * (H_B - H_A) / K.Rg;
+  } else {
+    temp = D_C * K.vmR * c * (R(1) - c) * (H_B - H_A);
+  }
+}
+
+// One cell of the step (Warren1995.py:69-86) from its 3x3 phi neighbourhood
+// (rows "m" = i-1, "c" = i, "p" = i+1 along array axis 0; columns 0/1/2 =
+// x-1/x/x+1) and the 5-point c, D_C, temp neighbourhoods.  Derivatives are
+// raw grid differences with the dx scalings folded into the constants:
+// phixx = Dxx/dx^2, phix = gx/(2dx), phixy = gxy/(4dx^2), and products of two
+// first differences carry 1/(4dx^2).
+template <typename R>
+__device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R pm2, R pc0, R ph, R pc2, R pp0,
+                                            R pp1, R pp2, R cm, R c0, R c, R c2v, R cp, R Dm, R D0, R D1, R D2,
+                                            R Dp, R tm, R t0, R t1, R t2, R tp, uint64_t seed, int64_t step,
+                                            uint64_t ncells, uint64_t cell, R& nph, R& nc) {
+  const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
+  const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
+  const R Hmix = (R(1) - c) * (K.W_A * gp + K.kA * g) + c * (K.W_B * gp + K.kB * g);
+  const R Dxx = pm1 + pp1 - R(2) * ph;
+  const R Dyy = pc0 + pc2 - R(2) * ph;
+  const R lap = Dxx + Dyy;
+  const R gx = pm1 - pp1;
+  const R gy = pc0 - pc2;
+  const R gxy = (pm0 - pp0) - (pm2 - pp2);
+  // change in c (Warren1995.py:69-71):
+  // D_C lap c + grad D_C . grad c + temp lap phi + grad temp . grad phi
+  const R clap = cm + cp + c0 + c2v - R(4) * c;
+  const R cross = (Dm - Dp) * (cm - cp) + (D0 - D2) * (c0 - c2v) + (tm - tp) * gx + (t0 - t2) * gy;
+  const R deltac = D1 * clap + t1 * lap + R(0.25) * cross;
+  // change in phi (Warren1995.py:76-86); theta = atan2(phiy, phix) evaluated
+  // algebraically from the gradient, with the arctan2(0, 0) = 0 case giving
+  // cos = 1, sin = 0; eta'' = -16 (eta - 1) = -16 y_e cos(4 theta).
+  const R r2 = gx * gx + gy * gy;
+  R c2 = R(1), s2 = R(0);
+  if (r2 > R(0)) {
+    const R ir2 = recip(r2);
+    c2 = (gx * gx - gy * gy) * ir2;
+    s2 = R(2) * gx * gy * ir2;
+  }
+  const R c4 = c2 * c2 - s2 * s2;
+  const R s4 = R(2) * s2 * c2;
+  const R eta = R(1) + K.y_e * c4;
+  const R etap = R(-4) * K.y_e * s4;
+  const R etapp = R(-16) * K.y_e * c4;
+  const R dd = Dyy - Dxx;
+  const R B = eta * eta * lap + eta * etap * (s2 * dd + R(0.5) * c2 * gxy) +
+              R(0.5) * (etap * etap + eta * etapp) * (lap - R(0.5) * s2 * gxy + c2 * dd);
+  const R M_phi = (R(1) - c) * K.M_A + c * K.M_B;
+  R deltaphi = M_phi * (K.A * B - Hmix);
+  // noise (Warren1995.py:85-86); its factor g vanishes in the bulk, where
+  // the hash is skipped (the term is exactly zero there)
+  if (K.noise && g != R(0)) {
+    const R u = uniform01<R>(hash64(seed, rng_counter(step, S_WARREN_PHI, ncells, cell)));
+    const R rnd = R(2) * u - R(1);
+    deltaphi = deltaphi + M_phi * K.alpha * rnd * (R(16) * g) * Hmix;
+  }
+  nph = ph + deltaphi * K.dt;
+  nc = c + deltac * K.dtc;
+}
+
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R* __restrict__ out,
                                                      const R* __restrict__ b, Dom d, R dx, WarrenK p,
@@ -51,19 +138,13 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
   __shared__ R sc[HH][W];
   __shared__ R sD[HH][W];
   __shared__ R stemp[HH][W];
-#if PPF_WARREN_STORE_GH
-  __shared__ R sg[HH][W];
-  __shared__ R shm[HH][W];
-#endif
   uint32_t nbx = uint32_t((d.ax.n + kTX - 1) / kTX);
   uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
   const int64_t x0 = 1 + int64_t(blk % nbx) * kTX;
   const int64_t y0 = 1 + int64_t(blk / nbx) * kWTY;
   const bool interior = x0 - 1 >= 1 && x0 + kTX <= d.ax.n && y0 - 1 >= 1 && y0 + kWTY <= d.ay.n;
   const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
-  const R D_L = R(p.D_L), D_S = R(p.D_S), v_m = R(p.v_m), Rg = R(p.R);
-  const R W_A = R(p.W_A), W_B = R(p.W_B), kA = R(p.kA), kB = R(p.kB);
-  const R vmR = v_m / Rg;
+  const WarrenC<R> K(p, dx);
   const int64_t xs = x0 - 1, ys = y0 - 1;
   // 1) phi and c of the (64+2) x (16+2) tile, and the pointwise D_C and temp
   //    (Warren1995.py:69-70) derived from them, into LDS.  The tile is walked
@@ -76,26 +157,12 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
   R* const fD = &sD[0][0];
   R* const ftemp = &stemp[0][0];
   auto pointwise = [&](int t, R ph, R c) {
-    const R pp = ph * ph * ph * (R(10) - R(15) * ph + R(6) * ph * ph);
-    const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
-    const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
-    const R H_A = W_A * gp + kA * g;
-    const R H_B = W_B * gp + kB * g;
-    const R D_C = D_S + pp * (D_L - D_S);
-    R temp;
-    if constexpr (sizeof(R) == 8) {
-      temp = D_C * v_m * c * (R(1) - c) * (H_B - H_A) / Rg;
-    } else {
-      temp = D_C * vmR * c * (R(1) - c) * (H_B - H_A);
-    }
+    R D_C, temp;
+    warren_point(K, ph, c, D_C, temp);
     fphi[t] = ph;
     fc[t] = c;
     fD[t] = D_C;
     ftemp[t] = temp;
-#if PPF_WARREN_STORE_GH
-    (&sg[0][0])[t] = g;
-    (&shm[0][0])[t] = (R(1) - c) * H_A + c * H_B;
-#endif
   };
   if (interior) {
     const int32_t sy32 = int32_t(d.ay.stride);
@@ -136,14 +203,6 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
 
   const int64_t x = x0 + tx;
   if (x > d.ax.n) return;
-  // Derivatives are raw grid differences; the dx scalings are folded into
-  // the constants: phixx = Dxx/dx^2, phix = gx/(2dx), phixy = gxy/(4dx^2),
-  // and products of two first differences carry 1/(4dx^2).
-  const R idx2 = R(1) / (dx * dx);
-  const R y_e = R(p.y_e), M_A = R(p.M_A), M_B = R(p.M_B), dt = R(p.dt);
-  const R A = R(p.ebar2) * idx2;      // ebar^2 / dx^2
-  const R dtc = R(p.dt) * idx2;       // dt / dx^2 for the c update
-  const R alpha = R(p.alpha);
   const uint64_t ncells = uint64_t(d.ax.n + 2) * uint64_t(d.ay.n + 2);
   const int lx = tx + 1;
   const int ly0 = 1 + kWRpt * ty;
@@ -162,58 +221,13 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
     const R pp0 = sphi[ly + 1][lx - 1], pp1 = sphi[ly + 1][lx], pp2 = sphi[ly + 1][lx + 1];
     const R cp = sc[ly + 1][lx], Dp = sD[ly + 1][lx], tp = stemp[ly + 1][lx];
     if (interior || y <= d.ay.n) {
-      const R ph = pc1, c = c1;
-#if PPF_WARREN_STORE_GH
-      const R g = sg[ly][lx];
-      const R Hmix = shm[ly][lx];  // (1-c) H_A + c H_B
-#else
-      const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
-      const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
-      const R Hmix = (R(1) - c) * (W_A * gp + kA * g) + c * (W_B * gp + kB * g);
-#endif
-      // rows: pm = "i-1" (roll +1 along axis 0), pp = "i+1"
-      const R Dxx = pm1 + pp1 - R(2) * ph;
-      const R Dyy = pc0 + pc2 - R(2) * ph;
-      const R lap = Dxx + Dyy;
-      const R gx = pm1 - pp1;
-      const R gy = pc0 - pc2;
-      const R gxy = (pm0 - pp0) - (pm2 - pp2);
-      // change in c (Warren1995.py:69-71):
-      // D_C lap c + grad D_C . grad c + temp lap phi + grad temp . grad phi
-      const R clap = cm + cp + c0 + c2v - R(4) * c;
-      const R cross = (Dm - Dp) * (cm - cp) + (D0 - D2) * (c0 - c2v) + (tm - tp) * gx + (t0 - t2) * gy;
-      const R deltac = D1 * clap + t1 * lap + R(0.25) * cross;
-      // change in phi (Warren1995.py:76-86); theta = atan2(phiy, phix) evaluated
-      // algebraically from the gradient, with the arctan2(0, 0) = 0 case giving
-      // cos = 1, sin = 0; eta'' = -16 (eta - 1) = -16 y_e cos(4 theta).
-      const R r2 = gx * gx + gy * gy;
-      R c2 = R(1), s2 = R(0);
-      if (r2 > R(0)) {
-        const R ir2 = recip(r2);
-        c2 = (gx * gx - gy * gy) * ir2;
-        s2 = R(2) * gx * gy * ir2;
-      }
-      const R c4 = c2 * c2 - s2 * s2;
-      const R s4 = R(2) * s2 * c2;
-      const R eta = R(1) + y_e * c4;
-      const R etap = R(-4) * y_e * s4;
-      const R etapp = R(-16) * y_e * c4;
-      const R dd = Dyy - Dxx;
-      const R B = eta * eta * lap + eta * etap * (s2 * dd + R(0.5) * c2 * gxy) +
-                  R(0.5) * (etap * etap + eta * etapp) * (lap - R(0.5) * s2 * gxy + c2 * dd);
-      const R M_phi = (R(1) - c) * M_A + c * M_B;
-      R deltaphi = M_phi * (A * B - Hmix);
-      // noise (Warren1995.py:85-86); its factor g vanishes in the bulk, where
-      // the hash is skipped (the term is exactly zero there)
-      if (p.alpha != 0.0 && g != R(0)) {
-        const uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
-        const R u = uniform01<R>(hash64(seed, rng_counter(step, S_WARREN_PHI, ncells, cell)));
-        const R rnd = R(2) * u - R(1);
-        deltaphi = deltaphi + M_phi * alpha * rnd * (R(16) * g) * Hmix;
-      }
+      const uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
+      R nph, nc;
+      warren_cell(K, pm0, pm1, pm2, pc0, pc1, pc2, pp0, pp1, pp2, cm, c0, c1, c2v, cp, Dm, D0, D1, D2, Dp, tm, t0,
+                  t1, t2, tp, seed, step, ncells, cell, nph, nc);
       R* __restrict__ o = orow + r * int32_t(d.ay.stride);
-      o[0] = ph + deltaphi * dt;
-      o[d.sf] = c + deltac * dtc;
+      o[0] = nph;
+      o[d.sf] = nc;
     }
     // slide the window down one row
     pm0 = pc0; pm1 = pc1; pm2 = pc2;

@@ -284,6 +284,38 @@ def test_warren_fp64_parity(gpu, alpha):
     assert rel_l2(sim.fields[1].data, want[1]) <= FP64_TOL
 
 
+@pytest.mark.parametrize("bc", ["PERIODIC", ["PERIODIC", "NEUMANN"], ["NEUMANN", "DIRCHLET"],
+                                ["DIRCHLET", "PERIODIC"]])
+def test_warren_fp64_boundaries(gpu, bc):
+    """Warren step with partial tiles in x and y, every boundary kind with bc
+    values, noise on, over two simulate() calls: interior and (materialised on
+    retrieval) halo ring match the oracle."""
+    from pyphasefield_amd.Engines.Warren1995 import warren_derived
+    E = _engines()
+    ny, nx = 70, 130
+    sim = E.Warren1995(dimensions=[ny, nx], seed=11)
+    sim.set_user_data({"alpha": 0.3, "y_e": 0.04, "seed_shape": "circle", "seed_radius": 8,
+                       "c_solid": 0.40, "c_liquid": 0.41})
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(4.6e-8)
+    sim.set_boundary_conditions(bc)
+    sim.set_precision("FP64")
+    sim.initialize_fields_and_imported_data()
+    rng = np.random.default_rng(21)
+    bca = np.stack([1.0 - 0.05 * rng.random((ny + 2, nx + 2)), 0.41 + 0.01 * rng.random((ny + 2, nx + 2))])
+    sim._boundary_conditions_array = bca.copy()
+    phi0 = sim.fields[0].get_cells().copy()
+    c0 = sim.fields[1].get_cells().copy()
+    sim.simulate(30)
+    sim.simulate(30)
+    p = warren_derived(sim.user_data, sim.dx)
+    want = run_warren(phi0, c0, 60, bc, sim.dx, p, seed=11, bcarray=bca)
+    for f in range(2):
+        got = sim.fields[f].data
+        assert rel_l2(got[1:-1, 1:-1], want[f][1:-1, 1:-1]) <= FP64_TOL
+        assert rel_l2(got, want[f]) <= FP64_TOL
+
+
 def test_warren_fp32_vs_fp64_oracle(gpu):
     """FP32 Warren: phi and c within stated tolerances, interface within 1 cell."""
     from pyphasefield_amd.Engines.Warren1995 import warren_derived
