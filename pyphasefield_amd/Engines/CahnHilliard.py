@@ -30,6 +30,10 @@ class CahnHilliard(Simulation):
             self.dx = 1.0
         super().just_before_simulating()
 
+    def _graph_key(self):
+        u = self.user_data
+        return (float(u["M"]), float(u["epsilon"]), float(self.dt))
+
     def simulation_loop(self):
         _gpu.solver_check(self)
         u = self.user_data

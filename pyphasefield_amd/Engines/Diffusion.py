@@ -27,6 +27,9 @@ class Diffusion(Simulation):
             self.dx = 1.0
         super().just_before_simulating()
 
+    def _graph_key(self):
+        return (float(self.user_data["D"]), float(self.dt))
+
     def simulation_loop(self):
         _gpu.solver_check(self)
         _gpu.stencil_step(self, "ppf_diffusion_step", float(self.user_data["D"]), float(self.dt), nactive=1)

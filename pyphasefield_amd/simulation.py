@@ -18,6 +18,7 @@ the design notes):
 New, optional: ``precision`` ("FP64" default, or "FP32") and ``seed``.
 """
 
+import gc
 from pathlib import Path
 
 import numpy as np
@@ -218,22 +219,18 @@ class Simulation:
         self._device_newer = True
         n = int(number_of_timesteps)
         chunk = self._graph_chunk()
-        if chunk and n >= 2 * chunk:
+        if chunk and n >= chunk:
             # engines whose step launches do not depend on the step counter:
-            # `chunk` steps captured once in a HIP graph and replayed (the two
-            # buffers are back in place after an even number of steps)
-            import torch
-            graph = torch.cuda.CUDAGraph()
-            torch.cuda.synchronize()
-            with torch.cuda.graph(graph):
-                for _ in range(chunk):
-                    self.simulation_loop()
+            # `chunk` steps captured in a HIP graph and replayed (the two
+            # buffers are back in place after an even number of steps).  The
+            # graph is kept for later simulate() calls while everything baked
+            # into its launches (buffers, grid, engine scalars) is unchanged.
+            graph = self._cached_graph(chunk)
             reps = n // chunk
             for _ in range(reps):
                 graph.replay()
             self.time_step_counter += reps * chunk
             n -= reps * chunk
-            del graph
         for _ in range(n):
             self.time_step_counter += 1
             self.simulation_loop()
@@ -247,6 +244,43 @@ class Simulation:
     # steps per captured HIP graph for engines that declare their step launches
     # independent of the step counter (0: always eager)
     _GRAPH_CHUNK = 0
+
+    def _graph_key(self):
+        """Engine scalars the captured step launches bake in (None: the graph
+        is captured afresh on every simulate() call)."""
+        return None
+
+    def _cached_graph(self, chunk):
+        import torch
+        key = self._graph_key()
+        if key is not None:
+            bcd = self._boundary_conditions_gpu_device
+            key = (key, chunk, self._fields_gpu_device.data_ptr(), self._fields_out_gpu_device.data_ptr(),
+                   0 if bcd is None else bcd.data_ptr(), bytes(self._grid), tuple(self._fields_gpu_device.shape))
+        cached = getattr(self, "_graph_cache", None)
+        if key is not None and cached is not None and cached[0] == key:
+            return cached[1]
+        if cached is not None:
+            # release the stale graph now, not whenever it is collected
+            torch.cuda.synchronize()
+            cached[1].reset()
+            self._graph_cache = None
+        graph = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        # no garbage collection inside the capture: a graph kept by another
+        # (unreachable) simulation must not be destroyed while this one records
+        gc_was_on = gc.isenabled()
+        gc.disable()
+        try:
+            with torch.cuda.graph(graph):
+                for _ in range(chunk):
+                    self.simulation_loop()
+        finally:
+            if gc_was_on:
+                gc.enable()
+        if key is not None:
+            self._graph_cache = (key, graph)
+        return graph
 
     def _graph_chunk(self):
         if not self._GRAPH_CHUNK or (self._autosave_flag and self._autosave_rate):

@@ -484,6 +484,34 @@ def test_graph_replay_equals_eager_steps(gpu, engine):
         assert sims[0].time_step_counter == sims[1].time_step_counter
 
 
+def test_graph_cache_reused_and_invalidated(gpu):
+    """The captured graph is replayed again by later simulate() calls while
+    the buffers, grid and engine scalars are unchanged, and captured afresh
+    when one of them (here dt) changes; results equal eager steps throughout."""
+    E = _engines()
+    sims = []
+    for graph in (True, False):
+        sim = E.CahnHilliard(dimensions=[37, 70], seed=4)
+        sim.set_user_data({"M": 0.7, "epsilon": 1.1})
+        _setup(sim, "PERIODIC", 1.0, 0.02, "FP32")
+        if not graph:
+            sim._GRAPH_CHUNK = 0
+        sim.fields[0].data[1:-1, 1:-1] = 0.5 + 0.1 * np.random.default_rng(5).random((37, 70))
+        sims.append(sim)
+    for sim in sims:
+        sim.simulate(40)
+    g1 = sims[0]._graph_cache[1]
+    for sim in sims:
+        sim.simulate(40)
+    assert sims[0]._graph_cache[1] is g1
+    assert np.array_equal(sims[0].fields[0].get_cells(), sims[1].fields[0].get_cells())
+    for sim in sims:
+        sim.set_dt(0.01)
+        sim.simulate(40)
+    assert sims[0]._graph_cache[1] is not g1
+    assert np.array_equal(sims[0].fields[0].get_cells(), sims[1].fields[0].get_cells())
+
+
 # ---------------------------------------------------- checkpoint and resume --
 def test_checkpoint_resume_is_bit_exact(gpu, tmp_path):
     """save_simulation at step 12 (step_12.npz, keys = field names), load into a
