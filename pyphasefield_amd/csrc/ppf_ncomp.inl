@@ -324,8 +324,19 @@ struct Nb5 {
       : c(s[ly][lx]), xp(s[ly][lx + 1]), xm(s[ly][lx - 1]), yp(s[ly + 1][lx]), ym(s[ly - 1][lx]) {}
 };
 
+// occupancy hint for the step (waves per SIMD the register allocation must
+// allow; 0 = compiler's choice) -- chosen by sweep
+#ifndef PPF_NC_STEP_WAVES
+#define PPF_NC_STEP_WAVES 5
+#endif
+#if PPF_NC_STEP_WAVES
+#define PPF_NC_STEP_ATTR __attribute__((amdgpu_waves_per_eu(PPF_NC_STEP_WAVES, 8)))
+#else
+#define PPF_NC_STEP_ATTR
+#endif
+
 template <typename R, int N>
-__global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ in, R* __restrict__ out,
+__global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const R* __restrict__ in, R* __restrict__ out,
                                                          const R* __restrict__ b, Dom d, R dx, ppf_temperature t,
                                                          const R* __restrict__ tr, int64_t tsy, int64_t tsf,
                                                          NcK k, uint64_t seed, int64_t step) {
@@ -406,6 +417,7 @@ __global__ __launch_bounds__(kThreads) void k_ncomp_step(const R* __restrict__ i
   int64_t x = x0 + tx;
   if (x > d.ax.n) return;
   const int lx = tx + 1;
+#pragma unroll 1
   for (int kk = 0; kk < kNcTY / 4; ++kk) {
     int ly = tyb + 4 * kk + 1;
     int64_t y = y0 + ly - 1;
