@@ -156,7 +156,7 @@ def bench_diffusion(args):
     sim.simulate(args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
-    roof = _roofline(16.0 * cells, t / args.steps, "k_point2d<double,0>")
+    roof = _roofline(16.0 * cells, t / args.steps, "k_point2d<double,0,4>")
     cpu = None if args.no_cpu else cpu_diffusion(n, 1000)
     return _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f64",
                  {"workload": "Diffusion 2D 256x256 FP64 periodic, D=1 dx=1 dt=0.1", "engine": "Diffusion",

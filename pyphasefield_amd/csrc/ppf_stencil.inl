@@ -173,11 +173,12 @@ __device__ __forceinline__ void load_tile(R (*s)[W], const R* __restrict__ f, co
   }
 }
 
+template <int TX = kTX, int TY = kTY>
 __device__ __forceinline__ void tile_coords(const Dom& d, int64_t* x0, int64_t* y0) {
-  uint32_t nbx = uint32_t((d.ax.n + kTX - 1) / kTX);
+  uint32_t nbx = uint32_t((d.ax.n + TX - 1) / TX);
   uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
-  *x0 = 1 + int64_t(b % nbx) * kTX;
-  *y0 = 1 + int64_t(b / nbx) * kTY;
+  *x0 = 1 + int64_t(b % nbx) * TX;
+  *y0 = 1 + int64_t(b / nbx) * TY;
 }
 
 template <int TX = kTX, int TY = kTY>
@@ -237,14 +238,18 @@ __device__ __forceinline__ void point_tile(R* smem, const R* __restrict__ in, R*
   }
 }
 
-template <typename R, int KIND>
+template <typename R, int KIND, int TY = kTY>
 __global__ __launch_bounds__(kThreads) void k_point2d(const R* __restrict__ in, R* __restrict__ out,
                                                       const R* __restrict__ b, Dom d, R dx, PointParams p) {
-  __shared__ R s[(kTY + 2) * (kTX + 2)];
+  __shared__ R s[(TY + 2) * (kTX + 2)];
   int64_t x0, y0;
-  tile_coords(d, &x0, &y0);
-  point_tile<R, KIND, kTX, kTY>(s, in, out, b, d, dx, p, x0, y0);
+  tile_coords<kTX, TY>(d, &x0, &y0);
+  point_tile<R, KIND, kTX, TY>(s, in, out, b, d, dx, p, x0, y0);
 }
+
+// grids with fewer 64 x 16 tiles than this take 64 x 4 tiles (four times the
+// blocks; a 256^2 grid is otherwise 64 blocks on 256 CUs)
+constexpr unsigned kSmallGridTiles = 512;
 
 // generic 1D / 3D version (one cell per thread, folded global loads)
 template <typename R, int KIND>
@@ -274,7 +279,11 @@ template <typename R, int KIND>
 int point_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const PointParams& p,
                hipStream_t s, const char* what) {
   Dom d = make_dom(g);
-  if (g.ndim == 2) {
+  if (g.ndim == 2 && tiles2d(g) < kSmallGridTiles) {
+    const unsigned blocks = unsigned(((g.n[0] + kTX - 1) / kTX) * ((g.n[1] + 3) / 4));
+    hipLaunchKernelGGL((k_point2d<R, KIND, 4>), dim3(blocks), dim3(kThreads), 0, s, static_cast<const R*>(in),
+                       static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), p);
+  } else if (g.ndim == 2) {
     hipLaunchKernelGGL((k_point2d<R, KIND>), dim3(tiles2d(g)), dim3(kThreads), 0, s,
                        static_cast<const R*>(in), static_cast<R*>(out), static_cast<const R*>(bcvals), d,
                        R(g.dx), p);
