@@ -153,10 +153,68 @@ constexpr int kTX = 64;
 constexpr int kTY = 16;
 constexpr int kThreads = 256;
 
+// Interior coordinate a halo coordinate reads under a periodic or zero-flux
+// rule without bc values: resolve()'s interior coordinate.
+__device__ __forceinline__ int32_t remap_coord32(int32_t p, int32_t n, int bc) {
+  if (p >= 1 && p <= n) return p;
+  if (bc == PPF_PERIODIC) {
+    int32_t q = p < 1 ? p + n : p - n;
+    q = q < 1 ? 1 : q;
+    return q > n ? n : q;
+  }
+  return p < 1 ? 1 : n;
+}
+
 template <typename R, int W, int HH>
 __device__ __forceinline__ void load_tile(R (*s)[W], const R* __restrict__ f, const R* __restrict__ b,
                                           const Dom& d, R dx, int64_t xs, int64_t ys, bool interior) {
-  // s[ly][lx] = padded cell (xs + lx, ys + ly)
+  // s[ly][lx] = padded cell (xs + lx, ys + ly).  Interior tiles, and tiles
+  // whose rules are a pure coordinate remap (periodic wrap, or zero-flux
+  // Neumann copy when there are no bc values: the fold then adds -/+dx*0), read
+  // every element with its loads issued before the first LDS store; other
+  // tiles fold element by element.
+  constexpr int NP = (W * HH + kThreads - 1) / kThreads, NFULL = (W * HH) / kThreads;
+  R* flat = &s[0][0];
+  if (interior) {
+    const R* __restrict__ base = f + ys * d.ay.stride + xs;
+    const int32_t sy = int32_t(d.ay.stride);
+    R v[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int t = threadIdx.x + j * kThreads;
+      if (j < NFULL || t < W * HH) {
+        const int ly = t / W, lx = t - ly * W;
+        v[j] = base[ly * sy + lx];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int t = threadIdx.x + j * kThreads;
+      if (j < NFULL || t < W * HH) flat[t] = v[j];
+    }
+    return;
+  }
+  if (!b && d.ax.bc != PPF_DIRICHLET && d.ay.bc != PPF_DIRICHLET &&
+      (d.ay.n + 2) * d.ay.stride < (int64_t(1) << 31)) {  // 32-bit offsets
+    const int32_t nx = int32_t(d.ax.n), ny = int32_t(d.ay.n), sy = int32_t(d.ay.stride);
+    const int32_t xs32 = int32_t(xs), ys32 = int32_t(ys);
+    R v[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int t = threadIdx.x + j * kThreads;
+      if (j < NFULL || t < W * HH) {
+        const int ly = t / W, lx = t - ly * W;
+        const int32_t x = remap_coord32(xs32 + lx, nx, d.ax.bc), y = remap_coord32(ys32 + ly, ny, d.ay.bc);
+        v[j] = f[uint32_t(y * sy + x)];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int t = threadIdx.x + j * kThreads;
+      if (j < NFULL || t < W * HH) flat[t] = v[j];
+    }
+    return;
+  }
   for (int t = threadIdx.x; t < W * HH; t += kThreads) {
     int ly = t / W;
     int lx = t - ly * W;
