@@ -467,8 +467,9 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(const f
 }
 
 // Cells the surrogate kernel marked (NaN in G_S - G_L): the plain FP64
-// evaluation, same outputs.  One thread per padded cell (a scan of one FP32
-// array when nothing is marked).
+// evaluation, same outputs.  A grid-stride scan of the padded cells (a few
+// blocks per CU: when the step's mark counter is zero every block returns
+// after one load).
 __global__ __launch_bounds__(256) void k_ncomp_thermo_fallback(const float* __restrict__ in,
                                                                const float* __restrict__ b, Dom d, float dx,
                                                                ppf_temperature t, float* __restrict__ tr,
@@ -485,11 +486,10 @@ __global__ __launch_bounds__(256) void k_ncomp_thermo_fallback(const float* __re
     if (blockIdx.x == 0 && threadIdx.x == 0) marks[(step + 1) & 1] = 0;
     if (marks[step & 1] == 0) return;
   }
-  const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
-  if (cell < ncell) {
+  for (uint32_t cell = blockIdx.x * 256 + threadIdx.x; cell < ncell; cell += gridDim.x * 256) {
     const uint32_t y = cell / px, x = cell - y * px;
     const uint32_t o = y * uint32_t(tsy) + x;
-    if (!__builtin_isnan(tr[o])) return;
+    if (!__builtin_isnan(tr[o])) continue;
     const float phi = fold(in, b, d, int64_t(x), int64_t(y), int64_t(0), dx);
     const float c1 = fold(in + 3 * d.sf, b ? b + 3 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
     const float c2 = fold(in + 4 * d.sf, b ? b + 4 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
@@ -606,7 +606,7 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
                      static_cast<const float*>(bcvals), d, float(g->dx), *t, static_cast<float*>(transfer), tg->sy,
                      tg->sf, k, m, seed, step, ntiles);
   if (int rc = check_launch("ncomp_thermo_mlp")) return rc;
-  const unsigned fblocks = unsigned((ncell + 255) / 256);
+  const unsigned fblocks = unsigned(std::min<int64_t>((ncell + 255) / 256, int64_t(4) * device_cus()));
   hipLaunchKernelGGL(k_ncomp_thermo_fallback, dim3(fblocks), dim3(256), 0, stream_of(s),
                      static_cast<const float*>(in), static_cast<const float*>(bcvals), d, float(g->dx), *t,
                      static_cast<float*>(transfer), tg->sy, tg->sf, k, *th, net->marks, seed, step);

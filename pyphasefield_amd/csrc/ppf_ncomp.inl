@@ -368,22 +368,28 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
     }
     const R* fb = in + (y0 - 1) * d.ay.stride + (x0 - 1);
     const R* tb = tr + (y0 - 1) * tsy + (x0 - 1);
+    // all loads of the tile issued before the first LDS store (only the last
+    // slot of each array is partial)
+    constexpr int NFULL = (W * HH) / kThreads;
+    R vf[NF][NE], vt[NT][NE];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      R* dst = &sf[f][0][0];
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * kThreads;
+      if (j < NFULL || e < W * HH) {
 #pragma unroll
-      for (int j = 0; j < NE; ++j) {
-        const int e = threadIdx.x + j * kThreads;
-        if (e < W * HH) dst[e] = fb[f * d.sf + eo[j]];
+        for (int f = 0; f < NF; ++f) vf[f][j] = fb[f * d.sf + eo[j]];
+#pragma unroll
+        for (int f = 0; f < NT; ++f) vt[f][j] = tb[(1 + f) * tsf + et[j]];
       }
     }
 #pragma unroll
-    for (int f = 0; f < NT; ++f) {
-      R* dst = &st[f][0][0];
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * kThreads;
+      if (j < NFULL || e < W * HH) {
 #pragma unroll
-      for (int j = 0; j < NE; ++j) {
-        const int e = threadIdx.x + j * kThreads;
-        if (e < W * HH) dst[e] = tb[(1 + f) * tsf + et[j]];
+        for (int f = 0; f < NF; ++f) (&sf[f][0][0])[e] = vf[f][j];
+#pragma unroll
+        for (int f = 0; f < NT; ++f) (&st[f][0][0])[e] = vt[f][j];
       }
     }
 #pragma unroll
