@@ -71,3 +71,69 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
     g.sy = 3  # smaller than nx + 2
     g.dx = 1.0
     assert L.ppf_diffusion_step(g, None, None, None, 1.0, 0.1, None) == 1000
+
+
+def _grid2(L, n=(8, 6), bc=(0, 0), nf=1):
+    from pyphasefield_amd import _lib
+    g = _lib.Grid()
+    g.ndim = 2
+    g.dtype = _lib.PPF_F32
+    g.nfields = nf
+    g.n[0], g.n[1], g.n[2] = n[0], n[1], 1
+    g.bc[0], g.bc[1], g.bc[2] = bc[0], bc[1], 0
+    g.sy = n[0] + 2
+    g.sz = 0
+    g.sf = (n[0] + 2) * (n[1] + 2)
+    g.dx = 1.0
+    return g
+
+
+def test_edge_case_arguments_are_rejected():
+    """Empty axes, bad boundary kinds, aliased or missing buffers, too few
+    fields, bad reduction kinds, negative counts and malformed ensembles are
+    refused with PPF_EINVAL and a message, before any launch (placeholder
+    device pointers are never dereferenced)."""
+    from pyphasefield_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libppf_hip.so not built")
+    L = _lib.lib()
+    EINVAL = 1000
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+
+    def rejected(rc, word):
+        assert rc == EINVAL
+        assert word.encode() in L.ppf_last_error(), L.ppf_last_error()
+
+    g = _grid2(L, n=(0, 6))
+    rejected(L.ppf_cahnhilliard_step(g, a, b, None, 1.0, 1.0, 0.01, None), "empty axis")
+    g = _grid2(L, bc=(0, 7))
+    rejected(L.ppf_allencahn_step(g, a, b, None, 1.0, 1.0, 1.0, 0.01, None), "boundary kind")
+    g = _grid2(L)
+    g.dx = 0.0
+    rejected(L.ppf_diffusion_step(g, a, b, None, 1.0, 0.1, None), "dx")
+    g = _grid2(L)
+    rejected(L.ppf_diffusion_step(g, a, a, None, 1.0, 0.1, None), "distinct")
+    rejected(L.ppf_cahnhilliard_step(g, None, b, None, 1.0, 1.0, 0.01, None), "distinct")
+    g3 = _grid2(L)
+    g3.ndim = 3
+    g3.n[2] = 4
+    g3.sz = 10  # smaller than a plane
+    rejected(L.ppf_cahnhilliard_step(g3, a, b, None, 1.0, 1.0, 0.01, None), "plane stride")
+    p = _lib.WarrenParams()
+    rejected(L.ppf_warren_step(_grid2(L, nf=1), a, b, None, p, 1, 1, None), "needs fields")
+    rejected(L.ppf_warren_step(_grid2(L, nf=2), a, b, None, None, 1, 1, None), "params")
+    out = (ctypes.c_double * 2)()
+    rejected(L.ppf_reduce(_grid2(L), a, None, 5, 1.0, out, b, None), "kind")
+    th = _lib.Thermo()
+    rejected(L.ppf_thermo_eval(th, -1, a, a, a, None), "negative")
+    shape = (ctypes.c_int32 * 4)()
+    rejected(L.ppf_ensemble_config(L.ppf_ensemble_num_configs(), shape), "config")
+    seg = (ctypes.c_int32 * 9)(*([0] * 8 + [4]))
+    rejected(L.ppf_ensemble_step(a, 1, a, 4, seg, 0, 8, a, a, None), "aliased")
+    rejected(L.ppf_ensemble_step(a, 0, a, 4, seg, 0, 8, a, b, None), "empty")
+    rejected(L.ppf_ensemble_step(a, 1, a, 4, seg, -1, 8, a, b, None), "tile configuration")
+    bad = (ctypes.c_int32 * 9)(0, 3, 2, 4, 4, 4, 4, 4, 4)
+    rejected(L.ppf_ensemble_step(a, 1, a, 4, bad, 0, 8, a, b, None), "non-decreasing")
+    short = (ctypes.c_int32 * 9)(*([0] * 8 + [3]))
+    rejected(L.ppf_ensemble_step(a, 1, a, 4, short, 0, 8, a, b, None), "cover")
+    rejected(L.ppf_mlp_gemm(0, 4, 4, a, 1, 1, a, 1, 1, b, 1, 1.0, 0.0, 1, 0, None), "bad shape")
