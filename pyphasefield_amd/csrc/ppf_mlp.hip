@@ -156,11 +156,12 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float
 //   slot (l/32, t) -> row 32h + 16p + 8(t/4) + 4(l/32) + t%4.
 // Packed weights of one phase network (bf16x8 words, 64 lanes each; host
 // packing in pyphasefield_amd.surrogate.pack_phase):
-//   set 0..1    layer 1, hidden half h: slot (0, t) = input t of
-//               (T_hi, c1_hi, c2_hi, T_lo, c1_lo, c2_lo, 0, 0), lanes >= 32 zero
+//   set 0..1    layer 1, hidden half h: slot (0, t) = weight of input t of
+//               (T_hi, c1_hi, c2_hi, T_lo, c1_lo, c2_lo, 1, 1): slots 6 and 7
+//               hold the layer-1 bias as a BF16 hi + lo pair; lanes >= 32 zero
 //   set 2..9    layer 2, output half m = (set-2)/4, K block kb = (set-2)%4
 //   set 10..13  layer 3, K block kb, output rows 0..2 (rows 3..31 zero)
-// followed by FP32 b1[64], b2[64], b3[4] (b3[3] unused).  The inputs enter
+// followed by FP32 b1[64] (unused: packed above), b2[64], b3[4] (b3[3] unused).  The inputs enter
 // layer 1 as BF16 pairs hi + lo; the activations entering layers 2 and 3 are
 // either such pairs (two MFMAs with the same weights: the only BF16 rounding
 // is then that of the weights, which training sees) or BF16 only.
@@ -259,12 +260,13 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
                                          const bf16x8 (&x)[G], int lane, float (&out)[G][3]) {
   const int u = lane >> 5;
   f32x16 h[G][2];
+  const f32x16 zero = {};
 #pragma unroll
   for (int hh = 0; hh < 2; ++hh) {
-    const f32x16 bt = bias_tile(b, 32 * hh, u);
+    // the layer-1 bias comes in through the packed words (constant-1 inputs)
     const bf16x8 a = w[hh * 64 + lane];
 #pragma unroll
-    for (int g = 0; g < G; ++g) h[g][hh] = mfma16(a, x[g], bt);
+    for (int g = 0; g < G; ++g) h[g][hh] = mfma16(a, x[g], zero);
   }
   bf16x8 bh[G][4], bl[G][4];
 #pragma unroll
@@ -287,7 +289,6 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
 #pragma unroll
   for (int g = 0; g < G; ++g) pack_layer<S3>(h[g], bh[g], bl[g]);
   f32x16 y[G];
-  const f32x16 zero = {};
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
     const bf16x8 a = w[(10 + kb) * 64 + lane];
@@ -423,8 +424,9 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(const f
         xin[g][q] = u == 0 ? hq : static_cast<__bf16>(0.f);
         xin[g][3 + q] = u == 0 ? lq : static_cast<__bf16>(0.f);
       }
-      xin[g][6] = static_cast<__bf16>(0.f);
-      xin[g][7] = static_cast<__bf16>(0.f);
+      // constant inputs of the packed layer-1 bias (hi, lo)
+      xin[g][6] = u == 0 ? static_cast<__bf16>(1.f) : static_cast<__bf16>(0.f);
+      xin[g][7] = xin[g][6];
     }
     // the two networks one after the other (not unrolled: one network's
     // operands and activations live at a time)

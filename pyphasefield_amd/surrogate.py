@@ -40,25 +40,32 @@ def _slot_rows(h, p):
     return 32 * h + 16 * p + 8 * (t // 4) + 4 * u + t % 4
 
 
-def pack_phase(W1, W2, W3):
+def pack_phase(W1, W2, W3, b1):
     """BF16 A-operand lane layout of ``ppf_ncomp_thermo_mlp`` for one network.
 
     32x32x16 MFMA A operand: lane l, element t holds A[l % 32][slot (l // 32, t)];
     slot (u, t) of K block kb = 2h + p is hidden row 32h + 16p + 8(t//4) + 4u + t%4
     (the row the previous layer's D tile leaves in that lane and element).
     Sets of 64 lanes x 8: layer 1 (2 hidden halves; slots (0, 0..5) carry the
-    inputs T, c1, c2, T, c1, c2 for the hi and lo parts), layer 2 (2 output
-    halves x 4 K blocks), layer 3 (4 K blocks, output rows 0..2, rest zero).
+    inputs T, c1, c2, T, c1, c2 for the hi and lo parts, slots (0, 6..7) the
+    layer-1 bias as a BF16 hi + lo pair against two constant-1 inputs, so the
+    bias needs no accumulator tile), layer 2 (2 output halves x 4 K blocks),
+    layer 3 (4 K blocks, output rows 0..2, rest zero).
     """
     lanes = np.arange(64)
     r = lanes % 32
     u = lanes // 32
     sets = []
     col_of_t = [0, 1, 2, 0, 1, 2]
+    b1f = np.asarray(b1, np.float64).astype(np.float32).astype(np.float64)
+    b1_hi = _bf16(b1f)
+    b1_lo = b1f - b1_hi  # rounded to BF16 with the rest of the set
     for h in range(2):
         blk = np.zeros((64, 8), np.float32)
         for t, col in enumerate(col_of_t):
             blk[u == 0, t] = W1[32 * h + r[u == 0], col]
+        blk[u == 0, 6] = b1_hi[32 * h + r[u == 0]]
+        blk[u == 0, 7] = b1_lo[32 * h + r[u == 0]]
         sets.append(blk)
     for m in range(2):
         for kb in range(4):
@@ -324,10 +331,10 @@ class SurrogateThermo:
         keep = []
         for p, pn in enumerate(self.nets):
             h = pn.numpy()
-            words = pack_phase(h["W1"], h["W2"], h["W3"])
+            words = pack_phase(h["W1"], h["W2"], h["W3"], h["b1"])
             wdev = torch.from_numpy(words.view(np.int16).copy()).to(self.device)
             bias = np.zeros(132, np.float32)
-            bias[:64] = h["b1"]
+            bias[:64] = h["b1"]  # unused by the kernel: layer 1 takes it from the packed words
             bias[64:128] = h["b2"]
             bias[128:131] = h["b3"]
             bdev = torch.from_numpy(bias).to(self.device)

@@ -55,7 +55,8 @@ def _emulate(words, b1, b2, b3, X):
     lo = _bf16(xs - hi)
     xin[:32, 0:3] = hi.T
     xin[:32, 3:6] = lo.T
-    h = [_mfma(W[hh], xin, _bias_tile(b1, 32 * hh)) for hh in range(2)]
+    xin[:32, 6:8] = 1.0  # constant inputs of the packed layer-1 bias (hi, lo)
+    h = [_mfma(W[hh], xin, np.zeros((64, 16))) for hh in range(2)]
     packs = [_relu_split(h[kb // 2], kb % 2) for kb in range(4)]
     z = []
     for m in range(2):
@@ -80,17 +81,21 @@ def test_packed_layout_reproduces_forward():
     W3 = rng.normal(0, 0.2, (3, 64))
     b1, b2, b3 = rng.normal(0, 0.1, 64), rng.normal(0, 0.1, 64), rng.normal(0, 0.1, 4)
     X = rng.uniform(-1, 1, (3, 32))
-    words = pack_phase(W1, W2, W3)
+    words = pack_phase(W1, W2, W3, b1)
     assert words.shape == (14 * 64, 8)
     got = _emulate(words, b1, b2, b3, X)
-    H1 = np.maximum(_bf16(W1) @ _split(X) + b1[:, None], 0)
+    # the layer-1 bias travels as a BF16 hi + lo pair
+    H1 = np.maximum(_bf16(W1) @ _split(X) + _split(b1)[:, None], 0)
     H2 = np.maximum(_bf16(W2) @ _split(H1) + b2[:, None], 0)
     want = _bf16(W3) @ _split(H2) + b3[:3, None]
     # equal up to FP32 rounding of the accumulators and summation order
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-6 * np.abs(want).max())
     # a transposed hidden layer is caught
-    bad = _emulate(pack_phase(W1, W2.T, W3), b1, b2, b3, X)
+    bad = _emulate(pack_phase(W1, W2.T, W3, b1), b1, b2, b3, X)
     assert np.abs(bad - want).max() > 1e-3
+    # so is a layer-1 bias left out of the packed words
+    nob = _emulate(pack_phase(W1, W2, W3, np.zeros(64)), b1, b2, b3, X)
+    assert np.abs(nob - want).max() > 1e-3
 
 
 def test_bf16_rounding_is_nearest_even():
