@@ -165,20 +165,32 @@ __device__ __forceinline__ int32_t remap_coord32(int32_t p, int32_t n, int bc) {
   return p < 1 ? 1 : n;
 }
 
-template <typename R, int W, int HH>
-__device__ __forceinline__ void load_tile(R (*s)[W], const R* __restrict__ f, const R* __restrict__ b,
-                                          const Dom& d, R dx, int64_t xs, int64_t ys, bool interior) {
-  // s[ly][lx] = padded cell (xs + lx, ys + ly).  Interior tiles, and tiles
-  // whose rules are a pure coordinate remap (periodic wrap, or zero-flux
-  // Neumann copy when there are no bc values: the fold then adds -/+dx*0), read
-  // every element with its loads issued before the first LDS store; other
-  // tiles fold element by element.
-  constexpr int NP = (W * HH + kThreads - 1) / kThreads, NFULL = (W * HH) / kThreads;
-  R* flat = &s[0][0];
+// register slots per thread of a W x HH tile
+template <int W, int HH>
+constexpr int tile_slots() {
+  return (W * HH + kThreads - 1) / kThreads;
+}
+
+// whether a tile's halo is a pure coordinate remap (periodic wrap, or zero-flux
+// Neumann copy when there are no bc values: the fold then adds -/+dx*0) that
+// 32-bit offsets can address
+template <typename R>
+__device__ __forceinline__ bool tile_remappable(const R* __restrict__ b, const Dom& d) {
+  return !b && d.ax.bc != PPF_DIRICHLET && d.ay.bc != PPF_DIRICHLET &&
+         (d.ay.n + 2) * d.ay.stride < (int64_t(1) << 31);
+}
+
+// this thread's elements of the W x HH tile at padded (xs, ys) into registers,
+// all loads issued together: interior tiles directly, other tiles through the
+// remap (the caller checks tile_remappable)
+template <typename R, int W, int HH, int NPX>
+__device__ __forceinline__ void tile_fetch(R (&v)[NPX], const R* __restrict__ f, const Dom& d, int64_t xs,
+                                           int64_t ys, bool interior) {
+  constexpr int NP = tile_slots<W, HH>(), NFULL = (W * HH) / kThreads;
+  static_assert(NP <= NPX, "register slots");
   if (interior) {
     const R* __restrict__ base = f + ys * d.ay.stride + xs;
     const int32_t sy = int32_t(d.ay.stride);
-    R v[NP];
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
       const int t = threadIdx.x + j * kThreads;
@@ -187,32 +199,42 @@ __device__ __forceinline__ void load_tile(R (*s)[W], const R* __restrict__ f, co
         v[j] = base[ly * sy + lx];
       }
     }
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      const int t = threadIdx.x + j * kThreads;
-      if (j < NFULL || t < W * HH) flat[t] = v[j];
-    }
     return;
   }
-  if (!b && d.ax.bc != PPF_DIRICHLET && d.ay.bc != PPF_DIRICHLET &&
-      (d.ay.n + 2) * d.ay.stride < (int64_t(1) << 31)) {  // 32-bit offsets
-    const int32_t nx = int32_t(d.ax.n), ny = int32_t(d.ay.n), sy = int32_t(d.ay.stride);
-    const int32_t xs32 = int32_t(xs), ys32 = int32_t(ys);
+  const int32_t nx = int32_t(d.ax.n), ny = int32_t(d.ay.n), sy = int32_t(d.ay.stride);
+  const int32_t xs32 = int32_t(xs), ys32 = int32_t(ys);
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    if (j < NFULL || t < W * HH) {
+      const int ly = t / W, lx = t - ly * W;
+      const int32_t x = remap_coord32(xs32 + lx, nx, d.ax.bc), y = remap_coord32(ys32 + ly, ny, d.ay.bc);
+      v[j] = f[uint32_t(y * sy + x)];
+    }
+  }
+}
+
+template <typename R, int W, int HH, int NPX>
+__device__ __forceinline__ void tile_put(R* __restrict__ flat, const R (&v)[NPX]) {
+  constexpr int NP = tile_slots<W, HH>(), NFULL = (W * HH) / kThreads;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    if (j < NFULL || t < W * HH) flat[t] = v[j];
+  }
+}
+
+template <typename R, int W, int HH>
+__device__ __forceinline__ void load_tile(R (*s)[W], const R* __restrict__ f, const R* __restrict__ b,
+                                          const Dom& d, R dx, int64_t xs, int64_t ys, bool interior) {
+  // s[ly][lx] = padded cell (xs + lx, ys + ly).  Interior tiles, and tiles
+  // whose rules are a pure coordinate remap, read every element with its loads
+  // issued before the first LDS store; other tiles fold element by element.
+  if (interior || tile_remappable(b, d)) {
+    constexpr int NP = tile_slots<W, HH>();
     R v[NP];
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      const int t = threadIdx.x + j * kThreads;
-      if (j < NFULL || t < W * HH) {
-        const int ly = t / W, lx = t - ly * W;
-        const int32_t x = remap_coord32(xs32 + lx, nx, d.ax.bc), y = remap_coord32(ys32 + ly, ny, d.ay.bc);
-        v[j] = f[uint32_t(y * sy + x)];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      const int t = threadIdx.x + j * kThreads;
-      if (j < NFULL || t < W * HH) flat[t] = v[j];
-    }
+    tile_fetch<R, W, HH, NP>(v, f, d, xs, ys, interior);
+    tile_put<R, W, HH, NP>(&s[0][0], v);
     return;
   }
   for (int t = threadIdx.x; t < W * HH; t += kThreads) {
@@ -266,6 +288,10 @@ __device__ __forceinline__ R point_update(R v, R lap, const PointParams& p) {
   }
 }
 
+template <typename R, int KIND, int TX, int TY>
+__device__ __forceinline__ void point_tile_body(const R* smem, R* __restrict__ out, const Dom& d, R dx,
+                                                const PointParams& p, int64_t x0, int64_t y0);
+
 // One TX x TY output tile at padded (x0, y0) of a 2D field: 1-cell halo
 // staged in LDS (s holds (TY + 2) x (TX + 2) values), 256 threads as TX
 // columns x (256 / TX) row groups.
@@ -278,6 +304,15 @@ __device__ __forceinline__ void point_tile(R* smem, const R* __restrict__ in, R*
   R(*s)[W] = reinterpret_cast<R(*)[W]>(smem);
   load_tile<R, W, HH>(s, in, b, d, dx, x0 - H, y0 - H, tile_interior<TX, TY>(d, x0, y0, H));
   __syncthreads();
+  point_tile_body<R, KIND, TX, TY>(smem, out, d, dx, p, x0, y0);
+}
+
+// the update of a point tile whose (TY + 2) x (TX + 2) values are in LDS
+template <typename R, int KIND, int TX, int TY>
+__device__ __forceinline__ void point_tile_body(const R* smem, R* __restrict__ out, const Dom& d, R dx,
+                                                const PointParams& p, int64_t x0, int64_t y0) {
+  constexpr int H = 1, W = TX + 2 * H, RP = kThreads / TX;
+  const R(*s)[W] = reinterpret_cast<const R(*)[W]>(smem);
   const R idx = R(1) / dx;
   const R idx2 = idx * idx;
   int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -374,6 +409,10 @@ __device__ __forceinline__ R ch_mu(R c, R lap, R eps2) {
   return -eps2 * lap + (R(4) * c * c * c - R(6) * c * c + R(2) * c);
 }
 
+template <typename R, int TX, int TY>
+__device__ __forceinline__ void ch_tile_body(R* smem, R* __restrict__ out, const Dom& d, R dx, R eps, R dtM,
+                                             int64_t x0, int64_t y0);
+
 // One TX x TY Cahn-Hilliard output tile: 2-cell halo of c in LDS, mu on the
 // tile + 1 ring in LDS (smem holds (TY + 4)(TX + 4) + (TY + 2)(TX + 2) values).
 template <typename R, int TX, int TY>
@@ -381,12 +420,22 @@ __device__ __forceinline__ void ch_tile(R* smem, const R* __restrict__ in, R* __
                                         const R* __restrict__ b, const Dom& d, R dx, R eps, R dtM, int64_t x0,
                                         int64_t y0) {
   constexpr int H = 2, W = TX + 2 * H, HH = TY + 2 * H, RP = kThreads / TX;
-  constexpr int MW = TX + 2, MH = TY + 2;
   static_assert(kThreads % TX == 0 && TY % RP == 0, "tile shape");
   R(*s)[W] = reinterpret_cast<R(*)[W]>(smem);
-  R(*mu)[MW] = reinterpret_cast<R(*)[MW]>(smem + W * HH);
   load_tile<R, W, HH>(s, in, b, d, dx, x0 - H, y0 - H, tile_interior<TX, TY>(d, x0, y0, H));
   __syncthreads();
+  ch_tile_body<R, TX, TY>(smem, out, d, dx, eps, dtM, x0, y0);
+}
+
+// mu ring and update of a Cahn-Hilliard tile whose (TY + 4) x (TX + 4) values
+// of c are in LDS (the mu ring follows them there)
+template <typename R, int TX, int TY>
+__device__ __forceinline__ void ch_tile_body(R* smem, R* __restrict__ out, const Dom& d, R dx, R eps, R dtM,
+                                             int64_t x0, int64_t y0) {
+  constexpr int H = 2, W = TX + 2 * H, HH = TY + 2 * H, RP = kThreads / TX;
+  constexpr int MW = TX + 2, MH = TY + 2;
+  R(*s)[W] = reinterpret_cast<R(*)[W]>(smem);
+  R(*mu)[MW] = reinterpret_cast<R(*)[MW]>(smem + W * HH);
   const R idx = R(1) / dx;
   const R idx2 = idx * idx;
   const R eps2 = eps * eps;
