@@ -44,10 +44,12 @@ template <typename R>
 struct WarrenC {
   R D_L, D_S, v_m, Rg, vmR, W_A, W_B, kA, kB;
   R y_e, M_A, M_B, dt, A, dtc, alpha;
+  R dD, dW, dk, dM;  // D_L - D_S, W_B - W_A, kB - kA, M_B - M_A (FP32 forms)
   bool noise;
   __device__ WarrenC(const WarrenK& p, R dx) {
     D_L = R(p.D_L); D_S = R(p.D_S); v_m = R(p.v_m); Rg = R(p.R); vmR = v_m / Rg;
     W_A = R(p.W_A); W_B = R(p.W_B); kA = R(p.kA); kB = R(p.kB);
+    dD = R(p.D_L - p.D_S); dW = R(p.W_B - p.W_A); dk = R(p.kB - p.kA); dM = R(p.M_B - p.M_A);
     const R idx2 = R(1) / (dx * dx);
     y_e = R(p.y_e); M_A = R(p.M_A); M_B = R(p.M_B); dt = R(p.dt);
     A = R(p.ebar2) * idx2;  // ebar^2 / dx^2
@@ -60,16 +62,24 @@ struct WarrenC {
 // pointwise D_C and temp (Warren1995.py:69-70)
 template <typename R>
 __device__ __forceinline__ void warren_point(const WarrenC<R>& K, R ph, R c, R& D_C, R& temp) {
-  const R pp = ph * ph * ph * (R(10) - R(15) * ph + R(6) * ph * ph);
-  const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
-  const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
-  const R H_A = K.W_A * gp + K.kA * g;
-  const R H_B = K.W_B * gp + K.kB * g;
-  D_C = K.D_S + pp * (K.D_L - K.D_S);
   if constexpr (sizeof(R) == 8) {
+    // FP64: the reference's expressions in its order
+    const R pp = ph * ph * ph * (R(10) - R(15) * ph + R(6) * ph * ph);
+    const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
+    const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
+    const R H_A = K.W_A * gp + K.kA * g;
+    const R H_B = K.W_B * gp + K.kB * g;
+    D_C = K.D_S + pp * (K.D_L - K.D_S);
     temp = D_C * K.v_m * c * (R(1) - c) * (H_B - H_A) / K.Rg;
   } else {
-    temp = D_C * K.vmR * c * (R(1) - c) * (H_B - H_A);
+    // FP32: Horner forms; H_B - H_A = (W_B - W_A) g' + (kB - kA) g
+    const R ph2 = ph * ph;
+    const R pp = ph2 * ph * __builtin_fmaf(ph, __builtin_fmaf(R(6), ph, R(-15)), R(10));
+    const R q = ph - ph2;  // ph (1 - ph)
+    const R g = q * q;
+    const R gp = ph * __builtin_fmaf(ph, __builtin_fmaf(R(4), ph, R(-6)), R(2));
+    D_C = __builtin_fmaf(pp, K.dD, K.D_S);
+    temp = D_C * K.vmR * (c - c * c) * __builtin_fmaf(K.dW, gp, K.dk * g);
   }
 }
 
@@ -84,9 +94,20 @@ __device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R
                                             R pp1, R pp2, R cm, R c0, R c, R c2v, R cp, R Dm, R D0, R D1, R D2,
                                             R Dp, R tm, R t0, R t1, R t2, R tp, uint64_t seed, int64_t step,
                                             uint64_t ncells, uint64_t cell, R& nph, R& nc) {
-  const R g = (ph * ph * (R(1) - ph) * (R(1) - ph));
-  const R gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
-  const R Hmix = (R(1) - c) * (K.W_A * gp + K.kA * g) + c * (K.W_B * gp + K.kB * g);
+  R g, gp, Hmix, M_phi;
+  if constexpr (sizeof(R) == 8) {
+    g = (ph * ph * (R(1) - ph) * (R(1) - ph));
+    gp = (R(4) * ph * ph * ph - R(6) * ph * ph + R(2) * ph);
+    Hmix = (R(1) - c) * (K.W_A * gp + K.kA * g) + c * (K.W_B * gp + K.kB * g);
+    M_phi = (R(1) - c) * K.M_A + c * K.M_B;
+  } else {
+    // FP32: (1-c) H_A + c H_B = g' (W_A + c dW) + g (kA + c dk)
+    const R q = ph - ph * ph;
+    g = q * q;
+    gp = ph * __builtin_fmaf(ph, __builtin_fmaf(R(4), ph, R(-6)), R(2));
+    Hmix = __builtin_fmaf(gp, __builtin_fmaf(c, K.dW, K.W_A), g * __builtin_fmaf(c, K.dk, K.kA));
+    M_phi = __builtin_fmaf(c, K.dM, K.M_A);
+  }
   const R Dxx = pm1 + pp1 - R(2) * ph;
   const R Dyy = pc0 + pc2 - R(2) * ph;
   const R lap = Dxx + Dyy;
@@ -116,7 +137,6 @@ __device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R
   const R dd = Dyy - Dxx;
   const R B = eta * eta * lap + eta * etap * (s2 * dd + R(0.5) * c2 * gxy) +
               R(0.5) * (etap * etap + eta * etapp) * (lap - R(0.5) * s2 * gxy + c2 * dd);
-  const R M_phi = (R(1) - c) * K.M_A + c * K.M_B;
   R deltaphi = M_phi * (K.A * B - Hmix);
   // noise (Warren1995.py:85-86); its factor g vanishes in the bulk, where
   // the hash is skipped (the term is exactly zero there)
