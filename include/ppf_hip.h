@@ -54,7 +54,7 @@ typedef struct ppf_grid {
 } ppf_grid;
 
 /* Temperature seen by an engine (simulation.py:126-177, 303-332). */
-enum { PPF_T_CONST = 0, PPF_T_LINEAR = 1, PPF_T_ARRAY = 2 };
+enum { PPF_T_CONST = 0, PPF_T_LINEAR = 1, PPF_T_ARRAY = 2, PPF_T_FRAMES = 3 };
 typedef struct ppf_temperature {
   int32_t kind;
   int32_t pad_;
@@ -67,6 +67,13 @@ typedef struct ppf_temperature {
      interior is read (halos folded by the temperature rule). */
   const void* data;
   int64_t sy, sz;
+  /* FRAMES (thermal history, update_thermal_file_*_kernel,
+     ppf_gpu_utils.py:357-394): two padded frames like ARRAY (`data` at time
+     t_start, `data1` at t_end); step s reads, at cur = (s-1)*dt,
+       T = T_start*(1 - r) + T_end*r,  r = (cur - t_start)/(t_end - t_start)
+     in FP64.  The host keeps t_start <= cur <= t_end by advancing frames. */
+  const void* data1;
+  double t_start, t_end;
 } ppf_temperature;
 
 /* Warren1995 parameters after Warren1995.py:197-203 derivation. */

@@ -532,18 +532,51 @@ def ncomp_rates(fields, T, transfer, params, c_params, seed=0, step=0):
 # Simulation-level loop  (simulation.py:245-271 with the D5 resolution)
 # ---------------------------------------------------------------------------
 
-def temperature_linear(shape, T0, dTdx, dTdy, dx, dTdt, dt, step):
+def temperature_linear(shape, T0, dTdx, dTdy, dx, dTdt, dt, step, dTdz=0.0):
     """Interior temperature of a LINEAR_GRADIENT run after ``step`` steps
-    (simulation.py:134-156 with counter*dTdt*dt; ppf_gpu_utils.py:332-342).
+    (simulation.py:134-156 with counter*dTdt*dt; ppf_gpu_utils.py:332-355),
+    for 2D shapes (ny, nx) and 3D shapes (nz, ny, nx).
 
-    T = ((T0 + i*(dx*dTdx)) + j*(dx*dTdy)) + step*(dTdt*dt), i along x.
+    T = (((T0 + i*(dx*dTdx)) + j*(dx*dTdy)) + k*(dx*dTdz)) + step*(dTdt*dt),
+    i along x, j along y, k along z.
     """
-    ny, nx = shape
     T = np.zeros(shape) + T0
+    nd = len(shape)
     if dTdx:
-        T = T + (np.arange(nx, dtype=np.float64) * (dx * dTdx))[None, :]
-    if dTdy:
-        T = T + (np.arange(ny, dtype=np.float64) * (dx * dTdy))[:, None]
+        T = T + np.arange(shape[-1], dtype=np.float64) * (dx * dTdx)
+    if nd > 1 and dTdy:
+        T = T + (np.arange(shape[-2], dtype=np.float64) * (dx * dTdy))[:, None]
+    if nd > 2 and dTdz:
+        T = T + (np.arange(shape[-3], dtype=np.float64) * (dx * dTdz))[:, None, None]
     if dTdt:
         T = T + step * (dTdt * dt)
     return T
+
+
+def temperature_history(times, frames, time, form="kernel"):
+    """Interior temperature of an XDMF_FILE run at ``time``.
+
+    Window (simulation.py:169-174, 319-326; ppf_gpu_utils.py:534-543):
+    starting from frames (0, 1) it moves one frame at a time while time > the
+    later frame's time.  Interpolation between T_a (t_a) and T_b (t_b):
+
+    * form="kernel" -- update_thermal_file_*_kernel (ppf_gpu_utils.py:357-394),
+      the temperature a GPU run steps with:
+      r1 = (time - t_a)/(t_b - t_a), r0 = 1 - r1, T = T_a*r0 + T_b*r1;
+    * form="host" -- simulation.py:175,329 (CPU runs and the initial field):
+      T = T_a*(t_b - time)/(t_b - t_a) + T_b*(time - t_a)/(t_b - t_a).
+
+    The two agree to rounding (about one ulp of T)."""
+    a, b = 0, 1
+    while time > times[b]:
+        if b + 1 >= len(times):
+            raise ValueError("thermal history ends before t=%g" % time)
+        a, b = b, b + 1
+    ta, tb = times[a], times[b]
+    if form == "kernel":
+        r1 = (time - ta) / (tb - ta)
+        r0 = 1 - r1
+        return frames[a] * r0 + frames[b] * r1
+    if form == "host":
+        return frames[a] * (tb - time) / (tb - ta) + frames[b] * (time - ta) / (tb - ta)
+    raise ValueError("form must be 'kernel' or 'host'")

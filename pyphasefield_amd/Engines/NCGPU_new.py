@@ -144,6 +144,8 @@ class NCGPU_new(Simulation):
             self._setup_surrogate()
 
     def _temperature_range(self):
+        if self._thermal is not None:
+            return self._thermal.range()
         t = self.temperature_descriptor()
         if t.kind == _lib.PPF_T_CONST:
             return t.T0, t.T0

@@ -14,7 +14,7 @@ LIB_PATH = os.environ.get("PPF_LIB") or os.path.join(HERE, "libppf_hip.so")
 
 PPF_F32, PPF_F64 = 0, 1
 PPF_PERIODIC, PPF_NEUMANN, PPF_DIRICHLET = 0, 1, 2
-PPF_T_CONST, PPF_T_LINEAR, PPF_T_ARRAY = 0, 1, 2
+PPF_T_CONST, PPF_T_LINEAR, PPF_T_ARRAY, PPF_T_FRAMES = 0, 1, 2, 3
 PPF_DG_FORWARD_DIFF, PPF_DG_ANALYTIC = 0, 1
 PPF_MAX_COMP = 4
 ABI_VERSION = 1
@@ -29,7 +29,8 @@ class Grid(ctypes.Structure):
 
 class Temperature(ctypes.Structure):
     _fields_ = [("kind", c_i32), ("pad_", c_i32), ("T0", c_f64), ("dTdx", c_f64), ("dTdy", c_f64),
-                ("dTdz", c_f64), ("dTdt", c_f64), ("dt", c_f64), ("data", c_vp), ("sy", c_i64), ("sz", c_i64)]
+                ("dTdz", c_f64), ("dTdt", c_f64), ("dt", c_f64), ("data", c_vp), ("sy", c_i64), ("sz", c_i64),
+                ("data1", c_vp), ("t_start", c_f64), ("t_end", c_f64)]
 
 
 class WarrenParams(ctypes.Structure):

@@ -52,6 +52,16 @@ int check_grid(const ppf_grid* g, int lo, int hi, const char* what) {
   return 0;
 }
 
+int check_temperature(const ppf_temperature* t, const char* what) {
+  if (!t) return fail(PPF_EINVAL, std::string(what) + ": null temperature");
+  if (t->kind < PPF_T_CONST || t->kind > PPF_T_FRAMES)
+    return fail(PPF_EINVAL, std::string(what) + ": bad temperature kind");
+  if (t->kind == PPF_T_ARRAY && !t->data) return fail(PPF_EINVAL, std::string(what) + ": null T array");
+  if (t->kind == PPF_T_FRAMES && (!t->data || !t->data1 || !(t->t_end > t->t_start)))
+    return fail(PPF_EINVAL, std::string(what) + ": thermal history needs two frames with t_end > t_start");
+  return 0;
+}
+
 }  // namespace ppf
 
 using namespace ppf;
@@ -78,8 +88,8 @@ int ppf_apply_bc_temperature(const ppf_grid* g, void* T, ppf_stream_t s) {
 
 int ppf_fill_temperature(const ppf_grid* g, void* out, const ppf_temperature* t, int64_t step, ppf_stream_t s) {
   if (int rc = check_grid(g, 1, 3, "fill_temperature")) return rc;
-  if (!out || !t) return fail(PPF_EINVAL, "fill_temperature: null argument");
-  if (t->kind == PPF_T_ARRAY && !t->data) return fail(PPF_EINVAL, "fill_temperature: null T array");
+  if (!out) return fail(PPF_EINVAL, "fill_temperature: null argument");
+  if (int rc = check_temperature(t, "fill_temperature")) return rc;
   return g->dtype == PPF_F32 ? fill_temperature<float>(*g, out, *t, step, stream_of(s))
                              : fill_temperature<double>(*g, out, *t, step, stream_of(s));
 }
@@ -133,6 +143,7 @@ int ppf_ncomp_thermo(const ppf_grid* g, const void* in, const void* bcvals, cons
                      uint64_t seed, int64_t step, ppf_stream_t s) {
   if (int rc = check_ncomp(g, tg, p, "ncomp_thermo")) return rc;
   if (!in || !transfer || !t || !th) return fail(PPF_EINVAL, "ncomp_thermo: null argument");
+  if (int rc = check_temperature(t, "ncomp_thermo")) return rc;
   if (th->liquid.ncomp != p->ncomp || th->solid.ncomp != p->ncomp)
     return fail(PPF_EINVAL, "ncomp_thermo: thermodynamic model has a different component count");
   if (g->dtype == PPF_F32 && th->dg_mode == PPF_DG_FORWARD_DIFF)
@@ -147,6 +158,7 @@ int ppf_ncomp_step(const ppf_grid* g, const void* in, void* out, const void* bcv
                    int64_t step, ppf_stream_t s) {
   if (int rc = check_ncomp(g, tg, p, "ncomp_step")) return rc;
   if (!in || !out || in == out || !transfer || !t) return fail(PPF_EINVAL, "ncomp_step: null argument");
+  if (int rc = check_temperature(t, "ncomp_step")) return rc;
   return g->dtype == PPF_F32
              ? ncomp_step<float>(*g, in, out, bcvals, *t, *tg, transfer, *p, seed, step, stream_of(s))
              : ncomp_step<double>(*g, in, out, bcvals, *t, *tg, transfer, *p, seed, step, stream_of(s));

@@ -16,6 +16,7 @@ int fail(int code, const std::string& msg);
 int check_launch(const char* what);
 // compute units of the current device (cached per device id)
 int device_cus();
+int check_temperature(const ppf_temperature* t, const char* what);
 int check_grid(const ppf_grid* g, int want_ndim_min, int want_ndim_max, const char* what);
 
 inline hipStream_t stream_of(ppf_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
@@ -137,6 +138,15 @@ __device__ __forceinline__ R temperature_at(const ppf_temperature& t, const int6
   if (t.kind == PPF_T_ARRAY) {
     const R* d = static_cast<const R*>(t.data);
     return d[zi * t.sz + yi * t.sy + xi];
+  }
+  if (t.kind == PPF_T_FRAMES) {
+    // two frames of a thermal history, interpolated in time in FP64 in the
+    // order of update_thermal_file_*_kernel (ppf_gpu_utils.py:357-394)
+    const int64_t o = zi * t.sz + yi * t.sy + xi;
+    const double cur = double(step - 1) * t.dt;
+    const double r1 = (cur - t.t_start) / (t.t_end - t.t_start);
+    const double r0 = 1.0 - r1;
+    return R(double(static_cast<const R*>(t.data)[o]) * r0 + double(static_cast<const R*>(t.data1)[o]) * r1);
   }
   // LINEAR: evaluated in double in the same order as the host/oracle formula
   double T = t.T0;

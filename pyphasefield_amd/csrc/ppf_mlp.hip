@@ -567,6 +567,7 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
   if (!tg || tg->nfields != 5 || tg->n[0] != g->n[0] || tg->n[1] != g->n[1] || tg->dtype != PPF_F32)
     return fail(PPF_EINVAL, "ncomp_thermo_mlp: transfer grid does not match");
   if (!in || !transfer || !t || !th || !net) return fail(PPF_EINVAL, "ncomp_thermo_mlp: null argument");
+  if (int rc = check_temperature(t, "ncomp_thermo_mlp")) return rc;
   for (int q = 0; q < 2; ++q)
     if (!net->weights[q] || !net->bias[q]) return fail(PPF_EINVAL, "ncomp_thermo_mlp: missing network");
   Dom d = make_dom(*g);

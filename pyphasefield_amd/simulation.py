@@ -13,7 +13,10 @@ the design notes):
   halo kernels run per step; halos are materialised on retrieval;
 * LINEAR_GRADIENT temperature is T0 + x*dx*dTdx + y*dx*dTdy + counter*dTdt*dt
   (``simulation.py:154`` omits dt, D5), evaluated inside the kernels;
-* noise comes from a counter RNG keyed by (seed, step, cell).
+* noise comes from a counter RNG keyed by (seed, step, cell);
+* XDMF_FILE thermal histories are read without meshio (``thermal.py``;
+  inline XML or raw Binary heavy data) and interpolated in time inside the
+  step kernels from the two bracketing frames.
 
 New, optional: ``precision`` ("FP64" default, or "FP32") and ``seed``.
 """
@@ -79,6 +82,10 @@ class Simulation:
         self._host_dirty = False
         self._t_from_array = False
         self._grid = None
+        # XDMF_FILE: the two frames bracketing the current time, on the host
+        # (ThermalHistory) and in two padded device buffers [older, newer]
+        self._thermal = None
+        self._t_frames_gpu = None
 
     # ------------------------------------------------------------ config --
     # per-engine user_data defaults, applied at construction and again
@@ -131,12 +138,29 @@ class Simulation:
         elif t == "LINEAR_GRADIENT":
             array = self._linear_temperature(self.time_step_counter)
         elif t == "XDMF_FILE":
-            raise NotImplementedError("XDMF_FILE thermal histories are not supported by this engine path")
+            array = self._init_thermal_history()
         else:
             raise ValueError("Unknown temperature profile.")
         self.temperature = Field(data=array, simulation=self, colormap="jet",
                                  name="Temperature (" + str(self._temperature_units) + ")")
         self._apply_temperature_rule_host()
+
+    def _init_thermal_history(self):
+        """simulation.py:157-177: open the history (default ``T.xdmf``), move
+        to the frames bracketing counter*dt and interpolate between them."""
+        from .thermal import ThermalHistory
+        if self._temperature_path is None:
+            self._temperature_path = "T.xdmf"
+        self._thermal = ThermalHistory(self._temperature_path, self.dimensions)
+        now = float(self.dt or 0.0) * self.time_step_counter
+        self._thermal.advance_to(now)
+        return self._thermal.at(now)
+
+    def _upload_frame(self, k):
+        import torch
+        buf = self._t_frames_gpu[k]
+        inner = (0,) + (slice(1, -1),) * len(self.dimensions)
+        buf[inner].copy_(torch.from_numpy(np.ascontiguousarray(self._thermal.arrays[k])).to(buf.device, buf.dtype))
 
     def _linear_temperature(self, counter):
         dims = self.dimensions
@@ -287,6 +311,8 @@ class Simulation:
             return 0
         if self._t_from_array and self._dTdt:
             return 0
+        if self._thermal is not None:
+            return 0  # the frame window moves between steps
         return int(self._GRAPH_CHUNK)
 
     # ------------------------------------------------ device state --
@@ -348,6 +374,14 @@ class Simulation:
             self._temperature_gpu_device = Td
         else:
             self._temperature_gpu_device = None
+        if self._thermal is not None and not self._t_from_array:
+            # dt may have been set after the history was opened
+            self._thermal.advance_to(float(self.dt or 0.0) * self.time_step_counter)
+            self._t_frames_gpu = [_lib.alloc_stack(1, list(self.dimensions), dt, dev) for _ in range(2)]
+            self._upload_frame(0)
+            self._upload_frame(1)
+        else:
+            self._t_frames_gpu = None
         self._host_dirty = False
 
     def _swap_buffers(self):
@@ -373,6 +407,19 @@ class Simulation:
         if kind == "ISOTHERMAL":
             t.T0 = float(self._initial_T)
             return t
+        if kind == "XDMF_FILE":
+            if self._t_frames_gpu is None:
+                raise RuntimeError("the thermal history is not on the GPU yet (call send_fields_to_GPU)")
+            A, B = self._t_frames_gpu
+            t.kind = _lib.PPF_T_FRAMES
+            t.data = A.data_ptr()
+            t.data1 = B.data_ptr()
+            nd = len(self.dimensions)
+            t.sy = A.stride(-2) if nd >= 2 else 0
+            t.sz = A.stride(-3) if nd >= 3 else 0
+            t.dt = float(self.dt or 0.0)
+            t.t_start, t.t_end = float(self._thermal.bounds[0]), float(self._thermal.bounds[1])
+            return t
         t.kind = _lib.PPF_T_LINEAR
         t.T0 = float(self._initial_T)
         t.dTdx = float(self._dTdx or 0.0)
@@ -383,10 +430,22 @@ class Simulation:
         return t
 
     def update_temperature_field(self, force_cpu=False):
-        """ppf_gpu_utils.py:516-531: analytic modes need no work; an uploaded
-        temperature array advances by dTdt*dt."""
+        """ppf_gpu_utils.py:516-558: analytic modes need no work; an uploaded
+        temperature array advances by dTdt*dt; a thermal history moves its
+        frame window until it brackets counter*dt (the newer frame is read
+        from the file into the buffer of the frame that drops out; the step
+        kernels interpolate between the two)."""
         if self._t_from_array and self._dTdt and self._temperature_gpu_device is not None:
             self._temperature_gpu_device.add_(self._dTdt * self.dt)
+        elif self._thermal is not None and not self._t_from_array:
+            loaded = self._thermal.advance_to(float(self.dt) * self.time_step_counter)
+            if loaded and self._t_frames_gpu is not None:
+                if loaded == 1:
+                    self._t_frames_gpu.reverse()
+                    self._upload_frame(1)
+                else:
+                    self._upload_frame(0)
+                    self._upload_frame(1)
 
     def apply_boundary_conditions(self):
         """Materialise halo cells (ppf_gpu_utils.py:430-514); the step
@@ -423,7 +482,8 @@ class Simulation:
         if self._device_newer:
             self.retrieve_fields_from_GPU()
         self._host_dirty = True
-        if field is self.temperature and self.temperature is not None and self._begun_simulation:
+        if (field is self.temperature and self.temperature is not None and self._begun_simulation
+                and self._thermal is None):
             self._t_from_array = True
 
     # -------------------------------------------------- reductions --
@@ -482,6 +542,11 @@ class Simulation:
         if self._temperature_type is not None and str(self._temperature_type).upper() == "LINEAR_GRADIENT":
             self.temperature = Field(data=self._linear_temperature(self.time_step_counter), simulation=self,
                                      colormap="jet", name="Temperature (" + str(self._temperature_units) + ")")
+            self._apply_temperature_rule_host()
+        elif self._temperature_type is not None and str(self._temperature_type).upper() == "XDMF_FILE":
+            # simulation.py:394-395: the temperature follows the history to the loaded step
+            self.temperature = Field(data=self._init_thermal_history(), simulation=self, colormap="jet",
+                                     name="Temperature (" + str(self._temperature_units) + ")")
             self._apply_temperature_rule_host()
         if self._boundary_conditions_array is None or self._boundary_conditions_array.shape[0] != len(self.fields):
             self.init_boundary_conditions()

@@ -1,0 +1,233 @@
+"""Thermal histories for the ``XDMF_FILE`` temperature type.
+
+The reference reads temperature time series through meshio
+(``ppf_utils.py:63-123``: ``CSVtoXDMF`` converts a thermal solver's CSV dump
+into an XDMF time series, ``XDMFLoader`` reads it back frame by frame) and
+interpolates linearly in time between the two frames that bracket the current
+time (``simulation.py:157-177,316-330``; on the GPU
+``update_thermal_file_*_kernel``, ``ppf_gpu_utils.py:357-394,532-558``).
+
+meshio is not available here, so this module reads and writes the XDMF
+layout itself: a temporal collection of grids, each with a ``Time`` value and
+a node attribute ``T`` whose ``DataItem`` holds the values inline
+(``Format="XML"``) or in a raw file next to the ``.xdmf`` (``Format="Binary"``,
+with ``Endian`` and ``Seek``).  HDF5 heavy data (meshio's default) needs an
+HDF5 reader, which this environment does not have; such files are rejected
+with an explanation.
+"""
+
+import os
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+__all__ = ["read_xdmf_series", "write_xdmf_series", "XDMFLoader", "CSVtoXDMF", "ThermalHistory"]
+
+_NUMBER = {("Float", 4): np.float32, ("Float", 8): np.float64, ("Int", 4): np.int32, ("Int", 8): np.int64,
+           ("UInt", 4): np.uint32, ("UInt", 8): np.uint64, ("Char", 1): np.int8, ("UChar", 1): np.uint8}
+
+
+def _local(tag):
+    return tag.rsplit("}", 1)[-1]
+
+
+def _data_item(item, base_dir):
+    """Values of one XDMF DataItem (XML inline or raw Binary)."""
+    fmt = item.get("Format", "XML")
+    kind = item.get("DataType", item.get("NumberType", "Float"))
+    prec = int(item.get("Precision", "4" if kind == "Float" else "4"))
+    dims = [int(d) for d in item.get("Dimensions", "").split()]
+    dtype = _NUMBER.get((kind, prec))
+    if dtype is None:
+        raise ValueError("XDMF DataItem: unsupported number type %s/%d" % (kind, prec))
+    text = (item.text or "").strip()
+    if fmt == "XML":
+        values = np.array(text.split(), dtype=np.float64).astype(dtype)
+    elif fmt == "Binary":
+        path = text if os.path.isabs(text) else os.path.join(base_dir, text)
+        endian = item.get("Endian", "Native")
+        dt = np.dtype(dtype)
+        if endian == "Big":
+            dt = dt.newbyteorder(">")
+        elif endian == "Little":
+            dt = dt.newbyteorder("<")
+        count = int(np.prod(dims)) if dims else -1
+        with open(path, "rb") as f:
+            f.seek(int(item.get("Seek", "0")))
+            values = np.fromfile(f, dtype=dt, count=count).astype(dtype)
+    elif fmt == "HDF":
+        raise NotImplementedError(
+            "XDMF heavy data in HDF5 (%s) needs an HDF5 reader, which is not available here; write the "
+            "series with data_format='XML' or 'Binary' (CSVtoXDMF / write_xdmf_series)" % text)
+    else:
+        raise ValueError("XDMF DataItem: unknown Format %r" % fmt)
+    if dims and values.size != int(np.prod(dims)):
+        raise ValueError("XDMF DataItem: %d values for Dimensions %s" % (values.size, dims))
+    return values.reshape(dims) if dims else values
+
+
+def read_xdmf_series(path, name="T"):
+    """[(time, array)] of the temporal collection in ``path``, in file order."""
+    base = os.path.dirname(os.path.abspath(path))
+    root = ET.parse(path).getroot()
+    frames = []
+    for grid in root.iter():
+        if _local(grid.tag) != "Grid":
+            continue
+        time = None
+        attr = None
+        for child in grid:
+            tag = _local(child.tag)
+            if tag == "Time":
+                time = float(child.get("Value"))
+            elif tag == "Attribute" and child.get("Name") == name:
+                attr = child
+        if time is None or attr is None:
+            continue
+        items = [c for c in attr if _local(c.tag) == "DataItem"]
+        if not items:
+            raise ValueError("XDMF attribute %s at t=%g has no DataItem" % (name, time))
+        frames.append((time, _data_item(items[0], base)))
+    if not frames:
+        raise ValueError("%s holds no time steps with attribute %r" % (path, name))
+    return frames
+
+
+def write_xdmf_series(path, times, arrays, name="T", data_format="XML"):
+    """Write a temporal collection of node attribute ``name``; Binary data go
+    to ``<stem>_<name><k>.bin`` (little endian) next to ``path``."""
+    if data_format not in ("XML", "Binary"):
+        raise ValueError("data_format must be 'XML' or 'Binary'")
+    stem = os.path.splitext(os.path.basename(path))[0]
+    base = os.path.dirname(os.path.abspath(path))
+    xdmf = ET.Element("Xdmf", Version="3.0")
+    domain = ET.SubElement(xdmf, "Domain")
+    coll = ET.SubElement(domain, "Grid", Name="TimeSeries_" + name, GridType="Collection",
+                         CollectionType="Temporal")
+    for k, (t, a) in enumerate(zip(times, arrays)):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        g = ET.SubElement(coll, "Grid", Name="step_%d" % k, GridType="Uniform")
+        ET.SubElement(g, "Time", Value=repr(float(t)))
+        at = ET.SubElement(g, "Attribute", Name=name, AttributeType="Scalar", Center="Node")
+        item = ET.SubElement(at, "DataItem", DataType="Float", Precision="8",
+                             Dimensions=" ".join(str(n) for n in a.shape), Format=data_format)
+        if data_format == "XML":
+            item.text = " ".join(repr(float(v)) for v in a.ravel())
+        else:
+            fname = "%s_%s%d.bin" % (stem, name, k)
+            a.astype("<f8").tofile(os.path.join(base, fname))
+            item.set("Endian", "Little")
+            item.text = fname
+    ET.ElementTree(xdmf).write(path, xml_declaration=True, encoding="utf-8")
+
+
+class XDMFLoader:
+    """Frames of an XDMF thermal history (``ppf_utils.py:112-123``):
+    ``num_steps`` and ``data(step) -> (time, array)``."""
+
+    def __init__(self, t_file_path):
+        self.t_file_path = t_file_path
+        self._frames = read_xdmf_series(t_file_path)
+        self.num_steps = len(self._frames)
+
+    def data(self, step):
+        t, a = self._frames[step]
+        return t, np.squeeze(a)
+
+
+def CSVtoXDMF(csv_path, T_cutoffs=False, starting_T=None, ending_T=None, reflect_X=False,
+              xdmf_path="T.xdmf", data_format="XML"):
+    """Thermal-solver CSV dump -> XDMF time series (``ppf_utils.py:63-110``).
+
+    CSV layout: a header line, then label/value line pairs for the origins,
+    the spacings and the point counts (comma separated; the counts give the
+    array shape), a "time, temperature" header, then one row per time:
+    time in microseconds followed by the temperatures (in kK) of every point.
+    Each frame is written at time * 1e-6 s with T * 1000 K, the values laid out
+    with numpy.resize to the point counts and transposed.  With ``T_cutoffs``
+    the series starts (at t = 0) with the first frame whose maximum exceeds
+    ``starting_T`` and stops before the first later frame whose maximum is
+    below ``ending_T``; ``reflect_X`` mirrors every frame along its second axis
+    and prepends the mirror image."""
+    times, frames = [], []
+    with open(csv_path) as f:
+        lines = iter(f)
+        for _ in range(6):  # header, origins (label, values), spacings (label, values), counts label
+            next(lines)
+        dims = [int(v) for v in next(lines).strip().split(",") if v.strip()]
+        next(lines)  # time, temperature header
+        started = not T_cutoffs
+        offset = 0.0
+        for row in lines:
+            if not row.strip():
+                continue
+            head, rest = row.split(",", 1)
+            time = float(head) * 0.000001
+            vals = np.array([float(v) for v in rest.strip().split(",") if v.strip()])
+            T = np.transpose(1000 * np.resize(vals, dims))
+            if T_cutoffs:
+                if not started:
+                    if np.max(T) > starting_T:
+                        started = True
+                        offset = time
+                elif np.max(T) < ending_T:
+                    break
+            if reflect_X:
+                T = np.concatenate((np.flip(T, 1), T), axis=1)
+            if started:
+                times.append(time - offset)
+                frames.append(T)
+    write_xdmf_series(xdmf_path, times, frames, data_format=data_format)
+    return xdmf_path
+
+
+class ThermalHistory:
+    """The two frames bracketing the current time, advanced as the time grows
+    (``simulation.py:157-177,316-330``)."""
+
+    def __init__(self, path, shape):
+        self.loader = XDMFLoader(path)
+        if self.loader.num_steps < 2:
+            raise ValueError("a thermal history needs at least two frames (%s has %d)"
+                             % (path, self.loader.num_steps))
+        self.shape = tuple(shape)
+        self.index = 1
+        self.bounds = [0.0, 0.0]
+        self.arrays = [None, None]
+        self.bounds[0], self.arrays[0] = self._frame(0)
+        self.bounds[1], self.arrays[1] = self._frame(1)
+
+    def _frame(self, k):
+        t, a = self.loader.data(k)
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError("thermal history frame %d has shape %s, the simulation %s" % (k, a.shape, self.shape))
+        return t, a
+
+    def advance_to(self, time):
+        """Move the window until time <= the later frame's time; returns the
+        number of frames loaded."""
+        loaded = 0
+        while time > self.bounds[1]:
+            if self.index + 1 >= self.loader.num_steps:
+                raise ValueError("the thermal history ends at t=%g, before t=%g" % (self.bounds[1], time))
+            self.index += 1
+            self.bounds[0], self.arrays[0] = self.bounds[1], self.arrays[1]
+            self.bounds[1], self.arrays[1] = self._frame(self.index)
+            loaded += 1
+        return loaded
+
+    def range(self):
+        """(min, max) temperature over every frame of the history."""
+        lo, hi = np.inf, -np.inf
+        for k in range(self.loader.num_steps):
+            a = self.loader.data(k)[1]
+            lo, hi = min(lo, float(np.min(a))), max(hi, float(np.max(a)))
+        return lo, hi
+
+    def at(self, time):
+        """Interpolated interior temperature, in the GPU kernel's operation
+        order: T0 * (1 - r) + T1 * r, r = (t - t0) / (t1 - t0)."""
+        r1 = (time - self.bounds[0]) / (self.bounds[1] - self.bounds[0])
+        r0 = 1.0 - r1
+        return self.arrays[0] * r0 + self.arrays[1] * r1

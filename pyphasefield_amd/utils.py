@@ -1,13 +1,15 @@
 """Host utilities exported at package level (ppf_utils.py of the reference):
-the colormaps, ``make_seed`` and the ThermoModel container that stands in for
-TDBContainer on this engine path.  XDMF/CSV thermal-history readers are not
-part of this engine path."""
+the colormaps, ``make_seed``, the ThermoModel container that stands in for
+TDBContainer on this engine path, and the thermal-history helpers CSVtoXDMF
+and XDMFLoader (``ppf_utils.py:63-123``, implemented in ``thermal.py``)."""
 
 import numpy as np
 
 from .thermo import ThermoModel as TDBContainer  # noqa: F401
+from .thermal import CSVtoXDMF, XDMFLoader  # noqa: F401
 
 __all__ = ["COLORMAP_OTHER", "COLORMAP_PHASE", "COLORMAP_PHASE_INV", "make_seed", "TDBContainer",
+           "CSVtoXDMF", "XDMFLoader",
            "successfully_imported_pycalphad", "successfully_imported_numba"]
 
 _OTHER = [(0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 1, 0), (1, 0, 0)]

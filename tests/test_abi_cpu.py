@@ -40,7 +40,7 @@ def test_struct_layouts_match_header():
     """ctypes mirrors of the C structs have the sizes the C compiler gives."""
     from pyphasefield_amd import _lib
     assert ctypes.sizeof(_lib.Grid) == 4 * 6 + 8 * 3 + 8 * 4
-    assert ctypes.sizeof(_lib.Temperature) == 8 + 6 * 8 + 8 + 16
+    assert ctypes.sizeof(_lib.Temperature) == 8 + 6 * 8 + 8 + 16 + 8 + 16
     assert ctypes.sizeof(_lib.WarrenParams) == 17 * 8
     assert ctypes.sizeof(_lib.NcompParams) == 8 + 15 * 8 + 6 * 4 * 8
     assert ctypes.sizeof(_lib.RKPhase) == 8 + 8 + 4 * 5 * 8 + 6 * 3 * 2 * 8

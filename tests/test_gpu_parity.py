@@ -338,7 +338,7 @@ def test_warren_fp32_vs_fp64_oracle(gpu):
 
 
 # -------------------------------------------------------------- N-component --
-def _nc_sim(E, ny, nx, precision, noise, bc="NEUMANN", temp="LINEAR_GRADIENT", dg=None, seeds=3):
+def _nc_sim(E, ny, nx, precision, noise, bc="NEUMANN", temp="LINEAR_GRADIENT", dg=None, seeds=3, t_path=None):
     from pyphasefield_amd import thermo
     model, _ = thermo.synthetic_rk_model(seed=0)
     sim = E.NCGPU_new(dimensions=[ny, nx], seed=7)
@@ -346,6 +346,8 @@ def _nc_sim(E, ny, nx, precision, noise, bc="NEUMANN", temp="LINEAR_GRADIENT", d
     sim.set_dx(0.0000046)
     sim.set_temperature_type(temp)
     sim.set_temperature_initial_T(1574.)
+    if t_path is not None:
+        sim.set_temperature_path(t_path)
     if temp == "LINEAR_GRADIENT":
         sim.set_temperature_dTdx(1.0 / 0.0000046)   # 1 K per cell
         sim.set_temperature_dTdy(0.0)
@@ -389,6 +391,79 @@ def test_ncomp_fp64_parity(gpu, noise, bc):
     want = run()
     for i in range(len(sim.fields)):
         assert rel_l2(sim.fields[i].data, want[i]) <= FP64_TOL, sim.fields[i].name
+
+
+@pytest.mark.parametrize("fmt", ["XML", "Binary"])
+def test_ncomp_fp64_thermal_history(gpu, tmp_path, fmt):
+    """XDMF_FILE temperature: 30 steps across five frame windows; the step
+    kernels interpolate between the two frames on the GPU in the order of
+    update_thermal_file_*_kernel, as the oracle does on the host (the
+    reference's host formula differs by about one ulp of T, which the noisy
+    phi dynamics amplify to ~1e-9 over 30 steps)."""
+    from pyphasefield_amd import thermal
+    E = _engines()
+    ny, nx = 36, 44
+    probe, _ = _nc_sim(E, ny, nx, "FP64", 0.0, temp="ISOTHERMAL")
+    dt = probe.dt
+    y, x = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    times = [k * 6.5 * dt for k in range(7)]
+    frames = [1574.0 + x + 3.0 * np.sin(1.3 * k) + 0.2 * y * k for k in range(7)]
+    path = str(tmp_path / "T.xdmf")
+    thermal.write_xdmf_series(path, times, frames, data_format=fmt)
+    sim, model = _nc_sim(E, ny, nx, "FP64", 1.0, temp="XDMF_FILE", t_path=path)
+    assert sim.dt == dt
+    sim.just_before_simulating()
+    sim._begun_simulation = True
+    u = sim.user_data
+    fields0 = np.stack([f.get_cells().copy() for f in sim.fields])
+    liq = (model.liquid.g0, model.liquid.L, model.liquid.R)
+    sol = (model.solid.g0, model.solid.L, model.solid.R)
+    T_of = lambda s: padded_T(oe.temperature_history(times, frames, (s - 1) * dt), "NEUMANN")
+    sim.simulate(30)
+    assert sim._thermal.index == 5  # 30*dt lies in (t4, t5]
+    want = run_ncomp(fields0, 30, "NEUMANN", T_of, u["params"], u["c_params"], liq, sol, seed=sim.seed)
+    for i in range(len(sim.fields)):
+        assert rel_l2(sim.fields[i].data, want[i]) <= FP64_TOL, sim.fields[i].name
+    T = sim.temperature.get_cells()
+    assert np.allclose(T, oe.temperature_history(times, frames, 30 * dt), rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("kind", ["LINEAR_GRADIENT", "XDMF_FILE"])
+def test_temperature_3d_providers(gpu, tmp_path, kind):
+    """3D temperature as the step kernels see it (materialised by
+    ppf_fill_temperature on retrieval): a linear gradient along x, y, z and t
+    (simulation.py:134-156), and a thermal history of 3D frames advanced
+    over three windows (ppf_gpu_utils.py:382-394); halos by the temperature
+    rule of each axis."""
+    from pyphasefield_amd import thermal
+    E = _engines()
+    shape = [6, 7, 9]
+    bc = ["PERIODIC", "NEUMANN", "DIRCHLET"]
+    dx, dt = 2e-6, 3e-7
+    sim = E.Diffusion(dimensions=shape)
+    sim.set_temperature_type(kind)
+    if kind == "LINEAR_GRADIENT":
+        sim.set_temperature_initial_T(1600.0)
+        sim.set_temperature_dTdx(1.0e5)
+        sim.set_temperature_dTdy(-3.0e5)
+        sim.set_temperature_dTdz(7.0e5)
+        sim.set_temperature_dTdt(-2.0e6)
+    else:
+        rng = np.random.default_rng(5)
+        times = [0.0, 7 * dt, 13 * dt, 30 * dt]
+        frames = [1500.0 + 100.0 * rng.random(shape) for _ in times]
+        path = str(tmp_path / "T3.xdmf")
+        thermal.write_xdmf_series(path, times, frames, data_format="Binary")
+        sim.set_temperature_path(path)
+    _setup(sim, bc, dx, dt, "FP64")
+    sim.simulate(25)
+    if kind == "LINEAR_GRADIENT":
+        Ti = oe.temperature_linear(tuple(shape), 1600.0, 1.0e5, -3.0e5, dx, -2.0e6, dt, 25, dTdz=7.0e5)
+    else:
+        assert sim._thermal.index == 3
+        Ti = oe.temperature_history(times, frames, 25 * dt)
+    want = padded_T(Ti, bc)
+    assert np.allclose(sim.temperature.data, want, rtol=1e-15, atol=0)
 
 
 def interface_error_cells(phi_a, phi_b, level=0.5):
