@@ -210,6 +210,56 @@ int64_t ppf_reduce_work_bytes(const ppf_grid* g);
 int ppf_reduce(const ppf_grid* g, const void* in, const void* bcvals, int kind, double epsilon,
                double* out, void* work, ppf_stream_t s);
 
+/* ---- implicit and Crank-Nicolson diffusion (Engines/Diffusion.py:18-633) ----
+   Every matrix the reference builds (diffusion_matrix_1d/2d/3d,
+   Diffusion.py:18-196) is periodic; these entry points require PERIODIC on
+   every axis and at least 3 cells per axis.  They act on field 0 of the
+   stack; `in` and `out` are distinct padded stacks with the layout of g and
+   the halos of `out` are left untouched. */
+
+/* Host only (no device work): coefficients of the cyclic tridiagonal system
+     -a x[i-1] + (1+2a) x[i] - a x[i+1] = d[i]   (indices mod n)
+   i.e. diffusion_matrix_1d(n, 1+2a, -a) (Diffusion.py:18-64), for the
+   corner-free Thomas elimination plus a Sherman-Morrison correction.
+   table must hold ppf_cyclic_table_len(n) doubles (-1 when n < 3). */
+int64_t ppf_cyclic_table_len(int64_t n);
+int ppf_cyclic_table(int64_t n, double a, double* table);
+
+/* For every grid line along `axis` (0 = x, 1 = y, 2 = z):
+     out = M^-1 rhs,  rhs = E(in),
+   M = the cyclic matrix of `table` (a device copy of ppf_cyclic_table(n[axis], a)),
+   E = identity (explicit_axis = -1) or the explicit half step along
+   explicit_axis, (1-2b) c + b (c[-1] + c[+1]) with periodic neighbours.
+   Replaces np.linalg.solve / the np.dot(inv, line) loops of the 1D and
+   alternating-direction solvers (Diffusion.py:210-241, 309-343, 413-633). */
+int ppf_diffusion_line_solve(const ppf_grid* g, const void* in, void* out, int axis, const double* table,
+                             int explicit_axis, double b, ppf_stream_t s);
+
+/* Full implicit / Crank-Nicolson systems of diffusion_matrix_2d/3d
+   (Diffusion.py:243-308, 344-412):
+     (1 + 2*ndim*a) x - a * sum_{6 or 4 neighbours} x = rhs,
+     rhs = in (b = 0) or (1 - 2*ndim*b) in + b * sum_nb in,
+   solved by matrix-free conjugate gradients from x0 = in.  work holds
+   ppf_diffusion_cg_work_bytes(g) bytes of device memory, the first
+   sizeof(ppf_cg_state) of them the solver state.  ppf_diffusion_cg_begin
+   writes x0 into out and the initial residual; ppf_diffusion_cg_iterate
+   enqueues `iterations` iterations numbered first_iteration.. (the caller
+   numbers them consecutively from 0 after a begin); once r.r <= tol^2 rhs.rhs
+   the state is marked converged and further iterations do nothing.  No host
+   synchronisation: the caller reads the state (e.g. every few dozen
+   iterations) to decide whether to continue. */
+typedef struct ppf_cg_state {
+  double bb;          /* rhs . rhs */
+  double rr;          /* r . r at the latest convergence test */
+  int32_t iterations; /* iterations taken once converged, else -1 */
+  int32_t converged;
+} ppf_cg_state;
+int64_t ppf_diffusion_cg_work_bytes(const ppf_grid* g);
+int ppf_diffusion_cg_begin(const ppf_grid* g, const void* in, void* out, double a, double b, void* work,
+                           ppf_stream_t s);
+int ppf_diffusion_cg_iterate(const ppf_grid* g, void* out, double a, double tol, int64_t first_iteration,
+                             int iterations, void* work, ppf_stream_t s);
+
 /* Plain thermodynamics at scattered points: for each of n points with
    temperature T[i] and component fractions c[i*(N-1)+l] (c_N implied),
    write for each phase (liquid then solid) G and dG/dc_l:

@@ -1,0 +1,144 @@
+"""Implicit and Crank-Nicolson diffusion, FP64 numpy restatement (oracle).
+
+Test infrastructure only: the product path never imports this module.
+
+Follows Engines/Diffusion.py:18-633.  The reference works on interior arrays
+(axes (z, y, x), x last) with periodic matrices; its defects D1 (`self` for
+`sim`) and the padded/interior size mismatch of `c.data` are resolved by
+operating on the interior cells.  a = D*dt/dx^2 throughout; Crank-Nicolson
+uses b = a/2 for both halves.
+
+Two independent formulations:
+  * ``step``: dense matrices built like diffusion_matrix_1d/2d/3d and the
+    reference's solve / inverse-per-line sequence (small sizes);
+  * ``spectral_step``: the same linear maps as Fourier multipliers (any size),
+    which pins the dense construction (neighbour offsets, wrap, centre value,
+    axis order of the alternating-direction sweeps) and serves as the oracle
+    at sizes where dense matrices do not fit.
+"""
+
+import numpy as np
+
+
+def _shift_sum(n):
+    """S + S^T for the cyclic shift S of size n (n >= 3)."""
+    e = np.eye(n)
+    return np.roll(e, 1, axis=1) + np.roll(e, -1, axis=1)
+
+
+def matrix_1d(n, center, neighbor):
+    """diffusion_matrix_1d (Diffusion.py:18-64): center on the diagonal,
+    neighbor at (i, i+-1 mod n)."""
+    if n < 3:
+        raise ValueError("periodic diffusion matrices need n >= 3")
+    return center * np.eye(n) + neighbor * _shift_sum(n)
+
+
+def matrix_nd(shape, center, neighbor):
+    """diffusion_matrix_2d / _3d (Diffusion.py:66-196): row-major flattening
+    (x fastest), neighbor coupling along every axis with periodic wrap."""
+    if any(n < 3 for n in shape):
+        raise ValueError("periodic diffusion matrices need n >= 3 on every axis")
+    size = int(np.prod(shape))
+    A = center * np.eye(size)
+    for ax in range(len(shape)):
+        term = np.ones((1, 1))
+        for k, n in enumerate(shape):
+            term = np.kron(term, _shift_sum(n) if k == ax else np.eye(n))
+        A = A + neighbor * term
+    return A
+
+
+def explicit_axis(c, b, axis):
+    """(1-2b) c + b (roll(c, 1, axis) + roll(c, -1, axis))  (Diffusion.py:321, 526, 532, 583...)."""
+    return (1 - 2 * b) * c + b * (np.roll(c, 1, axis) + np.roll(c, -1, axis))
+
+
+def explicit_all(c, b):
+    """(1-2*nd*b) c + b * sum of the 2*nd rolled copies (Diffusion.py:355, 389)."""
+    nd = c.ndim
+    s = np.roll(c, 1, 0) + np.roll(c, -1, 0)
+    for ax in range(1, nd):
+        s = s + np.roll(c, 1, ax) + np.roll(c, -1, ax)
+    return (1 - 2 * nd * b) * c + b * s
+
+
+def _apply_lines(c, inv, axis):
+    """np.dot(inv, line) for every line along `axis` (the reference loops)."""
+    return np.moveaxis(np.tensordot(inv, np.moveaxis(c, axis, 0), axes=(1, 0)), 0, axis)
+
+
+def step(c, a, solver, adi=False):
+    """One step of the reference's implicit / Crank-Nicolson engines on an
+    interior array c (1D, 2D or 3D); np.linalg.solve and gmres variants solve
+    the same system, so `gmres` has no counterpart here."""
+    nd = c.ndim
+    if solver == "implicit":
+        if nd == 1:  # engine_ImplicitDiffusion1D, :210-224
+            return np.linalg.solve(matrix_1d(c.shape[0], 1 + 2 * a, -a), c)
+        if not adi:  # :243-257, :276-290
+            A = matrix_nd(c.shape, 1 + 2 * nd * a, -a)
+            return np.linalg.solve(A, c.ravel()).reshape(c.shape)
+        # :413-433 (x lines, then y lines), :456-483 (x, y, z)
+        out = c.copy()
+        for axis in range(nd - 1, -1, -1):
+            inv = np.linalg.inv(matrix_1d(c.shape[axis], 1 + 2 * a, -a))
+            out = _apply_lines(out, inv, axis)
+        return out
+    if solver == "crank-nicolson":
+        b = 0.5 * a
+        if nd == 1:  # :309-324
+            rhs = explicit_axis(c, b, 0)
+            return np.linalg.solve(matrix_1d(c.shape[0], 1 + 2 * b, -b), rhs)
+        if not adi:  # :344-358, :378-393
+            A = matrix_nd(c.shape, 1 + 2 * nd * b, -b)
+            return np.linalg.solve(A, explicit_all(c, b).ravel()).reshape(c.shape)
+        inv = [np.linalg.inv(matrix_1d(n, 1 + 2 * b, -b)) for n in c.shape]
+        if nd == 2:  # :512-537: explicit axis 0 + lines along axis 0, then axis 1 + lines along 1
+            out = _apply_lines(explicit_axis(c, b, 0), inv[0], 0)
+            return _apply_lines(explicit_axis(out, b, 1), inv[1], 1)
+        # :565-599: explicit axis 0 + lines along 1; explicit 1 + lines along 0;
+        # explicit 2 + lines along 2
+        out = _apply_lines(explicit_axis(c, b, 0), inv[1], 1)
+        out = _apply_lines(explicit_axis(out, b, 1), inv[0], 0)
+        return _apply_lines(explicit_axis(out, b, 2), inv[2], 2)
+    raise ValueError("solver must be 'implicit' or 'crank-nicolson'")
+
+
+def _symbols(shape):
+    """2 (1 - cos(2 pi k / n)) per axis, broadcast to the FFT grid."""
+    out = []
+    for ax, n in enumerate(shape):
+        s = 2.0 * (1.0 - np.cos(2.0 * np.pi * np.arange(n) / n))
+        sh = [1] * len(shape)
+        sh[ax] = n
+        out.append(s.reshape(sh))
+    return out
+
+
+def _product(factors):
+    out = factors[0]
+    for f in factors[1:]:
+        out = out * f
+    return out
+
+
+def spectral_step(c, a, solver, adi=False):
+    """The same step as a Fourier multiplier: with s_ax = 2(1 - cos theta_ax),
+    implicit: 1 / (1 + a sum s); implicit ADI: prod 1 / (1 + a s_ax);
+    Crank-Nicolson (b = a/2): (1 - b sum s) / (1 + b sum s);
+    CN ADI: prod (1 - b s_ax) / (1 + b s_ax)  (all factors commute on a
+    periodic grid, so the sweep order only changes rounding)."""
+    s = _symbols(c.shape)
+    split = adi or c.ndim == 1
+    if solver == "implicit":
+        g = _product([1.0 / (1.0 + a * x) for x in s]) if split else 1.0 / (1.0 + a * sum(s))
+    elif solver == "crank-nicolson":
+        b = 0.5 * a
+        if split:
+            g = _product([(1.0 - b * x) / (1.0 + b * x) for x in s])
+        else:
+            g = (1.0 - b * sum(s)) / (1.0 + b * sum(s))
+    else:
+        raise ValueError("solver must be 'implicit' or 'crank-nicolson'")
+    return np.real(np.fft.ifftn(np.fft.fftn(c) * g))

@@ -1,12 +1,17 @@
-"""Diffusion engine (Engines/Diffusion.py:635-733 of the reference), explicit
-solver on the GPU: c <- c + dt*D*lap(c) (Diffusion.py:198-208, with the D1
-resolution: D from user_data).  Replaces DiffusionGPU (DiffusionGPU.py:18-117),
-which omitted dt and could not run."""
+"""Diffusion engine (Engines/Diffusion.py:635-733 of the reference) on the GPU.
+
+user_data["solver"]:
+* "explicit": c <- c + dt*D*lap(c) (Diffusion.py:198-208, with the D1
+  resolution: D from user_data).  Replaces DiffusionGPU (DiffusionGPU.py:18-117),
+  which omitted dt and could not run.
+* "implicit" / "crank-nicolson", with user_data["adi"] and ["gmres"]
+  (Diffusion.py:210-633): periodic line solves and conjugate gradients, see
+  ``_implicit``."""
 
 import numpy as np
 
 from ..simulation import Simulation
-from . import _gpu
+from . import _gpu, _implicit
 
 
 class Diffusion(Simulation):
@@ -30,6 +35,13 @@ class Diffusion(Simulation):
     def _graph_key(self):
         return (float(self.user_data["D"]), float(self.dt))
 
+    def _graph_chunk(self):
+        if self.user_data.get("solver", "explicit") != "explicit":
+            return 0  # conjugate gradients read their state back between iterations
+        return super()._graph_chunk()
+
     def simulation_loop(self):
-        _gpu.solver_check(self)
+        if self.user_data.get("solver", "explicit") != "explicit":
+            _implicit.step(self)
+            return
         _gpu.stencil_step(self, "ppf_diffusion_step", float(self.user_data["D"]), float(self.dt), nactive=1)

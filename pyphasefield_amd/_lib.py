@@ -105,6 +105,12 @@ def lib():
         "ppf_reduce_work_bytes": ([P(Grid)], c_i64),
         "ppf_reduce": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_f64, c_vp, c_vp, c_vp], c_i32),
         "ppf_thermo_eval": ([P(Thermo), c_i64, c_vp, c_vp, c_vp, c_vp], c_i32),
+        "ppf_cyclic_table_len": ([c_i64], c_i64),
+        "ppf_cyclic_table": ([c_i64, c_f64, c_vp], c_i32),
+        "ppf_diffusion_line_solve": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_f64, c_vp], c_i32),
+        "ppf_diffusion_cg_work_bytes": ([P(Grid)], c_i64),
+        "ppf_diffusion_cg_begin": ([P(Grid), c_vp, c_vp, c_f64, c_f64, c_vp, c_vp], c_i32),
+        "ppf_diffusion_cg_iterate": ([P(Grid), c_vp, c_f64, c_f64, c_i64, ctypes.c_int, c_vp, c_vp], c_i32),
     }
     optional = {}
     try:

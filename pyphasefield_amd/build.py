@@ -33,6 +33,7 @@ UNITS = {
     "ppf_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "ppf_f64.hip": ["-ffp-contract=off"],
     "ppf_mlp.hip": [],
+    "ppf_implicit.hip": [],
     # same arithmetic flags as the single-instance FP32 kernels it must match
     "ppf_batched.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }

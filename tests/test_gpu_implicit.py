@@ -1,0 +1,92 @@
+"""Implicit and Crank-Nicolson diffusion on the GPU (Engines/Diffusion.py:210-733)
+against the oracle: the reference's dense solves at small sizes, the
+spectral form of the same operators at larger ones."""
+
+import numpy as np
+import pytest
+
+from oracle import implicit as oi
+from tests.oracle_runner import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def _sim(shape, solver, adi, precision="FP64", a=2.5, gmres=False, seed=0):
+    import pyphasefield_amd.Engines as E
+    sim = E.Diffusion(dimensions=list(shape))
+    sim.set_user_data({"D": 1.0, "solver": solver, "adi": adi, "gmres": gmres})
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(1.0)
+    sim.set_dt(a)  # a = D dt / dx^2
+    sim.set_boundary_conditions("PERIODIC")
+    sim.set_precision(precision)
+    sim.initialize_fields_and_imported_data()
+    c0 = np.random.default_rng(seed).random(shape)
+    sim.fields[0].data[tuple(slice(1, -1) for _ in shape)] = c0
+    return sim, c0
+
+
+CASES = [((40,), False), ((18, 22), False), ((18, 22), True), ((7, 9, 11), False), ((7, 9, 11), True)]
+
+
+@pytest.mark.parametrize("solver", ["implicit", "crank-nicolson"])
+@pytest.mark.parametrize("shape,adi", CASES)
+def test_implicit_steps_match_reference_dense_solves(gpu, solver, shape, adi):
+    sim, c0 = _sim(shape, solver, adi)
+    steps = 3
+    sim.simulate(steps)
+    want = c0
+    for _ in range(steps):
+        want = oi.step(want, 2.5, solver, adi)
+    assert rel_l2(sim.fields[0].get_cells(), want) <= FP64_TOL
+
+
+def test_gmres_flag_solves_the_same_system(gpu):
+    sim, c0 = _sim((18, 22), "crank-nicolson", False, gmres=True)
+    sim.simulate(2)
+    want = oi.step(oi.step(c0, 2.5, "crank-nicolson"), 2.5, "crank-nicolson")
+    assert rel_l2(sim.fields[0].get_cells(), want) <= FP64_TOL
+
+
+@pytest.mark.parametrize("solver,adi,shape,a", [("crank-nicolson", False, (384, 640), 3.0),
+                                                ("implicit", False, (48, 40, 56), 1.5),
+                                                ("implicit", True, (48, 40, 56), 4.0),
+                                                ("crank-nicolson", True, (256, 320), 0.7)])
+def test_implicit_steps_at_larger_sizes_match_fourier_multipliers(gpu, solver, adi, shape, a):
+    """Beyond dense-matrix sizes: the spectral form of the same operators;
+    mass is conserved to rounding."""
+    sim, c0 = _sim(shape, solver, adi, a=a, seed=3)
+    sim.simulate(2)
+    got = sim.fields[0].get_cells()
+    want = oi.spectral_step(oi.spectral_step(c0, a, solver, adi), a, solver, adi)
+    assert rel_l2(got, want) <= FP64_TOL
+    assert abs(got.sum() - c0.sum()) <= 1e-11 * abs(c0.sum())
+
+
+def test_implicit_fp32_conjugate_gradients(gpu):
+    sim, c0 = _sim((256, 256), "implicit", False, precision="FP32", a=2.0, seed=5)
+    sim.simulate(2)
+    want = oi.spectral_step(oi.spectral_step(c0, 2.0, "implicit"), 2.0, "implicit")
+    assert rel_l2(sim.fields[0].get_cells(), want) <= 1e-5
+
+
+def test_fp32_line_solves(gpu):
+    sim, c0 = _sim((64, 48, 40), "crank-nicolson", True, precision="FP32", a=1.2, seed=6)
+    sim.simulate(2)
+    want = oi.spectral_step(oi.spectral_step(c0, 1.2, "crank-nicolson", True), 1.2, "crank-nicolson", True)
+    assert rel_l2(sim.fields[0].get_cells(), want) <= 1e-5
+
+
+def test_conjugate_gradients_stop_early_and_report(gpu):
+    """The device-side convergence test stops the iterations (later launches
+    do nothing) and records the iteration count; more iterations than needed
+    leave the solution unchanged."""
+    sim, c0 = _sim((64, 64), "implicit", False, a=0.5, seed=7)
+    sim.user_data["cg_check_every"] = 200
+    sim.simulate(1)
+    it = sim._cg_iterations
+    assert 0 < it < 200
+    want = oi.spectral_step(c0, 0.5, "implicit")
+    assert rel_l2(sim.fields[0].get_cells(), want) <= FP64_TOL
