@@ -210,6 +210,29 @@ int64_t ppf_reduce_work_bytes(const ppf_grid* g);
 int ppf_reduce(const ppf_grid* g, const void* in, const void* bcvals, int kind, double epsilon,
                double* out, void* work, ppf_stream_t s);
 
+/* ---- AnisoDorr binary polycrystalline solidification (Engines/AnisoDorrGPU.py) ----
+   2D; field stack (phi, q1, q4, c); transfer stack of 6 arrays with the layout
+   convention of g (tg->nfields = 6).  The reference hard-codes these constants
+   in both kernels (AnisoDorrGPU.py:63-105); derived ones (ebar, eqbar, W_A,
+   W_B, M_A, M_B, dt) are computed by the caller as there. */
+typedef struct ppf_aniso_params {
+  double T, M_qmax, H, T_mA, T_mB, L_A, L_B, D_L, D_S, v_m, R, y_e;
+  double beta;  /* clipping value of the squared quaternion gradients (1.5) */
+  double alpha; /* noise amplitude (0.3) */
+  double ebar, eqbar, W_A, W_B, M_A, M_B, dt;
+} ppf_aniso_params;
+
+/* Vertex and diffusion terms of every padded cell: replaces
+   AnisoDorr_helper_kernel (AnisoDorrGPU.py:262-375). */
+int ppf_aniso_helper(const ppf_grid* g, const void* in, const void* bcvals, const ppf_grid* tg, void* transfer,
+                     const ppf_aniso_params* p, ppf_stream_t s);
+
+/* One explicit step from the transfer stack ppf_aniso_helper wrote: replaces
+   AnisoDorr_kernel (AnisoDorrGPU.py:54-245).  Noise: uniform [0,1) from the
+   counter RNG keyed by (seed, step). */
+int ppf_aniso_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_grid* tg,
+                   const void* transfer, const ppf_aniso_params* p, uint64_t seed, int64_t step, ppf_stream_t s);
+
 /* ---- implicit and Crank-Nicolson diffusion (Engines/Diffusion.py:18-633) ----
    Every matrix the reference builds (diffusion_matrix_1d/2d/3d,
    Diffusion.py:18-196) is periodic; these entry points require PERIODIC on

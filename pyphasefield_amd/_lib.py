@@ -75,6 +75,11 @@ class Thermo(ctypes.Structure):
                 ("tables", ctypes.c_void_p)]
 
 
+class AnisoParams(ctypes.Structure):
+    _fields_ = [(n, c_f64) for n in ("T", "M_qmax", "H", "T_mA", "T_mB", "L_A", "L_B", "D_L", "D_S", "v_m", "R",
+                                     "y_e", "beta", "alpha", "ebar", "eqbar", "W_A", "W_B", "M_A", "M_B", "dt")]
+
+
 _lib = None
 
 
@@ -105,6 +110,8 @@ def lib():
         "ppf_reduce_work_bytes": ([P(Grid)], c_i64),
         "ppf_reduce": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_f64, c_vp, c_vp, c_vp], c_i32),
         "ppf_thermo_eval": ([P(Thermo), c_i64, c_vp, c_vp, c_vp, c_vp], c_i32),
+        "ppf_aniso_helper": ([P(Grid), c_vp, c_vp, P(Grid), c_vp, P(AnisoParams), c_vp], c_i32),
+        "ppf_aniso_step": ([P(Grid), c_vp, c_vp, c_vp, P(Grid), c_vp, P(AnisoParams), c_u64, c_i64, c_vp], c_i32),
         "ppf_cyclic_table_len": ([c_i64], c_i64),
         "ppf_cyclic_table": ([c_i64, c_f64, c_vp], c_i32),
         "ppf_diffusion_line_solve": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_f64, c_vp], c_i32),

@@ -164,6 +164,35 @@ int ppf_ncomp_step(const ppf_grid* g, const void* in, void* out, const void* bcv
              : ncomp_step<double>(*g, in, out, bcvals, *t, *tg, transfer, *p, seed, step, stream_of(s));
 }
 
+static int check_aniso(const ppf_grid* g, const ppf_grid* tg, const ppf_aniso_params* p, const char* what) {
+  if (int rc = check_grid(g, 2, 2, what)) return rc;
+  if (!p || !tg) return fail(PPF_EINVAL, std::string(what) + ": null params");
+  if (g->nfields != 4) return fail(PPF_EINVAL, std::string(what) + ": field stack must hold phi, q1, q4, c");
+  if (tg->nfields != 6 || tg->n[0] != g->n[0] || tg->n[1] != g->n[1] || tg->dtype != g->dtype ||
+      tg->sy < g->n[0] + 2 || tg->sf < tg->sy * (g->n[1] + 2))
+    return fail(PPF_EINVAL, std::string(what) + ": transfer grid does not match");
+  if (!(p->T > 0.0) || !(p->R > 0.0) || !(p->beta > 0.0))
+    return fail(PPF_EINVAL, std::string(what) + ": T, R and beta must be positive");
+  return 0;
+}
+
+int ppf_aniso_helper(const ppf_grid* g, const void* in, const void* bcvals, const ppf_grid* tg, void* transfer,
+                     const ppf_aniso_params* p, ppf_stream_t s) {
+  if (int rc = check_aniso(g, tg, p, "aniso_helper")) return rc;
+  if (!in || !transfer) return fail(PPF_EINVAL, "aniso_helper: null argument");
+  return g->dtype == PPF_F32 ? aniso_helper<float>(*g, in, bcvals, *tg, transfer, *p, stream_of(s))
+                             : aniso_helper<double>(*g, in, bcvals, *tg, transfer, *p, stream_of(s));
+}
+
+int ppf_aniso_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_grid* tg,
+                   const void* transfer, const ppf_aniso_params* p, uint64_t seed, int64_t step, ppf_stream_t s) {
+  if (int rc = check_aniso(g, tg, p, "aniso_step")) return rc;
+  if (!in || !out || in == out || !transfer) return fail(PPF_EINVAL, "aniso_step: null or aliased argument");
+  return g->dtype == PPF_F32
+             ? aniso_step<float>(*g, in, out, bcvals, *tg, transfer, *p, seed, step, stream_of(s))
+             : aniso_step<double>(*g, in, out, bcvals, *tg, transfer, *p, seed, step, stream_of(s));
+}
+
 int64_t ppf_reduce_work_bytes(const ppf_grid* g) {
   if (!g) return 0;
   return int64_t(kReduceBlocks) * int64_t(g->nfields > 0 ? g->nfields : 1) * int64_t(sizeof(double));

@@ -2,6 +2,7 @@
 // single precision; FP64 reductions inside the reduce kernels).
 #include "ppf_ncomp.inl"
 #include "ppf_warren.inl"
+#include "ppf_aniso.inl"
 
 namespace ppf {
 template int apply_bc<float>(const ppf_grid&, void*, const void*, hipStream_t);
@@ -18,6 +19,10 @@ template int warren_step<float>(const ppf_grid&, const void*, void*, const void*
 template int ncomp_thermo<float>(const ppf_grid&, const void*, const void*, const ppf_temperature&,
                                  const ppf_grid&, void*, const ppf_ncomp_params&, const ppf_thermo&, uint64_t,
                                  int64_t, hipStream_t);
+template int aniso_helper<float>(const ppf_grid&, const void*, const void*, const ppf_grid&, void*,
+                                 const ppf_aniso_params&, hipStream_t);
+template int aniso_step<float>(const ppf_grid&, const void*, void*, const void*, const ppf_grid&, const void*,
+                               const ppf_aniso_params&, uint64_t, int64_t, hipStream_t);
 template int ncomp_step<float>(const ppf_grid&, const void*, void*, const void*, const ppf_temperature&,
                                const ppf_grid&, const void*, const ppf_ncomp_params&, uint64_t, int64_t,
                                hipStream_t);

@@ -3,6 +3,7 @@
 // of the reference formulas (oracle parity mode).
 #include "ppf_ncomp.inl"
 #include "ppf_warren.inl"
+#include "ppf_aniso.inl"
 
 namespace ppf {
 template int apply_bc<double>(const ppf_grid&, void*, const void*, hipStream_t);
@@ -19,6 +20,10 @@ template int warren_step<double>(const ppf_grid&, const void*, void*, const void
 template int ncomp_thermo<double>(const ppf_grid&, const void*, const void*, const ppf_temperature&,
                                  const ppf_grid&, void*, const ppf_ncomp_params&, const ppf_thermo&, uint64_t,
                                  int64_t, hipStream_t);
+template int aniso_helper<double>(const ppf_grid&, const void*, const void*, const ppf_grid&, void*,
+                                 const ppf_aniso_params&, hipStream_t);
+template int aniso_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_grid&, const void*,
+                               const ppf_aniso_params&, uint64_t, int64_t, hipStream_t);
 template int ncomp_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_temperature&,
                                const ppf_grid&, const void*, const ppf_ncomp_params&, uint64_t, int64_t,
                                hipStream_t);

@@ -82,3 +82,15 @@ def composition_errors(c_got, c_want, phi_got, phi_want):
     bulk = max(float(np.abs(a - b)[~band].max()) for a, b in zip(c_got, c_want))
     l2 = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(c_got, c_want))
     return full, bulk, l2
+
+
+def run_aniso(fields_int, steps, bc, p, dx, seed=0, step0=0, bcarray=None):
+    """AnisoDorr oracle: fields_int [4, ny, nx] interiors (phi, q1, q4, c)."""
+    from oracle import aniso as oa
+    f = np.stack([obc.pad(x) for x in fields_int])
+    obc.apply_bc(f, bc, bcarray, dx)
+    for s in range(step0 + 1, step0 + steps + 1):
+        tr = oa.helper(f, p, dx)
+        f = oa.step(f, tr, p, dx, seed=seed, step=s)
+        obc.apply_bc(f, bc, bcarray, dx)
+    return f
