@@ -283,6 +283,33 @@ int ppf_diffusion_cg_begin(const ppf_grid* g, const void* in, void* out, double 
 int ppf_diffusion_cg_iterate(const ppf_grid* g, void* out, double a, double tol, int64_t first_iteration,
                              int iterations, void* work, ppf_stream_t s);
 
+/* Allen-Cahn semi-implicit steps (Engines/CahnAllen.py:137-280), periodic
+   boxes as for the diffusion solvers.  The double-well force
+   16 W (4 phi^3 - 6 phi^2 + 2 phi) is linearised as 16 W s(phi_old) phi_new,
+   s(phi) = 4 phi^2 - 6 phi + 2, which adds src_k * s to the diagonal
+   (src_k = 16 W M dt for a full solve, 8 W M dt per 2D alternating-direction
+   sweep, 16 W M dt / 3 per 3D sweep).
+
+   Line solve along `axis`: rhs d = E(in) (explicit half step along
+   explicit_axis with coefficient b, as in ppf_diffusion_line_solve), then
+   with imex_k != 0 d <- d - imex_k (4 d^3 - 6 d^2 + 2 d); the cyclic system
+   has off-diagonal -a and diagonal 1 + 2a + src_k s(d) per cell.  work holds
+   ppf_allencahn_line_work_bytes(g) bytes of device scratch. */
+int64_t ppf_allencahn_line_work_bytes(const ppf_grid* g);
+int ppf_allencahn_line_solve(const ppf_grid* g, const void* in, void* out, int axis, double a, int explicit_axis,
+                             double b, double src_k, double imex_k, void* work, ppf_stream_t s);
+
+/* Full 2D/3D semi-implicit system (the GMRES engine, CahnAllen.py:165-177):
+     (1 + 2*ndim*a + src_k s(in)) x - a * sum_nb x = in,
+   conjugate gradients exactly as ppf_diffusion_cg_begin/_iterate (same work
+   size and state) with the per-cell diagonal; `in` must stay unchanged until
+   the iterations are done.  Symmetric positive definite while
+   1 + src_k s(in) > 0 (src_k < 4); otherwise the state does not converge. */
+int ppf_allencahn_cg_begin(const ppf_grid* g, const void* in, void* out, double a, double src_k, void* work,
+                           ppf_stream_t s);
+int ppf_allencahn_cg_iterate(const ppf_grid* g, const void* in, void* out, double a, double src_k, double tol,
+                             int64_t first_iteration, int iterations, void* work, ppf_stream_t s);
+
 /* Plain thermodynamics at scattered points: for each of n points with
    temperature T[i] and component fractions c[i*(N-1)+l] (c_N implied),
    write for each phase (liquid then solid) G and dG/dc_l:

@@ -1,12 +1,17 @@
 """Allen-Cahn engine (class CahnAllen, Engines/CahnAllen.py:285-362 of the
-reference), explicit solver on the GPU:
+reference) on the GPU.
+
+user_data["solver"] = "explicit":
 phi <- phi + dt*M*(eps^2 lap(phi) - 16 W (4phi^3 - 6phi^2 + 2phi))
-(CahnAllen.py:128-135, D3 resolution: parameters from user_data)."""
+(CahnAllen.py:128-135, D3 resolution: parameters from user_data).
+"implicit" / "imex" / "crank-nicolson" (with "gmres" and "adi"): the
+semi-implicit schemes of CahnAllen.py:137-280 on periodic boxes, see
+``_implicit.allencahn_step``."""
 
 import numpy as np
 
 from ..simulation import Simulation
-from . import _gpu
+from . import _gpu, _implicit
 
 
 class CahnAllen(Simulation):
@@ -29,8 +34,15 @@ class CahnAllen(Simulation):
         u = self.user_data
         return (float(u["M"]), float(u["W"]), float(u["epsilon"]), float(self.dt))
 
+    def _graph_chunk(self):
+        if self.user_data.get("solver", "explicit") != "explicit":
+            return 0
+        return super()._graph_chunk()
+
     def simulation_loop(self):
-        _gpu.solver_check(self)
+        if self.user_data.get("solver", "explicit") != "explicit":
+            _implicit.allencahn_step(self)
+            return
         u = self.user_data
         _gpu.stencil_step(self, "ppf_allencahn_step", float(u["M"]), float(u["W"]), float(u["epsilon"]),
                           float(self.dt), nactive=1)

@@ -81,37 +81,109 @@ def line_steps(sim, solver, a):
 
 
 def cg_step(sim, solver, a):
-    import torch
     L = _lib.lib()
     am = a if solver == "implicit" else 0.5 * a
     b = 0.0 if solver == "implicit" else am
-    nbytes = int(L.ppf_diffusion_cg_work_bytes(sim._grid))
-    work = getattr(sim, "_cg_work", None)
-    if work is None or work.numel() != nbytes or work.device != sim._fields_gpu_device.device:
-        work = torch.empty(nbytes, dtype=torch.uint8, device=sim._fields_gpu_device.device)
-        sim._cg_work = work
+    work = _work(sim, "_cg_work", int(L.ppf_diffusion_cg_work_bytes(sim._grid)))
     u = sim.user_data
     tol = float(u.get("cg_tol", 1e-12 if sim._precision == "FP64" else 1e-6))
-    max_it = int(u.get("cg_max_iterations", 5000))
-    chunk = int(u.get("cg_check_every", 32))
     s = _lib.stream_handle()
     _lib.check(L.ppf_diffusion_cg_begin(sim._grid, _lib.ptr(sim._fields_gpu_device),
                                         _lib.ptr(sim._fields_out_gpu_device), am, b, _lib.ptr(work), s),
                "diffusion_cg_begin")
+    out = _lib.ptr(sim._fields_out_gpu_device)
+    _cg_loop(sim, lambda k, n: _lib.check(L.ppf_diffusion_cg_iterate(sim._grid, out, am, tol, k, n, _lib.ptr(work),
+                                                                        s), "diffusion_cg_iterate"), tol)
+    _passive(sim)
+    sim._swap_buffers()
+
+
+def _work(sim, attr, nbytes):
+    import torch
+    work = getattr(sim, attr, None)
+    if work is None or work.numel() != nbytes or work.device != sim._fields_gpu_device.device:
+        work = torch.empty(nbytes, dtype=torch.uint8, device=sim._fields_gpu_device.device)
+        setattr(sim, attr, work)
+    return work
+
+
+def _cg_loop(sim, iterate, tol):
+    """Run CG iterations in chunks until the device state reports convergence."""
+    u = sim.user_data
+    max_it = int(u.get("cg_max_iterations", 5000))
+    chunk = int(u.get("cg_check_every", 32))
+    work = sim._cg_work
     k = 0
     while True:
-        _lib.check(L.ppf_diffusion_cg_iterate(sim._grid, _lib.ptr(sim._fields_out_gpu_device), am, tol, k, chunk,
-                                              _lib.ptr(work), s), "diffusion_cg_iterate")
+        iterate(k, chunk)
         k += chunk
         st = np.frombuffer(work[:_STATE.itemsize].cpu().numpy().tobytes(), dtype=_STATE)[0]
         if st["converged"]:
             sim._cg_iterations = int(st["iterations"])
-            break
+            return
         if k >= max_it:
             raise RuntimeError("conjugate gradients did not reach a relative residual of %g in %d iterations "
                                "(residual %g)" % (tol, k, np.sqrt(st["rr"] / st["bb"]) if st["bb"] else np.inf))
-    _passive(sim)
-    sim._swap_buffers()
+
+
+# Allen-Cahn: (solve axis, explicit axis) and the fraction of K = 16 W M dt per sweep
+_AC_PLANS = {
+    ("implicit", 2): ([(0, -1), (1, -1)], 0.5),          # CahnAllen.py:179-196
+    ("implicit", 3): ([(0, -1), (1, -1), (2, -1)], 1 / 3.),  # :240-265
+    ("crank-nicolson", 2): ([(0, 0), (1, 1)], 0.5),       # :198-218
+}
+
+
+def allencahn_step(sim):
+    """One semi-implicit Allen-Cahn step with the reference's dispatch
+    (CahnAllen.py:320-362): implicit 1D, 2D (gmres and not adi: full system;
+    otherwise alternating directions), 3D (alternating directions); imex 1D;
+    crank-nicolson 2D.  The reference's remaining branches do nothing
+    (crank-nicolson 1D returns, imex 2D/3D and crank-nicolson 3D print a
+    message); they raise here."""
+    u = sim.user_data
+    solver = u.get("solver", "explicit")
+    nd = len(sim.dimensions)
+    _check(sim)
+    M, W, eps = float(u["M"]), float(u["W"]), float(u["epsilon"])
+    dt, dx = float(sim.dt), float(sim.dx)
+    a = M * dt * eps * eps / (dx * dx)
+    K = 16. * W * M * dt
+    L = _lib.lib()
+    s = _lib.stream_handle()
+    lw = _work(sim, "_ac_line_work", int(L.ppf_allencahn_line_work_bytes(sim._grid)))
+
+    def line(axis, e, aa, b, src, imex):
+        rc = L.ppf_allencahn_line_solve(sim._grid, _lib.ptr(sim._fields_gpu_device),
+                                        _lib.ptr(sim._fields_out_gpu_device), axis, aa, e, b, src, imex,
+                                        _lib.ptr(lw), s)
+        _lib.check(rc, "allencahn_line_solve")
+        _passive(sim)
+        sim._swap_buffers()
+
+    if solver == "implicit" and nd == 1:
+        line(0, -1, a, 0.0, K, 0.0)
+    elif solver == "imex" and nd == 1:
+        line(0, -1, a, 0.0, 0.0, K)
+    elif solver == "implicit" and nd == 2 and u.get("gmres", False) and not u.get("adi", False):
+        tol = float(u.get("cg_tol", 1e-12 if sim._precision == "FP64" else 1e-6))
+        sim._cg_work = _work(sim, "_cg_work", int(L.ppf_diffusion_cg_work_bytes(sim._grid)))
+        fin, fout = _lib.ptr(sim._fields_gpu_device), _lib.ptr(sim._fields_out_gpu_device)
+        _lib.check(L.ppf_allencahn_cg_begin(sim._grid, fin, fout, a, K, _lib.ptr(sim._cg_work), s),
+                   "allencahn_cg_begin")
+        _cg_loop(sim, lambda k, n: _lib.check(L.ppf_allencahn_cg_iterate(sim._grid, fin, fout, a, K, tol, k, n,
+                                                                            _lib.ptr(sim._cg_work), s),
+                                              "allencahn_cg_iterate"), tol)
+        _passive(sim)
+        sim._swap_buffers()
+    elif (solver, nd) in _AC_PLANS:
+        plan, frac = _AC_PLANS[(solver, nd)]
+        aa = a if solver == "implicit" else 0.5 * a
+        for axis, e in plan:
+            line(axis, e, aa, aa if e >= 0 else 0.0, K * frac, 0.0)
+    else:
+        raise NotImplementedError("the reference's Allen-Cahn engine has no working %r solver for %dD "
+                                  "(CahnAllen.py:320-362)" % (solver, nd))
 
 
 def step(sim):

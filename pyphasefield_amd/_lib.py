@@ -116,6 +116,12 @@ def lib():
         "ppf_cyclic_table": ([c_i64, c_f64, c_vp], c_i32),
         "ppf_diffusion_line_solve": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_f64, c_vp], c_i32),
         "ppf_diffusion_cg_work_bytes": ([P(Grid)], c_i64),
+        "ppf_allencahn_line_work_bytes": ([P(Grid)], c_i64),
+        "ppf_allencahn_line_solve": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_f64, ctypes.c_int, c_f64, c_f64, c_f64,
+                                      c_vp, c_vp], c_i32),
+        "ppf_allencahn_cg_begin": ([P(Grid), c_vp, c_vp, c_f64, c_f64, c_vp, c_vp], c_i32),
+        "ppf_allencahn_cg_iterate": ([P(Grid), c_vp, c_vp, c_f64, c_f64, c_f64, c_i64, ctypes.c_int, c_vp, c_vp],
+                                     c_i32),
         "ppf_diffusion_cg_begin": ([P(Grid), c_vp, c_vp, c_f64, c_f64, c_vp, c_vp], c_i32),
         "ppf_diffusion_cg_iterate": ([P(Grid), c_vp, c_f64, c_f64, c_i64, ctypes.c_int, c_vp, c_vp], c_i32),
     }

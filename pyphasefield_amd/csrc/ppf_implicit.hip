@@ -121,6 +121,84 @@ __global__ __launch_bounds__(kLineThreads) void k_line_solve(const R* __restrict
   }
 }
 
+// Allen-Cahn semi-implicit line solve: the diagonal of line element i is
+// 1 + 2a + srck (4 d^2 - 6 d + 2), d = its right-hand side (the linearised
+// double-well force, Engines/CahnAllen.py:146-148), so the elimination
+// factors are per line.  rhs: the explicit half step as in k_line_solve, or
+// with imexk != 0 d = c - imexk (4 c^3 - 6 c^2 + 2 c).  Scratch s1 holds the
+// elimination factors, s2 the correction vector (same layout as the field).
+template <typename R>
+__global__ __launch_bounds__(kLineThreads) void k_line_solve_var(const R* __restrict__ in, R* __restrict__ out,
+                                                                Line L, double a, double bexp, double srck,
+                                                                double imexk, R* __restrict__ s1,
+                                                                R* __restrict__ s2) {
+  const int64_t l = int64_t(blockIdx.x) * kLineThreads + threadIdx.x;
+  if (l >= L.lines) return;
+  const int64_t i0 = l % L.m0, i1 = l / L.m0;
+  const int64_t o = L.base + i0 * L.s0 + i1 * L.s1;
+  const int64_t n = L.n, st = L.stride;
+  int64_t dm = 0, dp = 0;
+  if (L.eslot >= 0) {
+    const int64_t c = L.eslot == 0 ? i0 : i1;
+    const int64_t s = L.eslot == 0 ? L.s0 : L.s1;
+    dm = c == 0 ? (L.en - 1) * s : -s;
+    dp = c == L.en - 1 ? -(L.en - 1) * s : s;
+  }
+  const R off = R(-a), d0 = R(1.0 + 2.0 * a), K = R(srck), KI = R(imexk);
+  const R b = R(bexp), c0 = R(1.0 - 2.0 * bexp);
+  // pass 1: forward elimination of the corner-free matrix for the solution
+  // (into out) and the correction vector (into s2)
+  R gam = R(0), cp = R(0), dy = R(0), dq = R(0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t p = o + i * st;
+    R d = in[p];
+    if (L.eslot >= 0) {
+      d = c0 * d + b * (in[p + dm] + in[p + dp]);
+    } else if (L.ealong) {
+      const R xm = in[i == 0 ? o + (n - 1) * st : p - st];
+      const R xp = in[i == n - 1 ? o : p + st];
+      d = c0 * d + b * (xm + xp);
+    }
+    if (imexk != 0.0) d = d - KI * (R(4) * d * d * d - R(6) * d * d + R(2) * d);
+    R diag = d0 + K * (R(4) * d * d - R(6) * d + R(2));
+    R u = R(0);
+    if (i == 0) {
+      gam = -diag;
+      diag = diag - gam;
+      u = gam;
+    } else if (i == n - 1) {
+      diag = diag - off * off / gam;
+      u = off;
+    }
+    const R m = i == 0 ? diag : diag - off * cp;
+    const R rm = R(1) / m;
+    cp = off * rm;
+    dy = (d - off * dy) * rm;
+    dq = (u - off * dq) * rm;
+    out[p] = dy;
+    s1[p] = cp;
+    s2[p] = dq;
+  }
+  // pass 2: back substitution of both
+  const R ylast = dy, qlast = dq;
+  R y = dy, q = dq;
+  for (int64_t i = n - 2; i >= 0; --i) {
+    const int64_t p = o + i * st;
+    const R f = s1[p];
+    y = out[p] - f * y;
+    q = s2[p] - f * q;
+    out[p] = y;
+    s2[p] = q;
+  }
+  // pass 3: Sherman-Morrison correction x = y - (v.y / (1 + v.q)) q
+  const R r = off / gam;
+  const R f = (y + r * ylast) / (R(1) + q + r * qlast);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t p = o + i * st;
+    out[p] = out[p] - f * s2[p];
+  }
+}
+
 // ---------------------------------------------------- conjugate gradients --
 struct Dense {
   int nd;
@@ -192,9 +270,9 @@ __device__ __forceinline__ Cell cell_of(const Dense& d, int64_t t) {
 
 // x0 = in (written to out), rhs = in or the explicit half step of in,
 // r = rhs - A x0; partial sums of r.r and rhs.rhs
-template <typename R>
+template <typename R, bool VAR>
 __global__ __launch_bounds__(kCgThreads) void k_cg_begin(const R* __restrict__ in, R* __restrict__ out, Dense d,
-                                                        double a, double b, Work w) {
+                                                        double a, double b, double srck, Work w) {
   R* r = static_cast<R*>(w.r);
   double srr = 0.0, sbb = 0.0;
   const double na = 2.0 * d.nd;
@@ -214,7 +292,9 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_begin(const R* __restrict__ i
     }
     const R v = in[p];
     const R rhs = b != 0.0 ? R(1.0 - na * b) * v + R(b) * s : v;
-    const R ax = R(1.0 + na * a) * v - R(a) * s;
+    R diag = R(1.0 + na * a);
+    if (VAR) diag = diag + R(srck) * (R(4) * v * v - R(6) * v + R(2));
+    const R ax = diag * v - R(a) * s;
     const R rr = rhs - ax;
     out[p] = v;
     r[t] = rr;
@@ -237,8 +317,9 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_begin(const R* __restrict__ i
 
 // iteration k, first half: convergence test, p_k = r_k + beta p_{k-1},
 // q = A p_k, partial sums of p.q
-template <typename R>
-__global__ __launch_bounds__(kCgThreads) void k_cg_dir(Dense d, double a, double tol2, int64_t k, Work w) {
+template <typename R, bool VAR>
+__global__ __launch_bounds__(kCgThreads) void k_cg_dir(const R* __restrict__ in, Dense d, double a, double srck,
+                                                      double tol2, int64_t k, Work w) {
   const double rr = reduce_partials(w.prr + (k & 1) * kCgBlocks, d.nb);
   const double bb = k == 0 ? reduce_partials(w.pbb, d.nb) : w.st->bb;
   if (w.st->converged) return;
@@ -265,7 +346,12 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_dir(Dense d, double a, double
     const R pv = k == 0 ? r[t] : r[t] + bt * po[t];
     R s = R(0);
     for (int j = 0; j < c.cnt; ++j) s = s + (k == 0 ? r[c.nb[j]] : r[c.nb[j]] + bt * po[c.nb[j]]);
-    const R qv = cdiag * pv - ra * s;
+    R dg = cdiag;
+    if (VAR) {
+      const R v = in[c.pad];
+      dg = dg + R(srck) * (R(4) * v * v - R(6) * v + R(2));
+    }
+    const R qv = dg * pv - ra * s;
     pn[t] = pv;
     q[t] = qv;
     spq += double(pv) * double(qv);
@@ -325,9 +411,7 @@ int check_periodic(const ppf_grid* g, const char* what) {
   return 0;
 }
 
-template <typename R>
-int line_solve(const ppf_grid& g, const void* in, void* out, int axis, const double* table, int eaxis, double b,
-               hipStream_t s) {
+Line line_of(const ppf_grid& g, int axis, int eaxis) {
   const int64_t st[3] = {1, g.ndim >= 2 ? g.sy : 0, g.ndim >= 3 ? g.sz : 0};
   const int64_t nn[3] = {g.n[0], g.ndim >= 2 ? g.n[1] : 1, g.ndim >= 3 ? g.n[2] : 1};
   int other[2], k = 0;
@@ -351,30 +435,51 @@ int line_solve(const ppf_grid& g, const void* in, void* out, int axis, const dou
     L.eslot = eaxis == other[0] ? 0 : 1;
     L.en = nn[eaxis];
   }
+  return L;
+}
+
+template <typename R>
+int line_solve_var(const ppf_grid& g, const void* in, void* out, int axis, double a, int eaxis, double b,
+                   double srck, double imexk, void* work, hipStream_t s) {
+  const Line L = line_of(g, axis, eaxis);
+  R* s1 = static_cast<R*>(work);
+  R* s2 = s1 + g.sf;
+  const unsigned blocks = unsigned((L.lines + kLineThreads - 1) / kLineThreads);
+  hipLaunchKernelGGL(k_line_solve_var<R>, dim3(blocks), dim3(kLineThreads), 0, s, static_cast<const R*>(in),
+                     static_cast<R*>(out), L, a, b, srck, imexk, s1, s2);
+  return check_launch("allencahn_line_solve");
+}
+
+template <typename R>
+int line_solve(const ppf_grid& g, const void* in, void* out, int axis, const double* table, int eaxis, double b,
+               hipStream_t s) {
+  const Line L = line_of(g, axis, eaxis);
   const unsigned blocks = unsigned((L.lines + kLineThreads - 1) / kLineThreads);
   hipLaunchKernelGGL(k_line_solve<R>, dim3(blocks), dim3(kLineThreads), 0, s, static_cast<const R*>(in),
                      static_cast<R*>(out), L, table, b);
   return check_launch("diffusion_line_solve");
 }
 
-template <typename R>
-int cg_begin(const ppf_grid& g, const void* in, void* out, double a, double b, void* work, hipStream_t s) {
+template <typename R, bool VAR>
+int cg_begin(const ppf_grid& g, const void* in, void* out, double a, double b, double srck, void* work,
+             hipStream_t s) {
   Dense d = dense_of(g);
   Work w = carve(work, d.N, sizeof(R));
-  hipLaunchKernelGGL(k_cg_begin<R>, dim3(d.nb), dim3(kCgThreads), 0, s, static_cast<const R*>(in),
-                     static_cast<R*>(out), d, a, b, w);
-  return check_launch("diffusion_cg_begin");
+  hipLaunchKernelGGL((k_cg_begin<R, VAR>), dim3(d.nb), dim3(kCgThreads), 0, s, static_cast<const R*>(in),
+                     static_cast<R*>(out), d, a, b, srck, w);
+  return check_launch("cg_begin");
 }
 
-template <typename R>
-int cg_iterate(const ppf_grid& g, void* out, double a, double tol, int64_t k0, int iters, void* work,
-               hipStream_t s) {
+template <typename R, bool VAR>
+int cg_iterate(const ppf_grid& g, const void* in, void* out, double a, double srck, double tol, int64_t k0,
+               int iters, void* work, hipStream_t s) {
   Dense d = dense_of(g);
   Work w = carve(work, d.N, sizeof(R));
   const double tol2 = tol * tol;
   for (int i = 0; i < iters; ++i) {
     const int64_t k = k0 + i;
-    hipLaunchKernelGGL(k_cg_dir<R>, dim3(d.nb), dim3(kCgThreads), 0, s, d, a, tol2, k, w);
+    hipLaunchKernelGGL((k_cg_dir<R, VAR>), dim3(d.nb), dim3(kCgThreads), 0, s, static_cast<const R*>(in), d, a,
+                       srck, tol2, k, w);
     if (int rc = check_launch("diffusion_cg_iterate")) return rc;
     hipLaunchKernelGGL(k_cg_step<R>, dim3(d.nb), dim3(kCgThreads), 0, s, static_cast<R*>(out), d, k, w);
     if (int rc = check_launch("diffusion_cg_iterate")) return rc;
@@ -450,8 +555,8 @@ int ppf_diffusion_cg_begin(const ppf_grid* g, const void* in, void* out, double 
   if (in == out) return fail(PPF_EINVAL, "diffusion_cg_begin: in and out must differ");
   if (!(a >= 0.0) || !std::isfinite(a) || !(b >= 0.0) || !std::isfinite(b))
     return fail(PPF_EINVAL, "diffusion_cg_begin: a and b must be finite and >= 0");
-  return g->dtype == PPF_F32 ? cg_begin<float>(*g, in, out, a, b, work, stream_of(s))
-                             : cg_begin<double>(*g, in, out, a, b, work, stream_of(s));
+  return g->dtype == PPF_F32 ? cg_begin<float, false>(*g, in, out, a, b, 0.0, work, stream_of(s))
+                             : cg_begin<double, false>(*g, in, out, a, b, 0.0, work, stream_of(s));
 }
 
 int ppf_diffusion_cg_iterate(const ppf_grid* g, void* out, double a, double tol, int64_t first_iteration,
@@ -461,8 +566,55 @@ int ppf_diffusion_cg_iterate(const ppf_grid* g, void* out, double a, double tol,
   if (!(a >= 0.0) || !std::isfinite(a)) return fail(PPF_EINVAL, "diffusion_cg_iterate: a must be finite and >= 0");
   if (!(tol >= 0.0)) return fail(PPF_EINVAL, "diffusion_cg_iterate: tol must be >= 0");
   if (first_iteration < 0 || iterations < 0) return fail(PPF_EINVAL, "diffusion_cg_iterate: negative count");
-  return g->dtype == PPF_F32 ? cg_iterate<float>(*g, out, a, tol, first_iteration, iterations, work, stream_of(s))
-                             : cg_iterate<double>(*g, out, a, tol, first_iteration, iterations, work, stream_of(s));
+  return g->dtype == PPF_F32
+             ? cg_iterate<float, false>(*g, nullptr, out, a, 0.0, tol, first_iteration, iterations, work, stream_of(s))
+             : cg_iterate<double, false>(*g, nullptr, out, a, 0.0, tol, first_iteration, iterations, work,
+                                         stream_of(s));
+}
+
+int64_t ppf_allencahn_line_work_bytes(const ppf_grid* g) {
+  if (check_grid(g, 1, 3, "allencahn_line_work_bytes")) return -1;
+  return 2 * g->sf * (g->dtype == PPF_F32 ? 4 : 8);
+}
+
+int ppf_allencahn_line_solve(const ppf_grid* g, const void* in, void* out, int axis, double a, int explicit_axis,
+                             double b, double src_k, double imex_k, void* work, ppf_stream_t s) {
+  if (int rc = check_periodic(g, "allencahn_line_solve")) return rc;
+  if (axis < 0 || axis >= g->ndim) return fail(PPF_EINVAL, "allencahn_line_solve: bad axis");
+  if (explicit_axis < -1 || explicit_axis >= g->ndim)
+    return fail(PPF_EINVAL, "allencahn_line_solve: bad explicit axis");
+  if (!in || !out || !work) return fail(PPF_EINVAL, "allencahn_line_solve: null pointer");
+  if (in == out) return fail(PPF_EINVAL, "allencahn_line_solve: in and out must differ");
+  if (!(a >= 0.0) || !std::isfinite(a) || !(b >= 0.0) || !std::isfinite(b) || !std::isfinite(src_k) ||
+      !std::isfinite(imex_k))
+    return fail(PPF_EINVAL, "allencahn_line_solve: coefficients must be finite, a and b >= 0");
+  return g->dtype == PPF_F32
+             ? line_solve_var<float>(*g, in, out, axis, a, explicit_axis, b, src_k, imex_k, work, stream_of(s))
+             : line_solve_var<double>(*g, in, out, axis, a, explicit_axis, b, src_k, imex_k, work, stream_of(s));
+}
+
+int ppf_allencahn_cg_begin(const ppf_grid* g, const void* in, void* out, double a, double src_k, void* work,
+                           ppf_stream_t s) {
+  if (int rc = check_periodic(g, "allencahn_cg_begin")) return rc;
+  if (!in || !out || !work) return fail(PPF_EINVAL, "allencahn_cg_begin: null pointer");
+  if (in == out) return fail(PPF_EINVAL, "allencahn_cg_begin: in and out must differ");
+  if (!(a >= 0.0) || !std::isfinite(a) || !std::isfinite(src_k))
+    return fail(PPF_EINVAL, "allencahn_cg_begin: a must be finite and >= 0, src_k finite");
+  return g->dtype == PPF_F32 ? cg_begin<float, true>(*g, in, out, a, 0.0, src_k, work, stream_of(s))
+                             : cg_begin<double, true>(*g, in, out, a, 0.0, src_k, work, stream_of(s));
+}
+
+int ppf_allencahn_cg_iterate(const ppf_grid* g, const void* in, void* out, double a, double src_k, double tol,
+                             int64_t first_iteration, int iterations, void* work, ppf_stream_t s) {
+  if (int rc = check_periodic(g, "allencahn_cg_iterate")) return rc;
+  if (!in || !out || !work) return fail(PPF_EINVAL, "allencahn_cg_iterate: null pointer");
+  if (!(a >= 0.0) || !std::isfinite(a) || !std::isfinite(src_k) || !(tol >= 0.0))
+    return fail(PPF_EINVAL, "allencahn_cg_iterate: bad coefficients");
+  if (first_iteration < 0 || iterations < 0) return fail(PPF_EINVAL, "allencahn_cg_iterate: negative count");
+  return g->dtype == PPF_F32
+             ? cg_iterate<float, true>(*g, in, out, a, src_k, tol, first_iteration, iterations, work, stream_of(s))
+             : cg_iterate<double, true>(*g, in, out, a, src_k, tol, first_iteration, iterations, work,
+                                        stream_of(s));
 }
 
 }  // extern "C"

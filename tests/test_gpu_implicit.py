@@ -90,3 +90,51 @@ def test_conjugate_gradients_stop_early_and_report(gpu):
     assert 0 < it < 200
     want = oi.spectral_step(c0, 0.5, "implicit")
     assert rel_l2(sim.fields[0].get_cells(), want) <= FP64_TOL
+
+
+# ------------------------------------------------- Allen-Cahn semi-implicit --
+def _ac_sim(shape, solver, adi=False, gmres=False, precision="FP64", M=1.0, W=0.2, eps=1.0, dt=0.5, seed=4):
+    import pyphasefield_amd.Engines as E
+    sim = E.CahnAllen(dimensions=list(shape))
+    sim.set_user_data({"M": M, "W": W, "epsilon": eps, "solver": solver, "adi": adi, "gmres": gmres})
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(1.0)
+    sim.set_dt(dt)
+    sim.set_boundary_conditions("PERIODIC")
+    sim.set_precision(precision)
+    sim.initialize_fields_and_imported_data()
+    phi0 = 0.5 + 0.35 * (np.random.default_rng(seed).random(shape) - 0.5)
+    sim.fields[0].data[tuple(slice(1, -1) for _ in shape)] = phi0
+    return sim, phi0
+
+
+AC_CASES = [((40,), "implicit", False, False), ((40,), "imex", False, False),
+            ((18, 22), "implicit", False, True), ((18, 22), "implicit", False, False),
+            ((18, 22), "implicit", True, True), ((7, 9, 11), "implicit", False, False),
+            ((18, 22), "crank-nicolson", True, False)]
+
+
+@pytest.mark.parametrize("shape,solver,adi,gmres", AC_CASES)
+def test_allencahn_semi_implicit_matches_dense_solves(gpu, shape, solver, adi, gmres):
+    sim, phi = _ac_sim(shape, solver, adi, gmres)
+    a, K = 1.0 * 0.5 * 1.0, 16 * 0.2 * 1.0 * 0.5
+    full = solver == "implicit" and len(shape) == 2 and gmres and not adi
+    oadi = len(shape) > 1 and not full
+    sim.simulate(3)
+    for _ in range(3):
+        phi = oi.ac_step(phi, a / 2 if solver == "crank-nicolson" else a, K, solver, oadi)
+    assert rel_l2(sim.fields[0].get_cells(), phi) <= FP64_TOL
+
+
+def test_allencahn_semi_implicit_fp32(gpu):
+    sim, phi = _ac_sim((96, 80), "implicit", precision="FP32")
+    sim.simulate(3)
+    for _ in range(3):
+        phi = oi.ac_step(phi, 0.5, 1.6, "implicit", True)
+    assert rel_l2(sim.fields[0].get_cells(), phi) <= 1e-5
+
+
+def test_allencahn_missing_reference_branches_raise(gpu):
+    sim, _ = _ac_sim((20,), "crank-nicolson")
+    with pytest.raises(NotImplementedError):
+        sim.simulate(1)

@@ -160,3 +160,41 @@ def test_engine_rejects_non_periodic_implicit_runs():
         _implicit._check(sim)
     sim.set_user_data({"D": 1.0, "solver": "crank-nicolson"})
     assert sim._graph_chunk() == 0
+
+
+# ------------------------------------------------- Allen-Cahn semi-implicit --
+def _uniform_ac(phi0, K, sweeps):
+    """Closed form for a spatially uniform field: each sweep solves
+    (1 + K s(phi)) phi_new = phi (the neighbour couplings cancel)."""
+    p = phi0
+    for _ in range(sweeps):
+        p = p / (1 + K / sweeps * (4 * p * p - 6 * p + 2))
+    return p
+
+
+@pytest.mark.parametrize("shape,solver,adi,sweeps", [((9,), "implicit", False, 1), ((6, 7), "implicit", False, 1),
+                                                     ((6, 7), "implicit", True, 2), ((4, 5, 6), "implicit", True, 3),
+                                                     ((6, 7), "crank-nicolson", True, 2)])
+def test_allencahn_uniform_state_closed_form(shape, solver, adi, sweeps):
+    K = 1.3
+    for phi0 in (0.2, 0.75, 0.93):
+        got = oi.ac_step(np.full(shape, phi0), 0.7, K, solver, adi)
+        assert np.allclose(got, _uniform_ac(phi0, K, sweeps), rtol=1e-14, atol=0)
+
+
+def test_allencahn_imex_uniform_state_closed_form():
+    K, phi0 = 0.4, 0.3
+    got = oi.ac_step(np.full((8,), phi0), 0.9, K, "imex")
+    assert np.allclose(got, phi0 - K * (4 * phi0 ** 3 - 6 * phi0 ** 2 + 2 * phi0), rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("shape,solver,adi", [((11,), "implicit", False), ((6, 9), "implicit", False),
+                                              ((6, 9), "implicit", True), ((4, 5, 6), "implicit", True),
+                                              ((6, 9), "crank-nicolson", True)])
+def test_allencahn_without_double_well_is_the_diffusion_step(shape, solver, adi):
+    """K = 0 leaves the linear operator: the same Fourier multiplier as the
+    diffusion solver with a = M dt eps^2 / dx^2 (Crank-Nicolson: a/2 per half)."""
+    c = np.random.default_rng(9).random(shape)
+    a = 0.8
+    got = oi.ac_step(c, a / 2 if solver == "crank-nicolson" else a, 0.0, solver, adi)
+    assert np.allclose(got, oi.spectral_step(c, a, solver, adi), rtol=0, atol=1e-13)
