@@ -588,6 +588,48 @@ class Simulation:
                 plt.savefig(str(self._save_path) + "/" + f.name + "_" + str(self.time_step_counter) + ".png")
             plt.show()
 
+    def plot_field(self, f, save_path=None):
+        """simulation.py:642-660: save one field as <name>_step_<N>.png in
+        save_path (default: the simulation's save path); the reference reads a
+        nonexistent ``_time_step_counter`` here, the step counter is used."""
+        import matplotlib
+        import matplotlib.pyplot as plt
+        if save_path is None:
+            save_path = self._save_path
+        if save_path is None:
+            raise ValueError("plot_field needs a save path")
+        Path(save_path).mkdir(parents=True, exist_ok=True)
+        fig, ax = plt.subplots()
+        data = f.data  # device -> host when the GPU copy is newer
+        if data.ndim == 1:
+            ax.plot(data)
+        else:
+            img = ax.imshow(data if data.ndim == 2 else data[data.shape[0] // 2], interpolation="nearest",
+                            cmap=f.colormap if isinstance(f.colormap, (str, matplotlib.colors.Colormap)) else None)
+            fig.colorbar(img, ax=ax)
+        ax.set_title("Field: " + f.name + ", Step: " + str(self.time_step_counter))
+        out = Path(save_path).joinpath(f.name + "_step_" + str(self.time_step_counter) + ".png")
+        fig.savefig(str(out))
+        plt.close(fig)
+        return 0
+
+    def plot_all_fields(self):
+        """simulation.py:630-640: every field into save_path/images_step_<N>/."""
+        if not self._save_path:
+            raise ValueError("plot_all_fields needs a save path")
+        folder = Path(self._save_path).joinpath("images_step_" + str(self.time_step_counter))
+        for f in self.fields:
+            self.plot_field(f, folder)
+        return 0
+
+    def progress_bar(self):
+        """simulation.py:662-663 (a stub in the reference)."""
+        return
+
+    def generate_python_script(self):
+        """simulation.py:665-666 (a stub in the reference)."""
+        return
+
     # --------------------------------------------------- setters --
     def set_dimensions(self, dimensions):
         self.dimensions = list(dimensions)
