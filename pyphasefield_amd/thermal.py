@@ -66,11 +66,10 @@ def _data_item(item, base_dir):
     return values.reshape(dims) if dims else values
 
 
-def read_xdmf_series(path, name="T"):
-    """[(time, array)] of the temporal collection in ``path``, in file order."""
-    base = os.path.dirname(os.path.abspath(path))
+def _series_index(path, name="T"):
+    """[(time, DataItem element)] of the temporal collection in ``path``."""
     root = ET.parse(path).getroot()
-    frames = []
+    index = []
     for grid in root.iter():
         if _local(grid.tag) != "Grid":
             continue
@@ -87,10 +86,16 @@ def read_xdmf_series(path, name="T"):
         items = [c for c in attr if _local(c.tag) == "DataItem"]
         if not items:
             raise ValueError("XDMF attribute %s at t=%g has no DataItem" % (name, time))
-        frames.append((time, _data_item(items[0], base)))
-    if not frames:
+        index.append((time, items[0]))
+    if not index:
         raise ValueError("%s holds no time steps with attribute %r" % (path, name))
-    return frames
+    return index
+
+
+def read_xdmf_series(path, name="T"):
+    """[(time, array)] of the temporal collection in ``path``, in file order."""
+    base = os.path.dirname(os.path.abspath(path))
+    return [(t, _data_item(item, base)) for t, item in _series_index(path, name)]
 
 
 def write_xdmf_series(path, times, arrays, name="T", data_format="XML"):
@@ -127,12 +132,15 @@ class XDMFLoader:
 
     def __init__(self, t_file_path):
         self.t_file_path = t_file_path
-        self._frames = read_xdmf_series(t_file_path)
-        self.num_steps = len(self._frames)
+        self._base = os.path.dirname(os.path.abspath(t_file_path))
+        # the time index is read once; a frame's values are read when asked
+        # for (binary heavy data from disk), as the reference re-reads its file
+        self._index = _series_index(t_file_path)
+        self.num_steps = len(self._index)
 
     def data(self, step):
-        t, a = self._frames[step]
-        return t, np.squeeze(a)
+        t, item = self._index[step]
+        return t, np.squeeze(_data_item(item, self._base))
 
 
 def CSVtoXDMF(csv_path, T_cutoffs=False, starting_T=None, ending_T=None, reflect_X=False,
