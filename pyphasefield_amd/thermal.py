@@ -86,6 +86,14 @@ def _series_index(path, name="T"):
         items = [c for c in attr if _local(c.tag) == "DataItem"]
         if not items:
             raise ValueError("XDMF attribute %s at t=%g has no DataItem" % (name, time))
+        fmt = items[0].get("Format", "XML")
+        if fmt == "HDF":
+            raise NotImplementedError(
+                "XDMF heavy data in HDF5 (%s) needs an HDF5 reader, which is not available here; write the "
+                "series with data_format='XML' or 'Binary' (CSVtoXDMF / write_xdmf_series)"
+                % (items[0].text or "").strip())
+        if fmt not in ("XML", "Binary"):
+            raise ValueError("XDMF DataItem: unknown Format %r" % fmt)
         index.append((time, items[0]))
     if not index:
         raise ValueError("%s holds no time steps with attribute %r" % (path, name))
