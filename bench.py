@@ -352,8 +352,32 @@ def bench_copy(args):
                          "frac": gbs / HBM_PEAK_GBS, "traffic": 2 * nbytes}, "cpu_baseline": None}
 
 
+# ------------------------------------------------------------- AnisoDorr --
+def bench_aniso(args):
+    """AnisoDorrGPU 2D, 16 seeds, the reference's constants; FP32 (default) or
+    FP64 (--variant fp64).  4 fields read and written per cell-update."""
+    import pyphasefield_amd.Engines as engines
+    n = args.size or 2048
+    precision = "FP64" if args.variant == "fp64" else "FP32"
+    sim = engines.AnisoDorrGPU(dimensions=[n, n], seed=5)
+    sim.set_boundary_conditions("PERIODIC")
+    sim.set_precision(precision)
+    sim.set_user_data({"sim_type": "seeds", "number_of_seeds": 16, "initial_concentration_array": [0.40831]})
+    sim.initialize_fields_and_imported_data()
+    sim.simulate(args.warmup)
+    t = _timed(sim.simulate, args.steps)
+    cells = n * n
+    esize = 8 if precision == "FP64" else 4
+    kernel = "k_aniso_fused<%s>" % ("double" if esize == 8 else "float")
+    roof = _roofline(2 * 4 * esize * cells, t / args.steps, kernel)
+    return _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t,
+                 "f64" if precision == "FP64" else "f32",
+                 {"workload": "AnisoDorrGPU 2D %dx%d, 16 seeds, %s" % (n, n, precision), "engine": "AnisoDorrGPU",
+                  "grid": [n, n], "bytes_per_cell_update": 2 * 4 * esize}, roof, None)
+
+
 CONFIGS = {"warren": bench_warren, "diffusion": bench_diffusion, "ch3d": bench_ch3d, "copy": bench_copy,
-           "ncomp": bench_ncomp}
+           "ncomp": bench_ncomp, "aniso": bench_aniso}
 
 
 def main():
@@ -380,7 +404,7 @@ def main():
               "(the domain would shard by slabs with a 1-2 cell halo exchange; not built)", file=sys.stderr)
         sys.exit(2)
     defaults = {"warren": (2000, 100), "diffusion": (1000, 50), "ch3d": (50, 5), "copy": (50, 5),
-                "ncomp": (500, 20), "ensemble": (5000, 100)}
+                "ncomp": (500, 20), "ensemble": (5000, 100), "aniso": (500, 20)}
     st, wu = defaults[args.config]
     args.steps = args.steps if args.steps is not None else st
     args.warmup = args.warmup if args.warmup is not None else wu

@@ -233,6 +233,12 @@ int ppf_aniso_helper(const ppf_grid* g, const void* in, const void* bcvals, cons
 int ppf_aniso_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_grid* tg,
                    const void* transfer, const ppf_aniso_params* p, uint64_t seed, int64_t step, ppf_stream_t s);
 
+/* The same step as ppf_aniso_helper + ppf_aniso_step in one launch: the
+   vertex and diffusion terms of each 64 x 8 tile stay in LDS (no transfer
+   stack; identical arithmetic). */
+int ppf_aniso_fused_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                         const ppf_aniso_params* p, uint64_t seed, int64_t step, ppf_stream_t s);
+
 /* ---- implicit and Crank-Nicolson diffusion (Engines/Diffusion.py:18-633) ----
    Every matrix the reference builds (diffusion_matrix_1d/2d/3d,
    Diffusion.py:18-196) is periodic; these entry points require PERIODIC on

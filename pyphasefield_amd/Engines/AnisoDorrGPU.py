@@ -5,7 +5,9 @@ phi, q1, q4 (orientation quaternion) and c_CU, using Warren1995's ideal
 solution thermodynamics.  Per step a vertex pass (replaces
 AnisoDorr_helper_kernel, :262-375) writes the staggered anisotropy and
 diffusion terms for every padded cell, and the step kernel (replaces
-AnisoDorr_kernel, :54-245) updates the four fields.
+AnisoDorr_kernel, :54-245) updates the four fields.  By default both run as
+one tiled launch that keeps those terms in LDS; user_data["aniso_two_pass"]
+selects the two launches through a transfer stack (same arithmetic).
 
 The reference hard-codes every material constant inside both kernels
 (:63-105); here they are user_data entries with those values as defaults
@@ -107,6 +109,14 @@ class AnisoDorrGPU(Simulation):
         L = _lib.lib()
         s = _lib.stream_handle()
         bc = _lib.ptr(self._boundary_conditions_gpu_device)
+        if not self.user_data.get("aniso_two_pass", False):
+            # one tiled launch; the two-pass path below keeps the reference's
+            # helper/kernel split (same arithmetic)
+            _lib.check(L.ppf_aniso_fused_step(self._grid, _lib.ptr(self._fields_gpu_device),
+                                              _lib.ptr(self._fields_out_gpu_device), bc, self._params, self.seed,
+                                              int(self.time_step_counter), s), "aniso_fused_step")
+            self._swap_buffers()
+            return
         _lib.check(L.ppf_aniso_helper(self._grid, _lib.ptr(self._fields_gpu_device), bc, self._tgrid,
                                       _lib.ptr(self._fields_transfer_gpu_device), self._params, s), "aniso_helper")
         _lib.check(L.ppf_aniso_step(self._grid, _lib.ptr(self._fields_gpu_device), _lib.ptr(self._fields_out_gpu_device),

@@ -193,6 +193,18 @@ int ppf_aniso_step(const ppf_grid* g, const void* in, void* out, const void* bcv
              : aniso_step<double>(*g, in, out, bcvals, *tg, transfer, *p, seed, step, stream_of(s));
 }
 
+int ppf_aniso_fused_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                         const ppf_aniso_params* p, uint64_t seed, int64_t step, ppf_stream_t s) {
+  if (int rc = check_grid(g, 2, 2, "aniso_fused_step")) return rc;
+  if (!p) return fail(PPF_EINVAL, "aniso_fused_step: null params");
+  if (g->nfields != 4) return fail(PPF_EINVAL, "aniso_fused_step: field stack must hold phi, q1, q4, c");
+  if (!(p->T > 0.0) || !(p->R > 0.0) || !(p->beta > 0.0))
+    return fail(PPF_EINVAL, "aniso_fused_step: T, R and beta must be positive");
+  if (!in || !out || in == out) return fail(PPF_EINVAL, "aniso_fused_step: null or aliased argument");
+  return g->dtype == PPF_F32 ? aniso_fused_step<float>(*g, in, out, bcvals, *p, seed, step, stream_of(s))
+                             : aniso_fused_step<double>(*g, in, out, bcvals, *p, seed, step, stream_of(s));
+}
+
 int64_t ppf_reduce_work_bytes(const ppf_grid* g) {
   if (!g) return 0;
   return int64_t(kReduceBlocks) * int64_t(g->nfields > 0 ? g->nfields : 1) * int64_t(sizeof(double));

@@ -1,7 +1,9 @@
 // AnisoDorr binary polycrystalline solidification (2D; fields phi, q1, q4, c):
 // the vertex pass replacing AnisoDorr_helper_kernel
 // (Engines/AnisoDorrGPU.py:262-375) and the step replacing AnisoDorr_kernel
-// (:54-245, device helpers :25-52).  Expressions keep the reference's
+// (:54-245, device helpers :25-52), as two kernels through a transfer stack
+// (the reference's structure) or as one tiled kernel (k_aniso_fused) that
+// keeps the transfer terms in LDS; both evaluate the same device functions.  Expressions keep the reference's
 // operation order (FP64 parity); its hard-coded constants arrive in
 // ppf_aniso_params.
 //
@@ -74,6 +76,137 @@ struct AdLoad {
   }
 };
 
+// Pointwise terms of one cell (AnisoDorrGPU.py:338-345).
+template <typename R>
+__device__ __forceinline__ void aniso_point(const AnisoK& k, R phi, R c, R& temp, R& D_C) {
+  const R g = ad_g(phi), gp = ad_gprime(phi);
+  const R m = R(1) - ad_h(phi);
+  const R H_A = R(k.W_A) * gp - R(k.kA) * g;
+  const R H_B = R(k.W_B) * gp - R(k.kB) * g;
+  D_C = R(k.D_S) + m * R(k.dLS);
+  temp = D_C * R(k.v_m) * c * (R(1) - c) * (H_B - H_A) / R(k.Rg);
+}
+
+// Vertex terms at (x-1/2, y-1/2) (AnisoDorrGPU.py:346-375) from the cells
+// (x,y), (x-1,y), (x,y-1), (x-1,y-1) -- the reference's [i][j], [i][j-1],
+// [i-1][j], [i-1][j-1] with i = y, j = x.
+template <typename R>
+__device__ __forceinline__ void aniso_vertex(const AnisoK& k, R idx, R phi, R pl, R pu, R pul, R q1, R q1u, R q1l,
+                                             R q1ul, R q4, R q4u, R q4l, R q4ul, R& pfx, R& pfy, R& t1t, R& t4t) {
+  const R gx = R(0.5) * (phi - pl + pu - pul) * idx;
+  const R gy = R(0.5) * (phi - pu + pl - pul) * idx;
+  const R q1m = R(0.25) * (q1 + q1u + q1l + q1ul);
+  const R q4m = R(0.25) * (q4 + q4u + q4l + q4ul);
+  const R a2_b2 = q1m * q1m - q4m * q4m;
+  const R ab2 = R(2) * q1m * q4m;
+  const R sx = a2_b2 * gx - ab2 * gy;
+  const R sy = a2_b2 * gy + ab2 * gx;
+  const R psi_xxy = sx * sx * sy;
+  const R psi_xyy = sx * sy * sy;
+  R mg2 = sx * sx + sy * sy;
+  mg2 = mg2 > R(0.000000001) ? mg2 : R(0.000000001);
+  const R sx3 = sx * sx * sx, sy3 = sy * sy * sy;
+  const R ye4 = R(k.ye4);
+  pfx = ye4 * ((R(2) * a2_b2 * sx3 + R(2) * ab2 * sy3) / mg2 - gx * (sx3 * gx + sy3 * gy) / (mg2 * mg2));
+  pfy = ye4 * ((R(2) * a2_b2 * sy3 - R(2) * ab2 * sx3) / mg2 - gy * (sx3 * gx + sy3 * gy) / (mg2 * mg2));
+  const R q1px = q1m * gx, q1py = q1m * gy, q4px = q4m * gx, q4py = q4m * gy;
+  const R kk = R(k.k16) / mg2;
+  t1t = kk * (psi_xyy * (q1px - q4py) + psi_xxy * (q1py + q4px));
+  t4t = kk * (psi_xyy * (-q4px - q1py) + psi_xxy * (-q4py + q1px));
+}
+
+// Five-point neighbourhood: centre, x+1, x-1, y+1, y-1.
+template <typename R>
+struct Ad5 {
+  R c, xp, xm, yp, ym;
+};
+// Four vertices of a cell: (x,y), (x+1,y), (x,y+1), (x+1,y+1).
+template <typename R>
+struct Ad4 {
+  R v00, v10, v01, v11;
+};
+
+// One cell of AnisoDorr_kernel (AnisoDorrGPU.py:136-245).  The noise
+// (uniform [0,1), counter RNG) multiplies 16 g, so it is drawn only where
+// g != 0.
+template <typename R>
+__device__ __forceinline__ void aniso_cell(const AnisoK& k, R idx, const Ad5<R>& P, const Ad5<R>& A,
+                                           const Ad5<R>& B, const Ad5<R>& C, const Ad5<R>& T4, const Ad5<R>& DC,
+                                           const Ad4<R>& X, const Ad4<R>& Y, const Ad4<R>& T1, const Ad4<R>& T4v,
+                                           uint64_t seed, int64_t step, uint64_t ncell, uint64_t cell, R& nphi,
+                                           R& nq1, R& nq4, R& nc) {
+  const R idx2 = idx * idx;
+  const R hidx2 = R(0.5) * idx * idx;
+  const R p = P.c, a = A.c, b = B.c, c = C.c;
+  const R g = ad_g(p), h = ad_h(p);
+  const R m = R(1) - h;
+  const R M_q = R(k.Mq0) + R(k.dMq) * m;
+  const R lq1 = (A.xp + A.xm + A.yp + A.ym - R(4) * a) * idx2;
+  const R lq4 = (B.xp + B.xm + B.yp + B.ym - R(4) * b) * idx2;
+  const R pp = R(2) * p;
+  const R gp = ad_gprime(p);
+  const R H_A = R(k.W_A) * gp - R(k.kA) * g;
+  const R H_B = R(k.W_B) * gp - R(k.kB) * g;
+  // quaternion gradient magnitudes per face
+  const R g1xm = (a - A.xm) * idx, g1ym = (a - A.ym) * idx, g1xp = (a - A.xp) * idx, g1yp = (a - A.yp) * idx;
+  const R g4xm = (b - B.xm) * idx, g4ym = (b - B.ym) * idx, g4xp = (b - B.xp) * idx, g4yp = (b - B.yp) * idx;
+  R sxm = g1xm * g1xm + g4xm * g4xm;
+  R sym = g1ym * g1ym + g4ym * g4ym;
+  R sxp = g1xp * g1xp + g4xp * g4xp;
+  R syp = g1yp * g1yp + g4yp * g4yp;
+  const R rg0 = sqrt(R(0.5) * (sxm + sym + sxp + syp));
+  const R beta = R(k.beta);
+  sxm = sxm > beta ? sxm : beta;
+  sym = sym > beta ? sym : beta;
+  sxp = sxp > beta ? sxp : beta;
+  syp = syp > beta ? syp : beta;
+  const R rxm = sqrt(sxm), rym = sqrt(sym), rxp = sqrt(sxp), ryp = sqrt(syp);
+
+  // dc/dt: div(temp grad phi) + div(D_C grad c)
+  R dc = hidx2 * ((T4.xp + T4.c) * (P.xp - p) - (T4.c + T4.xm) * (p - P.xm) + (T4.yp + T4.c) * (P.yp - p) -
+                  (T4.c + T4.ym) * (p - P.ym));
+  dc = dc + hidx2 * ((DC.xp + DC.c) * (C.xp - c) - (DC.c + DC.xm) * (c - C.xm) + (DC.yp + DC.c) * (C.yp - c) -
+                     (DC.c + DC.ym) * (c - C.ym));
+
+  // dphi/dt
+  const R lphi = (P.xp + P.xm + P.yp + P.ym - R(4) * p) * idx2;
+  const R pfx = R(0.5) * idx * (X.v10 + X.v11 - X.v00 - X.v01);
+  const R pfy = R(0.5) * idx * (Y.v01 + Y.v11 - Y.v00 - Y.v10);
+  const R M_phi = (R(1) - c) * R(k.M_A) + c * R(k.M_B);
+  R dphi = M_phi * (R(k.ebar2) * (R(k.one3ye) * lphi + pfx + pfy) - (R(1) - c) * H_A - c * H_B -
+                    R(k.twoHT) * pp * rg0);
+  if (g != R(0)) {
+    const R rnd = uniform01<R>(hash64(seed, rng_counter(step, S_ANISO_PHI, ncell, cell)));
+    dphi = dphi + M_phi * R(k.alpha) * rnd * (R(16) * g) * ((R(1) - c) * H_A + c * H_B);
+  }
+
+  // dq/dt
+  const R base = R(k.twoHT);
+  const R Dq = base * (p * p);
+  const R Dxp = base * P.xp * P.xp, Dxm = base * P.xm * P.xm, Dyp = base * P.yp * P.yp, Dym = base * P.ym * P.ym;
+  const R gaq1 = hidx2 * ((Dq + Dxp) * (A.xp - a) / rxp - (Dq + Dxm) * (a - A.xm) / rxm +
+                          (Dq + Dyp) * (A.yp - a) / ryp - (Dq + Dym) * (a - A.ym) / rym);
+  const R gaq4 = hidx2 * ((Dq + Dxp) * (B.xp - b) / rxp - (Dq + Dxm) * (b - B.xm) / rxm +
+                          (Dq + Dyp) * (B.yp - b) / ryp - (Dq + Dym) * (b - B.ym) / rym);
+  const R cc1 = R(0.25) * (T1.v00 + T1.v01 + T1.v10 + T1.v11);
+  const R cc4 = R(0.25) * (T4v.v00 + T4v.v01 + T4v.v10 + T4v.v11);
+  const R t1 = R(k.eqbar2) * lq1 + gaq1 + cc1;
+  const R t4 = R(k.eqbar2) * lq4 + gaq4 + cc4;
+  const R lm = a * t1 + b * t4;
+  const R dq1 = M_q * (t1 - a * lm);
+  const R dq4 = M_q * (t4 - b * lm);
+
+  const R dt = R(k.dt);
+  nphi = p + dphi * dt;
+  const R q1n = a + dq1 * dt;
+  const R q4n = b + dq4 * dt;
+  nc = c + dc * dt;
+  const R rn = sqrt(q1n * q1n + q4n * q4n);
+  nq1 = q1n / rn;
+  nq4 = q4n / rn;
+}
+
+// ------------------------------------------------------- two-pass path --
 // one thread per padded cell, 64 x 4 cells per block
 template <typename R>
 __global__ __launch_bounds__(256) void k_aniso_helper(const R* __restrict__ in, const R* __restrict__ bv, Dom d,
@@ -88,37 +221,13 @@ __global__ __launch_bounds__(256) void k_aniso_helper(const R* __restrict__ in, 
   const AdLoad<R> F{in, bv, &d, dx};
   const R idx = R(1) / dx;
   const R phi = F(0, x, y), c = F(3, x, y);
-  const R g = ad_g(phi), gp = ad_gprime(phi);
-  const R m = R(1) - ad_h(phi);
-  const R H_A = R(k.W_A) * gp - R(k.kA) * g;
-  const R H_B = R(k.W_B) * gp - R(k.kB) * g;
-  const R D_C = R(k.D_S) + m * R(k.dLS);
-  const R temp = D_C * R(k.v_m) * c * (R(1) - c) * (H_B - H_A) / R(k.Rg);
+  R temp, D_C;
+  aniso_point(k, phi, c, temp, D_C);
   R pfx = R(0), pfy = R(0), t1t = R(0), t4t = R(0);
-  if (x >= 1 && y >= 1) {
-    // cells [i][j], [i][j-1], [i-1][j], [i-1][j-1] of the reference (i = y, j = x)
-    const R pl = F(0, x - 1, y), pu = F(0, x, y - 1), pul = F(0, x - 1, y - 1);
-    const R gx = R(0.5) * (phi - pl + pu - pul) * idx;
-    const R gy = R(0.5) * (phi - pu + pl - pul) * idx;
-    const R q1m = R(0.25) * (F(1, x, y) + F(1, x, y - 1) + F(1, x - 1, y) + F(1, x - 1, y - 1));
-    const R q4m = R(0.25) * (F(2, x, y) + F(2, x, y - 1) + F(2, x - 1, y) + F(2, x - 1, y - 1));
-    const R a2_b2 = q1m * q1m - q4m * q4m;
-    const R ab2 = R(2) * q1m * q4m;
-    const R sx = a2_b2 * gx - ab2 * gy;
-    const R sy = a2_b2 * gy + ab2 * gx;
-    const R psi_xxy = sx * sx * sy;
-    const R psi_xyy = sx * sy * sy;
-    R mg2 = sx * sx + sy * sy;
-    mg2 = mg2 > R(0.000000001) ? mg2 : R(0.000000001);
-    const R sx3 = sx * sx * sx, sy3 = sy * sy * sy;
-    const R ye4 = R(k.ye4);
-    pfx = ye4 * ((R(2) * a2_b2 * sx3 + R(2) * ab2 * sy3) / mg2 - gx * (sx3 * gx + sy3 * gy) / (mg2 * mg2));
-    pfy = ye4 * ((R(2) * a2_b2 * sy3 - R(2) * ab2 * sx3) / mg2 - gy * (sx3 * gx + sy3 * gy) / (mg2 * mg2));
-    const R q1px = q1m * gx, q1py = q1m * gy, q4px = q4m * gx, q4py = q4m * gy;
-    const R kk = R(k.k16) / mg2;
-    t1t = kk * (psi_xyy * (q1px - q4py) + psi_xxy * (q1py + q4px));
-    t4t = kk * (psi_xyy * (-q4px - q1py) + psi_xxy * (-q4py + q1px));
-  }
+  if (x >= 1 && y >= 1)
+    aniso_vertex(k, idx, phi, F(0, x - 1, y), F(0, x, y - 1), F(0, x - 1, y - 1), F(1, x, y), F(1, x, y - 1),
+                 F(1, x - 1, y), F(1, x - 1, y - 1), F(2, x, y), F(2, x, y - 1), F(2, x - 1, y),
+                 F(2, x - 1, y - 1), pfx, pfy, t1t, t4t);
   const int64_t o = y * tsy + x;
   tr[o] = pfx;
   tr[o + tsf] = pfy;
@@ -141,90 +250,122 @@ __global__ __launch_bounds__(256) void k_aniso_step(const R* __restrict__ in, R*
   const int64_t y = 1 + int64_t(blk / nbx) * 4 + (threadIdx.x >> 6);
   if (x > nx || y > ny) return;
   const AdLoad<R> F{in, bv, &d, dx};
-  const R idx = R(1) / dx;
-  const R idx2 = idx * idx;
-  const R hidx2 = R(0.5) * idx * idx;
-  // five-point neighbourhoods: centre, x+, x-, y+, y-
-  const R p = F(0, x, y), pxp = F(0, x + 1, y), pxm = F(0, x - 1, y), pyp = F(0, x, y + 1), pym = F(0, x, y - 1);
-  const R a = F(1, x, y), axp = F(1, x + 1, y), axm = F(1, x - 1, y), ayp = F(1, x, y + 1), aym = F(1, x, y - 1);
-  const R b = F(2, x, y), bxp = F(2, x + 1, y), bxm = F(2, x - 1, y), byp = F(2, x, y + 1), bym = F(2, x, y - 1);
-  const R c = F(3, x, y), cxp = F(3, x + 1, y), cxm = F(3, x - 1, y), cyp = F(3, x, y + 1), cym = F(3, x, y - 1);
+  auto five = [&](int f) { return Ad5<R>{F(f, x, y), F(f, x + 1, y), F(f, x - 1, y), F(f, x, y + 1), F(f, x, y - 1)}; };
   const int64_t o = y * tsy + x;
-  const R* T4 = tr + 4 * tsf;
-  const R* DC = tr + 5 * tsf;
-
-  const R g = ad_g(p), h = ad_h(p);
-  const R m = R(1) - h;
-  const R M_q = R(k.Mq0) + R(k.dMq) * m;
-  const R lq1 = (axp + axm + ayp + aym - R(4) * a) * idx2;
-  const R lq4 = (bxp + bxm + byp + bym - R(4) * b) * idx2;
-  const R pp = R(2) * p;
-  const R gp = ad_gprime(p);
-  const R H_A = R(k.W_A) * gp - R(k.kA) * g;
-  const R H_B = R(k.W_B) * gp - R(k.kB) * g;
-  // quaternion gradient magnitudes per face
-  const R g1xm = (a - axm) * idx, g1ym = (a - aym) * idx, g1xp = (a - axp) * idx, g1yp = (a - ayp) * idx;
-  const R g4xm = (b - bxm) * idx, g4ym = (b - bym) * idx, g4xp = (b - bxp) * idx, g4yp = (b - byp) * idx;
-  R sxm = g1xm * g1xm + g4xm * g4xm;
-  R sym = g1ym * g1ym + g4ym * g4ym;
-  R sxp = g1xp * g1xp + g4xp * g4xp;
-  R syp = g1yp * g1yp + g4yp * g4yp;
-  const R rg0 = sqrt(R(0.5) * (sxm + sym + sxp + syp));
-  const R beta = R(k.beta);
-  sxm = sxm > beta ? sxm : beta;
-  sym = sym > beta ? sym : beta;
-  sxp = sxp > beta ? sxp : beta;
-  syp = syp > beta ? syp : beta;
-  const R rxm = sqrt(sxm), rym = sqrt(sym), rxp = sqrt(sxp), ryp = sqrt(syp);
-
-  // dc/dt: div(temp grad phi) + div(D_C grad c)
-  const R tc = T4[o], txp = T4[o + 1], txm = T4[o - 1], typ = T4[o + tsy], tym = T4[o - tsy];
-  R dc = hidx2 * ((txp + tc) * (pxp - p) - (tc + txm) * (p - pxm) + (typ + tc) * (pyp - p) - (tc + tym) * (p - pym));
-  const R dcc = DC[o], dxp = DC[o + 1], dxm = DC[o - 1], dyp = DC[o + tsy], dym = DC[o - tsy];
-  dc = dc + hidx2 * ((dxp + dcc) * (cxp - c) - (dcc + dxm) * (c - cxm) + (dyp + dcc) * (cyp - c) -
-                     (dcc + dym) * (c - cym));
-
-  // dphi/dt
-  const R lphi = (pxp + pxm + pyp + pym - R(4) * p) * idx2;
-  const R* X = tr;
-  const R* Y = tr + tsf;
-  const R pfx = R(0.5) * idx * (X[o + 1] + X[o + tsy + 1] - X[o] - X[o + tsy]);
-  const R pfy = R(0.5) * idx * (Y[o + tsy] + Y[o + tsy + 1] - Y[o] - Y[o + 1]);
-  const R M_phi = (R(1) - c) * R(k.M_A) + c * R(k.M_B);
-  R dphi = M_phi * (R(k.ebar2) * (R(k.one3ye) * lphi + pfx + pfy) - (R(1) - c) * H_A - c * H_B -
-                    R(k.twoHT) * pp * rg0);
-  const uint64_t ncell = uint64_t(nx + 2) * uint64_t(ny + 2);
-  const uint64_t cell = uint64_t(y) * uint64_t(nx + 2) + uint64_t(x);
-  const R rnd = uniform01<R>(hash64(seed, rng_counter(step, S_ANISO_PHI, ncell, cell)));
-  dphi = dphi + M_phi * R(k.alpha) * rnd * (R(16) * g) * ((R(1) - c) * H_A + c * H_B);
-
-  // dq/dt
-  const R base = R(k.twoHT);
-  const R Dq = base * (p * p);
-  const R Dxp = base * pxp * pxp, Dxm = base * pxm * pxm, Dyp = base * pyp * pyp, Dym = base * pym * pym;
-  const R gaq1 = hidx2 * ((Dq + Dxp) * (axp - a) / rxp - (Dq + Dxm) * (a - axm) / rxm +
-                          (Dq + Dyp) * (ayp - a) / ryp - (Dq + Dym) * (a - aym) / rym);
-  const R gaq4 = hidx2 * ((Dq + Dxp) * (bxp - b) / rxp - (Dq + Dxm) * (b - bxm) / rxm +
-                          (Dq + Dyp) * (byp - b) / ryp - (Dq + Dym) * (b - bym) / rym);
-  const R* T1 = tr + 2 * tsf;
-  const R* T4t = tr + 3 * tsf;
-  const R cc1 = R(0.25) * (T1[o] + T1[o + tsy] + T1[o + 1] + T1[o + tsy + 1]);
-  const R cc4 = R(0.25) * (T4t[o] + T4t[o + tsy] + T4t[o + 1] + T4t[o + tsy + 1]);
-  const R t1 = R(k.eqbar2) * lq1 + gaq1 + cc1;
-  const R t4 = R(k.eqbar2) * lq4 + gaq4 + cc4;
-  const R lm = a * t1 + b * t4;
-  const R dq1 = M_q * (t1 - a * lm);
-  const R dq4 = M_q * (t4 - b * lm);
-
-  const R dt = R(k.dt);
+  auto tfive = [&](int f) {
+    const R* t = tr + f * tsf;
+    return Ad5<R>{t[o], t[o + 1], t[o - 1], t[o + tsy], t[o - tsy]};
+  };
+  auto tfour = [&](int f) {
+    const R* t = tr + f * tsf;
+    return Ad4<R>{t[o], t[o + 1], t[o + tsy], t[o + tsy + 1]};
+  };
+  R nphi, nq1, nq4, nc;
+  aniso_cell(k, R(1) / dx, five(0), five(1), five(2), five(3), tfive(4), tfive(5), tfour(0), tfour(1), tfour(2),
+             tfour(3), seed, step, uint64_t(nx + 2) * uint64_t(ny + 2), uint64_t(y) * uint64_t(nx + 2) + uint64_t(x),
+             nphi, nq1, nq4, nc);
   const int64_t of = y * d.ay.stride + x;
-  out[of] = p + dphi * dt;
-  const R q1n = a + dq1 * dt;
-  const R q4n = b + dq4 * dt;
-  out[of + 3 * d.sf] = c + dc * dt;
-  const R rn = sqrt(q1n * q1n + q4n * q4n);
-  out[of + d.sf] = q1n / rn;
-  out[of + 2 * d.sf] = q4n / rn;
+  out[of] = nphi;
+  out[of + d.sf] = nq1;
+  out[of + 2 * d.sf] = nq4;
+  out[of + 3 * d.sf] = nc;
+}
+
+// --------------------------------------------------------- fused path --
+// One 64 x 8 output tile per 256-thread block.  The (64+2) x (8+2) cells of
+// the four fields are loaded once (boundary tiles fold the rules in), the
+// pointwise temp / D_C and the (64+1) x (8+1) vertex terms are computed into
+// LDS, and each thread then updates two cells: the six transfer arrays of
+// the two-pass path never touch HBM.
+constexpr int kAdTX = 64, kAdTY = 8;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_aniso_fused(const R* __restrict__ in, R* __restrict__ out,
+                                                     const R* __restrict__ bv, Dom d, R dx, AnisoK k, uint64_t seed,
+                                                     int64_t step) {
+  constexpr int CW = kAdTX + 2, CH = kAdTY + 2, VW = kAdTX + 1, VH = kAdTY + 1;
+  __shared__ R sf[4][CH][CW];
+  __shared__ R sp[2][CH][CW];  // temp, D_C
+  __shared__ R sv[4][VH][VW];  // pf_comp_x, pf_comp_y, t1_temp, t4_temp
+  const int64_t nx = d.ax.n, ny = d.ay.n;
+  const uint32_t nbx = uint32_t((nx + kAdTX - 1) / kAdTX);
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int64_t x0 = 1 + int64_t(blk % nbx) * kAdTX, y0 = 1 + int64_t(blk / nbx) * kAdTY;
+  const bool interior = x0 - 1 >= 1 && x0 + kAdTX <= nx && y0 - 1 >= 1 && y0 + kAdTY <= ny;
+  const R idx = R(1) / dx;
+  // 1) cells (x0-1 .. x0+64, y0-1 .. y0+8) and their pointwise terms; an
+  //    interior tile issues all its loads before the first LDS store
+  constexpr int NC = CW * CH, NP = (NC + 255) / 256;
+  if (interior) {
+    const R* __restrict__ base = in + ((y0 - 1) * d.ay.stride + (x0 - 1));
+    R f[NP][4];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int t = threadIdx.x + j * 256;
+      if (t < NC) {
+        const int ly = t / CW, lx = t - ly * CW;
+        const int64_t o = int64_t(ly) * d.ay.stride + lx;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) f[j][i] = base[o + i * d.sf];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int t = threadIdx.x + j * 256;
+      if (t < NC) {
+        const int ly = t / CW, lx = t - ly * CW;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sf[i][ly][lx] = f[j][i];
+        aniso_point(k, f[j][0], f[j][3], sp[0][ly][lx], sp[1][ly][lx]);
+      }
+    }
+  } else {
+    for (int t = threadIdx.x; t < NC; t += 256) {
+      const int ly = t / CW, lx = t - ly * CW;
+      const int64_t gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+      R f[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) f[i] = fold(in + i * d.sf, bv ? bv + i * d.sf : nullptr, d, gx, gy, int64_t(0), dx);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sf[i][ly][lx] = f[i];
+      aniso_point(k, f[0], f[3], sp[0][ly][lx], sp[1][ly][lx]);
+    }
+  }
+  __syncthreads();
+  // 2) vertices (x0 .. x0+64, y0 .. y0+8): vertex (vx, vy) belongs to cell
+  //    (vx+1, vy+1) of the tile
+  for (int t = threadIdx.x; t < VW * VH; t += 256) {
+    const int vy = t / VW, vx = t - vy * VW;
+    const int cx = vx + 1, cy = vy + 1;
+    aniso_vertex(k, idx, sf[0][cy][cx], sf[0][cy][cx - 1], sf[0][cy - 1][cx], sf[0][cy - 1][cx - 1], sf[1][cy][cx],
+                 sf[1][cy - 1][cx], sf[1][cy][cx - 1], sf[1][cy - 1][cx - 1], sf[2][cy][cx], sf[2][cy - 1][cx],
+                 sf[2][cy][cx - 1], sf[2][cy - 1][cx - 1], sv[0][vy][vx], sv[1][vy][vx], sv[2][vy][vx],
+                 sv[3][vy][vx]);
+  }
+  __syncthreads();
+  // 3) two output rows per thread
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const uint64_t ncell = uint64_t(nx + 2) * uint64_t(ny + 2);
+#pragma unroll
+  for (int r = 0; r < kAdTY / 4; ++r) {
+    const int oy = ty + 4 * r;
+    const int64_t gx = x0 + tx, gy = y0 + oy;
+    if (gx > nx || gy > ny) continue;
+    const int lx = tx + 1, ly = oy + 1;
+    auto five = [&](const R (&a)[CH][CW]) {
+      return Ad5<R>{a[ly][lx], a[ly][lx + 1], a[ly][lx - 1], a[ly + 1][lx], a[ly - 1][lx]};
+    };
+    auto four = [&](const R (&v)[VH][VW]) { return Ad4<R>{v[oy][tx], v[oy][tx + 1], v[oy + 1][tx], v[oy + 1][tx + 1]}; };
+    R nphi, nq1, nq4, nc;
+    aniso_cell(k, idx, five(sf[0]), five(sf[1]), five(sf[2]), five(sf[3]), five(sp[0]), five(sp[1]), four(sv[0]),
+               four(sv[1]), four(sv[2]), four(sv[3]), seed, step, ncell,
+               uint64_t(gy) * uint64_t(nx + 2) + uint64_t(gx), nphi, nq1, nq4, nc);
+    const int64_t of = gy * d.ay.stride + gx;
+    out[of] = nphi;
+    out[of + d.sf] = nq1;
+    out[of + 2 * d.sf] = nq4;
+    out[of + 3 * d.sf] = nc;
+  }
 }
 
 template <typename R>
@@ -236,6 +377,16 @@ int aniso_helper(const ppf_grid& g, const void* in, const void* bcvals, const pp
                      static_cast<const R*>(bcvals), d, R(g.dx), static_cast<R*>(transfer), tg.sy, tg.sf,
                      aniso_constants(p));
   return check_launch("aniso_helper");
+}
+
+template <typename R>
+int aniso_fused_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_aniso_params& p,
+                     uint64_t seed, int64_t step, hipStream_t s) {
+  Dom d = make_dom(g);
+  dim3 grid(unsigned(((g.n[0] + kAdTX - 1) / kAdTX) * ((g.n[1] + kAdTY - 1) / kAdTY)));
+  hipLaunchKernelGGL(k_aniso_fused<R>, grid, dim3(256), 0, s, static_cast<const R*>(in), static_cast<R*>(out),
+                     static_cast<const R*>(bcvals), d, R(g.dx), aniso_constants(p), seed, step);
+  return check_launch("aniso_fused_step");
 }
 
 template <typename R>

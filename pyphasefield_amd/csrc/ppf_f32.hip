@@ -21,6 +21,8 @@ template int ncomp_thermo<float>(const ppf_grid&, const void*, const void*, cons
                                  int64_t, hipStream_t);
 template int aniso_helper<float>(const ppf_grid&, const void*, const void*, const ppf_grid&, void*,
                                  const ppf_aniso_params&, hipStream_t);
+template int aniso_fused_step<float>(const ppf_grid&, const void*, void*, const void*, const ppf_aniso_params&,
+                                     uint64_t, int64_t, hipStream_t);
 template int aniso_step<float>(const ppf_grid&, const void*, void*, const void*, const ppf_grid&, const void*,
                                const ppf_aniso_params&, uint64_t, int64_t, hipStream_t);
 template int ncomp_step<float>(const ppf_grid&, const void*, void*, const void*, const ppf_temperature&,

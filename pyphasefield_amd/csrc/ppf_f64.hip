@@ -22,6 +22,8 @@ template int ncomp_thermo<double>(const ppf_grid&, const void*, const void*, con
                                  int64_t, hipStream_t);
 template int aniso_helper<double>(const ppf_grid&, const void*, const void*, const ppf_grid&, void*,
                                  const ppf_aniso_params&, hipStream_t);
+template int aniso_fused_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_aniso_params&,
+                                     uint64_t, int64_t, hipStream_t);
 template int aniso_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_grid&, const void*,
                                const ppf_aniso_params&, uint64_t, int64_t, hipStream_t);
 template int ncomp_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_temperature&,

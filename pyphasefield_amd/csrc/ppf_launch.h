@@ -40,6 +40,9 @@ template <typename R>
 int aniso_helper(const ppf_grid& g, const void* in, const void* bcvals, const ppf_grid& tg, void* transfer,
                  const ppf_aniso_params& p, hipStream_t s);
 template <typename R>
+int aniso_fused_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_aniso_params& p,
+                     uint64_t seed, int64_t step, hipStream_t s);
+template <typename R>
 int aniso_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_grid& tg,
                const void* transfer, const ppf_aniso_params& p, uint64_t seed, int64_t step, hipStream_t s);
 

@@ -84,3 +84,17 @@ def test_aniso_fp32_against_fp64_oracle(gpu):
     assert abs(int(got) - int(ref)) <= 1
     c = sim.fields[3].get_cells()
     assert abs(c.sum() - f0[3].sum()) <= 1e-6 * f0[3].sum()
+
+
+@pytest.mark.parametrize("bc", ["NEUMANN", "PERIODIC"])
+def test_aniso_fused_equals_two_pass_fp64(gpu, bc):
+    """The tiled single launch and the reference-shaped two launches evaluate
+    the same device functions: bit-identical FP64 fields (boundary tiles
+    included: 70 x 45 is not a multiple of the 64 x 8 tile)."""
+    a = _sim((45, 70), bc, sim_type="seeds", number_of_seeds=3)
+    b = _sim((45, 70), bc, sim_type="seeds", number_of_seeds=3)
+    b.user_data["aniso_two_pass"] = True
+    a.simulate(30)
+    b.simulate(30)
+    for i in range(4):
+        assert np.array_equal(a.fields[i].data, b.fields[i].data), a.fields[i].name
