@@ -376,8 +376,42 @@ def bench_aniso(args):
                   "grid": [n, n], "bytes_per_cell_update": 2 * 4 * esize}, roof, None)
 
 
+# ---------------------------------------------------- implicit diffusion --
+def bench_implicit(args):
+    """Diffusion 2D periodic, Crank-Nicolson: --variant adi (two cyclic line
+    solves per step, x then y) or cg (full system by conjugate gradients);
+    FP64, a = D dt / dx^2 = 4 (eight times the explicit stability limit)."""
+    import pyphasefield_amd.Engines as engines
+    n = args.size or 2048
+    adi = args.variant != "cg"
+    sim = engines.Diffusion(dimensions=[n, n], seed=3)
+    ud = {"D": 1.0, "solver": "crank-nicolson", "adi": adi}
+    if os.environ.get("PPF_SCAN_CHUNK"):
+        ud["scan_chunk"] = int(os.environ["PPF_SCAN_CHUNK"])
+    sim.set_user_data(ud)
+    sim.set_dx(1.0)
+    sim.set_dt(4.0)
+    sim.set_boundary_conditions("PERIODIC")
+    sim.set_precision("FP64")
+    sim.initialize_fields_and_imported_data()
+    sim.fields[0].data[1:-1, 1:-1] = np.random.default_rng(1).random((n, n))
+    sim.simulate(args.warmup)
+    t = _timed(sim.simulate, args.steps)
+    cells = n * n
+    # algorithmic bytes: ADI streams each cell 3 times per line solve (forward,
+    # back, correction: 6 x 8 B) for 2 solves; CG is reported per step
+    per = 2 * 48.0 if adi else 16.0
+    roof = _roofline(per * cells, t / args.steps, "k_line_solve<double>" if adi else "k_cg")
+    cfg = {"workload": "Diffusion 2D %dx%d FP64 periodic, Crank-Nicolson %s, a = 4" % (n, n, "ADI" if adi else "CG"),
+           "engine": "Diffusion", "grid": [n, n], "bytes_per_cell_update": per}
+    if not adi:
+        cfg["cg_iterations_last_step"] = getattr(sim, "_cg_iterations", None)
+    return _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f64", cfg,
+                 roof, None)
+
+
 CONFIGS = {"warren": bench_warren, "diffusion": bench_diffusion, "ch3d": bench_ch3d, "copy": bench_copy,
-           "ncomp": bench_ncomp, "aniso": bench_aniso}
+           "ncomp": bench_ncomp, "aniso": bench_aniso, "implicit": bench_implicit}
 
 
 def main():
@@ -404,7 +438,8 @@ def main():
               "(the domain would shard by slabs with a 1-2 cell halo exchange; not built)", file=sys.stderr)
         sys.exit(2)
     defaults = {"warren": (2000, 100), "diffusion": (1000, 50), "ch3d": (50, 5), "copy": (50, 5),
-                "ncomp": (500, 20), "ensemble": (5000, 100), "aniso": (500, 20)}
+                "ncomp": (500, 20), "ensemble": (5000, 100), "aniso": (500, 20),
+                "implicit": (50, 5)}
     st, wu = defaults[args.config]
     args.steps = args.steps if args.steps is not None else st
     args.warmup = args.warmup if args.warmup is not None else wu

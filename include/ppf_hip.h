@@ -264,6 +264,20 @@ int ppf_cyclic_table(int64_t n, double a, double* table);
 int ppf_diffusion_line_solve(const ppf_grid* g, const void* in, void* out, int axis, const double* table,
                              int explicit_axis, double b, ppf_stream_t s);
 
+/* The same line solves with parallelism along each line: the matrix factors
+   as (a/r)(I - r S)(I - r S^T), r = ((1+2a) - sqrt(1+4a))/(2a), S the cyclic
+   shift, so each solve is a forward and a backward first-order recurrence
+   closed around the ring, evaluated in chunks of m cells (one thread per
+   line and chunk) with carry scans between chunks.  table: a device copy of
+   ppf_scan_table(n[axis], a, m) (ppf_scan_table_len(m) doubles, host only);
+   work: ppf_line_scan_work_bytes(g, axis, m) bytes of device scratch.
+   Results equal ppf_diffusion_line_solve's to rounding. */
+int64_t ppf_scan_table_len(int64_t m);
+int ppf_scan_table(int64_t n, double a, int64_t m, double* table);
+int64_t ppf_line_scan_work_bytes(const ppf_grid* g, int axis, int64_t m);
+int ppf_diffusion_line_solve_scan(const ppf_grid* g, const void* in, void* out, int axis, const double* table,
+                                  int64_t m, int explicit_axis, double b, void* work, ppf_stream_t s);
+
 /* Full implicit / Crank-Nicolson systems of diffusion_matrix_2d/3d
    (Diffusion.py:243-308, 344-412):
      (1 + 2*ndim*a) x - a * sum_{6 or 4 neighbours} x = rhs,

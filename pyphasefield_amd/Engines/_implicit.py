@@ -7,9 +7,12 @@ solvers require PERIODIC boundaries on every axis.  With a = D*dt/dx^2
 
 * 1D and the alternating-direction variants (user_data["adi"]) are sequences
   of cyclic tridiagonal line solves, each optionally preceded by an explicit
-  half step, in the reference's axis order;
+  half step, in the reference's axis order (by default the scan-based solver,
+  parallel along each line; user_data["line_solver"] = "thomas" selects one
+  thread per line);
 * the full 2D/3D systems run matrix-free conjugate gradients to a relative
-  residual of user_data["cg_tol"] (default 1e-12 in FP64, 1e-6 in FP32).
+  residual of user_data["cg_tol"] (default 1e-12 in FP64, 1e-6 in FP32),
+  reading the device-side convergence state after 8, 16, 32, ... iterations.
 
 user_data["gmres"] selects scipy's approximate GMRES in the reference; both
 settings solve the same system here, to the stated tolerance.
@@ -65,17 +68,49 @@ def _passive(sim):
         sim._fields_out_gpu_device[1:].copy_(sim._fields_gpu_device[1:])
 
 
+# cells per chunk of the scan-based line solve
+_SCAN_CHUNK = 32
+
+
+def _scan_table(sim, n, a, m):
+    """Device copy of the scan solver's coefficients for (n, a, m), cached."""
+    import torch
+    cache = sim.__dict__.setdefault("_scan_tables", {})
+    key = (int(n), float(a), int(m))
+    t = cache.get(key)
+    if t is None:
+        L = _lib.lib()
+        host = np.zeros(int(L.ppf_scan_table_len(int(m))), dtype=np.float64)
+        _lib.check(L.ppf_scan_table(int(n), float(a), int(m), host.ctypes.data), "scan_table")
+        t = torch.from_numpy(host).to(sim._fields_gpu_device.device)
+        cache[key] = t
+    return t
+
+
 def line_steps(sim, solver, a):
+    """user_data["line_solver"]: "scan" (default; parallel along the lines) or
+    "thomas" (one thread per line)."""
     nd = len(sim.dimensions)
     am = a if solver == "implicit" else 0.5 * a
     L = _lib.lib()
+    scan = sim.user_data.get("line_solver", "scan") != "thomas"
     for axis, e in _LINE_PLANS[(solver, nd)]:
         n = sim._grid.n[axis]
-        tab = _table(sim, n, am)
-        rc = L.ppf_diffusion_line_solve(sim._grid, _lib.ptr(sim._fields_gpu_device),
-                                        _lib.ptr(sim._fields_out_gpu_device), axis, _lib.ptr(tab), e,
-                                        am if e >= 0 else 0.0, _lib.stream_handle())
-        _lib.check(rc, "diffusion_line_solve")
+        b = am if e >= 0 else 0.0
+        if scan:
+            m = int(sim.user_data.get("scan_chunk", _SCAN_CHUNK))
+            tab = _scan_table(sim, n, am, m)
+            work = _work(sim, "_scan_work_%d" % axis, int(L.ppf_line_scan_work_bytes(sim._grid, axis, m)))
+            rc = L.ppf_diffusion_line_solve_scan(sim._grid, _lib.ptr(sim._fields_gpu_device),
+                                                 _lib.ptr(sim._fields_out_gpu_device), axis, _lib.ptr(tab), m, e, b,
+                                                 _lib.ptr(work), _lib.stream_handle())
+            _lib.check(rc, "diffusion_line_solve_scan")
+        else:
+            tab = _table(sim, n, am)
+            rc = L.ppf_diffusion_line_solve(sim._grid, _lib.ptr(sim._fields_gpu_device),
+                                            _lib.ptr(sim._fields_out_gpu_device), axis, _lib.ptr(tab), e, b,
+                                            _lib.stream_handle())
+            _lib.check(rc, "diffusion_line_solve")
         _passive(sim)
         sim._swap_buffers()
 
@@ -108,15 +143,20 @@ def _work(sim, attr, nbytes):
 
 
 def _cg_loop(sim, iterate, tol):
-    """Run CG iterations in chunks until the device state reports convergence."""
+    """Run CG iterations in chunks until the device state reports convergence.
+    The chunks grow (8, 16, 32, ... up to user_data["cg_check_every"], default
+    256): few host reads of the state, few iterations launched past
+    convergence (those return at once)."""
     u = sim.user_data
     max_it = int(u.get("cg_max_iterations", 5000))
-    chunk = int(u.get("cg_check_every", 32))
+    cap = int(u.get("cg_check_every", 256))
+    chunk = min(8, cap)
     work = sim._cg_work
     k = 0
     while True:
         iterate(k, chunk)
         k += chunk
+        chunk = min(2 * chunk, cap)
         st = np.frombuffer(work[:_STATE.itemsize].cpu().numpy().tobytes(), dtype=_STATE)[0]
         if st["converged"]:
             sim._cg_iterations = int(st["iterations"])

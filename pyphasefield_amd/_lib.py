@@ -113,6 +113,11 @@ def lib():
         "ppf_aniso_helper": ([P(Grid), c_vp, c_vp, P(Grid), c_vp, P(AnisoParams), c_vp], c_i32),
         "ppf_aniso_step": ([P(Grid), c_vp, c_vp, c_vp, P(Grid), c_vp, P(AnisoParams), c_u64, c_i64, c_vp], c_i32),
         "ppf_aniso_fused_step": ([P(Grid), c_vp, c_vp, c_vp, P(AnisoParams), c_u64, c_i64, c_vp], c_i32),
+        "ppf_scan_table_len": ([c_i64], c_i64),
+        "ppf_scan_table": ([c_i64, c_f64, c_i64, c_vp], c_i32),
+        "ppf_line_scan_work_bytes": ([P(Grid), ctypes.c_int, c_i64], c_i64),
+        "ppf_diffusion_line_solve_scan": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_vp, c_i64, ctypes.c_int, c_f64, c_vp,
+                                           c_vp], c_i32),
         "ppf_cyclic_table_len": ([c_i64], c_i64),
         "ppf_cyclic_table": ([c_i64, c_f64, c_vp], c_i32),
         "ppf_diffusion_line_solve": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_vp, ctypes.c_int, c_f64, c_vp], c_i32),

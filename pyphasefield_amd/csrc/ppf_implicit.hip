@@ -199,6 +199,156 @@ __global__ __launch_bounds__(kLineThreads) void k_line_solve_var(const R* __rest
   }
 }
 
+// ----------------------------------------------- scan-based line solve --
+// The same constant-coefficient cyclic systems, with parallelism along each
+// line.  With r = ((1+2a) - sqrt(1+4a)) / (2a) < 1 (the root of
+// a r^2 - (1+2a) r + a = 0) the matrix factors as
+//   (1+2a) I - a (S + S^T) = (a/r) (I - r S)(I - r S^T),   (S x)_i = x_{i-1 mod n},
+// so x = (I - r S^T)^-1 (I - r S)^-1 (r/a) d: a forward and a backward
+// first-order recurrence, each closed around the ring (a recurrence started
+// from zero is off by r^(i+1) C at element i, C = end value / (1 - r^n)).
+// Each line is cut into chunks of m cells, one thread per (line, chunk):
+//   A  forward recurrence inside each chunk from zero, chunk end value E
+//   B  per line: carries between chunks and the ring closure
+//        K_k = carry into chunk k + r^(start_k) C_u
+//   C  forward fix-up u_i = v_i + r^(i+1-start) K_k, then the backward
+//      recurrence inside the chunk, chunk end value E'
+//   D  per line: backward carries and closure K'_k = carry + r^(n-end_k) C_x
+//   E  backward fix-up x_i = w_i + r^(end-i) K'_k
+// Table (host built, ppf_scan_table): r, r/a, r^n, m, then r^0 .. r^m.
+constexpr int64_t kScanHead = 4;
+
+struct Scan {
+  int64_t lines, nch, m;
+};
+
+__device__ __forceinline__ int64_t line_base(const Line& L, int64_t l) {
+  const int64_t i0 = l % L.m0, i1 = l / L.m0;
+  return L.base + i0 * L.s0 + i1 * L.s1;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_scan_a(const R* __restrict__ in, R* __restrict__ out, Line L, Scan S,
+                                                const double* __restrict__ tab, double bexp, double* __restrict__ E) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= S.lines * S.nch) return;
+  const int64_t l = t % S.lines, k = t / S.lines;
+  const int64_t o = line_base(L, l), n = L.n, st = L.stride;
+  int64_t dm = 0, dp = 0;
+  if (L.eslot >= 0) {
+    const int64_t i0 = l % L.m0, i1 = l / L.m0;
+    const int64_t c = L.eslot == 0 ? i0 : i1;
+    const int64_t s = L.eslot == 0 ? L.s0 : L.s1;
+    dm = c == 0 ? (L.en - 1) * s : -s;
+    dp = c == L.en - 1 ? -(L.en - 1) * s : s;
+  }
+  const R r = R(tab[0]), scale = R(tab[1]);
+  const R b = R(bexp), c0 = R(1.0 - 2.0 * bexp);
+  const int64_t i0 = k * S.m, i1 = i0 + S.m < n ? i0 + S.m : n;
+  R acc = R(0);
+  for (int64_t i = i0; i < i1; ++i) {
+    const int64_t p = o + i * st;
+    R d = in[p];
+    if (L.eslot >= 0) {
+      d = c0 * d + b * (in[p + dm] + in[p + dp]);
+    } else if (L.ealong) {
+      const R xm = in[i == 0 ? o + (n - 1) * st : p - st];
+      const R xp = in[i == n - 1 ? o : p + st];
+      d = c0 * d + b * (xm + xp);
+    }
+    acc = scale * d + r * acc;
+    out[p] = acc;
+  }
+  E[t] = double(acc);
+}
+
+// B (forward) / D (backward): one wavefront per line.  Chunk k in the order
+// of the recurrence acts on the carry as the affine map x -> r^(len_k) x + E_k;
+// each lane composes the maps of its consecutive chunks, a shuffle scan
+// composes across the lanes, and the lane's chunks are walked again to write
+// K_k = (carry into chunk k) + (product of the maps' slopes before k) * C,
+// C = (composition of all chunks applied to 0) / (1 - r^n) closing the ring.
+__global__ __launch_bounds__(256) void k_scan_carry(int64_t n, Scan S, const double* __restrict__ tab,
+                                                    const double* __restrict__ E, double* __restrict__ K,
+                                                    int backward) {
+  const int lane = threadIdx.x & 63;
+  const int64_t l = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (l >= S.lines) return;  // uniform per wavefront
+  const double rn = tab[2];
+  const double* P = tab + kScanHead;
+  const int64_t q = (S.nch + 63) / 64;
+  const int64_t j0 = int64_t(lane) * q, j1 = j0 + q < S.nch ? j0 + q : S.nch;
+  auto chunk = [&](int64_t j) { return backward ? S.nch - 1 - j : j; };
+  auto slope = [&](int64_t k) { return P[(k + 1) * S.m < n ? S.m : n - k * S.m]; };
+  double A = 1.0, B = 0.0;
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t k = chunk(j);
+    const double a = slope(k);
+    B = B * a + E[k * S.lines + l];
+    A *= a;
+  }
+#pragma unroll
+  for (int sft = 1; sft < 64; sft <<= 1) {
+    const double Ap = __shfl_up(A, sft, 64), Bp = __shfl_up(B, sft, 64);
+    if (lane >= sft) {
+      B = Bp * A + B;
+      A = Ap * A;
+    }
+  }
+  const double total = __shfl(B, 63, 64);
+  double Ar = __shfl_up(A, 1, 64), Br = __shfl_up(B, 1, 64);
+  if (lane == 0) {
+    Ar = 1.0;
+    Br = 0.0;
+  }
+  const double C = total / (1.0 - rn);
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t k = chunk(j);
+    const double e = E[k * S.lines + l];
+    K[k * S.lines + l] = Br + Ar * C;
+    const double a = slope(k);
+    Br = Br * a + e;
+    Ar *= a;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_scan_c(R* __restrict__ out, Line L, Scan S, const double* __restrict__ tab,
+                                                const double* __restrict__ K, double* __restrict__ E) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= S.lines * S.nch) return;
+  const int64_t l = t % S.lines, k = t / S.lines;
+  const int64_t o = line_base(L, l), n = L.n, st = L.stride;
+  const R r = R(tab[0]);
+  const double* P = tab + kScanHead;
+  const int64_t i0 = k * S.m, i1 = i0 + S.m < n ? i0 + S.m : n;
+  const R kk = R(K[t]);
+  R acc = R(0);
+  for (int64_t i = i1 - 1; i >= i0; --i) {
+    const int64_t p = o + i * st;
+    const R u = out[p] + R(P[i + 1 - i0]) * kk;
+    acc = u + r * acc;
+    out[p] = acc;
+  }
+  E[t] = double(acc);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_scan_e(R* __restrict__ out, Line L, Scan S, const double* __restrict__ tab,
+                                                const double* __restrict__ K) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= S.lines * S.nch) return;
+  const int64_t l = t % S.lines, k = t / S.lines;
+  const int64_t o = line_base(L, l), n = L.n, st = L.stride;
+  const double* P = tab + kScanHead;
+  const int64_t i0 = k * S.m, i1 = i0 + S.m < n ? i0 + S.m : n;
+  const R kk = R(K[t]);
+  for (int64_t i = i0; i < i1; ++i) {
+    const int64_t p = o + i * st;
+    out[p] = out[p] + R(P[i1 - i]) * kk;
+  }
+}
+
 // ---------------------------------------------------- conjugate gradients --
 struct Dense {
   int nd;
@@ -235,37 +385,32 @@ inline Work carve(void* w, int64_t N, int64_t esize) {
   return k;
 }
 
-struct Cell {
-  int64_t pad;      // padded offset
-  int64_t nb[6];    // dense indices of the neighbours
-  int cnt;
+// The CG kernels walk whole rows: block b takes rows (y, z) = b, b + nb, ...
+// in a fixed order and its threads stride along x, so a row's periodic
+// neighbour offsets are computed once and the per-block partial sums are
+// deterministic.
+struct Row {
+  int64_t t0;          // dense index of (0, y, z)
+  int64_t p0;          // padded offset of interior (0, y, z)
+  int64_t ym, yp, zm, zp;      // dense offsets of the y / z neighbours
+  int64_t pym, pyp, pzm, pzp;  // padded offsets of the same
 };
 
-__device__ __forceinline__ Cell cell_of(const Dense& d, int64_t t) {
-  Cell c;
-  const int64_t x = t % d.nx;
-  const int64_t y = d.nd >= 2 ? (t / d.nx) % d.ny : 0;
-  const int64_t z = d.nd >= 3 ? t / (d.nx * d.ny) : 0;
-  c.pad = (x + 1) + (d.nd >= 2 ? (y + 1) * d.sy : 0) + (d.nd >= 3 ? (z + 1) * d.sz : 0);
-  const int64_t row = t - x;
-  c.nb[0] = row + (x == 0 ? d.nx - 1 : x - 1);
-  c.nb[1] = row + (x == d.nx - 1 ? 0 : x + 1);
-  c.cnt = 2;
-  if (d.nd >= 2) {
-    const int64_t plane = t - (t % (d.nx * d.ny));
-    const int64_t inrow = t - plane - y * d.nx;
-    c.nb[2] = plane + (y == 0 ? d.ny - 1 : y - 1) * d.nx + inrow;
-    c.nb[3] = plane + (y == d.ny - 1 ? 0 : y + 1) * d.nx + inrow;
-    c.cnt = 4;
-  }
-  if (d.nd >= 3) {
-    const int64_t pl = d.nx * d.ny;
-    const int64_t inpl = t - z * pl;
-    c.nb[4] = (z == 0 ? d.nz - 1 : z - 1) * pl + inpl;
-    c.nb[5] = (z == d.nz - 1 ? 0 : z + 1) * pl + inpl;
-    c.cnt = 6;
-  }
-  return c;
+__device__ __forceinline__ Row row_of(const Dense& d, int64_t rw) {
+  Row r;
+  const int64_t y = d.nd >= 2 ? rw % d.ny : 0, z = d.nd >= 3 ? rw / d.ny : 0;
+  r.t0 = rw * d.nx;
+  r.p0 = 1 + (d.nd >= 2 ? (y + 1) * d.sy : 0) + (d.nd >= 3 ? (z + 1) * d.sz : 0);
+  const int64_t pl = d.nx * d.ny;
+  r.ym = (y == 0 ? d.ny - 1 : -1) * d.nx;
+  r.yp = (y == d.ny - 1 ? -(d.ny - 1) : 1) * d.nx;
+  r.zm = (z == 0 ? d.nz - 1 : -1) * pl;
+  r.zp = (z == d.nz - 1 ? -(d.nz - 1) : 1) * pl;
+  r.pym = (y == 0 ? d.ny - 1 : -1) * d.sy;
+  r.pyp = (y == d.ny - 1 ? -(d.ny - 1) : 1) * d.sy;
+  r.pzm = (z == 0 ? d.nz - 1 : -1) * d.sz;
+  r.pzp = (z == d.nz - 1 ? -(d.nz - 1) : 1) * d.sz;
+  return r;
 }
 
 // x0 = in (written to out), rhs = in or the explicit half step of in,
@@ -276,30 +421,26 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_begin(const R* __restrict__ i
   R* r = static_cast<R*>(w.r);
   double srr = 0.0, sbb = 0.0;
   const double na = 2.0 * d.nd;
-  for (int64_t t = int64_t(blockIdx.x) * kCgThreads + threadIdx.x; t < d.N; t += int64_t(d.nb) * kCgThreads) {
-    const int64_t x = t % d.nx;
-    const int64_t y = d.nd >= 2 ? (t / d.nx) % d.ny : 0;
-    const int64_t z = d.nd >= 3 ? t / (d.nx * d.ny) : 0;
-    const int64_t p = (x + 1) + (d.nd >= 2 ? (y + 1) * d.sy : 0) + (d.nd >= 3 ? (z + 1) * d.sz : 0);
-    // periodic neighbours in the padded array
-    // padded column of interior x' is x' + 1, so neighbour x' sits at p - x + x'
-    R s = in[p - x + (x == 0 ? d.nx - 1 : x - 1)] + in[p - x + (x == d.nx - 1 ? 0 : x + 1)];
-    if (d.nd >= 2) {
-      s = s + in[p + (y == 0 ? (d.ny - 1) * d.sy : -d.sy)] + in[p + (y == d.ny - 1 ? -(d.ny - 1) * d.sy : d.sy)];
+  const int64_t rows = d.ny * d.nz;
+  for (int64_t rw = blockIdx.x; rw < rows; rw += d.nb) {
+    const Row o = row_of(d, rw);
+    for (int64_t x = threadIdx.x; x < d.nx; x += kCgThreads) {
+      const int64_t p = o.p0 + x;
+      // periodic neighbours in the padded array
+      R s = in[p + (x == 0 ? d.nx - 1 : -1)] + in[p + (x == d.nx - 1 ? -(d.nx - 1) : 1)];
+      if (d.nd >= 2) s = s + in[p + o.pym] + in[p + o.pyp];
+      if (d.nd >= 3) s = s + in[p + o.pzm] + in[p + o.pzp];
+      const R v = in[p];
+      const R rhs = b != 0.0 ? R(1.0 - na * b) * v + R(b) * s : v;
+      R diag = R(1.0 + na * a);
+      if (VAR) diag = diag + R(srck) * (R(4) * v * v - R(6) * v + R(2));
+      const R ax = diag * v - R(a) * s;
+      const R rr = rhs - ax;
+      out[p] = v;
+      r[o.t0 + x] = rr;
+      srr += double(rr) * double(rr);
+      sbb += double(rhs) * double(rhs);
     }
-    if (d.nd >= 3) {
-      s = s + in[p + (z == 0 ? (d.nz - 1) * d.sz : -d.sz)] + in[p + (z == d.nz - 1 ? -(d.nz - 1) * d.sz : d.sz)];
-    }
-    const R v = in[p];
-    const R rhs = b != 0.0 ? R(1.0 - na * b) * v + R(b) * s : v;
-    R diag = R(1.0 + na * a);
-    if (VAR) diag = diag + R(srck) * (R(4) * v * v - R(6) * v + R(2));
-    const R ax = diag * v - R(a) * s;
-    const R rr = rhs - ax;
-    out[p] = v;
-    r[t] = rr;
-    srr += double(rr) * double(rr);
-    sbb += double(rhs) * double(rhs);
   }
   srr = block_sum_all(srr);
   sbb = block_sum_all(sbb);
@@ -340,21 +481,28 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_dir(const R* __restrict__ in,
   R* q = static_cast<R*>(w.q);
   const R bt = R(beta);
   const R cdiag = R(1.0 + 2.0 * d.nd * a), ra = R(a);
+  // direction at dense index j (k = 0: the residual)
+  auto dir = [&](int64_t j) { return k == 0 ? r[j] : r[j] + bt * po[j]; };
   double spq = 0.0;
-  for (int64_t t = int64_t(blockIdx.x) * kCgThreads + threadIdx.x; t < d.N; t += int64_t(d.nb) * kCgThreads) {
-    const Cell c = cell_of(d, t);
-    const R pv = k == 0 ? r[t] : r[t] + bt * po[t];
-    R s = R(0);
-    for (int j = 0; j < c.cnt; ++j) s = s + (k == 0 ? r[c.nb[j]] : r[c.nb[j]] + bt * po[c.nb[j]]);
-    R dg = cdiag;
-    if (VAR) {
-      const R v = in[c.pad];
-      dg = dg + R(srck) * (R(4) * v * v - R(6) * v + R(2));
+  const int64_t rows = d.ny * d.nz;
+  for (int64_t rw = blockIdx.x; rw < rows; rw += d.nb) {
+    const Row o = row_of(d, rw);
+    for (int64_t x = threadIdx.x; x < d.nx; x += kCgThreads) {
+      const int64_t t = o.t0 + x;
+      const R pv = dir(t);
+      R s = dir(t + (x == 0 ? d.nx - 1 : -1)) + dir(t + (x == d.nx - 1 ? -(d.nx - 1) : 1));
+      if (d.nd >= 2) s = s + dir(t + o.ym) + dir(t + o.yp);
+      if (d.nd >= 3) s = s + dir(t + o.zm) + dir(t + o.zp);
+      R dg = cdiag;
+      if (VAR) {
+        const R v = in[o.p0 + x];
+        dg = dg + R(srck) * (R(4) * v * v - R(6) * v + R(2));
+      }
+      const R qv = dg * pv - ra * s;
+      pn[t] = pv;
+      q[t] = qv;
+      spq += double(pv) * double(qv);
     }
-    const R qv = dg * pv - ra * s;
-    pn[t] = pv;
-    q[t] = qv;
-    spq += double(pv) * double(qv);
   }
   spq = block_sum_all(spq);
   if (threadIdx.x == 0) w.ppq[blockIdx.x] = spq;
@@ -371,15 +519,16 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_step(R* __restrict__ out, Den
   const R* q = static_cast<const R*>(w.q);
   const R al = R(alpha);
   double srr = 0.0;
-  for (int64_t t = int64_t(blockIdx.x) * kCgThreads + threadIdx.x; t < d.N; t += int64_t(d.nb) * kCgThreads) {
-    const int64_t x = t % d.nx;
-    const int64_t y = d.nd >= 2 ? (t / d.nx) % d.ny : 0;
-    const int64_t z = d.nd >= 3 ? t / (d.nx * d.ny) : 0;
-    const int64_t pp = (x + 1) + (d.nd >= 2 ? (y + 1) * d.sy : 0) + (d.nd >= 3 ? (z + 1) * d.sz : 0);
-    out[pp] = out[pp] + al * p[t];
-    const R rv = r[t] - al * q[t];
-    r[t] = rv;
-    srr += double(rv) * double(rv);
+  const int64_t rows = d.ny * d.nz;
+  for (int64_t rw = blockIdx.x; rw < rows; rw += d.nb) {
+    const Row o = row_of(d, rw);
+    for (int64_t x = threadIdx.x; x < d.nx; x += kCgThreads) {
+      const int64_t t = o.t0 + x, pp = o.p0 + x;
+      out[pp] = out[pp] + al * p[t];
+      const R rv = r[t] - al * q[t];
+      r[t] = rv;
+      srr += double(rv) * double(rv);
+    }
   }
   srr = block_sum_all(srr);
   if (threadIdx.x == 0) w.prr[((k + 1) & 1) * kCgBlocks + blockIdx.x] = srr;
@@ -394,8 +543,8 @@ Dense dense_of(const ppf_grid& g) {
   d.sy = g.sy;
   d.sz = g.sz;
   d.N = d.nx * d.ny * d.nz;
-  const int64_t want = (d.N + kCgThreads - 1) / kCgThreads;
-  d.nb = int(want < kCgBlocks ? want : kCgBlocks);
+  const int64_t rows = d.ny * d.nz;  // blocks take whole rows
+  d.nb = int(rows < kCgBlocks ? rows : kCgBlocks);
   return d;
 }
 
@@ -448,6 +597,33 @@ int line_solve_var(const ppf_grid& g, const void* in, void* out, int axis, doubl
   hipLaunchKernelGGL(k_line_solve_var<R>, dim3(blocks), dim3(kLineThreads), 0, s, static_cast<const R*>(in),
                      static_cast<R*>(out), L, a, b, srck, imexk, s1, s2);
   return check_launch("allencahn_line_solve");
+}
+
+template <typename R>
+int line_solve_scan(const ppf_grid& g, const void* in, void* out, int axis, const double* table, int eaxis,
+                    double b, void* work, int64_t m, hipStream_t s) {
+  const Line L = line_of(g, axis, eaxis);
+  Scan S;
+  S.lines = L.lines;
+  S.m = m;
+  S.nch = (L.n + m - 1) / m;
+  double* E = static_cast<double*>(work);
+  double* K = E + S.lines * S.nch;
+  const int64_t nt = S.lines * S.nch;
+  const unsigned gb = unsigned((nt + 255) / 256), lb = unsigned((S.lines + 3) / 4);
+  hipLaunchKernelGGL(k_scan_a<R>, dim3(gb), dim3(256), 0, s, static_cast<const R*>(in), static_cast<R*>(out), L, S,
+                     table, b, E);
+  if (int rc = check_launch("diffusion_line_solve_scan")) return rc;
+  hipLaunchKernelGGL(k_scan_carry, dim3(lb), dim3(256), 0, s, L.n, S, table, static_cast<const double*>(E), K, 0);
+  if (int rc = check_launch("diffusion_line_solve_scan")) return rc;
+  hipLaunchKernelGGL(k_scan_c<R>, dim3(gb), dim3(256), 0, s, static_cast<R*>(out), L, S, table,
+                     static_cast<const double*>(K), E);
+  if (int rc = check_launch("diffusion_line_solve_scan")) return rc;
+  hipLaunchKernelGGL(k_scan_carry, dim3(lb), dim3(256), 0, s, L.n, S, table, static_cast<const double*>(E), K, 1);
+  if (int rc = check_launch("diffusion_line_solve_scan")) return rc;
+  hipLaunchKernelGGL(k_scan_e<R>, dim3(gb), dim3(256), 0, s, static_cast<R*>(out), L, S, table,
+                     static_cast<const double*>(K));
+  return check_launch("diffusion_line_solve_scan");
 }
 
 template <typename R>
@@ -526,6 +702,53 @@ int ppf_cyclic_table(int64_t n, double a, double* table) {
   table[2] = r;
   table[3] = 0.0;
   return 0;
+}
+
+int64_t ppf_scan_table_len(int64_t m) { return m < 1 ? -1 : kScanHead + m + 1; }
+
+int ppf_scan_table(int64_t n, double a, int64_t m, double* table) {
+  if (n < 3) return fail(PPF_EINVAL, "scan_table: n must be at least 3");
+  if (m < 1) return fail(PPF_EINVAL, "scan_table: chunk length must be positive");
+  if (!(a >= 0.0) || !std::isfinite(a)) return fail(PPF_EINVAL, "scan_table: a must be finite and >= 0");
+  if (!table) return fail(PPF_EINVAL, "scan_table: null table");
+  // a = 0: the identity (the limit r -> 0 with r/a -> 1)
+  const double r = a == 0.0 ? 0.0 : ((1.0 + 2.0 * a) - std::sqrt(1.0 + 4.0 * a)) / (2.0 * a);
+  table[0] = r;
+  table[1] = a == 0.0 ? 1.0 : r / a;
+  table[2] = std::pow(r, double(n));
+  table[3] = double(m);
+  double pk = 1.0;
+  for (int64_t k = 0; k <= m; ++k) {
+    table[kScanHead + k] = pk;
+    pk *= r;
+  }
+  return 0;
+}
+
+int64_t ppf_line_scan_work_bytes(const ppf_grid* g, int axis, int64_t m) {
+  if (check_grid(g, 1, 3, "line_scan_work_bytes")) return -1;
+  if (axis < 0 || axis >= g->ndim || m < 1) return -1;
+  int64_t lines = 1;
+  for (int a = 0; a < g->ndim; ++a)
+    if (a != axis) lines *= g->n[a];
+  const int64_t nch = (g->n[axis] + m - 1) / m;
+  return 2 * lines * nch * int64_t(sizeof(double));
+}
+
+int ppf_diffusion_line_solve_scan(const ppf_grid* g, const void* in, void* out, int axis, const double* table,
+                                  int64_t m, int explicit_axis, double b, void* work, ppf_stream_t s) {
+  if (int rc = check_periodic(g, "diffusion_line_solve_scan")) return rc;
+  if (axis < 0 || axis >= g->ndim) return fail(PPF_EINVAL, "diffusion_line_solve_scan: bad axis");
+  if (explicit_axis < -1 || explicit_axis >= g->ndim)
+    return fail(PPF_EINVAL, "diffusion_line_solve_scan: bad explicit axis");
+  if (m < 1) return fail(PPF_EINVAL, "diffusion_line_solve_scan: chunk length must be positive");
+  if (!in || !out || !table || !work) return fail(PPF_EINVAL, "diffusion_line_solve_scan: null pointer");
+  if (in == out) return fail(PPF_EINVAL, "diffusion_line_solve_scan: in and out must differ");
+  if (!(b >= 0.0) || !std::isfinite(b))
+    return fail(PPF_EINVAL, "diffusion_line_solve_scan: b must be finite and >= 0");
+  return g->dtype == PPF_F32
+             ? line_solve_scan<float>(*g, in, out, axis, table, explicit_axis, b, work, m, stream_of(s))
+             : line_solve_scan<double>(*g, in, out, axis, table, explicit_axis, b, work, m, stream_of(s));
 }
 
 int ppf_diffusion_line_solve(const ppf_grid* g, const void* in, void* out, int axis, const double* table,

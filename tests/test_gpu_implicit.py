@@ -138,3 +138,21 @@ def test_allencahn_missing_reference_branches_raise(gpu):
     sim, _ = _ac_sim((20,), "crank-nicolson")
     with pytest.raises(NotImplementedError):
         sim.simulate(1)
+
+
+@pytest.mark.parametrize("shape,solver", [((333, 250), "crank-nicolson"), ((45, 37, 70), "implicit"),
+                                          ((1000,), "implicit")])
+def test_scan_line_solver_equals_thomas(gpu, shape, solver):
+    """The chunked-scan line solver (default) and the one-thread-per-line
+    Thomas solver agree to rounding (lengths that are not multiples of the
+    32-cell chunk included)."""
+    a_, c0 = _sim(shape, solver, True, a=3.0, seed=8)
+    b_, _ = _sim(shape, solver, True, a=3.0, seed=8)
+    b_.user_data["line_solver"] = "thomas"
+    a_.simulate(3)
+    b_.simulate(3)
+    assert rel_l2(a_.fields[0].get_cells(), b_.fields[0].get_cells()) <= 1e-13
+    want = c0
+    for _ in range(3):
+        want = oi.spectral_step(want, 3.0, solver, True)
+    assert rel_l2(a_.fields[0].get_cells(), want) <= FP64_TOL

@@ -198,3 +198,27 @@ def test_allencahn_without_double_well_is_the_diffusion_step(shape, solver, adi)
     a = 0.8
     got = oi.ac_step(c, a / 2 if solver == "crank-nicolson" else a, 0.0, solver, adi)
     assert np.allclose(got, oi.spectral_step(c, a, solver, adi), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("n,a,m", [(3, 0.5, 1), (37, 2.0, 8), (2048, 4.0, 32), (100, 0.0, 7)])
+def test_scan_table_holds_the_factorisation(n, a, m):
+    """r solves a r^2 - (1+2a) r + a = 0 with r < 1, so that
+    (1+2a) I - a (S + S^T) = (a/r)(I - r S)(I - r S^T); then r/a, r^n and
+    the powers r^0..r^m (a = 0: r = 0, r/a -> 1, the identity)."""
+    L = _lib.lib()
+    t = np.zeros(int(L.ppf_scan_table_len(m)))
+    assert L.ppf_scan_table(n, a, m, t.ctypes.data) == 0
+    r, scale, rn, mm = t[:4]
+    assert mm == m
+    if a == 0:
+        assert r == 0 and scale == 1
+    else:
+        assert 0 < r < 1 and abs(a * r * r - (1 + 2 * a) * r + a) <= 1e-15 * (1 + 2 * a)
+        assert scale == pytest.approx(r / a, rel=1e-15)
+        S = np.roll(np.eye(min(n, 40)), 1, axis=0)
+        k = S.shape[0]
+        lhs = (a / r) * (np.eye(k) - r * S) @ (np.eye(k) - r * S.T)
+        assert np.allclose(lhs, oi.matrix_1d(k, 1 + 2 * a, -a), rtol=0, atol=1e-12 * (1 + a))
+    assert rn == pytest.approx(r ** n, rel=1e-12, abs=1e-300)
+    assert np.allclose(t[4:], r ** np.arange(m + 1), rtol=1e-14, atol=0)
+    assert L.ppf_scan_table(n, a, 0, t.ctypes.data) == EINVAL
