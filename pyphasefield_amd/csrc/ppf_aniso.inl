@@ -277,7 +277,10 @@ __global__ __launch_bounds__(256) void k_aniso_step(const R* __restrict__ in, R*
 // pointwise temp / D_C and the (64+1) x (8+1) vertex terms are computed into
 // LDS, and each thread then updates two cells: the six transfer arrays of
 // the two-pass path never touch HBM.
-constexpr int kAdTX = 64, kAdTY = 8;
+#ifndef PPF_AD_TY
+#define PPF_AD_TY 8
+#endif
+constexpr int kAdTX = 64, kAdTY = PPF_AD_TY;
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_aniso_fused(const R* __restrict__ in, R* __restrict__ out,
