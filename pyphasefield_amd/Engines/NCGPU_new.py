@@ -21,7 +21,6 @@ import numpy as np
 from .. import _lib
 from ..simulation import Simulation
 from ..utils import make_seed
-from . import _gpu
 
 
 class NCGPU_new(Simulation):

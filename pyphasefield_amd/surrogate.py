@@ -15,8 +15,6 @@ project's elementwise and Adam kernels; inference is the fused MFMA kernel
 cells outside the trained range.  torch only provides device memory.
 """
 
-import ctypes
-
 import numpy as np
 
 from . import _lib

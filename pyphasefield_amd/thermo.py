@@ -93,7 +93,6 @@ class TablePhase:
     def tables(self):
         """(g0, L) in the oracle's table format: [(T_hi, c, n)] per component and
         [[(T_hi, c, n)] * 3] per pair"""
-        from .tdb import Piecewise
         n = self.ncomp
         zero = (np.full(_lib.PPF_TPOLY_RANGES, np.inf), np.zeros((_lib.PPF_TPOLY_RANGES, _lib.PPF_TPOLY_TERMS)), 0)
         g0 = [f.device_table(_lib.PPF_TPOLY_RANGES) for f in self.g0]

@@ -2,8 +2,6 @@
 engines for every tile configuration, close to the FP64 oracle, HIP-graph
 replays equal to eager steps, autotuner and Simulation interoperation."""
 
-import os
-
 import numpy as np
 import pytest
 
