@@ -222,3 +222,41 @@ def test_scan_table_holds_the_factorisation(n, a, m):
     assert rn == pytest.approx(r ** n, rel=1e-12, abs=1e-300)
     assert np.allclose(t[4:], r ** np.arange(m + 1), rtol=1e-14, atol=0)
     assert L.ppf_scan_table(n, a, 0, t.ctypes.data) == EINVAL
+
+
+def test_new_entry_points_reject_bad_arguments_without_a_device():
+    """Allen-Cahn line/CG solves, the scan line solve, the AnisoDorr fused step
+    and the work-size queries validate before any launch."""
+    L = _lib.lib()
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    w = ctypes.c_void_p(0x3000)
+    gp = _grid(_lib.PPF_PERIODIC)
+    gn = _grid(_lib.PPF_NEUMANN)
+    # Allen-Cahn
+    assert L.ppf_allencahn_line_solve(gn, a, b, 0, 1.0, -1, 0.0, 1.0, 0.0, w, None) == EINVAL
+    assert L.ppf_allencahn_line_solve(gp, a, a, 0, 1.0, -1, 0.0, 1.0, 0.0, w, None) == EINVAL
+    assert L.ppf_allencahn_line_solve(gp, a, b, 0, float("inf"), -1, 0.0, 1.0, 0.0, w, None) == EINVAL
+    assert L.ppf_allencahn_line_solve(gp, a, b, 0, 1.0, -1, 0.0, float("nan"), 0.0, w, None) == EINVAL
+    assert L.ppf_allencahn_line_solve(gp, a, b, 0, 1.0, -1, 0.0, 1.0, 0.0, None, None) == EINVAL
+    assert L.ppf_allencahn_cg_begin(gp, a, a, 1.0, 1.0, w, None) == EINVAL
+    assert L.ppf_allencahn_cg_iterate(gp, a, b, 1.0, 1.0, 1e-12, 0, -1, w, None) == EINVAL
+    assert L.ppf_allencahn_line_work_bytes(gp) == 2 * gp.sf * 8
+    # scan line solve
+    assert L.ppf_diffusion_line_solve_scan(gp, a, b, 0, w, 0, -1, 0.0, w, None) == EINVAL  # chunk length
+    assert L.ppf_diffusion_line_solve_scan(gp, a, b, 0, w, 32, -1, 0.0, None, None) == EINVAL
+    assert L.ppf_diffusion_line_solve_scan(gn, a, b, 0, w, 32, -1, 0.0, w, None) == EINVAL
+    assert L.ppf_line_scan_work_bytes(gp, 0, 32) == 2 * 8 * 1 * 8  # 8 lines x 1 chunk x (E, K) doubles
+    assert L.ppf_line_scan_work_bytes(gp, 2, 32) == -1
+    # AnisoDorr fused step: 4-field stack, 2D only, positive T/R/beta, distinct buffers
+    p = _lib.AnisoParams()
+    p.T, p.R, p.beta = 1574.0, 8.314, 1.5
+    g4 = _grid(_lib.PPF_PERIODIC)
+    g4.nfields = 4
+    assert L.ppf_aniso_fused_step(gp, a, b, None, p, 0, 1, None) == EINVAL   # 1 field
+    assert L.ppf_aniso_fused_step(g4, a, a, None, p, 0, 1, None) == EINVAL   # aliased
+    p.beta = 0.0
+    assert L.ppf_aniso_fused_step(g4, a, b, None, p, 0, 1, None) == EINVAL
+    g3 = _grid(_lib.PPF_PERIODIC, n=(8, 8, 8), ndim=3)
+    g3.nfields = 4
+    p.beta = 1.5
+    assert L.ppf_aniso_fused_step(g3, a, b, None, p, 0, 1, None) == EINVAL   # 3D
