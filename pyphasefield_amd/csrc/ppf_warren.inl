@@ -13,6 +13,7 @@ namespace ppf {
 #define PPF_WARREN_RPT 4
 #endif
 constexpr int kWRpt = PPF_WARREN_RPT;
+
 constexpr int kWTY = 4 * kWRpt;
 
 struct WarrenK {
@@ -149,78 +150,97 @@ __device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R
   nc = c + deltac * K.dtc;
 }
 
+// ---- pieces of the tile step
+constexpr int kWW = kTX + 2, kWH = kWTY + 2;  // LDS tile: (64+2) x (16+2)
+constexpr int kWNP = (kWW * kWH + kThreads - 1) / kThreads, kWNFULL = (kWW * kWH) / kThreads;
+
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R* __restrict__ out,
-                                                     const R* __restrict__ b, Dom d, R dx, WarrenK p,
-                                                     uint64_t seed, int64_t step) {
-  constexpr int W = kTX + 2, HH = kWTY + 2;
-  __shared__ R sphi[HH][W];
-  __shared__ R sc[HH][W];
-  __shared__ R sD[HH][W];
-  __shared__ R stemp[HH][W];
-  uint32_t nbx = uint32_t((d.ax.n + kTX - 1) / kTX);
-  uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int64_t x0 = 1 + int64_t(blk % nbx) * kTX;
-  const int64_t y0 = 1 + int64_t(blk / nbx) * kWTY;
-  const bool interior = x0 - 1 >= 1 && x0 + kTX <= d.ax.n && y0 - 1 >= 1 && y0 + kWTY <= d.ay.n;
-  const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
-  const WarrenC<R> K(p, dx);
-  const int64_t xs = x0 - 1, ys = y0 - 1;
-  // 1) phi and c of the (64+2) x (16+2) tile, and the pointwise D_C and temp
-  //    (Warren1995.py:69-70) derived from them, into LDS.  The tile is walked
-  //    linearly (LDS index t = ly * W + lx), so every lane of every pass does
-  //    useful work.  Interior tiles read directly; tiles touching the boundary
-  //    fold the rules in, in a loop kept out of line so the interior path
-  //    stays short.
-  R* const fphi = &sphi[0][0];
-  R* const fc = &sc[0][0];
-  R* const fD = &sD[0][0];
-  R* const ftemp = &stemp[0][0];
-  auto pointwise = [&](int t, R ph, R c) {
-    R D_C, temp;
-    warren_point(K, ph, c, D_C, temp);
-    fphi[t] = ph;
-    fc[t] = c;
-    fD[t] = D_C;
-    ftemp[t] = temp;
-  };
-  if (interior) {
-    const int32_t sy32 = int32_t(d.ay.stride);
-    const R* __restrict__ gphi = in + (ys * d.ay.stride + xs);
-    const R* __restrict__ gc = gphi + d.sf;
-    // all loads first (only the last pass is partial), then the arithmetic
-    constexpr int NP = (W * HH + kThreads - 1) / kThreads, NFULL = (W * HH) / kThreads;
-    R vph[NP], vc[NP];
+struct WarrenLds {
+  R phi[kWH][kWW];
+  R c[kWH][kWW];
+  R D[kWH][kWW];
+  R temp[kWH][kWW];
+};
+
+// tile t (after the XCD swizzle over ntiles): output origin and whether its
+// halo ring lies inside the interior (no boundary rule needed)
+__device__ __forceinline__ void warren_tile(const Dom& d, uint32_t t, uint32_t ntiles, int64_t& x0, int64_t& y0,
+                                            bool& interior) {
+  const uint32_t nbx = uint32_t((d.ax.n + kTX - 1) / kTX);
+  const uint32_t blk = xcd_swizzle(t, ntiles);
+  x0 = 1 + int64_t(blk % nbx) * kTX;
+  y0 = 1 + int64_t(blk / nbx) * kWTY;
+  interior = x0 - 1 >= 1 && x0 + kTX <= d.ax.n && y0 - 1 >= 1 && y0 + kWTY <= d.ay.n;
+}
+
+// phi and c of an interior tile (halo included), kWNP per thread, walked
+// linearly (LDS index t = ly * kWW + lx); only the last pass is partial
+template <typename R>
+__device__ __forceinline__ void warren_load(const R* __restrict__ in, const Dom& d, int64_t x0, int64_t y0,
+                                            R (&vph)[kWNP], R (&vc)[kWNP]) {
+  const int32_t sy32 = int32_t(d.ay.stride);
+  const R* __restrict__ gphi = in + ((y0 - 1) * d.ay.stride + (x0 - 1));
+  const R* __restrict__ gc = gphi + d.sf;
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      const uint32_t t = threadIdx.x + uint32_t(j * kThreads);
-      if (j < NFULL || t < uint32_t(W * HH)) {
-        const uint32_t ly = t / uint32_t(W), lx = t - ly * uint32_t(W);
-        const int32_t o = int32_t(ly) * sy32 + int32_t(lx);
-        vph[j] = gphi[o];
-        vc[j] = gc[o];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      const uint32_t t = threadIdx.x + uint32_t(j * kThreads);
-      if (j < NFULL || t < uint32_t(W * HH)) pointwise(int(t), vph[j], vc[j]);
-    }
-  } else {
-#pragma unroll 1
-    for (int t = threadIdx.x; t < W * HH; t += kThreads) {
-      const int ly = t / W, lx = t - (t / W) * W;
-      const int64_t x = xs + lx, y = ys + ly;
-      R ph = R(1), c = R(0);
-      if (x <= d.ax.n + 1 && y <= d.ay.n + 1) {
-        ph = fold(in, b, d, x, y, int64_t(0), dx);
-        c = fold(in + d.sf, b ? b + d.sf : nullptr, d, x, y, int64_t(0), dx);
-      }
-      pointwise(t, ph, c);
+  for (int j = 0; j < kWNP; ++j) {
+    const uint32_t t = threadIdx.x + uint32_t(j * kThreads);
+    if (j < kWNFULL || t < uint32_t(kWW * kWH)) {
+      const uint32_t ly = t / uint32_t(kWW), lx = t - ly * uint32_t(kWW);
+      const int32_t o = int32_t(ly) * sy32 + int32_t(lx);
+      vph[j] = gphi[o];
+      vc[j] = gc[o];
     }
   }
-  __syncthreads();
+}
 
+template <typename R>
+__device__ __forceinline__ void warren_put(WarrenLds<R>& S, const WarrenC<R>& K, int t, R ph, R c) {
+  R D_C, temp;
+  warren_point(K, ph, c, D_C, temp);
+  (&S.phi[0][0])[t] = ph;
+  (&S.c[0][0])[t] = c;
+  (&S.D[0][0])[t] = D_C;
+  (&S.temp[0][0])[t] = temp;
+}
+
+// pointwise D_C and temp (Warren1995.py:69-70) of a loaded interior tile
+template <typename R>
+__device__ __forceinline__ void warren_fill(WarrenLds<R>& S, const WarrenC<R>& K, const R (&vph)[kWNP],
+                                            const R (&vc)[kWNP]) {
+#pragma unroll
+  for (int j = 0; j < kWNP; ++j) {
+    const uint32_t t = threadIdx.x + uint32_t(j * kThreads);
+    if (j < kWNFULL || t < uint32_t(kWW * kWH)) warren_put(S, K, int(t), vph[j], vc[j]);
+  }
+}
+
+// the same for a tile touching the boundary: the rules are folded into the
+// loads, in a loop kept out of line so the interior path stays short
+template <typename R>
+__device__ __forceinline__ void warren_fill_fold(WarrenLds<R>& S, const WarrenC<R>& K, const R* __restrict__ in,
+                                                 const R* __restrict__ b, const Dom& d, R dx, int64_t x0,
+                                                 int64_t y0) {
+#pragma unroll 1
+  for (int t = threadIdx.x; t < kWW * kWH; t += kThreads) {
+    const int ly = t / kWW, lx = t - (t / kWW) * kWW;
+    const int64_t x = x0 - 1 + lx, y = y0 - 1 + ly;
+    R ph = R(1), c = R(0);
+    if (x <= d.ax.n + 1 && y <= d.ay.n + 1) {
+      ph = fold(in, b, d, x, y, int64_t(0), dx);
+      c = fold(in + d.sf, b ? b + d.sf : nullptr, d, x, y, int64_t(0), dx);
+    }
+    warren_put(S, K, t, ph, c);
+  }
+}
+
+// the cell updates of one tile from LDS: each thread walks kWRpt
+// consecutive rows of one column keeping a 3-row register window, so a cell
+// costs 14 LDS reads
+template <typename R>
+__device__ __forceinline__ void warren_rows(const WarrenLds<R>& S, const WarrenC<R>& K, R* __restrict__ out,
+                                            const Dom& d, int64_t x0, int64_t y0, bool interior, uint64_t seed,
+                                            int64_t step) {
+  const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
   const int64_t x = x0 + tx;
   if (x > d.ax.n) return;
   const uint64_t ncells = uint64_t(d.ax.n + 2) * uint64_t(d.ay.n + 2);
@@ -229,17 +249,17 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
   R* __restrict__ orow = out + ((y0 + ly0 - 1) * d.ay.stride + x);
   // register window: phi rows ly-1, ly, ly+1 (3 columns each); c, D_C, temp:
   // centre of row ly-1, 3 columns of row ly, centre of row ly+1
-  R pm0 = sphi[ly0 - 1][lx - 1], pm1 = sphi[ly0 - 1][lx], pm2 = sphi[ly0 - 1][lx + 1];
-  R pc0 = sphi[ly0][lx - 1], pc1 = sphi[ly0][lx], pc2 = sphi[ly0][lx + 1];
-  R cm = sc[ly0 - 1][lx], c0 = sc[ly0][lx - 1], c1 = sc[ly0][lx], c2v = sc[ly0][lx + 1];
-  R Dm = sD[ly0 - 1][lx], D0 = sD[ly0][lx - 1], D1 = sD[ly0][lx], D2 = sD[ly0][lx + 1];
-  R tm = stemp[ly0 - 1][lx], t0 = stemp[ly0][lx - 1], t1 = stemp[ly0][lx], t2 = stemp[ly0][lx + 1];
+  R pm0 = S.phi[ly0 - 1][lx - 1], pm1 = S.phi[ly0 - 1][lx], pm2 = S.phi[ly0 - 1][lx + 1];
+  R pc0 = S.phi[ly0][lx - 1], pc1 = S.phi[ly0][lx], pc2 = S.phi[ly0][lx + 1];
+  R cm = S.c[ly0 - 1][lx], c0 = S.c[ly0][lx - 1], c1 = S.c[ly0][lx], c2v = S.c[ly0][lx + 1];
+  R Dm = S.D[ly0 - 1][lx], D0 = S.D[ly0][lx - 1], D1 = S.D[ly0][lx], D2 = S.D[ly0][lx + 1];
+  R tm = S.temp[ly0 - 1][lx], t0 = S.temp[ly0][lx - 1], t1 = S.temp[ly0][lx], t2 = S.temp[ly0][lx + 1];
 #pragma unroll
   for (int r = 0; r < kWRpt; ++r) {
     const int ly = ly0 + r;
     const int64_t y = y0 + ly - 1;
-    const R pp0 = sphi[ly + 1][lx - 1], pp1 = sphi[ly + 1][lx], pp2 = sphi[ly + 1][lx + 1];
-    const R cp = sc[ly + 1][lx], Dp = sD[ly + 1][lx], tp = stemp[ly + 1][lx];
+    const R pp0 = S.phi[ly + 1][lx - 1], pp1 = S.phi[ly + 1][lx], pp2 = S.phi[ly + 1][lx + 1];
+    const R cp = S.c[ly + 1][lx], Dp = S.D[ly + 1][lx], tp = S.temp[ly + 1][lx];
     if (interior || y <= d.ay.n) {
       const uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
       R nph, nc;
@@ -255,11 +275,46 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
     cm = c1; Dm = D1; tm = t1;
     c1 = cp; D1 = Dp; t1 = tp;
     if (r < kWRpt - 1) {
-      c0 = sc[ly + 1][lx - 1]; c2v = sc[ly + 1][lx + 1];
-      D0 = sD[ly + 1][lx - 1]; D2 = sD[ly + 1][lx + 1];
-      t0 = stemp[ly + 1][lx - 1]; t2 = stemp[ly + 1][lx + 1];
+      c0 = S.c[ly + 1][lx - 1]; c2v = S.c[ly + 1][lx + 1];
+      D0 = S.D[ly + 1][lx - 1]; D2 = S.D[ly + 1][lx + 1];
+      t0 = S.temp[ly + 1][lx - 1]; t2 = S.temp[ly + 1][lx + 1];
     }
   }
+}
+
+// One 64 x 16 tile per 256-thread block: load phi/c (halo included), derive
+// D_C and temp there into LDS, then the row walk.
+// Occupancy hint (waves per SIMD the allocation must allow; 0 = compiler's
+// choice).  Left to itself the compiler takes 102 SGPRs, which caps the
+// kernel at 7 waves per SIMD (7 blocks per CU: a 2048^2 grid's 4096 tiles then
+// run in 2.3 rounds); with 8 it keeps 78 SGPRs, spilling the rest to VGPR
+// lanes, and 4096 tiles fill exactly two rounds.
+#ifndef PPF_WARREN_WAVES
+#define PPF_WARREN_WAVES 8
+#endif
+#if PPF_WARREN_WAVES
+#define PPF_WARREN_ATTR __attribute__((amdgpu_waves_per_eu(PPF_WARREN_WAVES, 8)))
+#else
+#define PPF_WARREN_ATTR
+#endif
+template <typename R>
+__global__ __launch_bounds__(kThreads) PPF_WARREN_ATTR void k_warren(const R* __restrict__ in, R* __restrict__ out,
+                                                     const R* __restrict__ b, Dom d, R dx, WarrenK p,
+                                                     uint64_t seed, int64_t step) {
+  __shared__ WarrenLds<R> S;
+  const WarrenC<R> K(p, dx);
+  int64_t x0, y0;
+  bool interior;
+  warren_tile(d, blockIdx.x, gridDim.x, x0, y0, interior);
+  if (interior) {
+    R vph[kWNP], vc[kWNP];
+    warren_load(in, d, x0, y0, vph, vc);  // all loads first, then the arithmetic
+    warren_fill(S, K, vph, vc);
+  } else {
+    warren_fill_fold(S, K, in, b, d, dx, x0, y0);
+  }
+  __syncthreads();
+  warren_rows(S, K, out, d, x0, y0, interior, seed, step);
 }
 
 template <typename R>
@@ -281,8 +336,8 @@ int warren_step(const ppf_grid& g, const void* in, void* out, const void* bcvals
   k.M_A = p.M_A;
   k.M_B = p.M_B;
   k.dt = p.dt;
-  unsigned blocks = unsigned(((g.n[0] + kTX - 1) / kTX) * ((g.n[1] + kWTY - 1) / kWTY));
-  hipLaunchKernelGGL(k_warren<R>, dim3(blocks), dim3(kThreads), 0, s, static_cast<const R*>(in),
+  const unsigned tiles = unsigned(((g.n[0] + kTX - 1) / kTX) * ((g.n[1] + kWTY - 1) / kWTY));
+  hipLaunchKernelGGL(k_warren<R>, dim3(tiles), dim3(kThreads), 0, s, static_cast<const R*>(in),
                      static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), k, seed, step);
   return check_launch("warren_step");
 }

@@ -120,6 +120,23 @@ def test_diffusion_3d_1d_fp64(gpu, shape):
     assert rel_l2(sim.fields[0].data, want) <= FP64_TOL
 
 
+
+def test_diffusiongpu_engine_fp64(gpu):
+    """DiffusionGPU (Engines/DiffusionGPU.py:76-117): its default start (c = 1
+    in the middle half, D = 0.1) stepped with dt, as the explicit Diffusion."""
+    E = _engines()
+    sim = E.DiffusionGPU(dimensions=[40, 56])
+    _setup(sim, "NEUMANN", 1.0, 0.5, "FP64")
+    c0 = sim.fields[0].get_cells().copy()
+    assert c0.sum() == 20 * 28 and c0[10:30, 14:42].min() == 1.0
+    sim.simulate(60)
+    want = run_point("diffusion", c0, 60, "NEUMANN", 1.0, 0.5, D=0.1)[1:-1, 1:-1]
+    assert rel_l2(sim.fields[0].get_cells(), want) <= FP64_TOL
+    assert abs(sim.fields[0].get_cells().sum() - c0.sum()) <= 1e-11 * c0.sum()
+    sim.user_data["solver"] = "implicit"
+    with pytest.raises(ValueError):
+        sim.simulate(1)
+
 def test_diffusion_fp32(gpu):
     E = _engines()
     sim = E.Diffusion(dimensions=[130, 200])

@@ -260,3 +260,19 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
     g3.nfields = 4
     p.beta = 1.5
     assert L.ppf_aniso_fused_step(g3, a, b, None, p, 0, 1, None) == EINVAL   # 3D
+
+
+def test_engine_registry_exports_the_reference_engine_classes():
+    """Engines/__init__.py discovery: every engine class of the reference is
+    importable by name; the legacy NCGPU refuses construction with a message."""
+    import pyphasefield_amd.Engines as E
+    for name in ("AnisoDorrGPU", "CahnAllen", "CahnHilliard", "Diffusion", "DiffusionGPU", "NCGPU", "NCGPU_new",
+                 "Template", "Warren1995"):
+        assert isinstance(getattr(E, name), type), name
+    with pytest.raises(NotImplementedError, match="NCGPU_new"):
+        E.NCGPU(dimensions=[8, 8])
+    s = E.DiffusionGPU(dimensions=[8, 12])
+    assert s.user_data["D"] == 0.1 and s.get_framework() == "GPU_SERIAL"
+    s.initialize_fields_and_imported_data()
+    c = s.fields[0].get_cells()
+    assert c.sum() == 4 * 6 and c[2:6, 3:9].min() == 1.0
