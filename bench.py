@@ -112,10 +112,19 @@ def cpu_warren(n, k=3):
                       "interpolants (Warren1995.py:26-28), single-threaded numpy" % (k, n, n)}
 
 
+def _warm(sim, warmup):
+    """Untimed steps; engines that replay captured HIP graphs also capture
+    theirs here, so the timed region holds replays only."""
+    sim.simulate(warmup)
+    chunk = sim._graph_chunk()
+    if chunk:
+        sim.simulate(chunk)
+
+
 def bench_warren(args):
     n = 2048
     sim = warren_sim(n, "FP32")
-    sim.simulate(args.warmup)
+    _warm(sim, args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     value = cells * args.steps / t
@@ -153,7 +162,7 @@ def bench_diffusion(args):
     sim.initialize_fields_and_imported_data()
     sim.fields[0].data[1:-1, 1:-1] = np.random.default_rng(0).random((n, n))
     args.warmup = max(args.warmup, sim._GRAPH_CHUNK)  # one graph capture inside the warm-up
-    sim.simulate(args.warmup)
+    _warm(sim, args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     roof = _roofline(16.0 * cells, t / args.steps, "k_point2d<double,0,4>")
@@ -197,7 +206,7 @@ def bench_ch3d(args):
     # the warm-up covers at least one graph chunk, so the captured step graph
     # is recorded there and replayed in the timed region
     args.warmup = max(args.warmup, sim._GRAPH_CHUNK)
-    sim.simulate(args.warmup)
+    _warm(sim, args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n ** 3
     roof = _roofline(8.0 * cells, t / args.steps, "k_ch3w<float,16>")
@@ -290,14 +299,14 @@ def bench_ncomp(args):
         sim._surrogate.split2 = "2" in args.mlp_split
         sim._surrogate.split3 = "3" in args.mlp_split
         sim._surrogate._packed = None
-    sim.simulate(args.warmup)
+    _warm(sim, args.warmup)
     if variant == "surrogate" and args.surrogate and not os.path.exists(args.surrogate):
         sim.user_data["surrogate"].save(args.surrogate)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     # per-kernel averages from separate timed loops on the same stream
     k = max(10, args.steps // 10)
-    t_thermo = _timed(lambda m: [sim.thermo_pass() for _ in range(m)], k) / k
+    t_thermo = 0.0 if sim._kernel_mode() == "fused" else _timed(lambda m: [sim.thermo_pass() for _ in range(m)], k) / k
     t_total = t / args.steps
     esize = 8 if precision == "FP64" else 4
     nf = len(sim.fields)
@@ -321,7 +330,7 @@ def bench_ncomp(args):
         for name, prec, mode in (("plain_fp64_forward_difference", "FP64", "forward_difference"),
                                  ("plain_fp32_analytic", "FP32", "analytic")):
             other = ncomp_sim(n, prec, mode)
-            other.simulate(max(2, args.warmup // 2))
+            _warm(other, max(2, args.warmup // 2))
             tt = _timed(other.simulate, ks) / ks
             same[name] = {"ms_per_step": tt * 1e3, "surrogate_speedup": tt / t_total, "steps": ks}
             del other
@@ -364,7 +373,7 @@ def bench_aniso(args):
     sim.set_precision(precision)
     sim.set_user_data({"sim_type": "seeds", "number_of_seeds": 16, "initial_concentration_array": [0.40831]})
     sim.initialize_fields_and_imported_data()
-    sim.simulate(args.warmup)
+    _warm(sim, args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     esize = 8 if precision == "FP64" else 4
@@ -395,7 +404,7 @@ def bench_implicit(args):
     sim.set_precision("FP64")
     sim.initialize_fields_and_imported_data()
     sim.fields[0].data[1:-1, 1:-1] = np.random.default_rng(1).random((n, n))
-    sim.simulate(args.warmup)
+    _warm(sim, args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     # algorithmic bytes: ADI streams each cell 3 times per line solve (forward,

@@ -185,6 +185,18 @@ int ppf_cahnhilliard_step(const ppf_grid* g, const void* in, void* out, const vo
 int ppf_warren_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
                     const ppf_warren_params* p, uint64_t seed, int64_t step, ppf_stream_t s);
 
+/* N-component step in one launch: replaces NComponent_helper_kernel +
+   NComponent_kernel (Engines/NCGPU_new.py:256-313 and :37-250, launched in
+   that order by simulation_loop :521-554).  The thermodynamic quantities of
+   the helper (G_S - G_L, M_c, dF/dc with its noise) are evaluated for each
+   output tile and its +1 ring into LDS from the same fields the step loads,
+   with the arithmetic of ppf_ncomp_thermo, so no transfer stack exists.
+   Results equal ppf_ncomp_thermo followed by ppf_ncomp_step (bit for bit in
+   FP64). */
+int ppf_ncomp_fused_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                         const ppf_temperature* t, const ppf_ncomp_params* p, const ppf_thermo* th,
+                         uint64_t seed, int64_t step, ppf_stream_t s);
+
 /* N-component thermodynamic pass: replaces NComponent_helper_kernel
    (Engines/NCGPU_new.py:256-313).  Writes, for every padded cell including the
    halo ring, transfer[0] = G_S - G_L, transfer[1+l] = M_c,l and
@@ -199,6 +211,28 @@ int ppf_ncomp_thermo(const ppf_grid* g, const void* in, const void* bcvals, cons
 int ppf_ncomp_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
                    const ppf_temperature* t, const ppf_grid* tg, const void* transfer,
                    const ppf_ncomp_params* p, uint64_t seed, int64_t step, ppf_stream_t s);
+
+/* ---- device-resident step counter (HIP graph replay) ----
+   The reference's loop increments a host counter before each step
+   (simulation.py:262-264) and hands it to kernels that key their noise and
+   temperature ramp on it.  The _dev variants below take the step from device
+   memory instead: step = *step_base + step_offset (step_base: a device
+   int64), so a chunk of k steps can be captured once (offsets 1..k) followed
+   by ppf_step_counter_add(step_base, k) and replayed; each replay then takes
+   the next k steps.  Results equal the host-counter entry points'. */
+int ppf_step_counter_add(int64_t* counter, int64_t inc, ppf_stream_t s);
+int ppf_warren_step_dev(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                        const ppf_warren_params* p, uint64_t seed, const int64_t* step_base, int64_t step_offset,
+                        ppf_stream_t s);
+int ppf_ncomp_thermo_dev(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                         const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                         uint64_t seed, const int64_t* step_base, int64_t step_offset, ppf_stream_t s);
+int ppf_ncomp_step_dev(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_temperature* t,
+                       const ppf_grid* tg, const void* transfer, const ppf_ncomp_params* p, uint64_t seed,
+                       const int64_t* step_base, int64_t step_offset, ppf_stream_t s);
+int ppf_ncomp_fused_step_dev(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                             const ppf_temperature* t, const ppf_ncomp_params* p, const ppf_thermo* th,
+                             uint64_t seed, const int64_t* step_base, int64_t step_offset, ppf_stream_t s);
 
 /* Global FP64 reductions over the interior (conservation checks):
    kind 0: out[f] = sum of field f (mass), for every field of the stack;
@@ -238,6 +272,10 @@ int ppf_aniso_step(const ppf_grid* g, const void* in, void* out, const void* bcv
    stack; identical arithmetic). */
 int ppf_aniso_fused_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
                          const ppf_aniso_params* p, uint64_t seed, int64_t step, ppf_stream_t s);
+/* the same with the step counter in device memory (see ppf_step_counter_add) */
+int ppf_aniso_fused_step_dev(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                             const ppf_aniso_params* p, uint64_t seed, const int64_t* step_base,
+                             int64_t step_offset, ppf_stream_t s);
 
 /* ---- implicit and Crank-Nicolson diffusion (Engines/Diffusion.py:18-633) ----
    Every matrix the reference builds (diffusion_matrix_1d/2d/3d,
@@ -393,6 +431,11 @@ typedef struct ppf_mlp_net {
 int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
                          const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
                          const ppf_mlp_net* net, uint64_t seed, int64_t step, ppf_stream_t s);
+/* the same with the step counter in device memory (see ppf_step_counter_add) */
+int ppf_ncomp_thermo_mlp_dev(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                             const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                             const ppf_mlp_net* net, uint64_t seed, const int64_t* step_base, int64_t step_offset,
+                             ppf_stream_t s);
 
 /* ---- Batched ensemble (no reference counterpart: the reference runs one
    Simulation per instance, each stepping with engine_CahnAllenExplicit,
@@ -422,7 +465,8 @@ int ppf_ensemble_config(int config, int32_t* shape);
    first output cell (1 + multiples of the configuration's tile shape for the
    instance's kind); segments: host array of PPF_ENS_XCDS + 1 tile indices,
    0 = s[0] <= s[1] <= ... <= s[8] = n_tiles (tiles of XCD k are
-   [s[k], s[k+1])); blocks: persistent grid size.  Halo cells are never read
+   [s[k], s[k+1])); blocks: persistent grid size, >= 1 (with fewer than
+   PPF_ENS_XCDS blocks, block b walks segments b, b + blocks, ...).  Halo cells are never read
    (boundary rules folded into the loads); in and out must not alias. */
 int ppf_ensemble_step(const ppf_ens_instance* instances, int32_t n_instances, const int32_t* tiles,
                       int32_t n_tiles, const int32_t* segments, int32_t config, int32_t blocks,

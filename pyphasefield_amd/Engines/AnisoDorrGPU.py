@@ -105,6 +105,18 @@ class AnisoDorrGPU(Simulation):
         self._fields_transfer_gpu_device = _lib.alloc_stack(6, list(self.dimensions), self._torch_dtype(), dev)
         self._tgrid = _lib.make_grid(self._fields_transfer_gpu_device, 2, self._boundary_conditions_type, self.dx)
 
+    # the fused step replays from captured HIP graphs (step counter on the device)
+    _GRAPH_CHUNK = 20
+    _DEVICE_STEP = True
+
+    def _graph_key(self):
+        return (bytes(self._params), int(self.seed))
+
+    def _graph_chunk(self):
+        if self.user_data.get("aniso_two_pass", False):
+            return 0
+        return super()._graph_chunk()
+
     def simulation_loop(self):
         L = _lib.lib()
         s = _lib.stream_handle()
@@ -112,9 +124,14 @@ class AnisoDorrGPU(Simulation):
         if not self.user_data.get("aniso_two_pass", False):
             # one tiled launch; the two-pass path below keeps the reference's
             # helper/kernel split (same arithmetic)
-            _lib.check(L.ppf_aniso_fused_step(self._grid, _lib.ptr(self._fields_gpu_device),
-                                              _lib.ptr(self._fields_out_gpu_device), bc, self._params, self.seed,
-                                              int(self.time_step_counter), s), "aniso_fused_step")
+            args = (self._grid, _lib.ptr(self._fields_gpu_device), _lib.ptr(self._fields_out_gpu_device), bc,
+                    self._params, self.seed)
+            src = self._step_source()
+            if src[0] == "host":
+                rc = L.ppf_aniso_fused_step(*args, src[1], s)
+            else:
+                rc = L.ppf_aniso_fused_step_dev(*args, src[1], src[2], s)
+            _lib.check(rc, "aniso_fused_step")
             self._swap_buffers()
             return
         _lib.check(L.ppf_aniso_helper(self._grid, _lib.ptr(self._fields_gpu_device), bc, self._tgrid,

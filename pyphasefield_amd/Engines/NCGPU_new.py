@@ -173,36 +173,81 @@ class NCGPU_new(Simulation):
             self.user_data["surrogate"] = sur
         self._surrogate = sur
 
+    # steps per captured HIP graph; noise and temperature ramp read the step
+    # counter from the device inside a capture
+    _GRAPH_CHUNK = 20
+    _DEVICE_STEP = True
+
+    def _kernel_mode(self):
+        """"fused": one launch per step (ppf_ncomp_fused_step, the default in
+        FP32); "two_pass": the reference's helper + kernel structure
+        (ppf_ncomp_thermo + ppf_ncomp_step, the default in FP64, where the
+        forward-difference thermodynamics costs more than the transfer stack
+        saves); the surrogate always runs as its own pass."""
+        if getattr(self, "_surrogate", None) is not None:
+            return "surrogate"
+        mode = self.user_data.get("ncomp_kernel", "fused" if self._precision == "FP32" else "two_pass")
+        if mode not in ("fused", "two_pass"):
+            raise ValueError("ncomp_kernel must be 'fused' or 'two_pass'")
+        return mode
+
+    def _graph_key(self):
+        sur = getattr(self, "_surrogate", None)
+        net = bytes(sur.packed()[0]) if sur is not None else b""
+        return (self._kernel_mode(), bytes(self._ncparams), bytes(self._thermo), bytes(self.temperature_descriptor()),
+                int(self.seed), net, self._fields_transfer_gpu_device.data_ptr())
+
     def kernel_names(self):
         t = "float" if self._precision == "FP32" else "double"
-        if getattr(self, "_surrogate", None) is not None:
-            return ["k_ncomp_thermo_mlp", "k_ncomp_step<float,3>"]
-        return ["k_ncomp_thermo<%s,%d>" % (t, len(self._tdb_components)),
-                "k_ncomp_step<%s,%d>" % (t, len(self._tdb_components))]
+        n = len(self._tdb_components)
+        mode = self._kernel_mode()
+        if mode == "surrogate":
+            return ["k_ncomp_thermo_mlp", "k_ncomp_step<float,3,false>"]
+        if mode == "fused":
+            return ["k_ncomp_step<%s,%d,true>" % (t, n)]
+        return ["k_ncomp_thermo<%s,%d>" % (t, n), "k_ncomp_step<%s,%d,false>" % (t, n)]
 
     def thermo_pass(self):
         L = _lib.lib()
+        src = self._step_source()
+        head = (self._grid, _lib.ptr(self._fields_gpu_device), _lib.ptr(self._boundary_conditions_gpu_device),
+                self.temperature_descriptor(), self._tgrid, _lib.ptr(self._fields_transfer_gpu_device),
+                self._ncparams, self._thermo)
         if self._surrogate is not None:
             net, _keep = self._surrogate.packed()
-            rc = L.ppf_ncomp_thermo_mlp(self._grid, _lib.ptr(self._fields_gpu_device),
-                                        _lib.ptr(self._boundary_conditions_gpu_device),
-                                        self.temperature_descriptor(), self._tgrid,
-                                        _lib.ptr(self._fields_transfer_gpu_device), self._ncparams, self._thermo,
-                                        net, self.seed, int(self.time_step_counter), _lib.stream_handle())
+            if src[0] == "host":
+                rc = L.ppf_ncomp_thermo_mlp(*head, net, self.seed, src[1], _lib.stream_handle())
+            else:
+                rc = L.ppf_ncomp_thermo_mlp_dev(*head, net, self.seed, src[1], src[2], _lib.stream_handle())
             _lib.check(rc, "ncomp_thermo_mlp")
             return
-        rc = L.ppf_ncomp_thermo(self._grid, _lib.ptr(self._fields_gpu_device),
-                                _lib.ptr(self._boundary_conditions_gpu_device), self.temperature_descriptor(),
-                                self._tgrid, _lib.ptr(self._fields_transfer_gpu_device), self._ncparams,
-                                self._thermo, self.seed, int(self.time_step_counter), _lib.stream_handle())
+        if src[0] == "host":
+            rc = L.ppf_ncomp_thermo(*head, self.seed, src[1], _lib.stream_handle())
+        else:
+            rc = L.ppf_ncomp_thermo_dev(*head, self.seed, src[1], src[2], _lib.stream_handle())
         _lib.check(rc, "ncomp_thermo")
 
     def simulation_loop(self):
-        self.thermo_pass()
         L = _lib.lib()
-        rc = L.ppf_ncomp_step(self._grid, _lib.ptr(self._fields_gpu_device), _lib.ptr(self._fields_out_gpu_device),
-                              _lib.ptr(self._boundary_conditions_gpu_device), self.temperature_descriptor(),
-                              self._tgrid, _lib.ptr(self._fields_transfer_gpu_device), self._ncparams,
-                              self.seed, int(self.time_step_counter), _lib.stream_handle())
+        src = self._step_source()
+        if self._kernel_mode() == "fused":
+            args = (self._grid, _lib.ptr(self._fields_gpu_device), _lib.ptr(self._fields_out_gpu_device),
+                    _lib.ptr(self._boundary_conditions_gpu_device), self.temperature_descriptor(), self._ncparams,
+                    self._thermo, self.seed)
+            if src[0] == "host":
+                rc = L.ppf_ncomp_fused_step(*args, src[1], _lib.stream_handle())
+            else:
+                rc = L.ppf_ncomp_fused_step_dev(*args, src[1], src[2], _lib.stream_handle())
+            _lib.check(rc, "ncomp_fused_step")
+            self._swap_buffers()
+            return
+        self.thermo_pass()
+        args = (self._grid, _lib.ptr(self._fields_gpu_device), _lib.ptr(self._fields_out_gpu_device),
+                _lib.ptr(self._boundary_conditions_gpu_device), self.temperature_descriptor(), self._tgrid,
+                _lib.ptr(self._fields_transfer_gpu_device), self._ncparams, self.seed)
+        if src[0] == "host":
+            rc = L.ppf_ncomp_step(*args, src[1], _lib.stream_handle())
+        else:
+            rc = L.ppf_ncomp_step_dev(*args, src[1], src[2], _lib.stream_handle())
         _lib.check(rc, "ncomp_step")
         self._swap_buffers()

@@ -37,6 +37,9 @@ def warren_derived(ud, dx):
 
 class Warren1995(Simulation):
     _DEFAULT_USER_DATA = _DEFAULTS
+    # steps per captured HIP graph; the noise key comes from a device counter
+    _GRAPH_CHUNK = 20
+    _DEVICE_STEP = True
 
     def init_fields(self):
         dim = self.dimensions
@@ -79,11 +82,18 @@ class Warren1995(Simulation):
         p.dt = float(self.dt)
         return p
 
+    def _graph_key(self):
+        return (bytes(self._params), int(self.seed))
+
     def simulation_loop(self):
         L = _lib.lib()
-        rc = L.ppf_warren_step(self._grid, _lib.ptr(self._fields_gpu_device), _lib.ptr(self._fields_out_gpu_device),
-                               _lib.ptr(self._boundary_conditions_gpu_device), self._params, self.seed,
-                               int(self.time_step_counter), _lib.stream_handle())
+        args = (self._grid, _lib.ptr(self._fields_gpu_device), _lib.ptr(self._fields_out_gpu_device),
+                _lib.ptr(self._boundary_conditions_gpu_device), self._params, self.seed)
+        src = self._step_source()
+        if src[0] == "host":
+            rc = L.ppf_warren_step(*args, src[1], _lib.stream_handle())
+        else:
+            rc = L.ppf_warren_step_dev(*args, src[1], src[2], _lib.stream_handle())
         _lib.check(rc, "warren_step")
         _gpu.carry_passive(self, 2)
         self._swap_buffers()

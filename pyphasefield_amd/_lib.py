@@ -107,6 +107,17 @@ def lib():
                               c_u64, c_i64, c_vp], c_i32),
         "ppf_ncomp_step": ([P(Grid), c_vp, c_vp, c_vp, P(Temperature), P(Grid), c_vp, P(NcompParams), c_u64,
                             c_i64, c_vp], c_i32),
+        "ppf_warren_step_dev": ([P(Grid), c_vp, c_vp, c_vp, P(WarrenParams), c_u64, c_vp, c_i64, c_vp], c_i32),
+        "ppf_ncomp_thermo_dev": ([P(Grid), c_vp, c_vp, P(Temperature), P(Grid), c_vp, P(NcompParams), P(Thermo),
+                                  c_u64, c_vp, c_i64, c_vp], c_i32),
+        "ppf_ncomp_step_dev": ([P(Grid), c_vp, c_vp, c_vp, P(Temperature), P(Grid), c_vp, P(NcompParams), c_u64,
+                                c_vp, c_i64, c_vp], c_i32),
+        "ppf_ncomp_fused_step": ([P(Grid), c_vp, c_vp, c_vp, P(Temperature), P(NcompParams), P(Thermo), c_u64,
+                                  c_i64, c_vp], c_i32),
+        "ppf_ncomp_fused_step_dev": ([P(Grid), c_vp, c_vp, c_vp, P(Temperature), P(NcompParams), P(Thermo), c_u64,
+                                      c_vp, c_i64, c_vp], c_i32),
+        "ppf_aniso_fused_step_dev": ([P(Grid), c_vp, c_vp, c_vp, P(AnisoParams), c_u64, c_vp, c_i64, c_vp], c_i32),
+        "ppf_step_counter_add": ([c_vp, c_i64, c_vp], c_i32),
         "ppf_reduce_work_bytes": ([P(Grid)], c_i64),
         "ppf_reduce": ([P(Grid), c_vp, c_vp, ctypes.c_int, c_f64, c_vp, c_vp, c_vp], c_i32),
         "ppf_thermo_eval": ([P(Thermo), c_i64, c_vp, c_vp, c_vp, c_vp], c_i32),
@@ -134,7 +145,7 @@ def lib():
     optional = {}
     try:
         from . import _lib_ext  # noqa: F401  (additional signatures: surrogate, ensemble)
-        optional = _lib_ext.SIGNATURES
+        optional = dict(_lib_ext.SIGNATURES)
     except ImportError:
         pass
     for name, (args, res) in list(sigs.items()) + list(optional.items()):

@@ -29,6 +29,8 @@ SIGNATURES = {
     "ppf_mlp_adam": ([c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_f32, c_i64, c_vp], c_i32),
     "ppf_ncomp_thermo_mlp": ([P(L.Grid), c_vp, c_vp, P(L.Temperature), P(L.Grid), c_vp, P(L.NcompParams),
                               P(L.Thermo), P(MlpNet), c_u64, c_i64, c_vp], c_i32),
+    "ppf_ncomp_thermo_mlp_dev": ([P(L.Grid), c_vp, c_vp, P(L.Temperature), P(L.Grid), c_vp, P(L.NcompParams),
+                                  P(L.Thermo), P(MlpNet), c_u64, c_vp, c_i64, c_vp], c_i32),
     "ppf_ensemble_num_configs": ([], ctypes.c_int),
     "ppf_ensemble_config": ([ctypes.c_int, P(c_i32)], ctypes.c_int),
     "ppf_ensemble_step": ([c_vp, c_i32, c_vp, c_i32, P(c_i32), c_i32, c_i32, c_vp, c_vp, c_vp], c_i32),

@@ -61,6 +61,10 @@ def build(verbose=False):
     if missing:
         raise FileNotFoundError("missing HIP sources: %s" % ", ".join(missing))
     units = dict(UNITS)
+    # any further translation unit in csrc/ builds with the common flags
+    for f in sorted(os.listdir(CSRC)):
+        if f.endswith(".hip") and f not in units:
+            units[f] = []
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(units))) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), units.items()))
     tmp = LIB + ".tmp"

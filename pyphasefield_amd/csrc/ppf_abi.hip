@@ -62,11 +62,23 @@ int check_temperature(const ppf_temperature* t, const char* what) {
   return 0;
 }
 
+// Advance a device-resident step counter by `inc` (one lane; the sum goes
+// out through an ordinary vector store).
+__global__ void k_step_advance(int64_t* __restrict__ counter, int64_t inc) {
+  if (threadIdx.x == 0) counter[threadIdx.x] += inc;
+}
+
 }  // namespace ppf
 
 using namespace ppf;
 
 extern "C" {
+
+int ppf_step_counter_add(int64_t* counter, int64_t inc, ppf_stream_t s) {
+  if (!counter) return fail(PPF_EINVAL, "step_counter_add: null counter");
+  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, stream_of(s), counter, inc);
+  return check_launch("step_counter_add");
+}
 
 int ppf_abi_version(void) { return PPF_ABI_VERSION; }
 
@@ -118,13 +130,25 @@ int ppf_cahnhilliard_step(const ppf_grid* g, const void* in, void* out, const vo
                              : cahnhilliard_step<double>(*g, in, out, bcvals, M, epsilon, dt, stream_of(s));
 }
 
-int ppf_warren_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_warren_params* p,
-                    uint64_t seed, int64_t step, ppf_stream_t s) {
+static int warren_dispatch(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                           const ppf_warren_params* p, uint64_t seed, StepRef step, ppf_stream_t s) {
   if (int rc = check_grid(g, 2, 2, "warren_step")) return rc;
   if (g->nfields < 2) return fail(PPF_EINVAL, "warren_step: needs fields (phi, c)");
   if (!in || !out || in == out || !p) return fail(PPF_EINVAL, "warren_step: need distinct in/out and params");
   return g->dtype == PPF_F32 ? warren_step<float>(*g, in, out, bcvals, *p, seed, step, stream_of(s))
                              : warren_step<double>(*g, in, out, bcvals, *p, seed, step, stream_of(s));
+}
+
+int ppf_warren_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_warren_params* p,
+                    uint64_t seed, int64_t step, ppf_stream_t s) {
+  return warren_dispatch(g, in, out, bcvals, p, seed, step_host(step), s);
+}
+
+int ppf_warren_step_dev(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                        const ppf_warren_params* p, uint64_t seed, const int64_t* step_base, int64_t step_offset,
+                        ppf_stream_t s) {
+  if (!step_base) return fail(PPF_EINVAL, "warren_step_dev: null step counter");
+  return warren_dispatch(g, in, out, bcvals, p, seed, StepRef{step_base, step_offset}, s);
 }
 
 static int check_ncomp(const ppf_grid* g, const ppf_grid* tg, const ppf_ncomp_params* p, const char* what) {
@@ -138,9 +162,9 @@ static int check_ncomp(const ppf_grid* g, const ppf_grid* tg, const ppf_ncomp_pa
   return 0;
 }
 
-int ppf_ncomp_thermo(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
-                     const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
-                     uint64_t seed, int64_t step, ppf_stream_t s) {
+static int ncomp_thermo_dispatch(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                                 const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                                 uint64_t seed, StepRef step, ppf_stream_t s) {
   if (int rc = check_ncomp(g, tg, p, "ncomp_thermo")) return rc;
   if (!in || !transfer || !t || !th) return fail(PPF_EINVAL, "ncomp_thermo: null argument");
   if (int rc = check_temperature(t, "ncomp_thermo")) return rc;
@@ -153,15 +177,71 @@ int ppf_ncomp_thermo(const ppf_grid* g, const void* in, const void* bcvals, cons
              : ncomp_thermo<double>(*g, in, bcvals, *t, *tg, transfer, *p, *th, seed, step, stream_of(s));
 }
 
-int ppf_ncomp_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_temperature* t,
-                   const ppf_grid* tg, const void* transfer, const ppf_ncomp_params* p, uint64_t seed,
-                   int64_t step, ppf_stream_t s) {
+int ppf_ncomp_thermo(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                     const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                     uint64_t seed, int64_t step, ppf_stream_t s) {
+  return ncomp_thermo_dispatch(g, in, bcvals, t, tg, transfer, p, th, seed, step_host(step), s);
+}
+
+int ppf_ncomp_thermo_dev(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                         const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                         uint64_t seed, const int64_t* step_base, int64_t step_offset, ppf_stream_t s) {
+  if (!step_base) return fail(PPF_EINVAL, "ncomp_thermo_dev: null step counter");
+  return ncomp_thermo_dispatch(g, in, bcvals, t, tg, transfer, p, th, seed, StepRef{step_base, step_offset}, s);
+}
+
+static int ncomp_step_dispatch(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                               const ppf_temperature* t, const ppf_grid* tg, const void* transfer,
+                               const ppf_ncomp_params* p, uint64_t seed, StepRef step, ppf_stream_t s) {
   if (int rc = check_ncomp(g, tg, p, "ncomp_step")) return rc;
   if (!in || !out || in == out || !transfer || !t) return fail(PPF_EINVAL, "ncomp_step: null argument");
   if (int rc = check_temperature(t, "ncomp_step")) return rc;
   return g->dtype == PPF_F32
              ? ncomp_step<float>(*g, in, out, bcvals, *t, *tg, transfer, *p, seed, step, stream_of(s))
              : ncomp_step<double>(*g, in, out, bcvals, *t, *tg, transfer, *p, seed, step, stream_of(s));
+}
+
+int ppf_ncomp_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_temperature* t,
+                   const ppf_grid* tg, const void* transfer, const ppf_ncomp_params* p, uint64_t seed,
+                   int64_t step, ppf_stream_t s) {
+  return ncomp_step_dispatch(g, in, out, bcvals, t, tg, transfer, p, seed, step_host(step), s);
+}
+
+int ppf_ncomp_step_dev(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_temperature* t,
+                       const ppf_grid* tg, const void* transfer, const ppf_ncomp_params* p, uint64_t seed,
+                       const int64_t* step_base, int64_t step_offset, ppf_stream_t s) {
+  if (!step_base) return fail(PPF_EINVAL, "ncomp_step_dev: null step counter");
+  return ncomp_step_dispatch(g, in, out, bcvals, t, tg, transfer, p, seed, StepRef{step_base, step_offset}, s);
+}
+
+static int ncomp_fused_dispatch(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                                const ppf_temperature* t, const ppf_ncomp_params* p, const ppf_thermo* th,
+                                uint64_t seed, StepRef step, ppf_stream_t s) {
+  if (int rc = check_grid(g, 2, 2, "ncomp_fused_step")) return rc;
+  if (!p || !th || !t) return fail(PPF_EINVAL, "ncomp_fused_step: null argument");
+  if (p->ncomp < 2 || p->ncomp > PPF_MAX_COMP) return fail(PPF_EINVAL, "ncomp_fused_step: ncomp out of range");
+  if (g->nfields != p->ncomp + 2) return fail(PPF_EINVAL, "ncomp_fused_step: field stack must hold N+2 fields");
+  if (!in || !out || in == out) return fail(PPF_EINVAL, "ncomp_fused_step: need distinct in/out buffers");
+  if (int rc = check_temperature(t, "ncomp_fused_step")) return rc;
+  if (th->liquid.ncomp != p->ncomp || th->solid.ncomp != p->ncomp)
+    return fail(PPF_EINVAL, "ncomp_fused_step: thermodynamic model has a different component count");
+  if (g->dtype == PPF_F32 && th->dg_mode == PPF_DG_FORWARD_DIFF)
+    return fail(PPF_EINVAL, "ncomp_fused_step: the h=1e-7 forward difference needs FP64; use PPF_DG_ANALYTIC");
+  return g->dtype == PPF_F32 ? ncomp_fused_step<float>(*g, in, out, bcvals, *t, *p, *th, seed, step, stream_of(s))
+                             : ncomp_fused_step<double>(*g, in, out, bcvals, *t, *p, *th, seed, step, stream_of(s));
+}
+
+int ppf_ncomp_fused_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, const ppf_temperature* t,
+                         const ppf_ncomp_params* p, const ppf_thermo* th, uint64_t seed, int64_t step,
+                         ppf_stream_t s) {
+  return ncomp_fused_dispatch(g, in, out, bcvals, t, p, th, seed, step_host(step), s);
+}
+
+int ppf_ncomp_fused_step_dev(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                             const ppf_temperature* t, const ppf_ncomp_params* p, const ppf_thermo* th,
+                             uint64_t seed, const int64_t* step_base, int64_t step_offset, ppf_stream_t s) {
+  if (!step_base) return fail(PPF_EINVAL, "ncomp_fused_step_dev: null step counter");
+  return ncomp_fused_dispatch(g, in, out, bcvals, t, p, th, seed, StepRef{step_base, step_offset}, s);
 }
 
 static int check_aniso(const ppf_grid* g, const ppf_grid* tg, const ppf_aniso_params* p, const char* what) {
@@ -193,8 +273,8 @@ int ppf_aniso_step(const ppf_grid* g, const void* in, void* out, const void* bcv
              : aniso_step<double>(*g, in, out, bcvals, *tg, transfer, *p, seed, step, stream_of(s));
 }
 
-int ppf_aniso_fused_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
-                         const ppf_aniso_params* p, uint64_t seed, int64_t step, ppf_stream_t s) {
+static int aniso_fused_dispatch(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                                const ppf_aniso_params* p, uint64_t seed, StepRef step, ppf_stream_t s) {
   if (int rc = check_grid(g, 2, 2, "aniso_fused_step")) return rc;
   if (!p) return fail(PPF_EINVAL, "aniso_fused_step: null params");
   if (g->nfields != 4) return fail(PPF_EINVAL, "aniso_fused_step: field stack must hold phi, q1, q4, c");
@@ -203,6 +283,18 @@ int ppf_aniso_fused_step(const ppf_grid* g, const void* in, void* out, const voi
   if (!in || !out || in == out) return fail(PPF_EINVAL, "aniso_fused_step: null or aliased argument");
   return g->dtype == PPF_F32 ? aniso_fused_step<float>(*g, in, out, bcvals, *p, seed, step, stream_of(s))
                              : aniso_fused_step<double>(*g, in, out, bcvals, *p, seed, step, stream_of(s));
+}
+
+int ppf_aniso_fused_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                         const ppf_aniso_params* p, uint64_t seed, int64_t step, ppf_stream_t s) {
+  return aniso_fused_dispatch(g, in, out, bcvals, p, seed, step_host(step), s);
+}
+
+int ppf_aniso_fused_step_dev(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                             const ppf_aniso_params* p, uint64_t seed, const int64_t* step_base,
+                             int64_t step_offset, ppf_stream_t s) {
+  if (!step_base) return fail(PPF_EINVAL, "aniso_fused_step_dev: null step counter");
+  return aniso_fused_dispatch(g, in, out, bcvals, p, seed, StepRef{step_base, step_offset}, s);
 }
 
 int64_t ppf_reduce_work_bytes(const ppf_grid* g) {

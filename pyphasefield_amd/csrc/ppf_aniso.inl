@@ -281,11 +281,14 @@ __global__ __launch_bounds__(256) void k_aniso_step(const R* __restrict__ in, R*
 #define PPF_AD_TY 8
 #endif
 constexpr int kAdTX = 64, kAdTY = PPF_AD_TY;
+// 256 threads cover 64 columns x 4 row groups; each thread walks kAdTY / 4 rows
+static_assert(kAdTY % 4 == 0 && kAdTY >= 4, "PPF_AD_TY must be a positive multiple of 4");
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_aniso_fused(const R* __restrict__ in, R* __restrict__ out,
                                                      const R* __restrict__ bv, Dom d, R dx, AnisoK k, uint64_t seed,
-                                                     int64_t step) {
+                                                     StepRef sr) {
+  const int64_t step = sr.get();
   constexpr int CW = kAdTX + 2, CH = kAdTY + 2, VW = kAdTX + 1, VH = kAdTY + 1;
   __shared__ R sf[4][CH][CW];
   __shared__ R sp[2][CH][CW];  // temp, D_C
@@ -384,7 +387,7 @@ int aniso_helper(const ppf_grid& g, const void* in, const void* bcvals, const pp
 
 template <typename R>
 int aniso_fused_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_aniso_params& p,
-                     uint64_t seed, int64_t step, hipStream_t s) {
+                     uint64_t seed, StepRef step, hipStream_t s) {
   Dom d = make_dom(g);
   dim3 grid(unsigned(((g.n[0] + kAdTX - 1) / kAdTX) * ((g.n[1] + kAdTY - 1) / kAdTY)));
   hipLaunchKernelGGL(k_aniso_fused<R>, grid, dim3(256), 0, s, static_cast<const R*>(in), static_cast<R*>(out),

@@ -35,29 +35,33 @@ __global__ __launch_bounds__(kThreads) void k_ensemble(const ppf_ens_instance* _
                                                        const int4* __restrict__ tiles, EnsSeg seg,
                                                        const float* __restrict__ in, float* __restrict__ out) {
   __shared__ float smem[EnsShape<ATX, ATY, CTX, CTY>::kLds];
-  const int xcd = int(blockIdx.x % kEnsXcds);
-  const int nb = int((gridDim.x - 1 - xcd) / kEnsXcds) + 1;  // blocks dispatched to this XCD
+  // With 8 or more blocks, block b walks segment b % 8 only.  A smaller grid
+  // has segments without a block of their own: block b then also walks
+  // segments b + G, b + 2G, ... (G = grid size), alone on each.
   const int j = int(blockIdx.x / kEnsXcds);
-  for (int t = seg.start[xcd] + j; t < seg.start[xcd + 1]; t += nb) {
-    const int4 td = tiles[t];
-    if (td.x < 0 || td.x >= ninst) continue;  // uniform per block
-    const ppf_ens_instance I = inst[td.x];
-    Dom d;
-    d.ndim = 2;
-    d.ax = Axis{I.n[0], 1, I.bc[0]};
-    d.ay = Axis{I.n[1], I.pitch, I.bc[1]};
-    d.az = Axis{1, 0, PPF_PERIODIC};
-    d.sf = 0;
-    const float* fin = in + I.offset;
-    float* fout = out + I.offset;
-    if (I.kind == PPF_ENS_AC) {
-      const PointParams p{I.M, I.W, I.eps, I.dt};
-      point_tile<float, 1, ATX, ATY>(smem, fin, fout, nullptr, d, float(I.dx), p, td.y, td.z);
-    } else {
-      ch_tile<float, CTX, CTY>(smem, fin, fout, nullptr, d, float(I.dx), float(I.eps), float(I.dt * I.M), td.y,
-                               td.z);
+  for (int xcd = int(blockIdx.x % kEnsXcds); xcd < kEnsXcds; xcd += int(gridDim.x)) {
+    const int nb = gridDim.x >= unsigned(kEnsXcds) ? int((gridDim.x - 1 - xcd) / kEnsXcds) + 1 : 1;
+    for (int t = seg.start[xcd] + j; t < seg.start[xcd + 1]; t += nb) {
+      const int4 td = tiles[t];
+      if (td.x < 0 || td.x >= ninst) continue;  // uniform per block
+      const ppf_ens_instance I = inst[td.x];
+      Dom d;
+      d.ndim = 2;
+      d.ax = Axis{I.n[0], 1, I.bc[0]};
+      d.ay = Axis{I.n[1], I.pitch, I.bc[1]};
+      d.az = Axis{1, 0, PPF_PERIODIC};
+      d.sf = 0;
+      const float* fin = in + I.offset;
+      float* fout = out + I.offset;
+      if (I.kind == PPF_ENS_AC) {
+        const PointParams p{I.M, I.W, I.eps, I.dt};
+        point_tile<float, 1, ATX, ATY>(smem, fin, fout, nullptr, d, float(I.dx), p, td.y, td.z);
+      } else {
+        ch_tile<float, CTX, CTY>(smem, fin, fout, nullptr, d, float(I.dx), float(I.eps), float(I.dt * I.M), td.y,
+                                 td.z);
+      }
+      __syncthreads();  // the next tile overwrites the LDS
     }
-    __syncthreads();  // the next tile overwrites the LDS
   }
 }
 

@@ -157,6 +157,17 @@ __device__ __forceinline__ R temperature_at(const ppf_temperature& t, const int6
   return R(T);
 }
 
+// ------------------------------------------------------------ step counter --
+// Step number seen by a step kernel: `off` alone, or *base + off when `base`
+// (a device int64) is set -- the form a captured HIP graph replays, with the
+// counter advanced on the device between replays (k_step_advance).
+struct StepRef {
+  const int64_t* base;
+  int64_t off;
+  __device__ __forceinline__ int64_t get() const { return base ? *base + off : off; }
+};
+inline StepRef step_host(int64_t step) { return StepRef{nullptr, step}; }
+
 // ------------------------------------------------------------ counter RNG --
 // Stateless hash of (seed, step, stream, cell); see the oracle's rng module
 // for the definition shared with the CPU side.

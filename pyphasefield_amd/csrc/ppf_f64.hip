@@ -15,20 +15,20 @@ template int allencahn_step<double>(const ppf_grid&, const void*, void*, const v
 template int cahnhilliard_step<double>(const ppf_grid&, const void*, void*, const void*, double, double, double,
                                       hipStream_t);
 template int reduce<double>(const ppf_grid&, const void*, const void*, int, double, double*, void*, hipStream_t);
-template int warren_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_warren_params&,
-                                uint64_t, int64_t, hipStream_t);
-template int ncomp_thermo<double>(const ppf_grid&, const void*, const void*, const ppf_temperature&,
-                                 const ppf_grid&, void*, const ppf_ncomp_params&, const ppf_thermo&, uint64_t,
-                                 int64_t, hipStream_t);
+template int warren_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_warren_params&, uint64_t,
+                            StepRef, hipStream_t);
+template int ncomp_thermo<double>(const ppf_grid&, const void*, const void*, const ppf_temperature&, const ppf_grid&,
+                             void*, const ppf_ncomp_params&, const ppf_thermo&, uint64_t, StepRef, hipStream_t);
 template int aniso_helper<double>(const ppf_grid&, const void*, const void*, const ppf_grid&, void*,
-                                 const ppf_aniso_params&, hipStream_t);
+                             const ppf_aniso_params&, hipStream_t);
 template int aniso_fused_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_aniso_params&,
-                                     uint64_t, int64_t, hipStream_t);
+                                 uint64_t, StepRef, hipStream_t);
 template int aniso_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_grid&, const void*,
-                               const ppf_aniso_params&, uint64_t, int64_t, hipStream_t);
+                           const ppf_aniso_params&, uint64_t, int64_t, hipStream_t);
 template int ncomp_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_temperature&,
-                               const ppf_grid&, const void*, const ppf_ncomp_params&, uint64_t, int64_t,
-                               hipStream_t);
+                           const ppf_grid&, const void*, const ppf_ncomp_params&, uint64_t, StepRef, hipStream_t);
+template int ncomp_fused_step<double>(const ppf_grid&, const void*, void*, const void*, const ppf_temperature&,
+                                 const ppf_ncomp_params&, const ppf_thermo&, uint64_t, StepRef, hipStream_t);
 // plain thermodynamics at scattered points (surrogate training data), FP64
 template <int N>
 __global__ __launch_bounds__(256) void k_thermo_eval(ppf_thermo th, int64_t n, const double* __restrict__ T,

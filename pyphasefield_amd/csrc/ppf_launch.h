@@ -26,29 +26,33 @@ int reduce(const ppf_grid& g, const void* in, const void* bcvals, int kind, doub
            void* work, hipStream_t s);
 template <typename R>
 int warren_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_warren_params& p,
-                uint64_t seed, int64_t step, hipStream_t s);
+                uint64_t seed, StepRef step, hipStream_t s);
 template <typename R>
 int ncomp_thermo(const ppf_grid& g, const void* in, const void* bcvals, const ppf_temperature& t, const ppf_grid& tg,
-                 void* transfer, const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, int64_t step,
+                 void* transfer, const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, StepRef step,
                  hipStream_t s);
 template <typename R>
 int ncomp_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
                const ppf_grid& tg, const void* transfer, const ppf_ncomp_params& p, uint64_t seed,
-               int64_t step, hipStream_t s);
+               StepRef step, hipStream_t s);
+template <typename R>
+int ncomp_fused_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
+                     const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, StepRef step, hipStream_t s);
 
 template <typename R>
 int aniso_helper(const ppf_grid& g, const void* in, const void* bcvals, const ppf_grid& tg, void* transfer,
                  const ppf_aniso_params& p, hipStream_t s);
 template <typename R>
 int aniso_fused_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_aniso_params& p,
-                     uint64_t seed, int64_t step, hipStream_t s);
+                     uint64_t seed, StepRef step, hipStream_t s);
 template <typename R>
 int aniso_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_grid& tg,
                const void* transfer, const ppf_aniso_params& p, uint64_t seed, int64_t step, hipStream_t s);
 
 int thermo_eval(const ppf_thermo& th, int64_t n, const double* T, const double* c, double* out, hipStream_t s);
 
-// number of partial sums the reduction kernels write (per field)
-constexpr int kReduceBlocks = 1024;
+// number of partial sums the reduction kernels write (per field): 8 blocks of
+// 256 threads per CU on a 256-CU device
+constexpr int kReduceBlocks = 2048;
 
 }  // namespace ppf

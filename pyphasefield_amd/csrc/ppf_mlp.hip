@@ -379,8 +379,9 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(const f
                                                              const float* __restrict__ b, Dom d, float dx,
                                                              ppf_temperature t, float* __restrict__ tr, int64_t tsy,
                                                              int64_t tsf, NcMlpK k, MlpK net, uint64_t seed,
-                                                             int64_t step, uint32_t ntiles) {
+                                                             StepRef sr, uint32_t ntiles) {
   constexpr int N = 3;
+  const int64_t step = sr.get();
   __shared__ bf16x8 sw[2][kMlpWords];
   __shared__ float sb[2][132];
   for (int i = threadIdx.x; i < 2 * kMlpWords; i += 256) sw[i / kMlpWords][i % kMlpWords] = net.w[i / kMlpWords][i % kMlpWords];
@@ -477,7 +478,8 @@ __global__ __launch_bounds__(256) void k_ncomp_thermo_fallback(const float* __re
                                                                ppf_temperature t, float* __restrict__ tr,
                                                                int64_t tsy, int64_t tsf, NcMlpK k, ppf_thermo th,
                                                                const int32_t* marks_in, uint64_t seed,
-                                                               int64_t step) {
+                                                               StepRef sr) {
+  const int64_t step = sr.get();
   int32_t* marks = const_cast<int32_t*>(marks_in);
   const uint32_t px = uint32_t(d.ax.n + 2), py = uint32_t(d.ay.n + 2), ncell = px * py;
   const int64_t n3[3] = {d.ax.n, d.ay.n, 1};
@@ -558,9 +560,10 @@ int ppf_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float 
   return check_launch("mlp_adam");
 }
 
-int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
-                         const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
-                         const ppf_mlp_net* net, uint64_t seed, int64_t step, ppf_stream_t s) {
+static int ncomp_thermo_mlp_dispatch(const ppf_grid* g, const void* in, const void* bcvals,
+                                     const ppf_temperature* t, const ppf_grid* tg, void* transfer,
+                                     const ppf_ncomp_params* p, const ppf_thermo* th, const ppf_mlp_net* net,
+                                     uint64_t seed, StepRef step, ppf_stream_t s) {
   if (int rc = check_grid(g, 2, 2, "ncomp_thermo_mlp")) return rc;
   if (g->dtype != PPF_F32) return fail(PPF_EINVAL, "ncomp_thermo_mlp: the surrogate path is FP32");
   if (!p || p->ncomp != 3 || g->nfields != 5) return fail(PPF_EINVAL, "ncomp_thermo_mlp: needs 3 components");
@@ -614,6 +617,21 @@ int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, 
                      static_cast<const float*>(in), static_cast<const float*>(bcvals), d, float(g->dx), *t,
                      static_cast<float*>(transfer), tg->sy, tg->sf, k, *th, net->marks, seed, step);
   return check_launch("ncomp_thermo_mlp");
+}
+
+int ppf_ncomp_thermo_mlp(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                         const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                         const ppf_mlp_net* net, uint64_t seed, int64_t step, ppf_stream_t s) {
+  return ncomp_thermo_mlp_dispatch(g, in, bcvals, t, tg, transfer, p, th, net, seed, step_host(step), s);
+}
+
+int ppf_ncomp_thermo_mlp_dev(const ppf_grid* g, const void* in, const void* bcvals, const ppf_temperature* t,
+                             const ppf_grid* tg, void* transfer, const ppf_ncomp_params* p, const ppf_thermo* th,
+                             const ppf_mlp_net* net, uint64_t seed, const int64_t* step_base, int64_t step_offset,
+                             ppf_stream_t s) {
+  if (!step_base) return fail(PPF_EINVAL, "ncomp_thermo_mlp_dev: null step counter");
+  return ncomp_thermo_mlp_dispatch(g, in, bcvals, t, tg, transfer, p, th, net, seed,
+                                   StepRef{step_base, step_offset}, s);
 }
 
 }  // extern "C"

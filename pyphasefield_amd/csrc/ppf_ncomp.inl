@@ -224,7 +224,8 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
 template <typename R, int N>
 __global__ __launch_bounds__(256) void k_ncomp_thermo(const R* __restrict__ in, const R* __restrict__ b, Dom d,
                                                       R dx, ppf_temperature t, R* __restrict__ tr, int64_t tsy,
-                                                      int64_t tsf, NcK k, uint64_t seed, int64_t step) {
+                                                      int64_t tsf, NcK k, uint64_t seed, StepRef sr) {
+  const int64_t step = sr.get();
   const int64_t px = d.ax.n + 2, py = d.ay.n + 2;
   const uint32_t nbx = uint32_t((px + 63) / 64);
   const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(256) void k_ncomp_thermo(const R* __restrict__ in, 
 template <typename R>
 int ncomp_thermo(const ppf_grid& g, const void* in, const void* bcvals, const ppf_temperature& t, const ppf_grid& tg,
                  void* transfer,
-                 const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, int64_t step, hipStream_t s) {
+                 const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, StepRef step, hipStream_t s) {
   Dom d = make_dom(g);
   NcK k{p, th};
   dim3 grid(unsigned(((g.n[0] + 2 + 63) / 64) * ((g.n[1] + 2 + 3) / 4)));
@@ -295,6 +296,8 @@ int ncomp_thermo(const ppf_grid& g, const void* in, const void* bcvals, const pp
 #define PPF_NC_TY 12
 #endif
 constexpr int kNcTY = PPF_NC_TY;
+// 256 threads cover 64 columns x 4 row groups; each thread walks kNcTY / 4 rows
+static_assert(kNcTY % 4 == 0 && kNcTY >= 4, "PPF_NC_TY must be a positive multiple of 4");
 
 template <typename R>
 __device__ __forceinline__ R divagradb(R a, R axp, R axm, R ayp, R aym, R b, R bxp, R bxm, R byp, R bym, R idx) {
@@ -340,17 +343,27 @@ struct Nb5 {
 #define PPF_NC_STEP_ATTR
 #endif
 
-template <typename R, int N>
+// The main step (NComponent_kernel) on a 64 x kNcTY tile.  The fields and T
+// of the tile plus its +1 ring sit in LDS.  The thermodynamic quantities the
+// step reads on that ring (M_c, dF/dc) and at its centre (G_S - G_L) come
+// either from the transfer stack of ppf_ncomp_thermo (FUSED = false: the
+// reference's two-kernel structure) or are evaluated here from the LDS tile
+// by the same device function (FUSED = true: the thermodynamics is pointwise,
+// so the ring needs no further halo and the transfer stack never reaches
+// memory).
+template <typename R, int N, bool FUSED>
 __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const R* __restrict__ in, R* __restrict__ out,
                                                          const R* __restrict__ b, Dom d, R dx, ppf_temperature t,
                                                          const R* __restrict__ tr, int64_t tsy, int64_t tsf,
-                                                         NcK k, uint64_t seed, int64_t step) {
+                                                         NcK k, uint64_t seed, StepRef sr) {
   constexpr int H = 1, W = kTX + 2 * H, HH = kNcTY + 2 * H;
   constexpr int NF = N + 2;         // phi, q1, q4, c_1..c_{N-1}
   constexpr int NT = 2 * (N - 1);   // M_c and dF/dc tiles
   __shared__ R sf[NF][HH][W];
   __shared__ R sT[HH][W];
   __shared__ R st[NT][HH][W];
+  __shared__ R sG[FUSED ? kNcTY : 1][FUSED ? kTX : 1];  // G_S - G_L of the outputs (fused)
+  const int64_t step = sr.get();
 
   uint32_t nbx = uint32_t((d.ax.n + kTX - 1) / kTX);
   uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -363,6 +376,7 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
     // every tile element of every array: one 32-bit offset per thread and slot,
     // computed once (the rows of the field and transfer stacks may differ in pitch)
     constexpr int NE = (W * HH + kThreads - 1) / kThreads;
+    constexpr int NTL = FUSED ? 1 : NT;  // transfer arrays loaded
     int32_t eo[NE], et[NE];
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
@@ -372,19 +386,21 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
       et[j] = ly * int32_t(tsy) + lx;
     }
     const R* fb = in + (y0 - 1) * d.ay.stride + (x0 - 1);
-    const R* tb = tr + (y0 - 1) * tsy + (x0 - 1);
+    const R* tb = FUSED ? nullptr : tr + (y0 - 1) * tsy + (x0 - 1);
     // all loads of the tile issued before the first LDS store (only the last
     // slot of each array is partial)
     constexpr int NFULL = (W * HH) / kThreads;
-    R vf[NF][NE], vt[NT][NE];
+    R vf[NF][NE], vt[NTL][NE];
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       const int e = threadIdx.x + j * kThreads;
       if (j < NFULL || e < W * HH) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) vf[f][j] = fb[f * d.sf + eo[j]];
+        if constexpr (!FUSED) {
 #pragma unroll
-        for (int f = 0; f < NT; ++f) vt[f][j] = tb[(1 + f) * tsf + et[j]];
+          for (int f = 0; f < NT; ++f) vt[f][j] = tb[(1 + f) * tsf + et[j]];
+        }
       }
     }
 #pragma unroll
@@ -393,8 +409,10 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
       if (j < NFULL || e < W * HH) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) (&sf[f][0][0])[e] = vf[f][j];
+        if constexpr (!FUSED) {
 #pragma unroll
-        for (int f = 0; f < NT; ++f) (&st[f][0][0])[e] = vt[f][j];
+          for (int f = 0; f < NT; ++f) (&st[f][0][0])[e] = vt[f][j];
+        }
       }
     }
 #pragma unroll
@@ -412,11 +430,36 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
       int64_t x = x0 - 1 + lx, y = y0 - 1 + ly;
       bool ok = x <= d.ax.n + 1 && y <= d.ay.n + 1;
       sT[ly][lx] = ok ? temperature_at<R>(t, n3, bc3, x, y, 1, double(dx), step) : R(1);
+      if constexpr (!FUSED) {
 #pragma unroll
-      for (int f = 0; f < NT; ++f) st[f][ly][lx] = ok ? tr[(1 + f) * tsf + y * tsy + x] : R(0);
+        for (int f = 0; f < NT; ++f) st[f][ly][lx] = ok ? tr[(1 + f) * tsf + y * tsy + x] : R(0);
+      }
     }
   }
   __syncthreads();
+  if constexpr (FUSED) {
+    // thermodynamics of every tile cell (NComponent_helper_kernel), from the
+    // fields and T just staged; cells past the padded domain get finite
+    // values that no output reads
+    const uint64_t pxy = uint64_t(d.ax.n + 2) * uint64_t(d.ay.n + 2);
+#pragma unroll 1
+    for (int e = threadIdx.x; e < W * HH; e += kThreads) {
+      const int ly = e / W, lx = e - ly * W;
+      R cin[N - 1];
+#pragma unroll
+      for (int l = 0; l < N - 1; ++l) cin[l] = sf[3 + l][ly][lx];
+      const uint64_t cell = uint64_t(y0 - 1 + ly) * uint64_t(d.ax.n + 2) + uint64_t(x0 - 1 + lx);
+      R dGc, Mc[N - 1], dF[N - 1];
+      thermo_cell<R, N>(cin, sf[0][ly][lx], sT[ly][lx], k, step, seed, pxy, cell, &dGc, Mc, dF);
+#pragma unroll
+      for (int l = 0; l < N - 1; ++l) {
+        st[l][ly][lx] = Mc[l];
+        st[N - 1 + l][ly][lx] = dF[l];
+      }
+      if (ly >= 1 && ly <= kNcTY && lx >= 1 && lx <= kTX) sG[ly - 1][lx - 1] = dGc;
+    }
+    __syncthreads();
+  }
 
   const ppf_ncomp_params& p = k.p;
   const R idx = R(1) / dx;
@@ -577,7 +620,12 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
     M_phi = M_phi + c_N * R(p.M[N - 1]);
     R M_q = M_qmax + (R(1e-6) - M_qmax) * h;
     // dphidt (:179-189)
-    R dG = tr[y * tsy + x];  // G_S - G_L
+    R dG;  // G_S - G_L
+    if constexpr (FUSED) {
+      dG = sG[ly - 1][lx - 1];
+    } else {
+      dG = tr[y * tsy + x];
+    }
     R dphidt = ebar2 * (R(1) - R(3) * y_e) * (Tc * lphi + dTdx * dphidx + dTdy * dphidy);
     dphidt = dphidt + d_term_dx + d_term_dy;
     dphidt = dphidt - hprime * dG / v_m;
@@ -653,36 +701,50 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
   }
 }
 
-template <typename R>
-int ncomp_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
-               const ppf_grid& tg, const void* transfer, const ppf_ncomp_params& p, uint64_t seed, int64_t step,
-               hipStream_t s) {
+template <typename R, bool FUSED>
+int ncomp_step_launch(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
+                      const ppf_grid* tg, const void* transfer, const ppf_ncomp_params& p, const ppf_thermo* th,
+                      uint64_t seed, StepRef step, hipStream_t s) {
   Dom d = make_dom(g);
   NcK k;
   k.p = p;
-  k.th = ppf_thermo{};
+  k.th = th ? *th : ppf_thermo{};
   unsigned blocks = unsigned(((g.n[0] + kTX - 1) / kTX) * ((g.n[1] + kNcTY - 1) / kNcTY));
   const R* fin = static_cast<const R*>(in);
   R* fout = static_cast<R*>(out);
   const R* bc = static_cast<const R*>(bcvals);
   const R* tr = static_cast<const R*>(transfer);
+  const int64_t tsy = tg ? tg->sy : 0, tsf = tg ? tg->sf : 0;
   switch (p.ncomp) {
     case 2:
-      hipLaunchKernelGGL((k_ncomp_step<R, 2>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t,
-                         tr, tg.sy, tg.sf, k, seed, step);
+      hipLaunchKernelGGL((k_ncomp_step<R, 2, FUSED>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d,
+                         R(g.dx), t, tr, tsy, tsf, k, seed, step);
       break;
     case 3:
-      hipLaunchKernelGGL((k_ncomp_step<R, 3>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t,
-                         tr, tg.sy, tg.sf, k, seed, step);
+      hipLaunchKernelGGL((k_ncomp_step<R, 3, FUSED>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d,
+                         R(g.dx), t, tr, tsy, tsf, k, seed, step);
       break;
     case 4:
-      hipLaunchKernelGGL((k_ncomp_step<R, 4>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t,
-                         tr, tg.sy, tg.sf, k, seed, step);
+      hipLaunchKernelGGL((k_ncomp_step<R, 4, FUSED>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d,
+                         R(g.dx), t, tr, tsy, tsf, k, seed, step);
       break;
     default:
       return fail(PPF_EINVAL, "ncomp_step: ncomp must be 2..4");
   }
-  return check_launch("ncomp_step");
+  return check_launch(FUSED ? "ncomp_fused_step" : "ncomp_step");
+}
+
+template <typename R>
+int ncomp_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
+               const ppf_grid& tg, const void* transfer, const ppf_ncomp_params& p, uint64_t seed, StepRef step,
+               hipStream_t s) {
+  return ncomp_step_launch<R, false>(g, in, out, bcvals, t, &tg, transfer, p, nullptr, seed, step, s);
+}
+
+template <typename R>
+int ncomp_fused_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
+                     const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, StepRef step, hipStream_t s) {
+  return ncomp_step_launch<R, true>(g, in, out, bcvals, t, nullptr, nullptr, p, &th, seed, step, s);
 }
 
 }  // namespace ppf

@@ -1072,37 +1072,49 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
+// Partial sums of one field: block b walks interior rows b, b + G, ... (a
+// row is a fixed (y, z); one division per row, uniform across the block) and
+// its threads walk x.  Forward neighbours are read directly, except on the
+// last cell of an axis, where the boundary rule is folded in.
 template <typename R>
 __global__ __launch_bounds__(256) void k_reduce_partial(const R* __restrict__ in, const R* __restrict__ b, Dom d,
                                                         R dx, int kind, double eps, double* __restrict__ work) {
-  int fi = blockIdx.y;
-  const R* f = in + fi * d.sf;
+  const int fi = blockIdx.y;
+  const R* __restrict__ f = in + fi * d.sf;
   const R* bb = b ? b + fi * d.sf : nullptr;
-  int64_t nx = d.ax.n, ny = d.ndim >= 2 ? d.ay.n : 1, nz = d.ndim >= 3 ? d.az.n : 1;
-  int64_t total = nx * ny * nz;
+  const int64_t nx = d.ax.n, ny = d.ndim >= 2 ? d.ay.n : 1, nz = d.ndim >= 3 ? d.az.n : 1;
+  const int64_t rows = ny * nz;
+  const int64_t sy = d.ndim >= 2 ? d.ay.stride : 0, sz = d.ndim >= 3 ? d.az.stride : 0;
   double acc = 0.0;
   const double idx = 1.0 / double(dx);
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
-       t += int64_t(gridDim.x) * blockDim.x) {
-    int64_t x = 1 + t % nx;
-    int64_t y = d.ndim >= 2 ? 1 + (t / nx) % ny : 0;
-    int64_t z = d.ndim >= 3 ? 1 + t / (nx * ny) : 0;
-    double c = double(f[offset(d, x, y, z)]);
+  const double ke = 0.5 * eps * eps;
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int64_t zq = r / ny;
+    const int64_t y = d.ndim >= 2 ? 1 + (r - zq * ny) : 0;
+    const int64_t z = d.ndim >= 3 ? 1 + zq : 0;
+    const R* __restrict__ row = f + z * sz + y * sy;
     if (kind == 0) {
-      acc += c;
-    } else {
-      double e = c * c * (1.0 - c) * (1.0 - c);
-      double gx = (double(fold(f, bb, d, x + 1, y, z, dx)) - c) * idx;
+      for (int64_t x = 1 + threadIdx.x; x <= nx; x += 256) acc += double(row[x]);
+      continue;
+    }
+    const bool ylast = d.ndim >= 2 && y == ny, zlast = d.ndim >= 3 && z == nz;
+    for (int64_t x = 1 + threadIdx.x; x <= nx; x += 256) {
+      const double c = double(row[x]);
+      const double e = c * c * (1.0 - c) * (1.0 - c);
+      const double xp = x < nx ? double(row[x + 1]) : double(fold(f, bb, d, x + 1, y, z, dx));
+      const double gx = (xp - c) * idx;
       double g2 = gx * gx;
       if (d.ndim >= 2) {
-        double gy = (double(fold(f, bb, d, x, y + 1, z, dx)) - c) * idx;
+        const double yp = ylast ? double(fold(f, bb, d, x, y + 1, z, dx)) : double(row[x + sy]);
+        const double gy = (yp - c) * idx;
         g2 += gy * gy;
       }
       if (d.ndim >= 3) {
-        double gz = (double(fold(f, bb, d, x, y, z + 1, dx)) - c) * idx;
+        const double zp = zlast ? double(fold(f, bb, d, x, y, z + 1, dx)) : double(row[x + sz]);
+        const double gz = (zp - c) * idx;
         g2 += gz * gz;
       }
-      acc += e + 0.5 * eps * eps * g2;
+      acc += e + ke * g2;
     }
   }
   double s = block_sum(acc);

@@ -284,25 +284,13 @@ __device__ __forceinline__ void warren_rows(const WarrenLds<R>& S, const WarrenC
 
 // One 64 x 16 tile per 256-thread block: load phi/c (halo included), derive
 // D_C and temp there into LDS, then the row walk.
-// Occupancy hint (waves per SIMD the allocation must allow; 0 = compiler's
-// choice).  Left to itself the compiler takes 102 SGPRs, which caps the
-// kernel at 7 waves per SIMD (7 blocks per CU: a 2048^2 grid's 4096 tiles then
-// run in 2.3 rounds); with 8 it keeps 78 SGPRs, spilling the rest to VGPR
-// lanes, and 4096 tiles fill exactly two rounds.
-#ifndef PPF_WARREN_WAVES
-#define PPF_WARREN_WAVES 8
-#endif
-#if PPF_WARREN_WAVES
-#define PPF_WARREN_ATTR __attribute__((amdgpu_waves_per_eu(PPF_WARREN_WAVES, 8)))
-#else
-#define PPF_WARREN_ATTR
-#endif
 template <typename R>
-__global__ __launch_bounds__(kThreads) PPF_WARREN_ATTR void k_warren(const R* __restrict__ in, R* __restrict__ out,
-                                                     const R* __restrict__ b, Dom d, R dx, WarrenK p,
-                                                     uint64_t seed, int64_t step) {
+__device__ __forceinline__ void warren_tile_step(const R* __restrict__ in, R* __restrict__ out,
+                                                 const R* __restrict__ b, const Dom& d, R dx, const WarrenK& p,
+                                                 uint64_t seed, StepRef sr) {
   __shared__ WarrenLds<R> S;
   const WarrenC<R> K(p, dx);
+  const int64_t step = sr.get();
   int64_t x0, y0;
   bool interior;
   warren_tile(d, blockIdx.x, gridDim.x, x0, y0, interior);
@@ -318,8 +306,42 @@ __global__ __launch_bounds__(kThreads) PPF_WARREN_ATTR void k_warren(const R* __
 }
 
 template <typename R>
+__global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R* __restrict__ out,
+                                                     const R* __restrict__ b, Dom d, R dx, WarrenK p,
+                                                     uint64_t seed, StepRef sr) {
+  warren_tile_step<R>(in, out, b, d, dx, p, seed, sr);
+}
+
+// FP32 occupancy hint (waves per SIMD the allocation must allow; 0 =
+// compiler's choice).  Left to itself the compiler takes 102 SGPRs, which
+// caps the kernel at 7 waves per SIMD (7 blocks per CU: a 2048^2 grid's 4096
+// tiles then run in 2.3 rounds); with 8 it keeps 78 SGPRs, spilling the rest
+// to VGPR lanes, and 4096 tiles fill exactly two rounds.  FP64 keeps the
+// compiler's choice: its registers allow 4 waves either way.
+#ifndef PPF_WARREN_WAVES
+#define PPF_WARREN_WAVES 8
+#endif
+#if PPF_WARREN_WAVES
+#define PPF_WARREN_ATTR __attribute__((amdgpu_waves_per_eu(PPF_WARREN_WAVES, 8)))
+#else
+#define PPF_WARREN_ATTR
+#endif
+// (defined in the FP32 translation unit only: an explicit specialization is
+// an ordinary function)
+#ifdef PPF_FP32_UNIT
+template <>
+__global__ __launch_bounds__(kThreads) PPF_WARREN_ATTR void k_warren<float>(const float* __restrict__ in,
+                                                                            float* __restrict__ out,
+                                                                            const float* __restrict__ b, Dom d,
+                                                                            float dx, WarrenK p, uint64_t seed,
+                                                                            StepRef sr) {
+  warren_tile_step<float>(in, out, b, d, dx, p, seed, sr);
+}
+#endif
+
+template <typename R>
 int warren_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_warren_params& p,
-                uint64_t seed, int64_t step, hipStream_t s) {
+                uint64_t seed, StepRef step, hipStream_t s) {
   Dom d = make_dom(g);
   WarrenK k;
   k.W_A = p.W_A;

@@ -244,13 +244,17 @@ class Simulation:
         n = int(number_of_timesteps)
         chunk = self._graph_chunk()
         if chunk and n >= chunk:
-            # engines whose step launches do not depend on the step counter:
             # `chunk` steps captured in a HIP graph and replayed (the two
             # buffers are back in place after an even number of steps).  The
             # graph is kept for later simulate() calls while everything baked
             # into its launches (buffers, grid, engine scalars) is unchanged.
+            # Engines whose launches read the step counter take it from a
+            # device int64 that the graph itself advances by `chunk`; it is
+            # set from the host counter before the replays.
             graph = self._cached_graph(chunk)
             reps = n // chunk
+            if self._DEVICE_STEP:
+                self._step_counter_device().fill_(int(self.time_step_counter))
             for _ in range(reps):
                 graph.replay()
             self.time_step_counter += reps * chunk
@@ -265,9 +269,29 @@ class Simulation:
                     self.save_simulation()
                     self._host_dirty = False
 
-    # steps per captured HIP graph for engines that declare their step launches
-    # independent of the step counter (0: always eager)
+    # steps per captured HIP graph (0: always eager)
     _GRAPH_CHUNK = 0
+    # the step launches read the step counter (noise, temperature ramp): inside
+    # a capture they take it from a device counter (see _step_source)
+    _DEVICE_STEP = False
+    # inside a capture: offset of the step being recorded from the counter
+    # value at the start of the chunk (1-based); None when launching eagerly
+    _graph_step_offset = None
+
+    def _step_counter_device(self):
+        import torch
+        c = getattr(self, "_step_counter_gpu", None)
+        if c is None:
+            c = torch.zeros(1, dtype=torch.int64, device=self._device())
+            self._step_counter_gpu = c
+        return c
+
+    def _step_source(self):
+        """Step-counter arguments of a step launch: ("host", step) eagerly,
+        ("dev", counter pointer, offset) while a graph chunk is recorded."""
+        if self._graph_step_offset is None:
+            return ("host", int(self.time_step_counter))
+        return ("dev", _lib.ptr(self._step_counter_device()), int(self._graph_step_offset))
 
     def _graph_key(self):
         """Engine scalars the captured step launches bake in (None: the graph
@@ -289,6 +313,8 @@ class Simulation:
             torch.cuda.synchronize()
             cached[1].reset()
             self._graph_cache = None
+        if self._DEVICE_STEP:
+            self._step_counter_device()  # allocated before the capture
         graph = torch.cuda.CUDAGraph()
         torch.cuda.synchronize()
         # no garbage collection inside the capture: a graph kept by another
@@ -297,9 +323,15 @@ class Simulation:
         gc.disable()
         try:
             with torch.cuda.graph(graph):
-                for _ in range(chunk):
+                for k in range(chunk):
+                    if self._DEVICE_STEP:
+                        self._graph_step_offset = k + 1
                     self.simulation_loop()
+                if self._DEVICE_STEP:
+                    _lib.check(_lib.lib().ppf_step_counter_add(_lib.ptr(self._step_counter_device()), chunk,
+                                                              _lib.stream_handle()), "step_counter_add")
         finally:
+            self._graph_step_offset = None
             if gc_was_on:
                 gc.enable()
         if key is not None:

@@ -63,7 +63,8 @@ def test_every_config_equals_single_instance_engines(gpu):
     steps = 12
     want = [_single_engine(s, f, steps) for s, f in zip(specs, init)]
     for config in range(_lib.lib().ppf_ensemble_num_configs()):
-        for blocks in (8, 13, 64):
+        # fewer blocks than XCD segments (1, 3), exactly one per segment (8), more
+        for blocks in (1, 3, 8, 13, 64):
             ens = Ensemble(specs, init)
             ens.step(steps, config=config, blocks=blocks)
             for i, got in enumerate(ens.fields()):
