@@ -244,6 +244,19 @@ int64_t ppf_reduce_work_bytes(const ppf_grid* g);
 int ppf_reduce(const ppf_grid* g, const void* in, const void* bcvals, int kind, double epsilon,
                double* out, void* work, ppf_stream_t s);
 
+/* The same reductions without a workspace: no counterpart in the reference;
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The
+   partial sums live in a file-scope device array of the library
+   (PPF_REDUCE_NOALLOC_MAX_FIELDS x 2048 doubles = 256 KiB per GPU, held until
+   the process ends); kind 0 on more than PPF_REDUCE_NOALLOC_MAX_FIELDS fields
+   is rejected.  Every call overwrites all the partial sums it reads, so
+   nothing needs clearing; calls that overlap on one GPU share the array and
+   race.  Allocates nothing (no hipMalloc, no library allocation). */
+#define PPF_REDUCE_NOALLOC_MAX_FIELDS 16
+int ppf_reduce_noalloc(const ppf_grid* g, const void* in, const void* bcvals, int kind, double epsilon,
+                       double* out, ppf_stream_t s);
+
 /* ---- AnisoDorr binary polycrystalline solidification (Engines/AnisoDorrGPU.py) ----
    2D; field stack (phi, q1, q4, c); transfer stack of 6 arrays with the layout
    convention of g (tg->nfields = 6).  The reference hard-codes these constants

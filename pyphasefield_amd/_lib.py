@@ -148,6 +148,8 @@ def lib():
         optional = dict(_lib_ext.SIGNATURES)
     except ImportError:
         pass
+    if hasattr(L, "ppf_reduce_noalloc"):
+        optional["ppf_reduce_noalloc"] = ([P(Grid), c_vp, c_vp, ctypes.c_int, c_f64, c_vp, c_vp], c_i32)
     for name, (args, res) in list(sigs.items()) + list(optional.items()):
         fn = getattr(L, name)
         fn.argtypes = args
