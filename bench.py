@@ -294,6 +294,8 @@ def bench_ncomp(args):
     sim = ncomp_sim(n, precision, dg, thermo_eval="surrogate" if variant == "surrogate" else None,
                     surrogate=args.surrogate if variant == "surrogate" and args.surrogate and
                     os.path.exists(args.surrogate) else None)
+    if args.nc_kernel:
+        sim.user_data["ncomp_kernel"] = args.nc_kernel
     if variant == "surrogate":
         sim._setup_surrogate()
         sim._surrogate.split2 = "2" in args.mlp_split
@@ -431,6 +433,8 @@ def main():
     ap.add_argument("--config", default="warren", choices=sorted(CONFIGS) + ["ensemble"])
     ap.add_argument("--variant", default="fp32")
     ap.add_argument("--size", type=int, default=None)
+    ap.add_argument("--nc-kernel", default=None, choices=["fused", "two_pass"],
+                    help="ncomp: one-launch step or the helper + step pair")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ens-config", type=int, default=None,
                     help="ensemble: tile configuration (skips the autotuner)")

@@ -46,8 +46,8 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src, extra):
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+def _compile(src, extra, objdir=OBJ):
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
     cmd = [_hipcc()] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -55,8 +55,11 @@ def _compile(src, extra):
     return obj
 
 
-def build(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose=False, defines=(), lib=LIB, objdir=OBJ):
+    """Compile every translation unit and link `lib`.  `defines`: extra
+    -D options (tile and occupancy knobs of the kernels) for a variant build
+    into another `lib` / `objdir`."""
+    os.makedirs(objdir, exist_ok=True)
     missing = [k for k in UNITS if not os.path.exists(os.path.join(CSRC, k))]
     if missing:
         raise FileNotFoundError("missing HIP sources: %s" % ", ".join(missing))
@@ -65,17 +68,18 @@ def build(verbose=False):
     for f in sorted(os.listdir(CSRC)):
         if f.endswith(".hip") and f not in units:
             units[f] = []
+    dflags = ["-D" + d for d in defines]
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(units))) as ex:
-        objs = list(ex.map(lambda kv: _compile(*kv), units.items()))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda kv: _compile(kv[0], kv[1] + dflags, objdir), units.items()))
+    tmp = lib + ".tmp"
     cmd = [_hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib)
+    return lib
 
 
 if __name__ == "__main__":
