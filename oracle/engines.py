@@ -249,13 +249,29 @@ PARAM_NAMES = ["dx", "d", "v_m", "M_qmax", "H", "y_e", "beta", "D_L", "D_S", "dt
                "ebar", "eqbar", "noise_phi", "noise_c", "noise_q"]  # NCGPU_new.py:497-511
 
 
+def nc_g(phi):
+    """Double-well g(phi) of NCGPU_new.py:298 (numba integer powers: products)."""
+    return (phi ** 2) * (1 - phi) ** 2
+
+
+def nc_h(phi):
+    """Interpolant h(phi) of NCGPU_new.py:299."""
+    return (phi * phi * phi) * (6. * phi ** 2 - 15. * phi + 10.)
+
+
 def ncomp_helper(fields, T, params, c_params, thermo_l, thermo_s, seed=0, step=0):
     """NComponent_helper_kernel (NCGPU_new.py:256-313) over every padded cell.
 
     fields: [N+2, *padded] = phi, q1, q4, c_1..c_{N-1};  T: padded.
     thermo_l / thermo_s: (g0, L, R) of the liquid (ufunc 1) and solid (ufunc 0)
     phases (NCGPU_new.py:353-354).
-    Returns transfer [2N, *padded] = G_L, G_S, M_c x (N-1), dFdc x (N-1).
+    Returns transfer [2N, *padded] = G_L, G_S, M_c x (N-1), dFdc x (N-1):
+      G_phase and its h = 1e-7 forward differences along e_l - e_N
+      (:295-305, thermo.fd_derivatives),
+      M_c,l = v_m c_l (D_L + h(phi) (D_S - D_L)) / (8.314 T)        (:308),
+      dF/dc_l = (dGL_l + h(phi) (dGS_l - dGL_l)) / v_m
+                + (W_l - W_N) g(phi) T + noise_c,l                  (:311),
+      noise_c,l = noise_amp_c sqrt(2 * 8.314 T / v_m) xi,  xi ~ N(0, 1) (:309).
     """
     pr = dict(zip(PARAM_NAMES, params))
     v_m, D_L, D_S, noise_amp_c = pr["v_m"], pr["D_L"], pr["D_S"], pr["noise_c"]
@@ -268,29 +284,24 @@ def ncomp_helper(fields, T, params, c_params, thermo_l, thermo_s, seed=0, step=0
     for cl in cs:
         c_N = c_N - cl
     arr = [c.copy() for c in cs] + [c_N]
-    G_L = _thermo.gibbs(arr, T, *thermo_l)
-    G_S = _thermo.gibbs(arr, T, *thermo_s)
-    g = (phi ** 2) * (1 - phi) ** 2
-    h = (phi * phi * phi) * (6. * phi ** 2 - 15. * phi + 10.)  # numba int power: products
+    G_L, dGL = _thermo.fd_derivatives(arr, T, *thermo_l)
+    G_S, dGS = _thermo.fd_derivatives(arr, T, *thermo_s)
+    g = nc_g(phi)
+    h = nc_h(phi)
     transfer = np.zeros((2 * ncomp,) + phi.shape)
     transfer[0] = G_L
     transfer[1] = G_S
     cells = _rng.padded_cells(phi.shape)
-    arr[ncomp - 1] = arr[ncomp - 1] - 0.0000001
     for l in range(ncomp - 1):
-        arr[l] = arr[l] + 0.0000001
-        dGLdc = 10000000. * (_thermo.gibbs(arr, T, *thermo_l) - G_L)
-        dGSdc = 10000000. * (_thermo.gibbs(arr, T, *thermo_s) - G_S)
         M_c = v_m * cs[l] * (D_L + h * (D_S - D_L)) / (8.314 * T)
         if noise_amp_c != 0:
             xi = _rng.normal_f64(seed, step, _rng.S_NC_C0 + l, phi.size, cells)
             noise_c = noise_amp_c * np.sqrt(2. * 8.314 * T / v_m) * xi
         else:
             noise_c = 0.
-        dFdc = (dGLdc + h * (dGSdc - dGLdc)) / v_m + (W[l] - W[ncomp - 1]) * g * T + noise_c
+        dFdc = (dGL[l] + h * (dGS[l] - dGL[l])) / v_m + (W[l] - W[ncomp - 1]) * g * T + noise_c
         transfer[2 + l] = M_c
         transfer[2 + l + ncomp - 1] = dFdc
-        arr[l] = arr[l] - 0.0000001
     return transfer
 
 

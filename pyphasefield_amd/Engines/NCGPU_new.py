@@ -179,14 +179,13 @@ class NCGPU_new(Simulation):
     _DEVICE_STEP = True
 
     def _kernel_mode(self):
-        """"fused": one launch per step (ppf_ncomp_fused_step, the default in
-        FP32); "two_pass": the reference's helper + kernel structure
-        (ppf_ncomp_thermo + ppf_ncomp_step, the default in FP64, where the
-        forward-difference thermodynamics costs more than the transfer stack
-        saves); the surrogate always runs as its own pass."""
+        """"fused" (default): one launch per step (ppf_ncomp_fused_step);
+        "two_pass": the reference's helper + kernel structure
+        (ppf_ncomp_thermo + ppf_ncomp_step, same results); the surrogate
+        runs its network pass before the step."""
         if getattr(self, "_surrogate", None) is not None:
             return "surrogate"
-        mode = self.user_data.get("ncomp_kernel", "fused" if self._precision == "FP32" else "two_pass")
+        mode = self.user_data.get("ncomp_kernel", "fused")
         if mode not in ("fused", "two_pass"):
             raise ValueError("ncomp_kernel must be 'fused' or 'two_pass'")
         return mode

@@ -158,15 +158,175 @@ __device__ __forceinline__ void gibbs_grad(const R (&c)[N], const PhaseAtT<R, N>
   for (int l = 0; l < N - 1; ++l) dG[l] = part[l] - part[N - 1];
 }
 
+// FP32 form of the inline Redlich-Kister phases, converted once on the host
+// (the FP32 kernels then do no double arithmetic per cell): coefficients as
+// float, and which temperature terms occur at all.
+struct PhaseF {
+  float g0[PPF_MAX_COMP][5];
+  float L[6][3][2];
+  float R;
+};
+struct ThermoF {
+  PhaseF liquid, solid;
+  int32_t has_TlnT;  // any c T ln T term
+  int32_t has_more;  // any d T^2 or e / T term
+  float inv_vm, D_L, dD, inv_R8314;  // 1/v_m, D_L, D_S - D_L, 1/8.314
+};
+
+inline PhaseF phase_f(const ppf_rk_phase& ph) {
+  PhaseF o{};
+  for (int i = 0; i < PPF_MAX_COMP; ++i)
+    for (int j = 0; j < 5; ++j) o.g0[i][j] = float(ph.g0[i][j]);
+  for (int p = 0; p < 6; ++p)
+    for (int k = 0; k < 3; ++k)
+      for (int j = 0; j < 2; ++j) o.L[p][k][j] = float(ph.L[p][k][j]);
+  o.R = float(ph.R);
+  return o;
+}
+
+inline ThermoF thermo_f(const ppf_thermo& th, const ppf_ncomp_params& p) {
+  ThermoF f{};
+  f.liquid = phase_f(th.liquid);
+  f.solid = phase_f(th.solid);
+  for (int i = 0; i < PPF_MAX_COMP; ++i) {
+    for (const ppf_rk_phase* ph : {&th.liquid, &th.solid}) {
+      if (ph->g0[i][2] != 0.0) f.has_TlnT = 1;
+      if (ph->g0[i][3] != 0.0 || ph->g0[i][4] != 0.0) f.has_more = 1;
+    }
+  }
+  f.inv_vm = float(1.0 / p.v_m);
+  f.D_L = float(p.D_L);
+  f.dD = float(p.D_S - p.D_L);
+  f.inv_R8314 = float(1.0 / 8.314);
+  return f;
+}
+
 struct NcK {
   ppf_ncomp_params p;
   ppf_thermo th;
+  ThermoF tf;  // FP32 kernels without TDB tables
 };
+
+inline NcK make_nck(const ppf_ncomp_params& p, const ppf_thermo& th) {
+  NcK k;
+  k.p = p;
+  k.th = th;
+  k.tf = thermo_f(th, p);
+  return k;
+}
+
+// FP32 phases at one temperature from the float coefficients
+template <int N>
+__device__ __forceinline__ void phase_at_f(const PhaseF& ph, float T, float lnT, bool tlnt, bool more,
+                                           PhaseAtT<float, N>& o) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    float g = __builtin_fmaf(ph.g0[i][1], T, ph.g0[i][0]);
+    if (tlnt) g = __builtin_fmaf(ph.g0[i][2], T * lnT, g);
+    if (more) g = g + ph.g0[i][3] * T * T + ph.g0[i][4] / T;
+    o.gi[i] = g;
+  }
+#pragma unroll
+  for (int p = 0; p < N * (N - 1) / 2; ++p) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.L[p][k] = __builtin_fmaf(ph.L[p][k][1], T, ph.L[p][k][0]);
+  }
+  o.RT = ph.R * T;
+}
+
+// natural log of a positive normal float: the hardware log2 scaled by ln 2
+// (the FP32 engines clamp compositions to >= 1e-10 and T is O(1e3))
+__device__ __forceinline__ float nlogf(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+
+// FP32 thermodynamics of one cell from the float coefficients (analytic
+// derivatives; same quantities and order of terms as thermo_cell)
+template <int N>
+__device__ __forceinline__ void thermo_cell_f32(const float (&cin)[N - 1], float phi, float T, const NcK& k,
+                                                int64_t step, uint64_t seed, uint64_t ncells, uint64_t cell,
+                                                float* dG, float* Mc, float* dF) {
+  const ThermoF& f = k.tf;
+  float c[N];
+  float cN = 1.f;
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) {
+    c[l] = cin[l];
+    cN = cN - cin[l];
+  }
+  c[N - 1] = cN;
+  const float lnT = f.has_TlnT ? nlogf(T) : 0.f;
+  PhaseAtT<float, N> PL, PS;
+  phase_at_f<N>(f.liquid, T, lnT, f.has_TlnT != 0, f.has_more != 0, PL);
+  phase_at_f<N>(f.solid, T, lnT, f.has_TlnT != 0, f.has_more != 0, PS);
+  // ln max(c, 1e-10) once per component, shared by G and both gradients
+  float cc[N], lc[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    cc[i] = fmaxf(c[i], float(kCMin));
+    lc[i] = nlogf(cc[i]);
+  }
+  float ideal = 0.f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) ideal = __builtin_fmaf(cc[i], lc[i], ideal);
+  float GL = 0.f, GS = 0.f;
+  float pL[N], pS[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    GL = __builtin_fmaf(c[i], PL.gi[i], GL);
+    GS = __builtin_fmaf(c[i], PS.gi[i], GS);
+    const float dmix = c[i] > float(kCMin) ? lc[i] + 1.f : 0.f;
+    pL[i] = __builtin_fmaf(PL.RT, dmix, PL.gi[i]);
+    pS[i] = __builtin_fmaf(PS.RT, dmix, PS.gi[i]);
+  }
+  int pr = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = i + 1; j < N; ++j, ++pr) {
+      const float dd = c[i] - c[j], cij = c[i] * c[j];
+      const float PLv = __builtin_fmaf(dd, __builtin_fmaf(dd, PL.L[pr][2], PL.L[pr][1]), PL.L[pr][0]);
+      const float PSv = __builtin_fmaf(dd, __builtin_fmaf(dd, PS.L[pr][2], PS.L[pr][1]), PS.L[pr][0]);
+      const float dPL = __builtin_fmaf(2.f * dd, PL.L[pr][2], PL.L[pr][1]);
+      const float dPS = __builtin_fmaf(2.f * dd, PS.L[pr][2], PS.L[pr][1]);
+      GL = __builtin_fmaf(cij, PLv, GL);
+      GS = __builtin_fmaf(cij, PSv, GS);
+      pL[i] = pL[i] + __builtin_fmaf(cij, dPL, c[j] * PLv);
+      pL[j] = pL[j] + __builtin_fmaf(-cij, dPL, c[i] * PLv);
+      pS[i] = pS[i] + __builtin_fmaf(cij, dPS, c[j] * PSv);
+      pS[j] = pS[j] + __builtin_fmaf(-cij, dPS, c[i] * PSv);
+    }
+  }
+  // G_S - G_L: the ideal-mixing terms enter each phase as in gibbs()
+  *dG = __builtin_fmaf(PS.RT, ideal, GS) - __builtin_fmaf(PL.RT, ideal, GL);
+  const float phi2 = phi * phi;
+  const float g = phi2 * ((1.f - phi) * (1.f - phi));
+  const float h = phi2 * phi * __builtin_fmaf(phi, __builtin_fmaf(6.f, phi, -15.f), 10.f);
+  const ppf_ncomp_params& p = k.p;
+  const float mob = float(p.v_m) * __builtin_fmaf(h, f.dD, f.D_L) * (f.inv_R8314 * __builtin_amdgcn_rcpf(T));
+  const float gT = g * T;
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) {
+    Mc[l] = mob * cin[l];
+    const float dGL = pL[l] - pL[N - 1], dGS = pS[l] - pS[N - 1];
+    float noise = 0.f;
+    if (p.noise_c != 0.0) {
+      const float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + l, ncells, cell)));
+      noise = float(p.noise_c) * sqrtf(2.f * 8.314f * T * f.inv_vm) * xi;
+    }
+    dF[l] = __builtin_fmaf(__builtin_fmaf(h, dGS - dGL, dGL), f.inv_vm,
+                           (float(p.W[l]) - float(p.W[N - 1])) * gT) + noise;
+  }
+}
 
 template <typename R, int N>
 __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, const NcK& k, int64_t step,
                                             uint64_t seed, uint64_t ncells, uint64_t cell, R* dG, R* Mc,
                                             R* dF) {
+  if constexpr (sizeof(R) == 4) {
+    if (!k.th.tables) {
+      thermo_cell_f32<N>(cin, phi, T, k, step, seed, ncells, cell, dG, Mc, dF);
+      return;
+    }
+  }
   const ppf_ncomp_params& p = k.p;
   R c[N];
   R cN = R(1);
@@ -265,7 +425,7 @@ int ncomp_thermo(const ppf_grid& g, const void* in, const void* bcvals, const pp
                  void* transfer,
                  const ppf_ncomp_params& p, const ppf_thermo& th, uint64_t seed, StepRef step, hipStream_t s) {
   Dom d = make_dom(g);
-  NcK k{p, th};
+  const NcK k = make_nck(p, th);
   dim3 grid(unsigned(((g.n[0] + 2 + 63) / 64) * ((g.n[1] + 2 + 3) / 4)));
   const R* fin = static_cast<const R*>(in);
   const R* bc = static_cast<const R*>(bcvals);
@@ -706,9 +866,7 @@ int ncomp_step_launch(const ppf_grid& g, const void* in, void* out, const void* 
                       const ppf_grid* tg, const void* transfer, const ppf_ncomp_params& p, const ppf_thermo* th,
                       uint64_t seed, StepRef step, hipStream_t s) {
   Dom d = make_dom(g);
-  NcK k;
-  k.p = p;
-  k.th = th ? *th : ppf_thermo{};
+  const NcK k = make_nck(p, th ? *th : ppf_thermo{});
   unsigned blocks = unsigned(((g.n[0] + kTX - 1) / kTX) * ((g.n[1] + kNcTY - 1) / kNcTY));
   const R* fin = static_cast<const R*>(in);
   R* fout = static_cast<R*>(out);
