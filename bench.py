@@ -222,12 +222,17 @@ def bench_ch3d(args):
 
 # ---------------------------------------------------------- N-component --
 NC_DX = 0.0000046
-NC_T0 = 1000.0
+# T0 about 31 K below the liquidus of the initial liquid (c = 0.30, 0.30, 0.40
+# melts at 1471 K in the synthetic model): solid grows at the cold edge and
+# melts back up the gradient; far colder starts drive |dG| to tens of times
+# the double-well barrier, where the reference scheme breaks interfaces into
+# checkerboards
+NC_T0 = 1440.0
 
 
-def ncomp_sim(n, precision, dg=None, seeds=16, noise=0.0, thermo_eval=None, surrogate=None):
+def ncomp_sim(n, precision, dg=None, seeds=16, noise=0.0, thermo_eval=None, surrogate=None, T0=None):
     """N-component 2048^2: AL-CU-NI synthetic Redlich-Kister model (seed 0),
-    16 random-orientation seeds of radius 6, T = 1000 K + 1 K/cell along x,
+    16 random-orientation seeds of radius 6, T = 1440 K + 1 K/cell along x,
     Neumann, engine parameters of the Ni-Cu test database."""
     import pyphasefield_amd.Engines as engines
     from pyphasefield_amd import thermo
@@ -236,7 +241,7 @@ def ncomp_sim(n, precision, dg=None, seeds=16, noise=0.0, thermo_eval=None, surr
     sim.set_framework("GPU_SERIAL")
     sim.set_dx(NC_DX)
     sim.set_temperature_type("LINEAR_GRADIENT")
-    sim.set_temperature_initial_T(NC_T0)
+    sim.set_temperature_initial_T(NC_T0 if T0 is None else T0)
     sim.set_temperature_dTdx(1.0 / NC_DX)
     sim.set_temperature_dTdy(0.0)
     sim.set_temperature_dTdt(0.0)

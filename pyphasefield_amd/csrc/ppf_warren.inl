@@ -122,10 +122,14 @@ __device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R
   const R deltac = D1 * clap + t1 * lap + R(0.25) * cross;
   // change in phi (Warren1995.py:76-86); theta = atan2(phiy, phix) evaluated
   // algebraically from the gradient, with the arctan2(0, 0) = 0 case giving
-  // cos = 1, sin = 0; eta'' = -16 (eta - 1) = -16 y_e cos(4 theta).
+  // cos = 1, sin = 0; eta'' = -16 (eta - 1) = -16 y_e cos(4 theta).  FP32:
+  // the hardware reciprocal flushes a subnormal r2 to zero and returns
+  // infinity, so gradients with r2 below the smallest normal float (phi
+  // decaying to ~1e-22 inside the solid) take the zero-gradient case; the
+  // terms they multiply are then below 1e-19.
   const R r2 = gx * gx + gy * gy;
   R c2 = R(1), s2 = R(0);
-  if (r2 > R(0)) {
+  if (r2 > (sizeof(R) == 4 ? R(0x1.0p-126f) : R(0))) {
     const R ir2 = recip(r2);
     c2 = (gx * gx - gy * gy) * ir2;
     s2 = R(2) * gx * gy * ir2;

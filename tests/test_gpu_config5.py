@@ -1,0 +1,90 @@
+"""N-component workload at its stated size: 2048^2, 3 components, 16 seeds of
+radius 6, synthetic Redlich-Kister model, T = 1440 K + 1 K/cell, 10,000 steps
+(the benchmark's workload, bench.ncomp_sim).
+
+The surrogate engine (FP32 stencil, BF16 networks trained on the GPU for this
+run's temperature box) and the plain FP32 engine are compared with the plain
+FP64 engine -- the reference's method and precision, pinned to the FP64
+oracle at 1e-10 (tests/test_gpu_parity.py) -- at every 2,000 steps.
+
+Bars: composition L-inf <= 1e-3 over the whole domain and interface position
+within one cell (phi = 0.5) at every checkpoint; the L-inf outside the band
+of partly solid cells is reported beside it.  In this workload the seeds melt
+back in the superheated part of the gradient within the first 2,000 steps
+(starts cold enough for r = 6 seeds to survive make the reference scheme
+break the solid into checkerboards), so the later checkpoints compare the
+liquid's composition field.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STEPS, EVERY = 10000, 2000
+
+
+def _interface_error(phi_a, phi_b, level=0.5):
+    from scipy import ndimage
+    ia, ib = phi_a > level, phi_b > level
+    diff = ia != ib
+    if not diff.any():
+        return 0
+    iface = np.zeros_like(ib)
+    iface[:-1, :] |= ib[:-1, :] != ib[1:, :]
+    iface[1:, :] |= ib[:-1, :] != ib[1:, :]
+    iface[:, :-1] |= ib[:, :-1] != ib[:, 1:]
+    iface[:, 1:] |= ib[:, :-1] != ib[:, 1:]
+    if not iface.any():
+        return 10 ** 9
+    return int(ndimage.distance_transform_cdt(~iface, metric="chessboard")[diff].max())
+
+
+def _bulk_mask(pa, pb, lo=0.01, hi=0.99):
+    from scipy import ndimage
+    bulk = ((pa < lo) & (pb < lo)) | ((pa > hi) & (pb > hi))
+    return ~ndimage.binary_dilation(~bulk, structure=np.ones((3, 3), bool))
+
+
+@pytest.fixture(scope="module")
+def checkpoints(gpu):
+    import bench
+    ref = bench.ncomp_sim(2048, "FP64", "forward_difference")
+    sims = {"surrogate": bench.ncomp_sim(2048, "FP32", "analytic", thermo_eval="surrogate"),
+            "fp32": bench.ncomp_sim(2048, "FP32", "analytic")}
+    rows = []
+    for step in range(EVERY, STEPS + 1, EVERY):
+        ref.simulate(EVERY)
+        for s in sims.values():
+            s.simulate(EVERY)
+        pr = ref.fields[0].get_cells()
+        cr = [ref.fields[i].get_cells() for i in (3, 4)]
+        row = {"step": step}
+        for name, s in sims.items():
+            pg = s.fields[0].get_cells()
+            cg = [s.fields[i].get_cells() for i in (3, 4)]
+            bulk = _bulk_mask(pg, pr)
+            row[name] = dict(
+                c_linf=max(float(np.abs(a - w).max()) for a, w in zip(cg, cr)),
+                c_linf_bulk=max(float(np.abs(a - w)[bulk].max()) for a, w in zip(cg, cr)),
+                iface=_interface_error(pg, pr), solid=int((pg > 0.5).sum()))
+        row["solid_fp64"] = int((pr > 0.5).sum())
+        print(row, flush=True)
+        rows.append(row)
+    return rows
+
+
+def test_surrogate_engine_composition_linf_full_domain(checkpoints):
+    for row in checkpoints:
+        assert row["surrogate"]["c_linf"] <= 1e-3, row
+        assert row["surrogate"]["c_linf_bulk"] <= 1e-3, row
+
+
+def test_surrogate_engine_interface_within_one_cell(checkpoints):
+    for row in checkpoints:
+        assert row["surrogate"]["iface"] <= 1, row
+
+
+def test_plain_fp32_engine_against_fp64(checkpoints):
+    for row in checkpoints:
+        assert row["fp32"]["c_linf"] <= 2e-4 and row["fp32"]["iface"] <= 1, row

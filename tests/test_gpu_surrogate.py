@@ -245,20 +245,14 @@ def _nc_oracle(sim, model, f0, steps):
     return run_ncomp(f0, steps, "NEUMANN", T_of, u["params"], u["c_params"], liq, sol, seed=sim.seed)
 
 
-def test_ncomp_surrogate_vs_fp64_oracle(gpu, trained):
-    """Surrogate engine (FP32 stencil + BF16 networks) and the plain FP32
-    engine against the FP64 forward-difference oracle after 200 steps.
-
-    Bars (surrogate): interface within 1 cell; composition L-inf <= 1e-3 in
-    the bulk, i.e. outside the band of cells that are not fully solid or
-    liquid in both runs (dilated by one cell).  Inside that band the interfaces
-    are 1-2 cells sharp and the composition jumps by ~0.2 across them, so the
-    full-domain L-inf measures sub-cell interface position: a constant
-    +1 J/mol offset of G_S - G_L alone moves it by 2e-3 in this case (oracle
-    study), below what a BF16 3-64-64-3 network resolves.  Even the plain
-    FP32 engine differs by ~1e-2 there (measured: surrogate 3.6-3.9e-2 /
-    bulk 2.8-4.1e-4 over the three activation precisions, plain FP32
-    1.1e-2 / bulk 3.3e-5); both figures are printed."""
+@pytest.fixture(scope="module")
+def growth_case(gpu, trained):
+    """The 48^2 growth case (three seeds near the liquidus, T = 1440 K +
+    1 K/cell) run 200 steps by the surrogate engine with three activation
+    precisions and by the plain FP32 engine, each compared with the FP64
+    forward-difference oracle: (full-domain c L-inf, bulk c L-inf, c rel L2,
+    interface error in cells) per variant.  "Bulk" excludes the band of cells
+    that are not fully solid or liquid in both runs, dilated by one cell."""
     steps = 200
     want = None
     results = {}
@@ -286,10 +280,30 @@ def test_ncomp_surrogate_vs_fp64_oracle(gpu, trained):
               % (name, full, bulk, l2, iface, (~interface_band(phi_got, phi_want)).sum()), flush=True)
     trained.split2 = trained.split3 = False
     trained._packed = None
-    for name, split in variants[:3]:
-        full, bulk, l2, iface = results[name]
+    return results
+
+
+SURROGATE_VARIANTS = ("surrogate, hidden inputs as BF16 pairs", "surrogate, layer-3 inputs BF16",
+                      "surrogate, hidden inputs BF16 (default)")
+
+
+@pytest.mark.xfail(strict=False, reason="unmet bar: full-domain composition L-inf of the surrogate engine is "
+                   "3.6-3.9e-2 against 1e-3 in this interface-bearing case (plain FP32: 1.1e-2); the interfaces "
+                   "of the reference scheme are 1-2 cells sharp with a 0.2 composition jump, so the pointwise "
+                   "error measures sub-cell interface position")
+def test_ncomp_surrogate_full_domain_linf(growth_case):
+    for name in SURROGATE_VARIANTS:
+        assert growth_case[name][0] <= 1e-3, (name, growth_case[name])
+
+
+def test_ncomp_surrogate_interface_bulk_and_l2(growth_case):
+    """Interface within one cell; outside the interface band the composition
+    L-inf stays within 1e-3; relative L2 within 2e-2.  Reported beside the
+    full-domain figure, not instead of it."""
+    for name in SURROGATE_VARIANTS:
+        full, bulk, l2, iface = growth_case[name]
         assert iface <= 1, name
         assert bulk <= 1e-3, name
         assert l2 <= 2e-2, name
-    full, bulk, l2, iface = results["fp32"]
+    full, bulk, l2, iface = growth_case["fp32"]
     assert iface <= 1 and bulk <= 1e-4 and l2 <= 5e-3
