@@ -7,9 +7,14 @@ Default workload (``--config warren``): Warren1995 binary solidification,
 c = 0.40 in the solid and 0.41 in the liquid, isothermal 1574 K, 4-fold
 anisotropy 0.04, interface noise amplitude 0.01, Neumann boundaries.  A step
 is one explicit-Euler time step of the whole grid; inputs are resident in HBM
-before the timed region.  Prints ONE JSON line with the metric
-(cell-updates/s), the roofline of the dominant kernel and the CPU baseline
-(the FP64 numpy restatement of the reference engine timed on this host).
+before the timed region; the steps replay a captured HIP graph.  Prints ONE
+JSON line with the metric (cell-updates/s), the roofline of the dominant
+kernel, the CPU baseline (the FP64 numpy restatement of the reference engine
+timed on this host, with its CPU model and thread counts), the achievable
+HBM bandwidth of a 1 GiB device copy measured in the same run, and
+``ncomp_surrogate_same_run``: the N-component 2048^2 workload with the
+surrogate thermodynamics against the plain FP64 and FP32 engines, timed in
+this process (``--no-surrogate`` skips it).
 
 Other configs (one JSON line each): ``diffusion`` (2D 256^2 FP64),
 ``ch3d`` (Cahn-Hilliard 3D 512^3 FP32), ``ncomp`` (N-component 2048^2, add
@@ -121,18 +126,108 @@ def _warm(sim, warmup):
         sim.simulate(chunk)
 
 
+def host_info():
+    """CPU model, logical CPUs and OpenMP threads of this host (the CPU
+    baseline's context)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def copy_peak(steps=20):
+    """Achievable HBM bandwidth on this GPU: a 1 GiB device-to-device copy
+    (read + write bytes), timed like the engines."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    t = _timed(lambda k: [b.copy_(a) for _ in range(k)], steps)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes * steps / t / 1e9
+
+
+# per network 3-64-64-3: (3*64 + 64*64 + 64*3) multiply-adds; two networks per cell
+MLP_FLOP_PER_CELL = 2 * 2 * (3 * 64 + 64 * 64 + 64 * 3)
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def ncomp_same_run(steps=60, warmup=20):
+    """The N-component workload (2048^2, T0 = 1440 K + 1 K/cell) with the
+    surrogate thermodynamics, timed in this process next to the plain FP64
+    engine (forward differences: the reference's method and precision) and
+    the plain FP32 engine (analytic derivatives).  The networks are trained
+    here on 1e6 samples of the plain evaluation (time reported)."""
+    import torch
+    out = {"workload": "N-component 2D 2048x2048, 3 components, 16 seeds r=6, T = %g K + 1 K/cell" % NC_T0}
+    t0 = time.perf_counter()
+    sur = ncomp_sim(2048, "FP32", "analytic", thermo_eval="surrogate")
+    sur.simulate(1)  # uploads, trains the networks, captures nothing yet
+    torch.cuda.synchronize()
+    out["training_seconds"] = time.perf_counter() - t0
+    _warm(sur, warmup)
+    t_sur = _timed(sur.simulate, steps) / steps
+    # cells of the last step handed to the FP64 fallback (outside the trained box)
+    padded = 2050 * 2050
+    marks = sur._surrogate.packed()[1][-1]
+    fallback = int(marks[int(sur.time_step_counter) & 1].item())
+    k = 20
+    t_mlp = _timed(lambda m: [sur.thermo_pass() for _ in range(m)], k) / k
+    tflops = MLP_FLOP_PER_CELL * padded / t_mlp / 1e12
+    out["surrogate"] = {"ms_per_step": t_sur * 1e3, "network_pass_ms": t_mlp * 1e3,
+                        "network_tflops": tflops, "network_frac_of_bf16_dense_peak": tflops / BF16_DENSE_PEAK_TFLOPS,
+                        "fallback_cell_fraction": fallback / padded}
+    del sur
+    torch.cuda.empty_cache()
+    for name, prec, dg in (("plain_fp64_forward_difference", "FP64", "forward_difference"),
+                           ("plain_fp32_analytic", "FP32", "analytic")):
+        sim = ncomp_sim(2048, prec, dg)
+        _warm(sim, warmup)
+        tt = _timed(sim.simulate, steps) / steps
+        out[name] = {"ms_per_step": tt * 1e3, "surrogate_speedup": tt / t_sur}
+        del sim
+        torch.cuda.empty_cache()
+    out["steps"] = steps
+    return out
+
+
 def bench_warren(args):
     n = 2048
     sim = warren_sim(n, "FP32")
+    if args.graph_chunk is not None:
+        sim._GRAPH_CHUNK = args.graph_chunk
     _warm(sim, args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     value = cells * args.steps / t
     roof = _roofline(16.0 * cells, t / args.steps, "k_warren<float>")
-    cpu = None if args.no_cpu else cpu_warren(n)
-    return _line("cell-updates/s", value, "cell-updates/s", args.steps, args.warmup, t, "f32",
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_warren(n)
+        cpu.update(host_info())
+    line = _line("cell-updates/s", value, "cell-updates/s", args.steps, args.warmup, t, "f32",
                  {"workload": "Warren1995 2D 2048x2048 FP32, Neumann, seed r=8, noise 0.01",
                   "engine": "Warren1995", "grid": [n, n], "bytes_per_cell_update": 16}, roof, cpu)
+    del sim
+    line["hbm_copy_peak_GBps"] = copy_peak()
+    roof["frac_of_copy_peak"] = roof["achieved"] / line["hbm_copy_peak_GBps"]
+    if not args.no_surrogate:
+        line["ncomp_surrogate_same_run"] = ncomp_same_run()
+    return line
 
 
 # --------------------------------------------------------------- Diffusion --
@@ -426,8 +521,72 @@ def bench_implicit(args):
                  roof, None)
 
 
+# ---------------------------------------------------------------- ensemble --
+def bench_ensemble(args):
+    """256-instance ensemble: autotune, then time args.steps graph-replayed steps."""
+    from pyphasefield_amd.ensemble import Ensemble, random_initial, random_specs
+    n = args.size or 256
+    specs = random_specs(n, seed=0)
+    ens = Ensemble(specs, random_initial(specs, seed=0))
+    if getattr(args, "ens_config", None) is not None:
+        ens.config = int(args.ens_config)
+        ens.blocks = int(args.ens_blocks or 8 * ens.cus)
+        best = {"config": ens.config, "blocks": ens.blocks, "shape": list(ens.shape(ens.config))}
+    else:
+        best = ens.autotune(steps=20, cache=False)
+    gs = getattr(args, "graph_steps", None)
+    if gs is None:
+        gs = 50 if args.steps % 50 == 0 else 2
+    ens.simulate(args.warmup - args.warmup % 2 if args.warmup > 1 else 0, graph_steps=gs)
+    t = _timed(lambda k: ens.simulate(k, graph_steps=gs), args.steps)
+    roof = _roofline(8.0 * ens.cells, t / args.steps, "k_ensemble")
+    cpu = None if args.no_cpu else cpu_ensemble(specs)
+    nac = sum(s.kind == "AC" for s in specs)
+    return _line("cell-updates/s", ens.cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f32",
+                 {"workload": "batched ensemble: %d 2D instances (%d Allen-Cahn, %d Cahn-Hilliard), sides "
+                              "log-uniform in [64, 1024] per axis, mixed periodic/Neumann, FP32" % (n, nac, n - nac),
+                  "instances": n, "cells": ens.cells, "bytes_per_cell_update": 8,
+                  "tile_config": best["config"], "tile_shape": best["shape"], "blocks": best["blocks"],
+                  "graph_steps": gs}, roof, cpu)
+
+
+def cpu_ensemble(specs, k=2, every=8):
+    """FP64 numpy oracle over every `every`-th instance, k steps each (one
+    warm-up) -- a bounded sample of the ensemble's work."""
+    from oracle import bc as obc
+    from oracle import engines as oe
+    from pyphasefield_amd.ensemble import random_initial
+    fields = random_initial(specs, seed=0)[::every]
+    specs = specs[::every]
+    padded = []
+    for s, f in zip(specs, fields):
+        p = obc.pad(f.astype(np.float64))
+        obc.apply_bc(p[None], list(s.bc), None, s.dx)
+        padded.append(p)
+
+    def one(i, p):
+        s = specs[i]
+        if s.kind == "AC":
+            q = oe.allencahn_step(p, s.M, s.W, s.eps, s.dt, s.dx)
+        else:
+            q = oe.cahnhilliard_step(p, s.M, s.eps, s.dt, s.dx, list(s.bc))
+        obc.apply_bc(q[None], list(s.bc), None, s.dx)
+        return q
+
+    padded = [one(i, p) for i, p in enumerate(padded)]
+    t0 = time.perf_counter()
+    for _ in range(k):
+        padded = [one(i, p) for i, p in enumerate(padded)]
+    dt = time.perf_counter() - t0
+    cells = sum(s.nx * s.ny for s in specs)
+    return {"value": cells * k / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d steps (+1 warm-up) of %d instances (every %dth of the ensemble, %d cells) with the "
+                      "FP64 numpy restatements of engine_CahnAllenExplicit / engine_CahnHilliardExplicit, "
+                      "single-threaded numpy" % (k, len(specs), every, cells)}
+
+
 CONFIGS = {"warren": bench_warren, "diffusion": bench_diffusion, "ch3d": bench_ch3d, "copy": bench_copy,
-           "ncomp": bench_ncomp, "aniso": bench_aniso, "implicit": bench_implicit}
+           "ncomp": bench_ncomp, "aniso": bench_aniso, "implicit": bench_implicit, "ensemble": bench_ensemble}
 
 
 def main():
@@ -435,12 +594,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--config", default="warren", choices=sorted(CONFIGS) + ["ensemble"])
+    ap.add_argument("--config", default="warren", choices=sorted(CONFIGS))
     ap.add_argument("--variant", default="fp32")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--nc-kernel", default=None, choices=["fused", "two_pass"],
                     help="ncomp: one-launch step or the helper + step pair")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph-chunk", type=int, default=None,
+                    help="warren: steps per captured HIP graph (0: eager launches)")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="warren: skip the same-run surrogate / plain N-component comparison")
     ap.add_argument("--ens-config", type=int, default=None,
                     help="ensemble: tile configuration (skips the autotuner)")
     ap.add_argument("--ens-blocks", type=int, default=None, help="ensemble: persistent grid size")
@@ -461,11 +624,7 @@ def main():
     st, wu = defaults[args.config]
     args.steps = args.steps if args.steps is not None else st
     args.warmup = args.warmup if args.warmup is not None else wu
-    if args.config == "ensemble":
-        from pyphasefield_amd.ensemble import bench_ensemble
-        line = bench_ensemble(args, _timed, _line, _roofline)
-    else:
-        line = CONFIGS[args.config](args)
+    line = CONFIGS[args.config](args)
     print(json.dumps(line), flush=True)
 
 

@@ -380,64 +380,3 @@ class Ensemble:
         for sim, f in zip(self.sims, self.fields()):
             sim.fields[0].data[1:-1, 1:-1] = f
             sim.time_step_counter = getattr(sim, "time_step_counter", 0) + self.steps_done
-
-
-def bench_ensemble(args, _timed, _line, _roofline):
-    """256-instance ensemble: autotune, then time args.steps graph-replayed steps."""
-    n = args.size or 256
-    specs = random_specs(n, seed=0)
-    ens = Ensemble(specs, random_initial(specs, seed=0))
-    if getattr(args, "ens_config", None) is not None:
-        ens.config = int(args.ens_config)
-        ens.blocks = int(args.ens_blocks or 8 * ens.cus)
-        best = {"config": ens.config, "blocks": ens.blocks, "shape": list(ens.shape(ens.config))}
-    else:
-        best = ens.autotune(steps=20, cache=False)
-    gs = getattr(args, "graph_steps", None)
-    if gs is None:
-        gs = 50 if args.steps % 50 == 0 else 2
-    ens.simulate(args.warmup - args.warmup % 2 if args.warmup > 1 else 0, graph_steps=gs)
-    t = _timed(lambda k: ens.simulate(k, graph_steps=gs), args.steps)
-    roof = _roofline(8.0 * ens.cells, t / args.steps, "k_ensemble")
-    cpu = None if args.no_cpu else cpu_ensemble(specs)
-    nac = sum(s.kind == "AC" for s in specs)
-    return _line("cell-updates/s", ens.cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f32",
-                 {"workload": "batched ensemble: %d 2D instances (%d Allen-Cahn, %d Cahn-Hilliard), sides "
-                              "log-uniform in [64, 1024] per axis, mixed periodic/Neumann, FP32" % (n, nac, n - nac),
-                  "instances": n, "cells": ens.cells, "bytes_per_cell_update": 8,
-                  "tile_config": best["config"], "tile_shape": best["shape"], "blocks": best["blocks"],
-                  "graph_steps": gs}, roof, cpu)
-
-
-def cpu_ensemble(specs, k=2, every=8):
-    """FP64 numpy oracle over every `every`-th instance, k steps each (one
-    warm-up) -- a bounded sample of the ensemble's work."""
-    from oracle import bc as obc
-    from oracle import engines as oe
-    fields = random_initial(specs, seed=0)[::every]
-    specs = specs[::every]
-    padded = []
-    for s, f in zip(specs, fields):
-        p = obc.pad(f.astype(np.float64))
-        obc.apply_bc(p[None], list(s.bc), None, s.dx)
-        padded.append(p)
-
-    def one(i, p):
-        s = specs[i]
-        if s.kind == "AC":
-            q = oe.allencahn_step(p, s.M, s.W, s.eps, s.dt, s.dx)
-        else:
-            q = oe.cahnhilliard_step(p, s.M, s.eps, s.dt, s.dx, list(s.bc))
-        obc.apply_bc(q[None], list(s.bc), None, s.dx)
-        return q
-
-    padded = [one(i, p) for i, p in enumerate(padded)]
-    t0 = time.perf_counter()
-    for _ in range(k):
-        padded = [one(i, p) for i, p in enumerate(padded)]
-    dt = time.perf_counter() - t0
-    cells = sum(s.nx * s.ny for s in specs)
-    return {"value": cells * k / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "%d steps (+1 warm-up) of %d instances (every %dth of the ensemble, %d cells) with the "
-                      "FP64 numpy restatements of engine_CahnAllenExplicit / engine_CahnHilliardExplicit, "
-                      "single-threaded numpy" % (k, len(specs), every, cells)}

@@ -253,21 +253,28 @@ class Simulation:
             # set from the host counter before the replays.
             graph = self._cached_graph(chunk)
             reps = n // chunk
-            if self._DEVICE_STEP:
+            if self._DEVICE_STEP and getattr(self, "_step_counter_mirror", None) != self.time_step_counter:
+                # the device counter is stale after eager steps or a reset of
+                # the host counter; replays keep it equal to the host's
                 self._step_counter_device().fill_(int(self.time_step_counter))
             for _ in range(reps):
                 graph.replay()
             self.time_step_counter += reps * chunk
+            if self._DEVICE_STEP:
+                self._step_counter_mirror = self.time_step_counter
             n -= reps * chunk
         for _ in range(n):
-            self.time_step_counter += 1
-            self.simulation_loop()
-            self.update_temperature_field()
-            if self._autosave_flag and self._autosave_rate:
-                if self.time_step_counter % self._autosave_rate == 0:
-                    self.retrieve_fields_from_GPU()
-                    self.save_simulation()
-                    self._host_dirty = False
+            self._eager_step()
+
+    def _eager_step(self):
+        self.time_step_counter += 1
+        self.simulation_loop()
+        self.update_temperature_field()
+        if self._autosave_flag and self._autosave_rate:
+            if self.time_step_counter % self._autosave_rate == 0:
+                self.retrieve_fields_from_GPU()
+                self.save_simulation()
+                self._host_dirty = False
 
     # steps per captured HIP graph (0: always eager)
     _GRAPH_CHUNK = 0

@@ -354,6 +354,23 @@ def test_warren_fp32_vs_fp64_oracle(gpu):
     assert abs(int(a) - int(b)) <= 1
 
 
+def test_warren_fp32_subnormal_gradients_stay_finite(gpu):
+    """Inside the solid phi decays towards 0; FP32 gradients whose squared
+    norm is subnormal take the zero-gradient anisotropy case (the hardware
+    reciprocal would return infinity and the step NaN).  A solid disc with
+    phi between 1e-25 and 1e-20 steps finite and matches the FP64 engine."""
+    E = _engines()
+    sims = [_warren_sim(E, 64, p, 0.0) for p in ("FP32", "FP64")]
+    tiny = 10.0 ** np.random.default_rng(2).uniform(-25, -20, (64, 64))
+    for s in sims:
+        inner = s.fields[0].data[1:-1, 1:-1]
+        s.fields[0].data[1:-1, 1:-1] = np.where(inner < 0.5, tiny, inner)
+        s.simulate(30)
+    p32, p64 = sims[0].fields[0].get_cells(), sims[1].fields[0].get_cells()
+    assert np.all(np.isfinite(p32)) and np.all(np.isfinite(sims[0].fields[1].get_cells()))
+    assert np.max(np.abs(p32 - p64)) <= 1e-3
+
+
 # -------------------------------------------------------------- N-component --
 def _nc_sim(E, ny, nx, precision, noise, bc="NEUMANN", temp="LINEAR_GRADIENT", dg=None, seeds=3, t_path=None):
     from pyphasefield_amd import thermo
