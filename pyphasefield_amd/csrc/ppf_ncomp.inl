@@ -495,7 +495,7 @@ struct Nb5 {
 // occupancy hint for the step (waves per SIMD the register allocation must
 // allow; 0 = compiler's choice) -- chosen by sweep
 #ifndef PPF_NC_STEP_WAVES
-#define PPF_NC_STEP_WAVES 5
+#define PPF_NC_STEP_WAVES 4
 #endif
 #if PPF_NC_STEP_WAVES
 #define PPF_NC_STEP_ATTR __attribute__((amdgpu_waves_per_eu(PPF_NC_STEP_WAVES, 8)))

@@ -333,11 +333,14 @@ def test_warren_fp64_boundaries(gpu, bc):
         assert rel_l2(got, want[f]) <= FP64_TOL
 
 
-def test_warren_fp32_vs_fp64_oracle(gpu):
-    """FP32 Warren: phi and c within stated tolerances, interface within 1 cell."""
+@pytest.mark.parametrize("n", [64, 200])
+def test_warren_fp32_vs_fp64_oracle(gpu, n):
+    """FP32 Warren: phi and c within stated tolerances, interface within 1 cell.
+    200^2 has tiles whose halo lies inside the domain (the FP32 kernel streams
+    those into LDS ahead of their turn) as well as boundary tiles."""
     from pyphasefield_amd.Engines.Warren1995 import warren_derived
     E = _engines()
-    sim = _warren_sim(E, 64, "FP32", 0.01)
+    sim = _warren_sim(E, n, "FP32", 0.01)
     phi0 = sim.fields[0].get_cells().copy()
     c0 = sim.fields[1].get_cells().copy()
     sim.simulate(400)
@@ -348,7 +351,7 @@ def test_warren_fp32_vs_fp64_oracle(gpu):
     assert np.max(np.abs(gphi - want[0][1:-1, 1:-1])) <= 1e-3
     assert np.max(np.abs(gc - want[1][1:-1, 1:-1])) <= 1e-4
     # interface (phi = 0.5 contour) position along the centre row within 1 cell
-    row = 32
+    row = n // 2
     a = np.argmax(gphi[row] < 0.5)
     b = np.argmax(want[0][1 + row, 1:-1] < 0.5)
     assert abs(int(a) - int(b)) <= 1
