@@ -381,6 +381,23 @@ int ppf_allencahn_cg_begin(const ppf_grid* g, const void* in, void* out, double 
 int ppf_allencahn_cg_iterate(const ppf_grid* g, const void* in, void* out, double a, double src_k, double tol,
                              int64_t first_iteration, int iterations, void* work, ppf_stream_t s);
 
+/* Cahn-Hilliard semi-implicit 1D (engine_CahnHilliardImplicit1D and its
+   GMRES and Crank-Nicolson variants, Engines/CahnHilliard.py:148-222, matrix
+   of implicit_matrix_1d, :20-70): solves, on a periodic 1D grid of n >= 5
+   cells, the pentadiagonal system whose row i is
+     (1 + alpha (2 s_i + 6 e)) x_i - alpha (s_i + 4 e)(x_{i-1} + x_{i+1})
+       + alpha e (x_{i-2} + x_{i+2}) = c_i,   s_i = 4 c_i^2 - 6 c_i + 2,
+   c = field 0 of `in`, x = field 0 of `out` (interior cells only; indices
+   mod n), alpha = M dt / dx^2, e = eps^2 / dx^2.  One work group: banded
+   elimination of the non-periodic band, then a rank-4 Sherman-Morrison-
+   Woodbury correction for the corner entries, in FP64 for both storage
+   types.  Needs ppf_cahnhilliard_line_work_bytes(g) bytes of device scratch.
+   Crank-Nicolson: ppf_cahnhilliard_step with dt/2, then this solve with
+   alpha = M (dt/2) / dx^2. */
+int64_t ppf_cahnhilliard_line_work_bytes(const ppf_grid* g);
+int ppf_cahnhilliard_line_solve(const ppf_grid* g, const void* in, void* out, double alpha, double e, void* work,
+                                ppf_stream_t s);
+
 /* Plain thermodynamics at scattered points: for each of n points with
    temperature T[i] and component fractions c[i*(N-1)+l] (c_N implied),
    write for each phase (liquid then solid) G and dG/dc_l:

@@ -212,3 +212,60 @@ def ac_step(phi, a, K, solver, adi=False, imex_K=None):
         out = explicit_axis(out, a, 0)
         return _solve_lines_var(out, a, K / 2, 0)
     raise ValueError("unknown solver")
+
+
+# ---------------------------------------------------------------------------
+# Cahn-Hilliard semi-implicit 1D (Engines/CahnHilliard.py:20-70, 148-222)
+# ---------------------------------------------------------------------------
+
+def ch_s(c):
+    """Frozen factor of the linearised chemical potential, f'(c) = c s(c)."""
+    return 4 * c ** 2 - 6 * c + 2
+
+
+def ch_matrix_1d(c, alpha, e2idx2):
+    """The periodic pentadiagonal system implicit_matrix_1d builds for
+    engine_CahnHilliardImplicit1D (CahnHilliard.py:20-70, 148-160): after its
+    sequence of fill_diagonal / roll operations, row i holds its own
+    coefficients
+
+      (i, i)       1 + alpha (2 s_i + 6 e)
+      (i, i -+ 1)  -alpha (s_i + 4 e)
+      (i, i -+ 2)  alpha e            (indices mod n; e = eps^2 / dx^2)
+
+    with s_i = s(c_i) and alpha = M dt / dx^2: the row of
+    c - dt M [s_i lap c - eps^2 lap^2 c] at cell i."""
+    n = len(c)
+    if n < 5:
+        raise ValueError("the periodic pentadiagonal system needs n >= 5")
+    s = ch_s(c)
+    A = np.zeros((n, n))
+    i = np.arange(n)
+    A[i, i] = 1 + alpha * (2 * s + 6 * e2idx2)
+    for d in (1, -1):
+        A[i, (i + d) % n] = -alpha * (s + 4 * e2idx2)
+    for d in (2, -2):
+        A[i, (i + d) % n] = alpha * e2idx2
+    return A
+
+
+def ch_step_1d(c, M, epsilon, dt, dx, solver):
+    """One step of engine_CahnHilliardImplicit1D ("implicit") or
+    engine_CahnHilliardCrankNicolson1D ("crank-nicolson"), CahnHilliard.py:
+    148-200, on the interior cells of a periodic line; the GMRES variants
+    (:162-174, :202-222) solve the same systems.  Crank-Nicolson first takes
+    the explicit half step c += (dt/2) M lap(mu) (the reference omits dt
+    there, defect D3) and then solves with dt/2 and s from the updated c."""
+    from . import engines as _oe
+    from . import bc as _obc
+    c = np.asarray(c, dtype=np.float64)
+    h = dt
+    if solver == "crank-nicolson":
+        h = dt / 2
+        p = _obc.pad(c)
+        _obc.apply_bc(p[None], "PERIODIC", None, dx)
+        c = _oe.cahnhilliard_step(p, M, epsilon, h, dx, "PERIODIC")[1:-1]
+    elif solver != "implicit":
+        raise ValueError("solver must be 'implicit' or 'crank-nicolson'")
+    A = ch_matrix_1d(c, M * h / dx ** 2, epsilon ** 2 / dx ** 2)
+    return np.linalg.solve(A, c)

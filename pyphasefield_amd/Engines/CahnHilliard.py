@@ -1,14 +1,19 @@
-"""Cahn-Hilliard engine (Engines/CahnHilliard.py:347-421 of the reference),
-explicit solver on the GPU:
+"""Cahn-Hilliard engine (Engines/CahnHilliard.py:347-421 of the reference)
+on the GPU.
+
+user_data["solver"] = "explicit":
 mu = -eps^2 lap(phi) + (4phi^3 - 6phi^2 + 2phi);  phi <- phi + dt*M*lap(mu)
 (CahnHilliard.py:134-146 with the D3 resolution: dt applied, mu on the +1
 ring from a consistent 2-cell halo -- periodic wrap, or zero flux on
-non-periodic sides, which conserves mass)."""
+non-periodic sides, which conserves mass).
+"implicit" / "imex" / "crank-nicolson" (with "gmres" and "adi"): the
+semi-implicit schemes of CahnHilliard.py:148-341 on periodic boxes, see
+``_implicit.cahnhilliard_step``."""
 
 import numpy as np
 
 from ..simulation import Simulation
-from . import _gpu
+from . import _gpu, _implicit
 
 
 class CahnHilliard(Simulation):
@@ -34,8 +39,15 @@ class CahnHilliard(Simulation):
         u = self.user_data
         return (float(u["M"]), float(u["epsilon"]), float(self.dt))
 
+    def _graph_chunk(self):
+        if self.user_data.get("solver", "explicit") != "explicit":
+            return 0
+        return super()._graph_chunk()
+
     def simulation_loop(self):
-        _gpu.solver_check(self)
+        if self.user_data.get("solver", "explicit") != "explicit":
+            _implicit.cahnhilliard_step(self)
+            return
         u = self.user_data
         _gpu.stencil_step(self, "ppf_cahnhilliard_step", float(u["M"]), float(u["epsilon"]), float(self.dt),
                           nactive=1)

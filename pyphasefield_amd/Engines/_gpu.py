@@ -21,9 +21,3 @@ def carry_passive(sim, nactive):
     F = sim._fields_gpu_device.shape[0]
     if nactive is not None and F > nactive:
         sim._fields_out_gpu_device[nactive:].copy_(sim._fields_gpu_device[nactive:])
-
-
-def solver_check(sim):
-    solver = sim.user_data.get("solver", "explicit")
-    if solver != "explicit":
-        raise NotImplementedError("only the explicit solver runs on this engine path (got %r)" % (solver,))

@@ -174,7 +174,7 @@ _AC_PLANS = {
 }
 
 
-def allencahn_step(sim):
+def allencahn_step(sim, where="CahnAllen.py:320-362"):
     """One semi-implicit Allen-Cahn step with the reference's dispatch
     (CahnAllen.py:320-362): implicit 1D, 2D (gmres and not adi: full system;
     otherwise alternating directions), 3D (alternating directions); imex 1D;
@@ -222,8 +222,51 @@ def allencahn_step(sim):
         for axis, e in plan:
             line(axis, e, aa, aa if e >= 0 else 0.0, K * frac, 0.0)
     else:
-        raise NotImplementedError("the reference's Allen-Cahn engine has no working %r solver for %dD "
-                                  "(CahnAllen.py:320-362)" % (solver, nd))
+        raise NotImplementedError("the reference's engine has no working %r solver for %dD (%s)"
+                                  % (solver, nd, where))
+
+
+def cahnhilliard_step(sim):
+    """One semi-implicit Cahn-Hilliard step with the reference's dispatch
+    (CahnHilliard.py:382-421).
+
+    * implicit 1D (with or without gmres, :148-174): the periodic
+      pentadiagonal system of implicit_matrix_1d (:20-70), alpha = M dt/dx^2,
+      e = eps^2/dx^2, s from the current field;
+    * crank-nicolson 1D (:176-222): the explicit step with dt/2 (the
+      reference omits dt there), then the same solve with dt/2 and s from the
+      updated field;
+    * implicit 2D/3D, imex 1D, crank-nicolson 2D (:224-341): the bodies of
+      these functions are those of the Allen-Cahn engine's (CahnAllen.py:
+      165-280), built with a three-argument implicit_matrix_1d /
+      implicit_matrix_2d that this module's four- and five-argument builders
+      reject; they run here as written, i.e. as the Allen-Cahn schemes with
+      this engine's M, W, epsilon (``allencahn_step``);
+    * imex 2D/3D and crank-nicolson 3D print a message and do nothing in the
+      reference; they raise here."""
+    u = sim.user_data
+    solver = u.get("solver", "explicit")
+    if len(sim.dimensions) != 1 or solver not in ("implicit", "crank-nicolson"):
+        allencahn_step(sim, where="CahnHilliard.py:382-421")
+        return
+    _check(sim)
+    if sim.dimensions[0] < 5:
+        raise ValueError("the periodic pentadiagonal Cahn-Hilliard system needs at least 5 cells")
+    from . import _gpu
+    M, eps = float(u["M"]), float(u["epsilon"])
+    dt, dx = float(sim.dt), float(sim.dx)
+    h = dt
+    if solver == "crank-nicolson":
+        h = 0.5 * dt
+        _gpu.stencil_step(sim, "ppf_cahnhilliard_step", M, eps, h, nactive=1)
+    L = _lib.lib()
+    work = _work(sim, "_ch_line_work", int(L.ppf_cahnhilliard_line_work_bytes(sim._grid)))
+    rc = L.ppf_cahnhilliard_line_solve(sim._grid, _lib.ptr(sim._fields_gpu_device),
+                                       _lib.ptr(sim._fields_out_gpu_device), M * h / (dx * dx),
+                                       eps * eps / (dx * dx), _lib.ptr(work), _lib.stream_handle())
+    _lib.check(rc, "cahnhilliard_line_solve")
+    _passive(sim)
+    sim._swap_buffers()
 
 
 def step(sim):

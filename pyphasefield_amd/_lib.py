@@ -139,6 +139,8 @@ def lib():
         "ppf_allencahn_cg_begin": ([P(Grid), c_vp, c_vp, c_f64, c_f64, c_vp, c_vp], c_i32),
         "ppf_allencahn_cg_iterate": ([P(Grid), c_vp, c_vp, c_f64, c_f64, c_f64, c_i64, ctypes.c_int, c_vp, c_vp],
                                      c_i32),
+        "ppf_cahnhilliard_line_work_bytes": ([P(Grid)], c_i64),
+        "ppf_cahnhilliard_line_solve": ([P(Grid), c_vp, c_vp, c_f64, c_f64, c_vp, c_vp], c_i32),
         "ppf_diffusion_cg_begin": ([P(Grid), c_vp, c_vp, c_f64, c_f64, c_vp, c_vp], c_i32),
         "ppf_diffusion_cg_iterate": ([P(Grid), c_vp, c_f64, c_f64, c_i64, ctypes.c_int, c_vp, c_vp], c_i32),
     }

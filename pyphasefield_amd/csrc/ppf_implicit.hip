@@ -663,6 +663,144 @@ int cg_iterate(const ppf_grid& g, const void* in, void* out, double a, double sr
   return 0;
 }
 
+// ---- Cahn-Hilliard semi-implicit 1D: the periodic pentadiagonal system
+// whose row i is (1 + a (2 s_i + 6 e)) x_i - a (s_i + 4 e)(x_{i-1} + x_{i+1})
+// + a e (x_{i-2} + x_{i+2}) = r_i, s_i = 4 r_i^2 - 6 r_i + 2.  Split
+// A = B + U V^T: B the same rows without the six corner entries (a plain
+// band of width 2, eliminated without pivoting), U = [e_0, e_1, e_{n-2},
+// e_{n-1}]; then x = z - Y (I + V^T Y)^-1 V^T z with B z = r and B Y = U
+// (Sherman-Morrison-Woodbury).  Thread 0 factors B; threads 0..4 substitute
+// the five right-hand sides; thread 0 solves the 4 x 4 system; all threads
+// correct.  Arithmetic in FP64 for both storage types.
+struct ChPenta {
+  double *u0, *u1, *u2, *l1, *l2, *z, *y;  // n each; y holds 4 columns
+};
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_ch_penta(const R* __restrict__ in, R* __restrict__ out, int64_t n, double a,
+                                                 double e, ChPenta w) {
+  const int tid = threadIdx.x;
+  auto s_of = [&](int64_t i) {
+    const double c = double(in[1 + i]);
+    return 4.0 * c * c - 6.0 * c + 2.0;
+  };
+  __shared__ double wsol[4];
+  if (tid == 0) {
+    // banded elimination of B without pivoting (row i: lower entries at
+    // i-2, i-1 when >= 0, upper entries at i+1, i+2 when < n); the two
+    // previous pivot rows stay in registers
+    double p2[3] = {1.0, 0.0, 0.0}, p1[3] = {1.0, 0.0, 0.0};
+    const double far = a * e;
+    for (int64_t i = 0; i < n; ++i) {
+      const double si = s_of(i);
+      const double nb = -a * (si + 4.0 * e);
+      double em1 = i >= 1 ? nb : 0.0;
+      double e0 = 1.0 + a * (2.0 * si + 6.0 * e);
+      double e1 = i + 1 < n ? nb : 0.0;
+      const double e2 = i + 2 < n ? far : 0.0;
+      double l2 = 0.0, l1 = 0.0;
+      if (i >= 2) {
+        l2 = far / p2[0];
+        em1 -= l2 * p2[1];
+        e0 -= l2 * p2[2];
+      }
+      if (i >= 1) {
+        l1 = em1 / p1[0];
+        e0 -= l1 * p1[1];
+        e1 -= l1 * p1[2];
+      }
+      w.u0[i] = e0;
+      w.u1[i] = e1;
+      w.u2[i] = e2;
+      w.l1[i] = l1;
+      w.l2[i] = l2;
+      for (int k = 0; k < 3; ++k) p2[k] = p1[k];
+      p1[0] = e0;
+      p1[1] = e1;
+      p1[2] = e2;
+    }
+  }
+  __syncthreads();
+  if (tid < 5) {
+    // right-hand side tid: 0 = the field, 1..4 = e_0, e_1, e_{n-2}, e_{n-1}
+    double* x = tid == 0 ? w.z : w.y + (tid - 1) * n;
+    const int64_t unit = tid == 1 ? 0 : tid == 2 ? 1 : tid == 3 ? n - 2 : n - 1;
+    double ym2 = 0.0, ym1 = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double r = tid == 0 ? double(in[1 + i]) : (i == unit ? 1.0 : 0.0);
+      const double yi = r - w.l2[i] * ym2 - w.l1[i] * ym1;
+      x[i] = yi;
+      ym2 = ym1;
+      ym1 = yi;
+    }
+    double xp1 = 0.0, xp2 = 0.0;
+    for (int64_t i = n - 1; i >= 0; --i) {
+      const double xi = (x[i] - w.u1[i] * xp1 - w.u2[i] * xp2) / w.u0[i];
+      x[i] = xi;
+      xp2 = xp1;
+      xp1 = xi;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // V^T applied to a vector v: the corner entries of rows 0, 1, n-2, n-1
+    const double s0 = s_of(0), s1 = s_of(n - 1);
+    const double far = a * e, nb0 = -a * (s0 + 4.0 * e), nbn = -a * (s1 + 4.0 * e);
+    auto vt = [&](const double* v, double (&o)[4]) {
+      o[0] = far * v[n - 2] + nb0 * v[n - 1];
+      o[1] = far * v[n - 1];
+      o[2] = far * v[0];
+      o[3] = nbn * v[0] + far * v[1];
+    };
+    double m[4][5];
+    for (int j = 0; j < 4; ++j) {
+      double col[4];
+      vt(w.y + j * n, col);
+      for (int r = 0; r < 4; ++r) m[r][j] = col[r] + (r == j ? 1.0 : 0.0);
+    }
+    double rhs[4];
+    vt(w.z, rhs);
+    for (int r = 0; r < 4; ++r) m[r][4] = rhs[r];
+    // Gaussian elimination with partial pivoting on the 4 x 5 system
+    for (int k = 0; k < 4; ++k) {
+      int p = k;
+      for (int r = k + 1; r < 4; ++r)
+        if (fabs(m[r][k]) > fabs(m[p][k])) p = r;
+      if (p != k)
+        for (int c = 0; c < 5; ++c) {
+          const double t = m[k][c];
+          m[k][c] = m[p][c];
+          m[p][c] = t;
+        }
+      for (int r = k + 1; r < 4; ++r) {
+        const double f = m[r][k] / m[k][k];
+        for (int c = k; c < 5; ++c) m[r][c] -= f * m[k][c];
+      }
+    }
+    for (int k = 3; k >= 0; --k) {
+      double v = m[k][4];
+      for (int c = k + 1; c < 4; ++c) v -= m[k][c] * wsol[c];
+      wsol[k] = v / m[k][k];
+    }
+  }
+  __syncthreads();
+  for (int64_t i = tid; i < n; i += blockDim.x) {
+    double v = w.z[i];
+    for (int j = 0; j < 4; ++j) v -= w.y[j * n + i] * wsol[j];
+    out[1 + i] = R(v);
+  }
+}
+
+template <typename R>
+int ch_penta(const ppf_grid& g, const void* in, void* out, double a, double e, void* work, hipStream_t s) {
+  const int64_t n = g.n[0];
+  double* b = static_cast<double*>(work);
+  ChPenta w{b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n};
+  hipLaunchKernelGGL(k_ch_penta<R>, dim3(1), dim3(64), 0, s, static_cast<const R*>(in), static_cast<R*>(out), n, a,
+                     e, w);
+  return check_launch("cahnhilliard_line_solve");
+}
+
 }  // namespace
 }  // namespace ppf
 
@@ -838,6 +976,22 @@ int ppf_allencahn_cg_iterate(const ppf_grid* g, const void* in, void* out, doubl
              ? cg_iterate<float, true>(*g, in, out, a, src_k, tol, first_iteration, iterations, work, stream_of(s))
              : cg_iterate<double, true>(*g, in, out, a, src_k, tol, first_iteration, iterations, work,
                                         stream_of(s));
+}
+
+int64_t ppf_cahnhilliard_line_work_bytes(const ppf_grid* g) {
+  if (check_grid(g, 1, 1, "cahnhilliard_line_work_bytes")) return -1;
+  return 10 * g->n[0] * int64_t(sizeof(double));
+}
+
+int ppf_cahnhilliard_line_solve(const ppf_grid* g, const void* in, void* out, double alpha, double e, void* work,
+                                ppf_stream_t s) {
+  if (int rc = check_periodic(g, "cahnhilliard_line_solve")) return rc;
+  if (g->ndim != 1) return fail(PPF_EINVAL, "cahnhilliard_line_solve: the pentadiagonal system is 1D");
+  if (g->n[0] < 5) return fail(PPF_EINVAL, "cahnhilliard_line_solve: needs at least 5 cells");
+  if (!in || !out || !work) return fail(PPF_EINVAL, "cahnhilliard_line_solve: null pointer");
+  if (in == out) return fail(PPF_EINVAL, "cahnhilliard_line_solve: in and out must differ");
+  return g->dtype == PPF_F32 ? ch_penta<float>(*g, in, out, alpha, e, work, stream_of(s))
+                             : ch_penta<double>(*g, in, out, alpha, e, work, stream_of(s));
 }
 
 }  // extern "C"
