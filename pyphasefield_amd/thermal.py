@@ -10,10 +10,11 @@ time (``simulation.py:157-177,316-330``; on the GPU
 meshio is not available here, so this module reads and writes the XDMF
 layout itself: a temporal collection of grids, each with a ``Time`` value and
 a node attribute ``T`` whose ``DataItem`` holds the values inline
-(``Format="XML"``) or in a raw file next to the ``.xdmf`` (``Format="Binary"``,
-with ``Endian`` and ``Seek``).  HDF5 heavy data (meshio's default) needs an
-HDF5 reader, which this environment does not have; such files are rejected
-with an explanation.
+(``Format="XML"``), in a raw file next to the ``.xdmf`` (``Format="Binary"``,
+with ``Endian`` and ``Seek``) or in a dataset of an HDF5 file next to it
+(``Format="HDF"``, text ``file.h5:/path``: meshio's default, which the
+reference's ``CSVtoXDMF`` writes), read by the package's own HDF5 reader
+(``pyphasefield_amd.hdf5``).
 """
 
 import os
@@ -31,8 +32,24 @@ def _local(tag):
     return tag.rsplit("}", 1)[-1]
 
 
-def _data_item(item, base_dir):
-    """Values of one XDMF DataItem (XML inline or raw Binary)."""
+def _hdf_item(text, base_dir, files=None):
+    """Values of a Format="HDF" DataItem, text "file.h5:/path/to/dataset"."""
+    from . import hdf5
+    if ":" not in text:
+        raise ValueError("XDMF HDF DataItem %r does not name file:/dataset" % text)
+    fname, dset = text.rsplit(":", 1)
+    path = fname if os.path.isabs(fname) else os.path.join(base_dir, fname)
+    if files is None:
+        return hdf5.read_dataset(path, dset)
+    f = files.get(path)
+    if f is None:
+        f = files[path] = hdf5.File(path)
+    return f.read(dset)
+
+
+def _data_item(item, base_dir, files=None):
+    """Values of one XDMF DataItem (XML inline, raw Binary or HDF5); ``files``
+    keeps HDF5 files open between calls."""
     fmt = item.get("Format", "XML")
     kind = item.get("DataType", item.get("NumberType", "Float"))
     prec = int(item.get("Precision", "4" if kind == "Float" else "4"))
@@ -56,9 +73,9 @@ def _data_item(item, base_dir):
             f.seek(int(item.get("Seek", "0")))
             values = np.fromfile(f, dtype=dt, count=count).astype(dtype)
     elif fmt == "HDF":
-        raise NotImplementedError(
-            "XDMF heavy data in HDF5 (%s) needs an HDF5 reader, which is not available here; write the "
-            "series with data_format='XML' or 'Binary' (CSVtoXDMF / write_xdmf_series)" % text)
+        values = _hdf_item(text, base_dir, files)
+        if values.dtype != np.dtype(dtype):
+            values = values.astype(dtype)
     else:
         raise ValueError("XDMF DataItem: unknown Format %r" % fmt)
     if dims and values.size != int(np.prod(dims)):
@@ -87,12 +104,7 @@ def _series_index(path, name="T"):
         if not items:
             raise ValueError("XDMF attribute %s at t=%g has no DataItem" % (name, time))
         fmt = items[0].get("Format", "XML")
-        if fmt == "HDF":
-            raise NotImplementedError(
-                "XDMF heavy data in HDF5 (%s) needs an HDF5 reader, which is not available here; write the "
-                "series with data_format='XML' or 'Binary' (CSVtoXDMF / write_xdmf_series)"
-                % (items[0].text or "").strip())
-        if fmt not in ("XML", "Binary"):
+        if fmt not in ("XML", "Binary", "HDF"):
             raise ValueError("XDMF DataItem: unknown Format %r" % fmt)
         index.append((time, items[0]))
     if not index:
@@ -107,12 +119,16 @@ def read_xdmf_series(path, name="T"):
 
 
 def write_xdmf_series(path, times, arrays, name="T", data_format="XML"):
-    """Write a temporal collection of node attribute ``name``; Binary data go
-    to ``<stem>_<name><k>.bin`` (little endian) next to ``path``."""
-    if data_format not in ("XML", "Binary"):
-        raise ValueError("data_format must be 'XML' or 'Binary'")
+    """Write a temporal collection of node attribute ``name``.  Binary data
+    go to ``<stem>_<name><k>.bin`` (little endian) next to ``path``; HDF data
+    to datasets ``data<k>`` of ``<stem>.h5`` next to it, named
+    ``<stem>.h5:/data<k>`` in the items, as meshio's TimeSeriesWriter names
+    them."""
+    if data_format not in ("XML", "Binary", "HDF"):
+        raise ValueError("data_format must be 'XML', 'Binary' or 'HDF'")
     stem = os.path.splitext(os.path.basename(path))[0]
     base = os.path.dirname(os.path.abspath(path))
+    heavy = {}
     xdmf = ET.Element("Xdmf", Version="3.0")
     domain = ET.SubElement(xdmf, "Domain")
     coll = ET.SubElement(domain, "Grid", Name="TimeSeries_" + name, GridType="Collection",
@@ -126,11 +142,17 @@ def write_xdmf_series(path, times, arrays, name="T", data_format="XML"):
                              Dimensions=" ".join(str(n) for n in a.shape), Format=data_format)
         if data_format == "XML":
             item.text = " ".join(repr(float(v)) for v in a.ravel())
+        elif data_format == "HDF":
+            heavy["data%d" % k] = a
+            item.text = "%s.h5:/data%d" % (stem, k)
         else:
             fname = "%s_%s%d.bin" % (stem, name, k)
             a.astype("<f8").tofile(os.path.join(base, fname))
             item.set("Endian", "Little")
             item.text = fname
+    if data_format == "HDF":
+        from . import hdf5
+        hdf5.write_datasets(os.path.join(base, stem + ".h5"), heavy)
     ET.ElementTree(xdmf).write(path, xml_declaration=True, encoding="utf-8")
 
 
@@ -145,14 +167,26 @@ class XDMFLoader:
         # for (binary heavy data from disk), as the reference re-reads its file
         self._index = _series_index(t_file_path)
         self.num_steps = len(self._index)
+        self._files = {}
 
     def data(self, step):
         t, item = self._index[step]
-        return t, np.squeeze(_data_item(item, self._base))
+        return t, np.squeeze(_data_item(item, self._base, self._files))
+
+    def close(self):
+        for f in self._files.values():
+            f.close()
+        self._files = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def CSVtoXDMF(csv_path, T_cutoffs=False, starting_T=None, ending_T=None, reflect_X=False,
-              xdmf_path="T.xdmf", data_format="XML"):
+              xdmf_path="T.xdmf", data_format="HDF"):
     """Thermal-solver CSV dump -> XDMF time series (``ppf_utils.py:63-110``).
 
     CSV layout: a header line, then label/value line pairs for the origins,
@@ -164,7 +198,9 @@ def CSVtoXDMF(csv_path, T_cutoffs=False, starting_T=None, ending_T=None, reflect
     the series starts (at t = 0) with the first frame whose maximum exceeds
     ``starting_T`` and stops before the first later frame whose maximum is
     below ``ending_T``; ``reflect_X`` mirrors every frame along its second axis
-    and prepends the mirror image."""
+    and prepends the mirror image.  The heavy data go to ``<stem>.h5`` by
+    default (the reference's meshio writer's default); ``data_format`` "XML"
+    or "Binary" selects the other XDMF forms."""
     times, frames = [], []
     with open(csv_path) as f:
         lines = iter(f)

@@ -430,7 +430,7 @@ def test_ncomp_fp64_parity(gpu, noise, bc):
         assert rel_l2(sim.fields[i].data, want[i]) <= FP64_TOL, sim.fields[i].name
 
 
-@pytest.mark.parametrize("fmt", ["XML", "Binary"])
+@pytest.mark.parametrize("fmt", ["XML", "Binary", "HDF"])
 def test_ncomp_fp64_thermal_history(gpu, tmp_path, fmt):
     """XDMF_FILE temperature: 30 steps across five frame windows; the step
     kernels interpolate between the two frames on the GPU in the order of

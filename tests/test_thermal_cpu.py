@@ -15,7 +15,7 @@ def _frames(shape, n, seed=0):
     return [1500.0 + 50.0 * rng.random(shape) for _ in range(n)]
 
 
-@pytest.mark.parametrize("fmt", ["XML", "Binary"])
+@pytest.mark.parametrize("fmt", ["XML", "Binary", "HDF"])
 def test_xdmf_round_trip(tmp_path, fmt):
     times = [0.0, 1.25e-7, 3.5e-7]
     frames = _frames((4, 6), 3)
@@ -77,13 +77,16 @@ def test_xdmf_reader_meshio_layout(tmp_path):
     assert np.array_equal(a1, np.array([[1000.0, 1000.5, 1001.0], [1001.5, 1002.0, 1002.5]]))
 
 
-def test_xdmf_hdf_heavy_data_is_rejected(tmp_path):
+def test_xdmf_hdf_heavy_data_missing_file(tmp_path):
+    """An HDF item whose file is absent fails when its frame is read."""
     path = tmp_path / "T.xdmf"
     path.write_text("""<Xdmf Version="3.0"><Domain><Grid GridType="Collection" CollectionType="Temporal">
 <Grid><Time Value="0"/><Attribute Name="T"><DataItem DataType="Float" Precision="8" Dimensions="2"
 Format="HDF">T.h5:/data0</DataItem></Attribute></Grid></Grid></Domain></Xdmf>""")
-    with pytest.raises(NotImplementedError, match="HDF5"):
-        thermal.XDMFLoader(str(path))
+    ld = thermal.XDMFLoader(str(path))
+    assert ld.num_steps == 1
+    with pytest.raises(FileNotFoundError):
+        ld.data(0)
 
 
 def _write_csv(path, rows, dims="3,2"):
