@@ -11,8 +11,9 @@ namespace ppf {
 
 // Partial sums of ppf_reduce_noalloc: a file-scope device array instead of a
 // caller's workspace (each GPU holds its own copy until the process ends).
-// k_reduce_partial overwrites every slot it later reads, so nothing needs
-// clearing between calls; calls that overlap on one GPU share the array and race.
+// k_reduce_sum / k_reduce_energy overwrite every slot the final pass reads,
+// so nothing needs clearing between calls; calls that overlap on one GPU
+// share the array and race.
 constexpr int kNoAllocFields = PPF_REDUCE_NOALLOC_MAX_FIELDS;
 static __device__ __align__(256) double scratch[kNoAllocFields * kReduceBlocks];
 

@@ -1073,52 +1073,169 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // Partial sums of one field: block b walks interior rows b, b + G, ... (a
-// row is a fixed (y, z); one division per row, uniform across the block) and
-// its threads walk x.  Forward neighbours are read directly, except on the
-// last cell of an axis, where the boundary rule is folded in.
+// row is a fixed (y, z)), kRRows of them at a time with all their loads
+// issued before the arithmetic; threads walk x.  The (y, z) of the next row
+// comes from an increment of G rows (no 64-bit division per row).  The two
+// sums are separate kernels: the free energy's boundary folds would otherwise
+// set the register allocation of the plain sum too.
+#ifndef PPF_REDUCE_ROWS
+#define PPF_REDUCE_ROWS 4
+#endif
+constexpr int kRRows = PPF_REDUCE_ROWS;
+
+struct RowWalk {
+  int64_t G, ny, dy, dz, cy, cz;
+  __device__ RowWalk(int64_t ny_) : G(gridDim.x), ny(ny_) {
+    dz = G / ny;
+    dy = G - dz * ny;
+    cz = int64_t(blockIdx.x) / ny;
+    cy = int64_t(blockIdx.x) - cz * ny;
+  }
+  // interior (y, z) (1-based) of the current row, then advance by G rows
+  __device__ __forceinline__ void next(int64_t& y, int64_t& z) {
+    y = 1 + cy;
+    z = 1 + cz;
+    cy += dy;
+    cz += dz;
+    if (cy >= ny) {
+      cy -= ny;
+      ++cz;
+    }
+  }
+};
+
+// sum of the interior cells of field blockIdx.y
 template <typename R>
-__global__ __launch_bounds__(256) void k_reduce_partial(const R* __restrict__ in, const R* __restrict__ b, Dom d,
-                                                        R dx, int kind, double eps, double* __restrict__ work) {
+__global__ __launch_bounds__(256) void k_reduce_sum(const R* __restrict__ in, Dom d, double* __restrict__ work) {
   const int fi = blockIdx.y;
   const R* __restrict__ f = in + fi * d.sf;
-  const R* bb = b ? b + fi * d.sf : nullptr;
   const int64_t nx = d.ax.n, ny = d.ndim >= 2 ? d.ay.n : 1, nz = d.ndim >= 3 ? d.az.n : 1;
   const int64_t rows = ny * nz;
   const int64_t sy = d.ndim >= 2 ? d.ay.stride : 0, sz = d.ndim >= 3 ? d.az.stride : 0;
+  RowWalk w(ny);
   double acc = 0.0;
-  const double idx = 1.0 / double(dx);
-  const double ke = 0.5 * eps * eps;
-  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-    const int64_t zq = r / ny;
-    const int64_t y = d.ndim >= 2 ? 1 + (r - zq * ny) : 0;
-    const int64_t z = d.ndim >= 3 ? 1 + zq : 0;
-    const R* __restrict__ row = f + z * sz + y * sy;
-    if (kind == 0) {
-      for (int64_t x = 1 + threadIdx.x; x <= nx; x += 256) acc += double(row[x]);
-      continue;
+  for (int64_t r = blockIdx.x; r < rows; r += kRRows * w.G) {
+    const R* rowp[kRRows];
+#pragma unroll
+    for (int k = 0; k < kRRows; ++k) {
+      int64_t y, z;
+      w.next(y, z);
+      rowp[k] = r + k * w.G < rows ? f + (d.ndim >= 3 ? z : 0) * sz + (d.ndim >= 2 ? y : 0) * sy : nullptr;
     }
-    const bool ylast = d.ndim >= 2 && y == ny, zlast = d.ndim >= 3 && z == nz;
     for (int64_t x = 1 + threadIdx.x; x <= nx; x += 256) {
-      const double c = double(row[x]);
-      const double e = c * c * (1.0 - c) * (1.0 - c);
-      const double xp = x < nx ? double(row[x + 1]) : double(fold(f, bb, d, x + 1, y, z, dx));
-      const double gx = (xp - c) * idx;
-      double g2 = gx * gx;
-      if (d.ndim >= 2) {
-        const double yp = ylast ? double(fold(f, bb, d, x, y + 1, z, dx)) : double(row[x + sy]);
-        const double gy = (yp - c) * idx;
-        g2 += gy * gy;
-      }
-      if (d.ndim >= 3) {
-        const double zp = zlast ? double(fold(f, bb, d, x, y, z + 1, dx)) : double(row[x + sz]);
-        const double gz = (zp - c) * idx;
-        g2 += gz * gz;
-      }
-      acc += e + ke * g2;
+      R v[kRRows];
+#pragma unroll
+      for (int k = 0; k < kRRows; ++k) v[k] = rowp[k] ? rowp[k][x] : R(0);
+#pragma unroll
+      for (int k = 0; k < kRRows; ++k) acc += double(v[k]);
     }
   }
   double s = block_sum(acc);
   if (threadIdx.x == 0) work[fi * kReduceBlocks + blockIdx.x] = s;
+}
+
+// free-energy density of one cell from its forward neighbours; the boundary
+// rule is folded in where a neighbour lies outside the interior
+template <typename R>
+__device__ __forceinline__ double fe_fold(const R* __restrict__ f, const R* bb, const Dom& d, int64_t x, int64_t y,
+                                          int64_t z, R dx, double idx, double ke) {
+  const double c = double(f[x + (d.ndim >= 2 ? y * d.ay.stride : 0) + (d.ndim >= 3 ? z * d.az.stride : 0)]);
+  const double gx = (double(fold(f, bb, d, x + 1, y, z, dx)) - c) * idx;
+  double g2 = gx * gx;
+  if (d.ndim >= 2) {
+    const double gy = (double(fold(f, bb, d, x, y + 1, z, dx)) - c) * idx;
+    g2 += gy * gy;
+  }
+  if (d.ndim >= 3) {
+    const double gz = (double(fold(f, bb, d, x, y, z + 1, dx)) - c) * idx;
+    g2 += gz * gz;
+  }
+  return c * c * (1.0 - c) * (1.0 - c) + ke * g2;
+}
+
+// sum of c^2 (1-c)^2 + eps^2/2 |grad c|^2 (forward differences) over the
+// interior of field 0 (ND dimensions), in three parts so that the common
+// case is a branch-free body whose loads are all in flight together:
+//  1. cells x < nx of every row by direct loads; rows on the last y or z
+//     layer (whose forward neighbour there is a halo cell, present in the
+//     padded layout but not necessarily initialised) and the rows past the
+//     end (pointed at row 0) are computed and dropped by a select;
+//  2. the cells of the rows on the last y or z layer, in the folded form,
+//     spread over the whole grid (with rows b, b + G, ... per block they
+//     would all fall to the few blocks whose b matches ny - 1);
+//  3. the last cell (x = nx) of the other rows, also spread over the grid.
+template <typename R, int ND>
+__global__ __launch_bounds__(256) void k_reduce_energy(const R* __restrict__ in, const R* __restrict__ b, Dom d,
+                                                       R dx, double eps, double* __restrict__ work) {
+  constexpr int kE = kRRows;
+  const R* __restrict__ f = in;
+  const int64_t nx = d.ax.n, ny = ND >= 2 ? d.ay.n : 1, nz = ND >= 3 ? d.az.n : 1;
+  const int64_t rows = ny * nz;
+  const int64_t sy = ND >= 2 ? d.ay.stride : 0, sz = ND >= 3 ? d.az.stride : 0;
+  const double idx = 1.0 / double(dx);
+  const double ke = 0.5 * eps * eps;
+  auto last_layer = [&](int64_t y, int64_t z) { return (ND >= 2 && y == ny) || (ND >= 3 && z == nz); };
+  RowWalk w(ny);
+  double acc = 0.0;
+  for (int64_t r = blockIdx.x; r < rows; r += kE * w.G) {
+    const R* rowp[kE];
+    int64_t yy[kE], zz[kE];
+    bool keep[kE];
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+      w.next(yy[k], zz[k]);
+      if (ND < 2) yy[k] = 0;
+      if (ND < 3) zz[k] = 0;
+      const bool valid = r + k * w.G < rows;
+      rowp[k] = f + (valid ? zz[k] * sz + yy[k] * sy : 0);
+      keep[k] = valid && !last_layer(yy[k], zz[k]);
+    }
+    for (int64_t x = 1 + threadIdx.x; x < nx; x += 256) {
+      R c[kE], xp[kE], yp[kE], zp[kE];
+#pragma unroll
+      for (int k = 0; k < kE; ++k) {
+        c[k] = rowp[k][x];
+        xp[k] = rowp[k][x + 1];
+        if (ND >= 2) yp[k] = rowp[k][x + sy];
+        if (ND >= 3) zp[k] = rowp[k][x + sz];
+      }
+#pragma unroll
+      for (int k = 0; k < kE; ++k) {
+        const double cc = double(c[k]);
+        const double gx = (double(xp[k]) - cc) * idx;
+        double g2 = gx * gx;
+        if (ND >= 2) {
+          const double gy = (double(yp[k]) - cc) * idx;
+          g2 += gy * gy;
+        }
+        if (ND >= 3) {
+          const double gz = (double(zp[k]) - cc) * idx;
+          g2 += gz * gz;
+        }
+        const double e = cc * cc * (1.0 - cc) * (1.0 - cc) + ke * g2;
+        acc += keep[k] ? e : 0.0;
+      }
+    }
+  }
+  const int64_t t0 = int64_t(blockIdx.x) * 256 + threadIdx.x, nt = w.G * 256;
+  // slow rows: j < nz -> (y = ny, z = 1 + j); then (y = 1 + j - nz, z = nz), y < ny
+  const int64_t nslow = ND == 1 ? 0 : ND == 2 ? 1 : nz + ny - 1;
+  for (int64_t i = t0; i < nslow * nx; i += nt) {
+    const int64_t j = i / nx, x = 1 + (i - j * nx);
+    int64_t y = ny, z = ND >= 3 ? 1 + j : 0;
+    if (ND >= 3 && j >= nz) {
+      y = 1 + (j - nz);
+      z = nz;
+    }
+    acc += fe_fold(f, b, d, x, y, z, dx, idx, ke);
+  }
+  for (int64_t r = t0; r < rows; r += nt) {
+    const int64_t zq = r / ny;
+    const int64_t y = ND >= 2 ? 1 + (r - zq * ny) : 0, z = ND >= 3 ? 1 + zq : 0;
+    if (!last_layer(y, z)) acc += fe_fold(f, b, d, nx, y, z, dx, idx, ke);
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) work[blockIdx.x] = s;
 }
 
 template <typename R>
@@ -1140,9 +1257,14 @@ int reduce(const ppf_grid& g, const void* in, const void* bcvals, int kind, doub
   if (kind == 1) {
     for (int a = 0; a < g.ndim; ++a) scale *= g.dx;
   }
-  hipLaunchKernelGGL(k_reduce_partial<R>, dim3(kReduceBlocks, unsigned(nf)), dim3(256), 0, s,
-                     static_cast<const R*>(in), static_cast<const R*>(bcvals), d, R(g.dx), kind, eps,
-                     static_cast<double*>(work));
+  if (kind == 0) {
+    hipLaunchKernelGGL(k_reduce_sum<R>, dim3(kReduceBlocks, unsigned(nf)), dim3(256), 0, s, static_cast<const R*>(in),
+                       d, static_cast<double*>(work));
+  } else {
+    auto kern = g.ndim == 1 ? k_reduce_energy<R, 1> : g.ndim == 2 ? k_reduce_energy<R, 2> : k_reduce_energy<R, 3>;
+    hipLaunchKernelGGL(kern, dim3(kReduceBlocks), dim3(256), 0, s, static_cast<const R*>(in),
+                       static_cast<const R*>(bcvals), d, R(g.dx), eps, static_cast<double*>(work));
+  }
   int rc = check_launch("reduce_partial");
   if (rc) return rc;
   hipLaunchKernelGGL(k_reduce_final<R>, dim3(unsigned(nf)), dim3(256), 0, s, static_cast<const double*>(work), out,
