@@ -83,11 +83,28 @@ typedef struct ppf_warren_params {
 } ppf_warren_params;
 
 /* N-component engine parameters: params[15] and c_params[6][N] of
-   Engines/NCGPU_new.py:495-519. */
+   Engines/NCGPU_new.py:495-519.  `flags` selects the legacy engine's
+   differences (class NCGPU, Engines/NCGPU.py); 0 = NCGPU_new:
+     PPF_NC_DERIV_FIRST  the free-energy derivative along e_1 - e_N (the first
+                         component) is used for every component
+                         (NCGPU.py:287-290, 303);
+     PPF_NC_SHARED_NOISE one composition noise draw per cell, added to every
+                         component (NCGPU.py:295-296);
+     PPF_NC_CLAMP_WIDE   phi clamped to [1e-6, 0.999999] instead of
+                         [1e-4, 0.9999] (NCGPU.py:229-232);
+     PPF_NC_EBAR_FROM_S  ebar^2 = 6 sqrt(2) S[1] d / T_M[1] and
+                         eqbar^2 = ebar^2 / 4 in that order (NCGPU.py:74-75)
+                         instead of ebar^2, eqbar^2 from ebar, eqbar.
+   The legacy engine's other differences are parameter values: no phi or
+   orientation noise (noise_phi = noise_q = 0), composition noise amplitude 1. */
 #define PPF_MAX_COMP 4
+#define PPF_NC_DERIV_FIRST 1
+#define PPF_NC_SHARED_NOISE 2
+#define PPF_NC_CLAMP_WIDE 4
+#define PPF_NC_EBAR_FROM_S 8
 typedef struct ppf_ncomp_params {
   int32_t ncomp; /* N components, 2..PPF_MAX_COMP; field stack = N+2 */
-  int32_t pad_;
+  int32_t flags; /* PPF_NC_* (legacy engine), 0 for NCGPU_new */
   double dx, d, v_m, M_qmax, H, y_e, beta, D_L, D_S, dt, ebar, eqbar;
   double noise_phi, noise_c, noise_q;
   double L[PPF_MAX_COMP], T_M[PPF_MAX_COMP], S[PPF_MAX_COMP];

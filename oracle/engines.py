@@ -406,21 +406,23 @@ def ncomp_anisotropy_terms(phi, q1, q4, T, dx, y_e, ebar2):
     return d_term_dx, d_term_dy
 
 
-def ncomp_main(fields, T, transfer, params, c_params, seed=0, step=0):
+def ncomp_main(fields, T, transfer, params, c_params, seed=0, step=0, clamp=(0.0001, 0.9999), ebar2=None,
+               eqbar2=None):
     """NComponent_kernel (NCGPU_new.py:37-250) over the interior cells (2D).
 
     Returns fields_out (padded; interior updated, halos copied from input).
+    clamp: the bounds of phi (NCGPU_new.py:234-237).
     """
     dt = dict(zip(PARAM_NAMES, params))["dt"]
-    dphidt, dcdt, dq1dt, dq4dt = ncomp_rates(fields, T, transfer, params, c_params, seed, step)
+    dphidt, dcdt, dq1dt, dq4dt = ncomp_rates(fields, T, transfer, params, c_params, seed, step, ebar2, eqbar2)
     F = fields.shape[0]
     out = fields.copy()
     p = fields[0][1:-1, 1:-1]
     for l in range(3, F):
         out[l][1:-1, 1:-1] = dcdt[l - 3] * dt + fields[l][1:-1, 1:-1]
     phi_new = p + dt * dphidt
-    phi_new = np.where(phi_new < 0.0001, 0.0001, phi_new)
-    phi_new = np.where(phi_new > 0.9999, 0.9999, phi_new)
+    phi_new = np.where(phi_new < clamp[0], clamp[0], phi_new)
+    phi_new = np.where(phi_new > clamp[1], clamp[1], phi_new)
     q1c, q4c = fields[1][1:-1, 1:-1], fields[2][1:-1, 1:-1]
     q1n = q1c + dt * dq1dt
     q4n = q4c + dt * dq4dt
@@ -431,8 +433,71 @@ def ncomp_main(fields, T, transfer, params, c_params, seed=0, step=0):
     return out
 
 
-def ncomp_rates(fields, T, transfer, params, c_params, seed=0, step=0):
-    """(dphi/dt, [dc_l/dt], dq1/dt, dq4/dt) on the interior (NCGPU_new.py:84-232)."""
+# ---- the legacy engine (class NCGPU, Engines/NCGPU.py) -----------------------
+# Its params vector has the first ten entries of PARAM_NAMES (NCGPU.py:
+# 486-497); c_params is the same.  Kernels: NCGPU.py:37-242 (main) and
+# :248-303 (helper).
+
+LEGACY_PARAM_NAMES = PARAM_NAMES[:10]
+
+
+def _legacy_full_params(params10):
+    """The NCGPU_new-layout vector the shared restatement reads: no phi or
+    orientation noise (NCGPU.py:185, 221-224), composition noise amplitude 1
+    (:296); ebar / eqbar are not read (the legacy values enter as ebar2,
+    eqbar2)."""
+    return np.concatenate([np.asarray(params10, dtype=np.float64)[:10], [0., 0., 0., 1., 0.]])
+
+
+def ncomp_helper_legacy(fields, T, params10, c_params, thermo_l, thermo_s, seed=0, step=0):
+    """NComponent_helper_kernel of the legacy engine (NCGPU.py:248-303): as
+    ncomp_helper, but one forward difference, along e_1 - e_N (c_1 + h,
+    c_N - h, :287-290), serves every component (:303), and one noise draw
+    per cell, sqrt(2 * 8.314 T / v_m) xi (:296), is added to every
+    component."""
+    pr = dict(zip(LEGACY_PARAM_NAMES, params10))
+    v_m, D_L, D_S = pr["v_m"], pr["D_L"], pr["D_S"]
+    W = c_params[4]
+    F = fields.shape[0]
+    ncomp = F - 2
+    phi = fields[0]
+    cs = [fields[l] for l in range(3, F)]
+    c_N = np.ones_like(phi)
+    for cl in cs:
+        c_N = c_N - cl
+    arr = [c.copy() for c in cs] + [c_N]
+    G_L, dGL = _thermo.fd_derivatives(arr, T, *thermo_l)
+    G_S, dGS = _thermo.fd_derivatives(arr, T, *thermo_s)
+    g = nc_g(phi)
+    h = nc_h(phi)
+    cells = _rng.padded_cells(phi.shape)
+    xi = _rng.normal_f64(seed, step, _rng.S_NC_C0, phi.size, cells)
+    noise_c = np.sqrt(2. * 8.314 * T / v_m) * xi
+    transfer = np.zeros((2 * ncomp,) + phi.shape)
+    transfer[0] = G_L
+    transfer[1] = G_S
+    for l in range(ncomp - 1):
+        transfer[2 + l] = v_m * cs[l] * (D_L + h * (D_S - D_L)) / (8.314 * T)
+        transfer[2 + l + ncomp - 1] = ((dGL[0] + h * (dGS[0] - dGL[0])) / v_m + (W[l] - W[ncomp - 1]) * g * T
+                                       + noise_c)
+    return transfer
+
+
+def ncomp_main_legacy(fields, T, transfer, params10, c_params, seed=0, step=0):
+    """NComponent_kernel of the legacy engine (NCGPU.py:37-242): the
+    NCGPU_new update with ebar^2 = 6 sqrt(2) S_1 d / T_M,1 and
+    eqbar^2 = ebar^2 / 4 (:74-75), no phi noise (:185), no orientation noise
+    (:221-224) and phi clamped to [1e-6, 0.999999] (:229-232)."""
+    pr = dict(zip(LEGACY_PARAM_NAMES, params10))
+    S, T_M = c_params[2], c_params[1]
+    ebar2 = 6. * np.sqrt(2.) * S[1] * pr["d"] / T_M[1]
+    return ncomp_main(fields, T, transfer, _legacy_full_params(params10), c_params, seed, step,
+                      clamp=(0.000001, 0.999999), ebar2=ebar2, eqbar2=0.25 * ebar2)
+
+
+def ncomp_rates(fields, T, transfer, params, c_params, seed=0, step=0, ebar2=None, eqbar2=None):
+    """(dphi/dt, [dc_l/dt], dq1/dt, dq4/dt) on the interior (NCGPU_new.py:84-232).
+    ebar2 / eqbar2 override ebar^2 / eqbar^2 (the legacy engine's values)."""
     pr = dict(zip(PARAM_NAMES, params))
     dx, v_m, M_qmax, H, y_e, beta = pr["dx"], pr["v_m"], pr["M_qmax"], pr["H"], pr["y_e"], pr["beta"]
     dt, ebar, eqbar = pr["dt"], pr["ebar"], pr["eqbar"]
@@ -442,8 +507,8 @@ def ncomp_rates(fields, T, transfer, params, c_params, seed=0, step=0):
     F = fields.shape[0]
     phi, q1, q4 = fields[0], fields[1], fields[2]
     P, Q1, Q4, TT = _Nb(phi), _Nb(q1), _Nb(q4), _Nb(T)
-    ebar2 = ebar * ebar
-    eqbar2 = eqbar * eqbar
+    ebar2 = ebar * ebar if ebar2 is None else ebar2
+    eqbar2 = eqbar * eqbar if eqbar2 is None else eqbar2
     idx = 1. / dx
 
     p = P.c

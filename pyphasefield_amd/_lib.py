@@ -44,7 +44,7 @@ _NC_VECTORS = ("L", "T_M", "S", "B", "W", "M")
 
 
 class NcompParams(ctypes.Structure):
-    _fields_ = ([("ncomp", c_i32), ("pad_", c_i32)] + [(n, c_f64) for n in _NC_SCALARS]
+    _fields_ = ([("ncomp", c_i32), ("flags", c_i32)] + [(n, c_f64) for n in _NC_SCALARS]
                 + [(n, c_f64 * PPF_MAX_COMP) for n in _NC_VECTORS])
 
 

@@ -151,10 +151,13 @@ int ppf_warren_step_dev(const ppf_grid* g, const void* in, void* out, const void
   return warren_dispatch(g, in, out, bcvals, p, seed, StepRef{step_base, step_offset}, s);
 }
 
+constexpr int32_t kNcFlags = PPF_NC_DERIV_FIRST | PPF_NC_SHARED_NOISE | PPF_NC_CLAMP_WIDE | PPF_NC_EBAR_FROM_S;
+
 static int check_ncomp(const ppf_grid* g, const ppf_grid* tg, const ppf_ncomp_params* p, const char* what) {
   if (int rc = check_grid(g, 2, 2, what)) return rc;
   if (!p || !tg) return fail(PPF_EINVAL, std::string(what) + ": null params");
   if (p->ncomp < 2 || p->ncomp > PPF_MAX_COMP) return fail(PPF_EINVAL, std::string(what) + ": ncomp out of range");
+  if (p->flags & ~kNcFlags) return fail(PPF_EINVAL, std::string(what) + ": unknown flags");
   if (g->nfields != p->ncomp + 2) return fail(PPF_EINVAL, std::string(what) + ": field stack must hold N+2 fields");
   if (tg->nfields != 2 * p->ncomp - 1 || tg->n[0] != g->n[0] || tg->n[1] != g->n[1] || tg->dtype != g->dtype ||
       tg->sy < g->n[0] + 2)
@@ -220,6 +223,7 @@ static int ncomp_fused_dispatch(const ppf_grid* g, const void* in, void* out, co
   if (int rc = check_grid(g, 2, 2, "ncomp_fused_step")) return rc;
   if (!p || !th || !t) return fail(PPF_EINVAL, "ncomp_fused_step: null argument");
   if (p->ncomp < 2 || p->ncomp > PPF_MAX_COMP) return fail(PPF_EINVAL, "ncomp_fused_step: ncomp out of range");
+  if (p->flags & ~kNcFlags) return fail(PPF_EINVAL, "ncomp_fused_step: unknown flags");
   if (g->nfields != p->ncomp + 2) return fail(PPF_EINVAL, "ncomp_fused_step: field stack must hold N+2 fields");
   if (!in || !out || in == out) return fail(PPF_EINVAL, "ncomp_fused_step: need distinct in/out buffers");
   if (int rc = check_temperature(t, "ncomp_fused_step")) return rc;

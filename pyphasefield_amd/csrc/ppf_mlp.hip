@@ -567,6 +567,8 @@ static int ncomp_thermo_mlp_dispatch(const ppf_grid* g, const void* in, const vo
   if (int rc = check_grid(g, 2, 2, "ncomp_thermo_mlp")) return rc;
   if (g->dtype != PPF_F32) return fail(PPF_EINVAL, "ncomp_thermo_mlp: the surrogate path is FP32");
   if (!p || p->ncomp != 3 || g->nfields != 5) return fail(PPF_EINVAL, "ncomp_thermo_mlp: needs 3 components");
+  if (p->flags & (PPF_NC_DERIV_FIRST | PPF_NC_SHARED_NOISE))
+    return fail(PPF_EINVAL, "ncomp_thermo_mlp: the surrogate has no legacy-engine derivative or noise form");
   if (!tg || tg->nfields != 5 || tg->n[0] != g->n[0] || tg->n[1] != g->n[1] || tg->dtype != PPF_F32)
     return fail(PPF_EINVAL, "ncomp_thermo_mlp: transfer grid does not match");
   if (!in || !transfer || !t || !th || !net) return fail(PPF_EINVAL, "ncomp_thermo_mlp: null argument");

@@ -303,13 +303,15 @@ __device__ __forceinline__ void thermo_cell_f32(const float (&cin)[N - 1], float
   const ppf_ncomp_params& p = k.p;
   const float mob = float(p.v_m) * __builtin_fmaf(h, f.dD, f.D_L) * (f.inv_R8314 * __builtin_amdgcn_rcpf(T));
   const float gT = g * T;
+  const bool first = (p.flags & PPF_NC_DERIV_FIRST) != 0, shared = (p.flags & PPF_NC_SHARED_NOISE) != 0;
 #pragma unroll
   for (int l = 0; l < N - 1; ++l) {
     Mc[l] = mob * cin[l];
-    const float dGL = pL[l] - pL[N - 1], dGS = pS[l] - pS[N - 1];
+    const int ld = first ? 0 : l;
+    const float dGL = pL[ld] - pL[N - 1], dGS = pS[ld] - pS[N - 1];
     float noise = 0.f;
     if (p.noise_c != 0.0) {
-      const float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + l, ncells, cell)));
+      const float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + (shared ? 0 : l), ncells, cell)));
       noise = float(p.noise_c) * sqrtf(2.f * 8.314f * T * f.inv_vm) * xi;
     }
     dF[l] = __builtin_fmaf(__builtin_fmaf(h, dGS - dGL, dGL), f.inv_vm,
@@ -348,12 +350,17 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
   R g = (phi * phi) * ((R(1) - phi) * (R(1) - phi));
   R h = phi3 * (R(6) * (phi * phi) - R(15) * phi + R(10));
   R dGL[N], dGS[N];
+  // legacy engine: one derivative, along e_1 - e_N, for every component
+  // (NCGPU.py:287-290, 303)
+  const bool first = (p.flags & PPF_NC_DERIV_FIRST) != 0, shared = (p.flags & PPF_NC_SHARED_NOISE) != 0;
   if (k.th.dg_mode == PPF_DG_FORWARD_DIFF) {
-    // NCGPU_new.py:301-313, with its in-place perturb / restore sequence
+    // NCGPU_new.py:301-313, with its in-place perturb / restore sequence;
+    // its first pass is NCGPU.py:287-290
     const R hstep = R(0.0000001);
     c[N - 1] = c[N - 1] - hstep;
 #pragma unroll
     for (int l = 0; l < N - 1; ++l) {
+      if (first && l > 0) break;
       c[l] = c[l] + hstep;
       R id = ideal_sum<R, N>(c);
       dGL[l] = R(10000000.) * (gibbs<R, N>(c, PL, id) - G_L);
@@ -364,13 +371,20 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
     gibbs_grad<R, N>(c, PL, dGL);
     gibbs_grad<R, N>(c, PS, dGS);
   }
+  if (first) {
+#pragma unroll
+    for (int l = 1; l < N - 1; ++l) {
+      dGL[l] = dGL[0];
+      dGS[l] = dGS[0];
+    }
+  }
   const R v_m = R(p.v_m), D_L = R(p.D_L), D_S = R(p.D_S);
 #pragma unroll
   for (int l = 0; l < N - 1; ++l) {
     Mc[l] = v_m * cin[l] * (D_L + h * (D_S - D_L)) / (R(8.314) * T);
     R noise = R(0);
     if (p.noise_c != 0.0) {
-      R xi = normal01<R>(hash64(seed, rng_counter(step, S_NC_C0 + l, ncells, cell)));
+      R xi = normal01<R>(hash64(seed, rng_counter(step, S_NC_C0 + (shared ? 0 : l), ncells, cell)));
       noise = R(p.noise_c) * sqrt(R(2.) * R(8.314) * T / v_m) * xi;
     }
     dF[l] = (dGL[l] + h * (dGS[l] - dGL[l])) / v_m + (R(p.W[l]) - R(p.W[N - 1])) * g * T + noise;
@@ -625,7 +639,12 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
   const R idx = R(1) / dx;
   const R v_m = R(p.v_m), M_qmax = R(p.M_qmax), Hh = R(p.H), y_e = R(p.y_e), beta = R(p.beta);
   const R dt = R(p.dt), ebar = R(p.ebar), eqbar = R(p.eqbar);
-  const R ebar2 = ebar * ebar, eqbar2 = eqbar * eqbar;
+  R ebar2 = ebar * ebar, eqbar2 = eqbar * eqbar;
+  if (p.flags & PPF_NC_EBAR_FROM_S) {
+    // the legacy engine's values, in its operation order (NCGPU.py:74-75)
+    ebar2 = R(6.0 * 1.4142135623730951 * p.S[1] * p.d / p.T_M[1]);
+    eqbar2 = R(0.25) * ebar2;
+  }
   const uint64_t ncells = uint64_t(d.ax.n + 2) * uint64_t(d.ay.n + 2);
   int tx = threadIdx.x % kTX, tyb = threadIdx.x / kTX;
   int64_t x = x0 + tx;
@@ -844,8 +863,11 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
     R dq1dt = M_q * ((R(1) - q1c * q1c) * t1 - q1c * q4c * t4);
     R dq4dt = M_q * ((R(1) - q4c * q4c) * t4 - q1c * q4c * t1);
     R phin = pc + dt * dphidt;
-    if (phin < R(0.0001)) phin = R(0.0001);
-    if (phin > R(0.9999)) phin = R(0.9999);
+    // NCGPU_new.py:234-237; the legacy engine's wider bounds, NCGPU.py:229-232
+    const bool wide = (p.flags & PPF_NC_CLAMP_WIDE) != 0;
+    const R plo = wide ? R(0.000001) : R(0.0001), phi_ = wide ? R(0.999999) : R(0.9999);
+    if (phin < plo) phin = plo;
+    if (phin > phi_) phin = phi_;
     R q1n = q1c + dt * dq1dt;
     R q4n = q4c + dt * dq4dt;
     out[o] = phin;

@@ -137,3 +137,42 @@ def test_edge_case_arguments_are_rejected():
     short = (ctypes.c_int32 * 9)(*([0] * 8 + [3]))
     rejected(L.ppf_ensemble_step(a, 1, a, 4, short, 0, 8, a, b, None), "cover")
     rejected(L.ppf_mlp_gemm(0, 4, 4, a, 1, 1, a, 1, 1, b, 1, 1.0, 0.0, 1, 0, None), "bad shape")
+
+
+def test_ncomp_flags_and_cahnhilliard_line_arguments_are_rejected():
+    """Unknown N-component flags, the legacy flags on the surrogate pass, and
+    Cahn-Hilliard line solves off a periodic 1D grid of at least 5 cells are
+    refused before any launch."""
+    from pyphasefield_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libppf_hip.so not built")
+    L = _lib.lib()
+    a, b, w = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), ctypes.c_void_p(0x3000)
+
+    def rejected(rc, word):
+        assert rc == 1000
+        assert word.encode() in L.ppf_last_error(), L.ppf_last_error()
+
+    p = _lib.NcompParams()
+    p.ncomp = 3
+    p.flags = 16
+    g = _grid2(L, nf=5)
+    t = _lib.Temperature()
+    th = _lib.Thermo()
+    rejected(L.ppf_ncomp_fused_step(g, a, b, None, t, p, th, 1, 1, None), "flags")
+    tg = _grid2(L, nf=5)
+    rejected(L.ppf_ncomp_step(g, a, b, None, t, tg, w, p, 1, 1, None), "flags")
+    p.flags = 1
+    from pyphasefield_amd._lib_ext import MlpNet
+    rejected(L.ppf_ncomp_thermo_mlp(g, a, None, t, tg, w, p, th, MlpNet(), 1, 1, None), "legacy")
+    g1 = _grid2(L, n=(4, 1))
+    g1.ndim = 1
+    rejected(L.ppf_cahnhilliard_line_solve(g1, a, b, 0.1, 1.0, w, None), "5 cells")
+    g1 = _grid2(L, n=(16, 1))
+    g1.ndim = 1
+    rejected(L.ppf_cahnhilliard_line_solve(g1, a, a, 0.1, 1.0, w, None), "differ")
+    g2 = _grid2(L)
+    rejected(L.ppf_cahnhilliard_line_solve(g2, a, b, 0.1, 1.0, w, None), "1D")
+    gn = _grid2(L, n=(16, 1), bc=(1, 0))
+    gn.ndim = 1
+    rejected(L.ppf_cahnhilliard_line_solve(gn, a, b, 0.1, 1.0, w, None), "PERIODIC")

@@ -645,3 +645,75 @@ def test_checkpoint_resume_is_bit_exact(gpu, tmp_path):
     resumed.simulate(18)
     for a, b in zip(full.fields, resumed.fields):
         assert np.array_equal(a.get_cells(), b.get_cells())
+
+
+def _legacy_sim(E, ny, nx, precision, bc="NEUMANN", mode="fused"):
+    from pyphasefield_amd import thermo
+    model, _ = thermo.synthetic_rk_model(seed=0)
+    sim = E.NCGPU(dimensions=[ny, nx], seed=11)
+    sim.set_framework("GPU_SERIAL")
+    sim.set_dx(0.0000046)
+    sim.set_temperature_type("LINEAR_GRADIENT")
+    sim.set_temperature_initial_T(1574.)
+    sim.set_temperature_dTdx(1.0 / 0.0000046)
+    sim.set_temperature_dTdy(0.0)
+    sim.set_temperature_dTdt(-2.0e5)
+    sim.set_tdb_container(model)
+    sim.set_boundary_conditions(bc)
+    sim.set_precision(precision)
+    sim.set_user_data({"d_ratio": 1.1, "sim_type": "seeds", "number_of_seeds": 3, "seed_radius": 4,
+                       "initial_concentration_array": [0.30, 0.30], "ncomp_kernel": mode})
+    sim.initialize_fields_and_imported_data()
+    return sim, model
+
+
+@pytest.mark.parametrize("mode,bc", [("fused", "NEUMANN"), ("two_pass", "NEUMANN"), ("fused", "PERIODIC")])
+def test_ncgpu_legacy_fp64_parity(gpu, mode, bc):
+    """The legacy engine (class NCGPU, Engines/NCGPU.py) against its
+    restatement (oracle.engines ncomp_helper_legacy / ncomp_main_legacy):
+    first-component derivative for every component, shared composition
+    noise, no phi / orientation noise, ebar from S_1 and T_M,1, the wide phi
+    clamp."""
+    E = _engines()
+    sim, model = _legacy_sim(E, 40, 52, "FP64", bc=bc, mode=mode)
+    sim.just_before_simulating()
+    sim._begun_simulation = True
+    u = sim.user_data
+    assert len(u["params"]) == 10 and sim._ncparams.flags == 15
+    fields0 = np.stack([f.get_cells().copy() for f in sim.fields])
+    ny, nx = sim.dimensions
+
+    def T_of(s):
+        Ti = oe.temperature_linear((ny, nx), sim._initial_T, sim._dTdx, sim._dTdy, sim.dx, sim._dTdt, sim.dt,
+                                   s - 1)
+        return padded_T(Ti, bc)
+
+    liq = (model.liquid.g0, model.liquid.L, model.liquid.R)
+    sol = (model.solid.g0, model.solid.L, model.solid.R)
+    f = np.stack([obc.pad(x) for x in fields0])
+    obc.apply_bc(f, bc, None, sim.dx)
+    steps = 30
+    for s in range(1, steps + 1):
+        T = T_of(s)
+        tr = oe.ncomp_helper_legacy(f, T, u["params"], u["c_params"], liq, sol, seed=sim.seed, step=s)
+        f = oe.ncomp_main_legacy(f, T, tr, u["params"], u["c_params"], seed=sim.seed, step=s)
+        obc.apply_bc(f, bc, None, sim.dx)
+    sim.simulate(steps)
+    for i in range(len(sim.fields)):
+        assert rel_l2(sim.fields[i].data, f[i]) <= FP64_TOL, sim.fields[i].name
+
+
+def test_ncgpu_legacy_differs_from_new_only_as_stated(gpu):
+    """Same start, 20 steps without noise in NCGPU_new: the engines differ
+    (the second component's potential, the clamp, ebar), and the legacy
+    engine's flags reach the ABI."""
+    E = _engines()
+    leg, _ = _legacy_sim(E, 32, 40, "FP64")
+    new, _ = _nc_sim(E, 32, 40, "FP64", 0.0)
+    leg.simulate(20)
+    new.simulate(20)
+    assert not np.array_equal(leg.fields[4].data, new.fields[4].data)
+    with pytest.raises(NotImplementedError):
+        s, _ = _legacy_sim(E, 16, 16, "FP32")
+        s.user_data["thermo_eval"] = "surrogate"
+        s.simulate(1)

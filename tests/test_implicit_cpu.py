@@ -264,13 +264,14 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
 
 def test_engine_registry_exports_the_reference_engine_classes():
     """Engines/__init__.py discovery: every engine class of the reference is
-    importable by name; the legacy NCGPU refuses construction with a message."""
+    importable by name; the legacy NCGPU shares NCGPU_new's machinery."""
     import pyphasefield_amd.Engines as E
     for name in ("AnisoDorrGPU", "CahnAllen", "CahnHilliard", "Diffusion", "DiffusionGPU", "NCGPU", "NCGPU_new",
                  "Template", "Warren1995"):
         assert isinstance(getattr(E, name), type), name
-    with pytest.raises(NotImplementedError, match="NCGPU_new"):
-        E.NCGPU(dimensions=[8, 8])
+    assert issubclass(E.NCGPU, E.NCGPU_new)
+    s = E.NCGPU(dimensions=[8, 8])
+    assert s.user_data["d_ratio"] == 4.
     s = E.DiffusionGPU(dimensions=[8, 12])
     assert s.user_data["D"] == 0.1 and s.get_framework() == "GPU_SERIAL"
     s.initialize_fields_and_imported_data()

@@ -230,3 +230,86 @@ def test_pins_catch_planted_slips(monkeypatch, slip):
         except AssertionError:
             caught = True
     assert caught, slip
+
+
+# ---------------------------------------------------------------- legacy engine --
+# NCGPU.py: one forward difference along e_1 - e_N for every component, one
+# shared noise draw; main kernel without phi / orientation noise, ebar^2 from
+# S_1, T_M,1, phi clamped to [1e-6, 0.999999].
+
+def _legacy_params(noise=1.0):
+    params, c_params = _params(noise_c=noise)
+    c_params = c_params.copy()
+    c_params[1] = [1728., 1358., 1500.]   # T_M
+    c_params[2] = [3.7e-5, 2.9e-5, 3.3e-5]  # S
+    return params[:10].copy(), params, c_params
+
+
+def _ternary_fields(shape=(6, 7), seed=4):
+    rng = np.random.default_rng(seed)
+    f = np.zeros((5,) + shape)
+    f[0] = rng.uniform(0.05, 0.95, shape)
+    f[1] = 1.0
+    f[3] = rng.uniform(0.2, 0.4, shape)
+    f[4] = rng.uniform(0.2, 0.4, shape)
+    return f
+
+
+def test_legacy_helper_uses_first_derivative_and_shared_noise():
+    """dF_2 - dF_1 = (W_2 - W_1) g T exactly: the same derivative and the same
+    noise enter both components; dF_1 and the mobilities equal NCGPU_new's
+    (its first component takes the same difference and the same stream)."""
+    liq, sol = _model()
+    p10, p15, cp = _legacy_params()
+    f = _ternary_fields()
+    T = np.full(f.shape[1:], T0)
+    leg = oe.ncomp_helper_legacy(f, T, p10, cp, liq, sol, seed=5, step=3)
+    new = oe.ncomp_helper(f, T, p15, cp, liq, sol, seed=5, step=3)
+    g = oe.nc_g(f[0])
+    np.testing.assert_allclose(leg[5] - leg[4], (W[1] - W[0]) * g * T, rtol=1e-12, atol=1e-9)
+    np.testing.assert_array_equal(leg[:5], new[:5])
+    assert np.abs(leg[5] - new[5]).max() > 1e-3  # the second component differs from NCGPU_new
+
+
+def test_legacy_binary_equals_ncgpu_new():
+    """With two components the legacy difference is the only one: the legacy
+    helper equals NCGPU_new's (noise included)."""
+    liq, sol = _model()
+    liq2 = (liq[0][:2], liq[1][:1], liq[2])
+    sol2 = (sol[0][:2], sol[1][:1], sol[2])
+    p10, p15, cp = _legacy_params()
+    cp2 = cp[:, :2].copy()
+    f = _ternary_fields()[:4]
+    T = np.full(f.shape[1:], T0)
+    leg = oe.ncomp_helper_legacy(f, T, p10, cp2, liq2, sol2, seed=2, step=9)
+    new = oe.ncomp_helper(f, T, p15, cp2, liq2, sol2, seed=2, step=9)
+    np.testing.assert_array_equal(leg, new)
+
+
+def test_legacy_main_kernel_values():
+    """ebar^2 = 6 sqrt(2) S_1 d / T_M,1 and eqbar^2 = ebar^2 / 4 enter as
+    NCGPU_new's ebar, eqbar would (to rounding); no phi or orientation noise
+    (any seed gives the same update); the clamp is [1e-6, 0.999999]."""
+    liq, sol = _model()
+    p10, p15, cp = _legacy_params()
+    f = _ternary_fields((9, 10))
+    f[1] = np.cos(0.3 * f[0])
+    f[2] = np.sin(0.3 * f[0])
+    T = np.full(f.shape[1:], T0)
+    tr = oe.ncomp_helper_legacy(f, T, p10, cp, liq, sol, seed=1, step=1)
+    a = oe.ncomp_main_legacy(f, T, tr, p10, cp, seed=1, step=1)
+    b = oe.ncomp_main_legacy(f, T, tr, p10, cp, seed=77, step=4)
+    np.testing.assert_array_equal(a, b)
+    ebar = np.sqrt(6. * np.sqrt(2.) * cp[2][1] * p10[1] / cp[1][1])
+    p = p15.copy()
+    i = oe.PARAM_NAMES.index
+    p[i("ebar")], p[i("eqbar")] = ebar, 0.5 * ebar
+    p[i("noise_phi")], p[i("noise_q")] = 0.0, 0.0
+    c = oe.ncomp_main(f, T, tr, p, cp, clamp=(0.000001, 0.999999))
+    np.testing.assert_allclose(a, c, rtol=1e-12, atol=1e-15)
+    # the clamp: phi driven below 1e-6 stops there
+    f2 = f.copy()
+    f2[0][1:-1, 1:-1] = 1e-7
+    tr2 = oe.ncomp_helper_legacy(f2, T, p10, cp, liq, sol, seed=1, step=1)
+    out = oe.ncomp_main_legacy(f2, T, tr2, p10, cp)
+    assert out[0][1:-1, 1:-1].min() >= 0.000001 and np.isclose(out[0][1:-1, 1:-1].min(), 0.000001)
