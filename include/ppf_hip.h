@@ -143,11 +143,22 @@ typedef struct ppf_tpoly {
 } ppf_tpoly; /* 440 bytes */
 
 /* A TDB phase of one sublattice: unary G of each component (PARAMETER
-   G(phase, i; 0)) and Redlich-Kister terms L^k of each pair (i < j in
-   component order, pairs as in ppf_rk_phase.L, orders 0..2). */
+   G(phase, i; 0)), binary Redlich-Kister terms L^k of each pair (i < j in
+   component order, pairs as in ppf_rk_phase.L, orders 0..4):
+     c_i c_j sum_k L^k_ij (c_i - c_j)^k,
+   and ternary terms of each triple (i < j < l in component order: (0,1,2),
+   (0,1,3), (0,2,3), (1,2,3)) in the Muggianu form pycalphad's
+   redlich_kister_sum builds:
+     c_i c_j c_l sum_k L^k_ijl v_k,  v_k = c_k + (1 - c_i - c_j - c_l)/3
+   (k = 0, 1, 2 naming i, j, l; a file giving only L^0 means L^1 = L^2 = L^0,
+   filled in on the host).  Evaluated as
+     w = (1 - ((c_i + c_j) + c_l)) / 3,
+     G += ((c_i c_j) c_l) (((c_i + w) L^0 + (c_j + w) L^1) + (c_l + w) L^2). */
+#define PPF_RK_ORDERS 5
 typedef struct ppf_tdb_phase {
   ppf_tpoly g0[PPF_MAX_COMP];
-  ppf_tpoly L[6][3];
+  ppf_tpoly L[6][PPF_RK_ORDERS];
+  ppf_tpoly T3[4][3];
 } ppf_tdb_phase;
 typedef struct ppf_tdb_tables {
   ppf_tdb_phase liquid, solid;

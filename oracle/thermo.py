@@ -24,6 +24,12 @@ the oracle and on the device.
 Coefficient arrays (float64):
     g0 : [N, 5]         a, b, c, d, e of G_i(T)
     L  : [N(N-1)/2, 3, 2]  pair (i<j) in lexicographic order, k = 0..2, (A, B)
+
+TDB phases (tables): binary orders 0..4, and three-species interaction
+parameters as pycalphad's redlich_kister_sum builds them (Muggianu):
+c_i c_j c_l sum_k v_k L^k_ijl, v_k = c_k + (1 - c_i - c_j - c_l)/3, where
+a file giving only L^0 stands for L^0 = L^1 = L^2 (since sum_k v_k = 1 the
+term is then c_i c_j c_l L^0).
 """
 
 import numpy as np
@@ -109,16 +115,52 @@ def _unary(g0, i, T, lnT):
     return a + b * T + c * T * lnT + d * T * T + e / T
 
 
+def triples(n):
+    return [(i, j, l) for i in range(n) for j in range(i + 1, n) for l in range(j + 1, n)]
+
+
+def _is_table_L(L):
+    return isinstance(L, dict)
+
+
 def _rk(L, p, T, lnT):
-    if isinstance(L, (list, tuple)):
-        return [tpoly(L[p][k], T, lnT) for k in range(3)]
+    """Binary orders of pair p: five (TDB tables: {"pairs": [[table] * 5],
+    "triples": [[table] * 3]}) or three (coefficient array [P, 3, 2])."""
+    if _is_table_L(L):
+        return [tpoly(tab, T, lnT) for tab in L["pairs"][p]]
     return [L[p, k, 0] + L[p, k, 1] * T for k in range(3)]
+
+
+def _rk_poly(Lk, dd):
+    """sum_k L^k dd^k by Horner from the highest order (the device's order)."""
+    if len(Lk) == 5:
+        return Lk[0] + dd * (Lk[1] + dd * (Lk[2] + dd * (Lk[3] + dd * Lk[4])))
+    return Lk[0] + dd * (Lk[1] + dd * Lk[2])
+
+
+def _rk_dpoly(Lk, dd):
+    if len(Lk) == 5:
+        return Lk[1] + dd * (2.0 * Lk[2] + dd * (3.0 * Lk[3] + dd * (4.0 * Lk[4])))
+    return Lk[1] + 2.0 * dd * Lk[2]
+
+
+def _ternary(L, t, T, lnT):
+    return [tpoly(tab, T, lnT) for tab in L["triples"][t]]
+
+
+def ternary_weights(ci, cj, cl):
+    """Muggianu weights v_k = c_k + (1 - c_i - c_j - c_l)/3 of a ternary
+    term (pycalphad's redlich_kister_sum for a three-species interaction)."""
+    w = (1.0 - ((ci + cj) + cl)) / 3.0
+    return ci + w, cj + w, cl + w
 
 
 def gibbs(cs, T, g0, L, R):
     """G for component fractions ``cs`` (list of N arrays, c_N last) and T.
-    g0 / L: the Redlich-Kister coefficient arrays, or TDB tables (lists of
-    (T_hi, c, nranges), see :func:`tpoly`)."""
+    g0 / L: the Redlich-Kister coefficient arrays, or TDB tables (g0: list of
+    (T_hi, c, nranges), see :func:`tpoly`; L: {"pairs", "triples"}).  TDB
+    phases add binary orders 3..4 and ternary terms
+    c_i c_j c_l sum_k v_k L^k_ijl (:func:`ternary_weights`)."""
     n = len(cs)
     lnT = plog(T)
     G = np.zeros(np.broadcast(cs[0], T).shape)
@@ -126,10 +168,15 @@ def gibbs(cs, T, g0, L, R):
         gi = _unary(g0, i, T, lnT)
         G = G + cs[i] * gi
     for p, (i, j) in enumerate(pairs(n)):
-        L0, L1, L2 = _rk(L, p, T, lnT)
         dd = cs[i] - cs[j]
-        P = L0 + dd * (L1 + dd * L2)
+        P = _rk_poly(_rk(L, p, T, lnT), dd)
         G = G + cs[i] * cs[j] * P
+    if _is_table_L(L):
+        for t, (i, j, l) in enumerate(triples(n)):
+            V0, V1, V2 = _ternary(L, t, T, lnT)
+            v0, v1, v2 = ternary_weights(cs[i], cs[j], cs[l])
+            Wt = (v0 * V0 + v1 * V1) + v2 * V2
+            G = G + ((cs[i] * cs[j]) * cs[l]) * Wt
     ideal = np.zeros_like(G)
     for i in range(n):
         ci = np.maximum(cs[i], CMIN)
@@ -167,12 +214,22 @@ def analytic_derivatives(cs, T, g0, L, R):
         clamp = (cs[i] > CMIN)
         part.append(gi + R * T * np.where(clamp, plog(ci) + 1.0, 0.0))
     for p, (i, j) in enumerate(pairs(n)):
-        L0, L1, L2 = _rk(L, p, T, lnT)
+        Lk = _rk(L, p, T, lnT)
         dd = cs[i] - cs[j]
-        P = L0 + dd * (L1 + dd * L2)
-        dP = L1 + 2.0 * dd * L2
+        P = _rk_poly(Lk, dd)
+        dP = _rk_dpoly(Lk, dd)
         part[i] = part[i] + cs[j] * P + cs[i] * cs[j] * dP
         part[j] = part[j] + cs[i] * P - cs[i] * cs[j] * dP
+    if _is_table_L(L):
+        for t, (i, j, l) in enumerate(triples(n)):
+            V0, V1, V2 = _ternary(L, t, T, lnT)
+            v0, v1, v2 = ternary_weights(cs[i], cs[j], cs[l])
+            Wt = (v0 * V0 + v1 * V1) + v2 * V2
+            Vm = ((V0 + V1) + V2) / 3.0
+            prod = (cs[i] * cs[j]) * cs[l]
+            part[i] = part[i] + (cs[j] * cs[l]) * Wt + prod * (V0 - Vm)
+            part[j] = part[j] + (cs[i] * cs[l]) * Wt + prod * (V1 - Vm)
+            part[l] = part[l] + (cs[i] * cs[j]) * Wt + prod * (V2 - Vm)
     for l in range(n - 1):
         dG.append(part[l] - part[n - 1])
     return dG

@@ -202,9 +202,10 @@ class NCGPU_new(Simulation):
         mode = self._kernel_mode()
         if mode == "surrogate":
             return ["k_ncomp_thermo_mlp", "k_ncomp_step<float,3,false>"]
+        tdb = getattr(self._tdb, "device_tables", None) is not None and self._tdb.device_tables() is not None
         if mode == "fused":
-            return ["k_ncomp_step<%s,%d,true>" % (t, n)]
-        return ["k_ncomp_thermo<%s,%d>" % (t, n), "k_ncomp_step<%s,%d,false>" % (t, n)]
+            return ["k_ncomp_step_tdb<%s,%d>" % (t, n) if tdb else "k_ncomp_step<%s,%d,true>" % (t, n)]
+        return ["k_ncomp_thermo%s<%s,%d>" % ("_tdb" if tdb else "", t, n), "k_ncomp_step<%s,%d,false>" % (t, n)]
 
     def thermo_pass(self):
         L = _lib.lib()

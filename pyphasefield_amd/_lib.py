@@ -62,8 +62,11 @@ class TPoly(ctypes.Structure):
                 ("c", (c_f64 * PPF_TPOLY_TERMS) * PPF_TPOLY_RANGES)]
 
 
+PPF_RK_ORDERS = 5
+
+
 class TdbPhase(ctypes.Structure):
-    _fields_ = [("g0", TPoly * PPF_MAX_COMP), ("L", (TPoly * 3) * 6)]
+    _fields_ = [("g0", TPoly * PPF_MAX_COMP), ("L", (TPoly * PPF_RK_ORDERS) * 6), ("T3", (TPoly * 3) * 4)]
 
 
 class TdbTables(ctypes.Structure):

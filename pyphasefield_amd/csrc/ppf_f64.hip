@@ -45,15 +45,15 @@ __global__ __launch_bounds__(256) void k_thermo_eval(ppf_thermo th, int64_t n, c
   cc[N - 1] = cN;
   double t = T[i];
   double lnT = plog(t);
-  PhaseAtT<double, N> PL, PS;
-  phases_at<double, N>(th, t, lnT, PL, PS);
   double id = ideal_sum<double, N>(cc);
   double dL[N], dS[N];
-  gibbs_grad<double, N>(cc, PL, dL);
-  gibbs_grad<double, N>(cc, PS, dS);
   double* o = out + i * 2 * N;
-  o[0] = gibbs<double, N>(cc, PL, id);
-  o[N] = gibbs<double, N>(cc, PS, id);
+  with_phases<double, N>(th, t, lnT, [&](const auto& PL, const auto& PS) {
+    gibbs_grad(cc, PL, dL);
+    gibbs_grad(cc, PS, dS);
+    o[0] = gibbs(cc, PL, id);
+    o[N] = gibbs(cc, PS, id);
+  });
 #pragma unroll
   for (int l = 0; l < N - 1; ++l) {
     o[1 + l] = dL[l];

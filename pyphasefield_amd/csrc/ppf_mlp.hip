@@ -316,13 +316,15 @@ __device__ __forceinline__ Plain5 plain_outputs(float c1, float c2, float T, con
   double c[N] = {double(c1), double(c2), 1.0 - double(c1) - double(c2)};
   double t = double(T);
   double lnT = plog(t);
-  PhaseAtT<double, N> PL, PS;
-  phases_at<double, N>(th, t, lnT, PL, PS);
   double id = ideal_sum<double, N>(c);
-  double gl = gibbs<double, N>(c, PL, id), gs = gibbs<double, N>(c, PS, id);
+  double gl = 0.0, gs = 0.0;
   double dl[N], ds[N];
-  gibbs_grad<double, N>(c, PL, dl);
-  gibbs_grad<double, N>(c, PS, ds);
+  with_phases<double, N>(th, t, lnT, [&](const auto& PL, const auto& PS) {
+    gl = gibbs(c, PL, id);
+    gs = gibbs(c, PS, id);
+    gibbs_grad(c, PL, dl);
+    gibbs_grad(c, PS, ds);
+  });
   Plain5 r;
   r.dG = float(gs - gl);
   for (int l = 0; l < N - 1; ++l) {

@@ -23,16 +23,21 @@ __device__ __forceinline__ double rlog<double>(double x) { return plog(x); }
 template <>
 __device__ __forceinline__ float rlog<float>(float x) { return logf(x); }
 
-// phase coefficients evaluated at one temperature
-template <typename R, int N>
+// phase coefficients evaluated at one temperature; TDB tables (EXT) add
+// binary orders 3..4 (L34) and ternary terms (T3)
+constexpr int nc_triples(int N) { return N * (N - 1) * (N - 2) / 6 > 0 ? N * (N - 1) * (N - 2) / 6 : 1; }
+
+template <typename R, int N, bool EXT = false>
 struct PhaseAtT {
   R gi[N];
   R L[N * (N - 1) / 2][3];
+  R L34[EXT ? N * (N - 1) / 2 : 1][2];
+  R T3[EXT ? nc_triples(N) : 1][3];
   R RT;
 };
 
 template <typename R, int N>
-__device__ __forceinline__ void phase_at(const ppf_rk_phase& ph, R T, R lnT, PhaseAtT<R, N>& o) {
+__device__ __forceinline__ void phase_at(const ppf_rk_phase& ph, R T, R lnT, PhaseAtT<R, N, false>& o) {
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     R a = R(ph.g0[i][0]), b = R(ph.g0[i][1]), c = R(ph.g0[i][2]), d = R(ph.g0[i][3]), e = R(ph.g0[i][4]);
@@ -73,27 +78,39 @@ __device__ __forceinline__ R tpoly_eval(const ppf_tpoly* __restrict__ f, R T, R 
 
 template <typename R, int N>
 __device__ __forceinline__ void phase_at_tables(const ppf_tdb_phase* __restrict__ ph, R Rg, R T, R lnT,
-                                                PhaseAtT<R, N>& o) {
+                                                PhaseAtT<R, N, true>& o) {
 #pragma unroll
   for (int i = 0; i < N; ++i) o.gi[i] = tpoly_eval<R>(&ph->g0[i], T, lnT);
 #pragma unroll
   for (int p = 0; p < N * (N - 1) / 2; ++p) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) o.L[p][k] = tpoly_eval<R>(&ph->L[p][k], T, lnT);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) o.L34[p][k] = tpoly_eval<R>(&ph->L[p][3 + k], T, lnT);
+  }
+#pragma unroll
+  for (int t = 0; t < nc_triples(N); ++t) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.T3[t][k] = N >= 3 ? tpoly_eval<R>(&ph->T3[t][k], T, lnT) : R(0);
   }
   o.RT = Rg * T;
 }
 
-// both phases at one temperature: TDB tables when present, else the inline
-// Redlich-Kister coefficients
-template <typename R, int N>
-__device__ __forceinline__ void phases_at(const ppf_thermo& th, R T, R lnT, PhaseAtT<R, N>& PL, PhaseAtT<R, N>& PS) {
+// f(PL, PS) with both phases at one temperature: from the TDB tables when
+// present (binary orders 0..4, ternary terms), else from the inline
+// Redlich-Kister coefficients (orders 0..2) -- two code paths, chosen once
+template <typename R, int N, typename F>
+__device__ __forceinline__ void with_phases(const ppf_thermo& th, R T, R lnT, F&& f) {
   if (th.tables) {
+    PhaseAtT<R, N, true> PL, PS;
     phase_at_tables<R, N>(&th.tables->liquid, R(th.liquid.R), T, lnT, PL);
     phase_at_tables<R, N>(&th.tables->solid, R(th.solid.R), T, lnT, PS);
+    f(PL, PS);
   } else {
+    PhaseAtT<R, N, false> PL, PS;
     phase_at<R, N>(th.liquid, T, lnT, PL);
     phase_at<R, N>(th.solid, T, lnT, PS);
+    f(PL, PS);
   }
 }
 
@@ -117,9 +134,23 @@ __device__ __forceinline__ R ideal_sum(const R (&c)[N]) {
   return s;
 }
 
+// binary Redlich-Kister polynomial sum_k L^k dd^k of pair p and its dd-derivative
+template <typename R, int N, bool EXT>
+__device__ __forceinline__ R rk_poly(const PhaseAtT<R, N, EXT>& ph, int p, R dd) {
+  if constexpr (EXT)
+    return ph.L[p][0] + dd * (ph.L[p][1] + dd * (ph.L[p][2] + dd * (ph.L34[p][0] + dd * ph.L34[p][1])));
+  return ph.L[p][0] + dd * (ph.L[p][1] + dd * ph.L[p][2]);
+}
+template <typename R, int N, bool EXT>
+__device__ __forceinline__ R rk_dpoly(const PhaseAtT<R, N, EXT>& ph, int p, R dd) {
+  if constexpr (EXT)
+    return ph.L[p][1] + dd * (R(2) * ph.L[p][2] + dd * (R(3) * ph.L34[p][0] + dd * (R(4) * ph.L34[p][1])));
+  return ph.L[p][1] + R(2) * dd * ph.L[p][2];
+}
+
 // G of one phase (same operation order as the oracle's thermo.gibbs)
-template <typename R, int N>
-__device__ __forceinline__ R gibbs(const R (&c)[N], const PhaseAtT<R, N>& ph, R ideal) {
+template <typename R, int N, bool EXT>
+__device__ __forceinline__ R gibbs(const R (&c)[N], const PhaseAtT<R, N, EXT>& ph, R ideal) {
   R G = R(0);
 #pragma unroll
   for (int i = 0; i < N; ++i) G = G + c[i] * ph.gi[i];
@@ -129,16 +160,31 @@ __device__ __forceinline__ R gibbs(const R (&c)[N], const PhaseAtT<R, N>& ph, R 
 #pragma unroll
     for (int j = i + 1; j < N; ++j, ++p) {
       R dd = c[i] - c[j];
-      R P = ph.L[p][0] + dd * (ph.L[p][1] + dd * ph.L[p][2]);
+      R P = rk_poly(ph, p, dd);
       G = G + c[i] * c[j] * P;
+    }
+  }
+  if constexpr (EXT) {
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+      for (int j = i + 1; j < N; ++j) {
+#pragma unroll
+        for (int l = j + 1; l < N; ++l, ++t) {
+          const R w = (R(1) - ((c[i] + c[j]) + c[l])) / R(3);
+          const R Wt = ((c[i] + w) * ph.T3[t][0] + (c[j] + w) * ph.T3[t][1]) + (c[l] + w) * ph.T3[t][2];
+          G = G + ((c[i] * c[j]) * c[l]) * Wt;
+        }
+      }
     }
   }
   return G + ph.RT * ideal;
 }
 
 // exact dG/dc_l along e_l - e_N
-template <typename R, int N>
-__device__ __forceinline__ void gibbs_grad(const R (&c)[N], const PhaseAtT<R, N>& ph, R (&dG)[N]) {
+template <typename R, int N, bool EXT>
+__device__ __forceinline__ void gibbs_grad(const R (&c)[N], const PhaseAtT<R, N, EXT>& ph, R (&dG)[N]) {
   R part[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) part[i] = ph.gi[i] + (c[i] > R(kCMin) ? ph.RT * (rlog<R>(c[i]) + R(1)) : R(0));
@@ -148,10 +194,29 @@ __device__ __forceinline__ void gibbs_grad(const R (&c)[N], const PhaseAtT<R, N>
 #pragma unroll
     for (int j = i + 1; j < N; ++j, ++p) {
       R dd = c[i] - c[j];
-      R P = ph.L[p][0] + dd * (ph.L[p][1] + dd * ph.L[p][2]);
-      R dP = ph.L[p][1] + R(2) * dd * ph.L[p][2];
+      R P = rk_poly(ph, p, dd);
+      R dP = rk_dpoly(ph, p, dd);
       part[i] = part[i] + c[j] * P + c[i] * c[j] * dP;
       part[j] = part[j] + c[i] * P - c[i] * c[j] * dP;
+    }
+  }
+  if constexpr (EXT) {
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+      for (int j = i + 1; j < N; ++j) {
+#pragma unroll
+        for (int l = j + 1; l < N; ++l, ++t) {
+          const R w = (R(1) - ((c[i] + c[j]) + c[l])) / R(3);
+          const R Wt = ((c[i] + w) * ph.T3[t][0] + (c[j] + w) * ph.T3[t][1]) + (c[l] + w) * ph.T3[t][2];
+          const R Vm = ((ph.T3[t][0] + ph.T3[t][1]) + ph.T3[t][2]) / R(3);
+          const R prod = (c[i] * c[j]) * c[l];
+          part[i] = part[i] + (c[j] * c[l]) * Wt + prod * (ph.T3[t][0] - Vm);
+          part[j] = part[j] + (c[i] * c[l]) * Wt + prod * (ph.T3[t][1] - Vm);
+          part[l] = part[l] + (c[i] * c[j]) * Wt + prod * (ph.T3[t][2] - Vm);
+        }
+      }
     }
   }
 #pragma unroll
@@ -218,7 +283,7 @@ inline NcK make_nck(const ppf_ncomp_params& p, const ppf_thermo& th) {
 // FP32 phases at one temperature from the float coefficients
 template <int N>
 __device__ __forceinline__ void phase_at_f(const PhaseF& ph, float T, float lnT, bool tlnt, bool more,
-                                           PhaseAtT<float, N>& o) {
+                                           PhaseAtT<float, N, false>& o) {
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     float g = __builtin_fmaf(ph.g0[i][1], T, ph.g0[i][0]);
@@ -254,7 +319,7 @@ __device__ __forceinline__ void thermo_cell_f32(const float (&cin)[N - 1], float
   }
   c[N - 1] = cN;
   const float lnT = f.has_TlnT ? nlogf(T) : 0.f;
-  PhaseAtT<float, N> PL, PS;
+  PhaseAtT<float, N, false> PL, PS;
   phase_at_f<N>(f.liquid, T, lnT, f.has_TlnT != 0, f.has_more != 0, PL);
   phase_at_f<N>(f.solid, T, lnT, f.has_TlnT != 0, f.has_more != 0, PS);
   // ln max(c, 1e-10) once per component, shared by G and both gradients
@@ -319,15 +384,45 @@ __device__ __forceinline__ void thermo_cell_f32(const float (&cin)[N - 1], float
   }
 }
 
-template <typename R, int N>
+// G_S - G_L and the chemical-potential derivatives of both phases at c:
+// the h = 1e-7 forward differences of NCGPU_new.py:301-313 with its in-place
+// perturb / restore sequence (fd; with `first` only its first pass, the
+// legacy engine's NCGPU.py:287-290), or the exact derivatives
+template <typename R, int N, bool EXT>
+__device__ __forceinline__ void phase_thermo(const PhaseAtT<R, N, EXT>& PL, const PhaseAtT<R, N, EXT>& PS, bool fd,
+                                             bool first, R (&c)[N], R* dG, R (&dGL)[N], R (&dGS)[N]) {
+  R id0 = ideal_sum<R, N>(c);
+  R G_L = gibbs(c, PL, id0);
+  R G_S = gibbs(c, PS, id0);
+  *dG = G_S - G_L;
+  if (fd) {
+    const R hstep = R(0.0000001);
+    c[N - 1] = c[N - 1] - hstep;
+#pragma unroll
+    for (int l = 0; l < N - 1; ++l) {
+      if (first && l > 0) break;
+      c[l] = c[l] + hstep;
+      R id = ideal_sum<R, N>(c);
+      dGL[l] = R(10000000.) * (gibbs(c, PL, id) - G_L);
+      dGS[l] = R(10000000.) * (gibbs(c, PS, id) - G_S);
+      c[l] = c[l] - hstep;
+    }
+  } else {
+    gibbs_grad(c, PL, dGL);
+    gibbs_grad(c, PS, dGS);
+  }
+}
+
+// TAB: the phases come from TDB tables (k.th.tables), else from the inline
+// Redlich-Kister coefficients; the two are separate kernels, so neither
+// path's registers are allocated for the other
+template <typename R, int N, bool TAB>
 __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, const NcK& k, int64_t step,
                                             uint64_t seed, uint64_t ncells, uint64_t cell, R* dG, R* Mc,
                                             R* dF) {
-  if constexpr (sizeof(R) == 4) {
-    if (!k.th.tables) {
-      thermo_cell_f32<N>(cin, phi, T, k, step, seed, ncells, cell, dG, Mc, dF);
-      return;
-    }
+  if constexpr (sizeof(R) == 4 && !TAB) {
+    thermo_cell_f32<N>(cin, phi, T, k, step, seed, ncells, cell, dG, Mc, dF);
+    return;
   }
   const ppf_ncomp_params& p = k.p;
   R c[N];
@@ -339,38 +434,25 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
   }
   c[N - 1] = cN;
   R lnT = needs_lnT(k.th, N) ? rlog<R>(T) : R(0);
-  PhaseAtT<R, N> PL, PS;
-  phases_at<R, N>(k.th, T, lnT, PL, PS);
-  R id0 = ideal_sum<R, N>(c);
-  R G_L = gibbs<R, N>(c, PL, id0);
-  R G_S = gibbs<R, N>(c, PS, id0);
-  *dG = G_S - G_L;
+  // legacy engine: one derivative, along e_1 - e_N, for every component
+  // (NCGPU.py:287-290, 303)
+  const bool first = (p.flags & PPF_NC_DERIV_FIRST) != 0, shared = (p.flags & PPF_NC_SHARED_NOISE) != 0;
+  R dGL[N], dGS[N];
+  if constexpr (TAB) {
+    PhaseAtT<R, N, true> PL, PS;
+    phase_at_tables<R, N>(&k.th.tables->liquid, R(k.th.liquid.R), T, lnT, PL);
+    phase_at_tables<R, N>(&k.th.tables->solid, R(k.th.solid.R), T, lnT, PS);
+    phase_thermo(PL, PS, k.th.dg_mode == PPF_DG_FORWARD_DIFF, first, c, dG, dGL, dGS);
+  } else {
+    PhaseAtT<R, N, false> PL, PS;
+    phase_at<R, N>(k.th.liquid, T, lnT, PL);
+    phase_at<R, N>(k.th.solid, T, lnT, PS);
+    phase_thermo(PL, PS, k.th.dg_mode == PPF_DG_FORWARD_DIFF, first, c, dG, dGL, dGS);
+  }
   // grouping of NCGPU_new.py:298-299 (numba integer powers are products)
   R phi3 = phi * phi * phi;
   R g = (phi * phi) * ((R(1) - phi) * (R(1) - phi));
   R h = phi3 * (R(6) * (phi * phi) - R(15) * phi + R(10));
-  R dGL[N], dGS[N];
-  // legacy engine: one derivative, along e_1 - e_N, for every component
-  // (NCGPU.py:287-290, 303)
-  const bool first = (p.flags & PPF_NC_DERIV_FIRST) != 0, shared = (p.flags & PPF_NC_SHARED_NOISE) != 0;
-  if (k.th.dg_mode == PPF_DG_FORWARD_DIFF) {
-    // NCGPU_new.py:301-313, with its in-place perturb / restore sequence;
-    // its first pass is NCGPU.py:287-290
-    const R hstep = R(0.0000001);
-    c[N - 1] = c[N - 1] - hstep;
-#pragma unroll
-    for (int l = 0; l < N - 1; ++l) {
-      if (first && l > 0) break;
-      c[l] = c[l] + hstep;
-      R id = ideal_sum<R, N>(c);
-      dGL[l] = R(10000000.) * (gibbs<R, N>(c, PL, id) - G_L);
-      dGS[l] = R(10000000.) * (gibbs<R, N>(c, PS, id) - G_S);
-      c[l] = c[l] - hstep;
-    }
-  } else {
-    gibbs_grad<R, N>(c, PL, dGL);
-    gibbs_grad<R, N>(c, PS, dGS);
-  }
   if (first) {
 #pragma unroll
     for (int l = 1; l < N - 1; ++l) {
@@ -395,10 +477,10 @@ __device__ __forceinline__ void thermo_cell(const R (&cin)[N - 1], R phi, R T, c
 // are all interior read the fields directly; the others fold the boundary
 // rules in (the halo ring of the transfer arrays is computed from folded
 // values, as the reference helper does from its BC-filled halos).
-template <typename R, int N>
-__global__ __launch_bounds__(256) void k_ncomp_thermo(const R* __restrict__ in, const R* __restrict__ b, Dom d,
-                                                      R dx, ppf_temperature t, R* __restrict__ tr, int64_t tsy,
-                                                      int64_t tsf, NcK k, uint64_t seed, StepRef sr) {
+template <typename R, int N, bool TAB>
+__device__ __forceinline__ void ncomp_thermo_body(const R* __restrict__ in, const R* __restrict__ b, Dom d, R dx,
+                                                  ppf_temperature t, R* __restrict__ tr, int64_t tsy, int64_t tsf,
+                                                  NcK k, uint64_t seed, StepRef sr) {
   const int64_t step = sr.get();
   const int64_t px = d.ax.n + 2, py = d.ay.n + 2;
   const uint32_t nbx = uint32_t((px + 63) / 64);
@@ -424,7 +506,7 @@ __global__ __launch_bounds__(256) void k_ncomp_thermo(const R* __restrict__ in, 
   R T = temperature_at<R>(t, n3, bc3, x, y, 1, double(dx), step);
   R dG, Mc[N - 1], dF[N - 1];
   const uint64_t cell = uint64_t(y) * uint64_t(px) + uint64_t(x);
-  thermo_cell<R, N>(cin, phi, T, k, step, seed, uint64_t(px * py), cell, &dG, Mc, dF);
+  thermo_cell<R, N, TAB>(cin, phi, T, k, step, seed, uint64_t(px * py), cell, &dG, Mc, dF);
   const int64_t o = y * tsy + x;
   tr[o] = dG;
 #pragma unroll
@@ -432,6 +514,19 @@ __global__ __launch_bounds__(256) void k_ncomp_thermo(const R* __restrict__ in, 
     tr[o + (1 + l) * tsf] = Mc[l];
     tr[o + (N + l) * tsf] = dF[l];
   }
+}
+
+template <typename R, int N>
+__global__ __launch_bounds__(256) void k_ncomp_thermo(const R* __restrict__ in, const R* __restrict__ b, Dom d,
+                                                      R dx, ppf_temperature t, R* __restrict__ tr, int64_t tsy,
+                                                      int64_t tsf, NcK k, uint64_t seed, StepRef sr) {
+  ncomp_thermo_body<R, N, false>(in, b, d, dx, t, tr, tsy, tsf, k, seed, sr);
+}
+template <typename R, int N>
+__global__ __launch_bounds__(256) void k_ncomp_thermo_tdb(const R* __restrict__ in, const R* __restrict__ b, Dom d,
+                                                          R dx, ppf_temperature t, R* __restrict__ tr, int64_t tsy,
+                                                          int64_t tsf, NcK k, uint64_t seed, StepRef sr) {
+  ncomp_thermo_body<R, N, true>(in, b, d, dx, t, tr, tsy, tsf, k, seed, sr);
 }
 
 template <typename R>
@@ -446,16 +541,16 @@ int ncomp_thermo(const ppf_grid& g, const void* in, const void* bcvals, const pp
   R* tr = static_cast<R*>(transfer);
   switch (p.ncomp) {
     case 2:
-      hipLaunchKernelGGL((k_ncomp_thermo<R, 2>), grid, dim3(256), 0, s, fin, bc, d, R(g.dx), t, tr, tg.sy,
-                         tg.sf, k, seed, step);
+      hipLaunchKernelGGL((th.tables ? k_ncomp_thermo_tdb<R, 2> : k_ncomp_thermo<R, 2>), grid, dim3(256), 0, s, fin,
+                         bc, d, R(g.dx), t, tr, tg.sy, tg.sf, k, seed, step);
       break;
     case 3:
-      hipLaunchKernelGGL((k_ncomp_thermo<R, 3>), grid, dim3(256), 0, s, fin, bc, d, R(g.dx), t, tr, tg.sy,
-                         tg.sf, k, seed, step);
+      hipLaunchKernelGGL((th.tables ? k_ncomp_thermo_tdb<R, 3> : k_ncomp_thermo<R, 3>), grid, dim3(256), 0, s, fin,
+                         bc, d, R(g.dx), t, tr, tg.sy, tg.sf, k, seed, step);
       break;
     case 4:
-      hipLaunchKernelGGL((k_ncomp_thermo<R, 4>), grid, dim3(256), 0, s, fin, bc, d, R(g.dx), t, tr, tg.sy,
-                         tg.sf, k, seed, step);
+      hipLaunchKernelGGL((th.tables ? k_ncomp_thermo_tdb<R, 4> : k_ncomp_thermo<R, 4>), grid, dim3(256), 0, s, fin,
+                         bc, d, R(g.dx), t, tr, tg.sy, tg.sf, k, seed, step);
       break;
     default:
       return fail(PPF_EINVAL, "ncomp_thermo: ncomp must be 2..4");
@@ -525,11 +620,11 @@ struct Nb5 {
 // by the same device function (FUSED = true: the thermodynamics is pointwise,
 // so the ring needs no further halo and the transfer stack never reaches
 // memory).
-template <typename R, int N, bool FUSED>
-__global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const R* __restrict__ in, R* __restrict__ out,
-                                                         const R* __restrict__ b, Dom d, R dx, ppf_temperature t,
-                                                         const R* __restrict__ tr, int64_t tsy, int64_t tsf,
-                                                         NcK k, uint64_t seed, StepRef sr) {
+template <typename R, int N, bool FUSED, bool TAB>
+__device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __restrict__ out,
+                                                const R* __restrict__ b, Dom d, R dx, ppf_temperature t,
+                                                const R* __restrict__ tr, int64_t tsy, int64_t tsf, NcK k,
+                                                uint64_t seed, StepRef sr) {
   constexpr int H = 1, W = kTX + 2 * H, HH = kNcTY + 2 * H;
   constexpr int NF = N + 2;         // phi, q1, q4, c_1..c_{N-1}
   constexpr int NT = 2 * (N - 1);   // M_c and dF/dc tiles
@@ -624,7 +719,7 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
       for (int l = 0; l < N - 1; ++l) cin[l] = sf[3 + l][ly][lx];
       const uint64_t cell = uint64_t(y0 - 1 + ly) * uint64_t(d.ax.n + 2) + uint64_t(x0 - 1 + lx);
       R dGc, Mc[N - 1], dF[N - 1];
-      thermo_cell<R, N>(cin, sf[0][ly][lx], sT[ly][lx], k, step, seed, pxy, cell, &dGc, Mc, dF);
+      thermo_cell<R, N, TAB>(cin, sf[0][ly][lx], sT[ly][lx], k, step, seed, pxy, cell, &dGc, Mc, dF);
 #pragma unroll
       for (int l = 0; l < N - 1; ++l) {
         st[l][ly][lx] = Mc[l];
@@ -883,6 +978,23 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const 
   }
 }
 
+template <typename R, int N, bool FUSED>
+__global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step(const R* __restrict__ in, R* __restrict__ out,
+                                                         const R* __restrict__ b, Dom d, R dx, ppf_temperature t,
+                                                         const R* __restrict__ tr, int64_t tsy, int64_t tsf,
+                                                         NcK k, uint64_t seed, StepRef sr) {
+  ncomp_step_body<R, N, FUSED, false>(in, out, b, d, dx, t, tr, tsy, tsf, k, seed, sr);
+}
+// the one-launch step with TDB-table phases
+template <typename R, int N>
+__global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step_tdb(const R* __restrict__ in,
+                                                             R* __restrict__ out, const R* __restrict__ b, Dom d,
+                                                             R dx, ppf_temperature t, const R* __restrict__ tr,
+                                                             int64_t tsy, int64_t tsf, NcK k, uint64_t seed,
+                                                             StepRef sr) {
+  ncomp_step_body<R, N, true, true>(in, out, b, d, dx, t, tr, tsy, tsf, k, seed, sr);
+}
+
 template <typename R, bool FUSED>
 int ncomp_step_launch(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
                       const ppf_grid* tg, const void* transfer, const ppf_ncomp_params& p, const ppf_thermo* th,
@@ -897,16 +1009,19 @@ int ncomp_step_launch(const ppf_grid& g, const void* in, void* out, const void* 
   const int64_t tsy = tg ? tg->sy : 0, tsf = tg ? tg->sf : 0;
   switch (p.ncomp) {
     case 2:
-      hipLaunchKernelGGL((k_ncomp_step<R, 2, FUSED>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d,
-                         R(g.dx), t, tr, tsy, tsf, k, seed, step);
+      hipLaunchKernelGGL((FUSED && th && th->tables ? k_ncomp_step_tdb<R, 2> : k_ncomp_step<R, 2, FUSED>),
+                         dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t, tr, tsy, tsf, k, seed,
+                         step);
       break;
     case 3:
-      hipLaunchKernelGGL((k_ncomp_step<R, 3, FUSED>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d,
-                         R(g.dx), t, tr, tsy, tsf, k, seed, step);
+      hipLaunchKernelGGL((FUSED && th && th->tables ? k_ncomp_step_tdb<R, 3> : k_ncomp_step<R, 3, FUSED>),
+                         dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t, tr, tsy, tsf, k, seed,
+                         step);
       break;
     case 4:
-      hipLaunchKernelGGL((k_ncomp_step<R, 4, FUSED>), dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d,
-                         R(g.dx), t, tr, tsy, tsf, k, seed, step);
+      hipLaunchKernelGGL((FUSED && th && th->tables ? k_ncomp_step_tdb<R, 4> : k_ncomp_step<R, 4, FUSED>),
+                         dim3(blocks), dim3(kThreads), 0, s, fin, fout, bc, d, R(g.dx), t, tr, tsy, tsf, k, seed,
+                         step);
       break;
     default:
       return fail(PPF_EINVAL, "ncomp_step: ncomp must be 2..4");

@@ -343,14 +343,35 @@ def parse_piecewise(text, resolve):
 # --------------------------------------------------------------- database --
 class Database:
     """elements, functions {name: Piecewise}, phases {name: constituents},
-    parameters {(kind 'G'|'L', phase, constituents tuple, order): Piecewise}"""
+    parameters {(kind 'G'|'L', phase, constituents tuple, order): Piecewise}.
+
+    Read leniently, as a database of many elements is used for a few: what
+    is outside the supported subset is recorded, not refused, and refused
+    only when a model for the components that touch it is built
+    (``thermo.load_tdb_model``):
+      * phases of several sublattices (``multi_sublattice``, name -> count)
+        and their parameters (``unsupported``);
+      * functions and parameters whose expressions fall outside the
+        temperature basis (``unsupported``, with the reason);
+      * parameter types other than G and L (TC, BMAGN, ...) are skipped
+        (``ignored``): the reference's phase model is
+        ``Model.redlich_kister_sum`` over G and L parameters plus the ideal
+        mixing energy (simulation.py:206-214), which leaves out the magnetic
+        and other contributions these describe.
+    A G parameter of several interacting species is an interaction term like
+    L with the same constituents (redlich_kister_sum takes both types
+    alike); it is stored under kind 'L'."""
 
     def __init__(self, text):
         self.elements = []
         self.functions = {}
         self._function_text = {}
+        self._function_error = {}
         self.phases = {}
+        self.multi_sublattice = {}
         self.parameters = {}
+        self.unsupported = {}
+        self.ignored = {}
         commands = _commands(text)
         # functions may reference functions defined later: resolve lazily
         for cmd in commands:
@@ -359,7 +380,11 @@ class Database:
                 name, rest = cmd.split(None, 2)[1:]
                 self._function_text[name.upper().rstrip("#")] = rest
         for name in list(self._function_text):
-            self._function(name)
+            try:
+                self._function(name)
+            except TDBError as e:
+                self._function_error[name] = e
+                self.functions.pop(name, None)
         for cmd in commands:
             word = cmd.split(None, 1)[0].upper()
             if word.startswith("ELEM"):
@@ -371,7 +396,8 @@ class Database:
                 name = f[1].upper().split(":")[0]
                 nsub = int(f[3])
                 if nsub != 1:
-                    raise TDBError("phase %s has %d sublattices; one is supported" % (name, nsub))
+                    self.multi_sublattice[name] = nsub
+                    continue
                 self.phases.setdefault(name, [])
             elif word.startswith("CONST"):
                 m = re.match(r"\S+\s+([^\s:]+)\s*:(.*):", cmd, re.S)
@@ -379,14 +405,18 @@ class Database:
                     raise TDBError("cannot read %r" % cmd)
                 name = m.group(1).upper().split(":")[0]
                 subl = [s for s in m.group(2).split(":") if s.strip()]
-                if len(subl) != 1:
-                    raise TDBError("phase %s: one sublattice is supported" % name)
+                if name in self.multi_sublattice or len(subl) != 1:
+                    self.multi_sublattice.setdefault(name, len(subl))
+                    self.phases.pop(name, None)
+                    continue
                 self.phases[name] = [c.strip().upper().rstrip("%") for c in subl[0].split(",") if c.strip()]
             elif word.startswith("PARA"):
                 self._parameter(cmd)
 
     def _function(self, name):
         name = name.upper().rstrip("#")
+        if name in self._function_error:
+            raise self._function_error[name]
         if name in self.functions:
             return self.functions[name]
         if name not in self._function_text:
@@ -401,20 +431,29 @@ class Database:
         return self._function(name)
 
     def _parameter(self, cmd):
-        m = re.match(r"\S+\s+([GL])\s*\(\s*([^,]+?)\s*,\s*([^;]+?)\s*;\s*(\d+)\s*\)\s*(.*)$", cmd, re.S | re.I)
+        m = re.match(r"\S+\s+(\w+)\s*\(\s*([^,]+?)\s*,\s*([^;]+?)\s*;\s*(\d+)\s*\)\s*(.*)$", cmd, re.S | re.I)
         if not m:
             raise TDBError("cannot read parameter %r" % cmd[:60])
         kind = m.group(1).upper()
         phase = m.group(2).upper()
-        cons = tuple(c.strip().upper() for c in m.group(3).split(","))
-        if any(":" in c for c in cons):
-            raise TDBError("parameter %r: one sublattice is supported" % cmd[:60])
+        cons_text = m.group(3).upper()
         order = int(m.group(4))
-        if kind == "G" and len(cons) != 1:
-            raise TDBError("G parameters of one constituent are supported: %r" % cmd[:60])
-        if kind == "L" and len(cons) != 2:
-            raise TDBError("binary L parameters are supported: %r" % cmd[:60])
-        self.parameters[(kind, phase, cons, order)] = parse_piecewise(m.group(5), self._resolve)
+        if kind not in ("G", "L"):
+            self.ignored[kind] = self.ignored.get(kind, 0) + 1
+            return
+        if ":" in cons_text:
+            self.unsupported[(phase, cons_text, order)] = "parameter of several sublattices"
+            return
+        cons = tuple(c.strip().rstrip("%") for c in cons_text.split(","))
+        key = ("G" if len(cons) == 1 else "L", phase, cons, order)
+        try:
+            f = parse_piecewise(m.group(5), self._resolve)
+        except TDBError as e:
+            self.unsupported[(phase, ",".join(cons), order)] = str(e)
+            return
+        if key in self.parameters:
+            raise TDBError("parameter %s(%s,%s;%d) given twice" % (key[0], phase, ",".join(cons), order))
+        self.parameters[key] = f
 
     def constant(self, name):
         """value of a constant FUNCTION (the engine parameters, NCGPU_new.py:358-383)"""

@@ -75,15 +75,22 @@ class RKPhase:
 R_PYCALPHAD = 8.3145
 
 
-class TablePhase:
-    """A TDB phase of one sublattice: unary G (Piecewise per component) and
-    Redlich-Kister terms L^k (Piecewise per component pair i < j and order k),
-    evaluated on the device from ppf_tdb_tables."""
+def triples(n):
+    """(i, j, l), i < j < l, in lexicographic order (ppf_tdb_phase.T3)."""
+    return [(i, j, l) for i in range(n) for j in range(i + 1, n) for l in range(j + 1, n)]
 
-    def __init__(self, name, g0, L, R=R_PYCALPHAD):
+
+class TablePhase:
+    """A TDB phase of one sublattice: unary G (Piecewise per component),
+    binary Redlich-Kister terms L^k (Piecewise per component pair i < j and
+    order k = 0..4) and ternary terms (Piecewise per triple i < j < l and
+    index k = 0..2), evaluated on the device from ppf_tdb_tables."""
+
+    def __init__(self, name, g0, L, R=R_PYCALPHAD, T3=None):
         self.name = name
         self.g0 = list(g0)
         self.L = dict(L)
+        self.T3 = dict(T3 or {})
         self.R = float(R)
 
     @property
@@ -91,18 +98,18 @@ class TablePhase:
         return len(self.g0)
 
     def tables(self):
-        """(g0, L) in the oracle's table format: [(T_hi, c, n)] per component and
-        [[(T_hi, c, n)] * 3] per pair"""
+        """(g0, L) in the oracle's table format: [(T_hi, c, n)] per component
+        and {"pairs": [[(T_hi, c, n)] * 5] per pair, "triples": [[...] * 3] per
+        triple}"""
         n = self.ncomp
         zero = (np.full(_lib.PPF_TPOLY_RANGES, np.inf), np.zeros((_lib.PPF_TPOLY_RANGES, _lib.PPF_TPOLY_TERMS)), 0)
-        g0 = [f.device_table(_lib.PPF_TPOLY_RANGES) for f in self.g0]
-        L = []
-        for p in range(n * (n - 1) // 2):
-            row = []
-            for k in range(3):
-                f = self.L.get((p, k))
-                row.append(f.device_table(_lib.PPF_TPOLY_RANGES) if f is not None else zero)
-            L.append(row)
+
+        def tab(f):
+            return f.device_table(_lib.PPF_TPOLY_RANGES) if f is not None else zero
+        g0 = [tab(f) for f in self.g0]
+        L = {"pairs": [[tab(self.L.get((p, k))) for k in range(_lib.PPF_RK_ORDERS)]
+                       for p in range(n * (n - 1) // 2)],
+             "triples": [[tab(self.T3.get((t, k))) for k in range(3)] for t in range(len(triples(n)))]}
         return g0, L
 
     def fill(self, st):
@@ -118,9 +125,12 @@ class TablePhase:
                     dst.c[r][k] = float(c[r, k])
         for i, tab in enumerate(g0):
             put(st.g0[i], tab)
-        for p, row in enumerate(L):
+        for p, row in enumerate(L["pairs"]):
             for k, tab in enumerate(row):
                 put(st.L[p][k], tab)
+        for t, row in enumerate(L["triples"]):
+            for k, tab in enumerate(row):
+                put(st.T3[t][k], tab)
 
     def to_ctypes(self):
         p = _lib.RKPhase()
@@ -284,49 +294,86 @@ def synthetic_rk_model(components=("AL", "CU", "NI"), seed=0, T_range=(1000.0, 3
 
 
 def load_tdb_model(path, components=None, phases=None):
-    """ThermoModel from a TDB file (one-sublattice phases, unary G and binary
-    Redlich-Kister L^0..L^2), replacing pycalphad's Database + Model
+    """ThermoModel from a TDB file (one-sublattice phases: unary G, binary
+    Redlich-Kister L^0..L^4 and ternary L^0..L^2), replacing pycalphad's
+    Database + Model.redlich_kister_sum + ideal_mixing_energy
     (simulation.py:196-214).  Components default to the file's elements and
     phases to its phases, both sorted alphabetically as the reference sorts
     them (simulation.py:201-202): the first phase is the solid (FCC_A1), the
-    second the liquid (NCGPU_new.py:353-354).  Engine parameters are the
-    file's constant FUNCTIONs (NCGPU_new.py:358-383)."""
+    second the liquid (NCGPU_new.py:353-354).  Only parameters whose
+    constituents are all among the components enter (pycalphad's Model does
+    the same); among those, what the model cannot represent (several
+    sublattices, four-species interactions, binary orders above 4,
+    expressions outside the temperature basis) is refused with its name.
+    A ternary parameter's order k names its k-th constituent as written;
+    when only order 0 of a triple is given, orders 1 and 2 take the same
+    function (pycalphad's rule for symmetric ternary parameters).  Engine
+    parameters are the file's constant FUNCTIONs (NCGPU_new.py:358-383)."""
     from . import tdb
     db = tdb.read(path)
     comps = sorted(c.upper() for c in (components or db.elements))
-    names = sorted(p.upper() for p in (phases or db.phases))
+    names = sorted(p.upper() for p in (phases or list(db.phases) + list(db.multi_sublattice)))
     if len(names) != 2:
         raise ValueError("the N-component engine needs two phases, %s has %s" % (path, names))
     built = {}
+    cset = set(comps)
+    index = {c: i for i, c in enumerate(comps)}
     for name in names:
+        if name in db.multi_sublattice:
+            raise ValueError("phase %s of %s has %d sublattices; one is supported"
+                             % (name, path, db.multi_sublattice[name]))
         cons = db.phases.get(name)
         if cons is None:
             raise ValueError("phase %s not in %s" % (name, path))
         missing = [c for c in comps if c not in cons]
         if missing:
             raise ValueError("phase %s has no constituent %s" % (name, missing))
+        for (ph, cons_text, order), why in db.unsupported.items():
+            species = set(x.strip() for x in cons_text.replace(":", ",").split(","))
+            if ph == name and species <= cset | {"VA", "*"}:
+                raise ValueError("parameter (%s, %s; %d) of %s: %s" % (name, cons_text, order, path, why))
         g0 = []
         for c in comps:
             f = db.parameters.get(("G", name, (c,), 0))
             if f is None:
                 raise ValueError("no G(%s,%s;0) in %s" % (name, c, path))
             g0.append(f)
-        L = {}
-        for p, (i, j) in enumerate(pairs(len(comps))):
-            a, b = comps[i], comps[j]
-            for k in range(3):
-                f = db.parameters.get(("L", name, (a, b), k))
-                g = db.parameters.get(("L", name, (b, a), k))
-                if f is not None and g is not None:
-                    raise ValueError("L(%s,%s,%s;%d) given twice" % (name, a, b, k))
-                if g is not None:  # L^k_ba = (-1)^k L^k_ab
-                    f = g if k % 2 == 0 else -g
-                if f is not None:
-                    L[(p, k)] = f
-        for key in db.parameters:
-            if key[0] == "L" and key[1] == name and key[3] > 2:
-                raise ValueError("Redlich-Kister order %d of %s is beyond the supported 0..2" % (key[3], name))
-        built[name] = TablePhase(name, g0, L)
+        L, T3, given3 = {}, {}, {}
+        pair_index = {pr: p for p, pr in enumerate(pairs(len(comps)))}
+        trip_index = {tr: t for t, tr in enumerate(triples(len(comps)))}
+        for (kind, ph, sp, k), f in db.parameters.items():
+            if kind != "L" or ph != name or not set(sp) <= cset:
+                continue
+            if len(set(sp)) != len(sp):
+                raise ValueError("parameter L(%s,%s;%d) repeats a species" % (name, ",".join(sp), k))
+            idx = [index[x] for x in sp]
+            if len(sp) == 2:
+                if k > 4:
+                    raise ValueError("Redlich-Kister order %d of %s is beyond the supported 0..4" % (k, name))
+                key = (pair_index[tuple(sorted(idx))], k)
+                if key in L:
+                    raise ValueError("L(%s,%s;%d) given twice" % (name, ",".join(sp), k))
+                # L^k_ba = (-1)^k L^k_ab
+                L[key] = f if idx[0] < idx[1] or k % 2 == 0 else -f
+            elif len(sp) == 3:
+                if k > 2:
+                    raise ValueError("ternary parameter L(%s,%s;%d): orders 0..2 name a constituent"
+                                     % (name, ",".join(sp), k))
+                t = trip_index[tuple(sorted(idx))]
+                slot = sorted(idx).index(idx[k])  # order k names constituent sp[k]
+                if (t, slot) in T3:
+                    raise ValueError("ternary L(%s,%s) given twice for %s" % (name, ",".join(sp), sp[k]))
+                T3[(t, slot)] = f
+                given3.setdefault(t, set()).add(k)
+            else:
+                raise ValueError("parameter L(%s,%s;%d): interactions of %d species are not supported"
+                                 % (name, ",".join(sp), k, len(sp)))
+        for t, ks in given3.items():
+            if ks == {0}:
+                f0 = [T3[(t, sl)] for sl in range(3) if (t, sl) in T3][0]
+                for sl in range(3):
+                    T3[(t, sl)] = f0
+        built[name] = TablePhase(name, g0, L, T3=T3)
     functions = {}
     for fname in db.functions:
         f = db.functions[fname]

@@ -46,7 +46,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.RKPhase) == 8 + 8 + 4 * 5 * 8 + 6 * 3 * 2 * 8
     assert ctypes.sizeof(_lib.Thermo) == 2 * ctypes.sizeof(_lib.RKPhase) + 16
     assert ctypes.sizeof(_lib.TPoly) == 440
-    assert ctypes.sizeof(_lib.TdbTables) == 2 * 22 * 440
+    assert ctypes.sizeof(_lib.TdbTables) == 2 * (4 + 6 * 5 + 4 * 3) * 440
     from pyphasefield_amd.ensemble import EnsInstance
     assert ctypes.sizeof(EnsInstance) == 88
     from pyphasefield_amd._lib_ext import MlpNet

@@ -16,11 +16,14 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden", "tdb")
 FP64_TOL = 1e-10
 
 
-def test_thermo_eval_with_tdb_tables(gpu):
+@pytest.mark.parametrize("name,comps", [("Ni-Cu-Al_Ideal.tdb", None), ("mc_simplified.tdb", ["AL", "CR", "NI"])])
+def test_thermo_eval_with_tdb_tables(gpu, name, comps):
+    """Device tables against the oracle's evaluation of the same tables;
+    mc_simplified adds binary orders up to 4 and a ternary term."""
     import torch
     from oracle import thermo as oth
     from pyphasefield_amd import _lib, thermo
-    model = thermo.load_tdb_model(os.path.join(GOLD, "Ni-Cu-Al_Ideal.tdb"))
+    model = thermo.load_tdb_model(os.path.join(GOLD, name), components=comps)
     rng = np.random.default_rng(1)
     n = 2000
     T = rng.uniform(400, 2900, n)   # every temperature range of the Al, Cu, Ni functions
@@ -43,10 +46,13 @@ def test_thermo_eval_with_tdb_tables(gpu):
             assert np.max(np.abs(o[:, k + 1 + l] - d[l])) <= 1e-12 * np.abs(d[l]).max()
 
 
+_COMPONENTS = {"mc_simplified.tdb": ["AL", "CR", "NI"]}
+
+
 def _tdb_sim(name, precision, ny=30, nx=44, T0=1300.0):
     import pyphasefield_amd.Engines as E
     path = os.path.join(GOLD, name)
-    sim = E.NCGPU_new(dimensions=[ny, nx], seed=11, tdb_path=path)
+    sim = E.NCGPU_new(dimensions=[ny, nx], seed=11, tdb_path=path, tdb_components=_COMPONENTS.get(name))
     sim.set_framework("GPU_SERIAL")
     sim.set_dx(0.0000046)
     sim.set_temperature_type("LINEAR_GRADIENT")
@@ -56,7 +62,7 @@ def _tdb_sim(name, precision, ny=30, nx=44, T0=1300.0):
     sim.set_temperature_dTdt(0.0)
     sim.set_boundary_conditions("NEUMANN")
     sim.set_precision(precision)
-    ncomp = 3 if "Al" in name else 2
+    ncomp = 3 if ("Al" in name or name in _COMPONENTS) else 2
     ica = [0.3, 0.3][:ncomp - 1]
     sim.set_user_data({"d_ratio": 1.1, "sim_type": "seeds", "number_of_seeds": 3, "seed_radius": 4,
                        "initial_concentration_array": ica, "noise_phi": 0.0, "noise_c": 0.0, "noise_q": 0.0})
@@ -64,7 +70,7 @@ def _tdb_sim(name, precision, ny=30, nx=44, T0=1300.0):
     return sim
 
 
-@pytest.mark.parametrize("name", ["Ni-Cu-Al_Ideal.tdb", "Ni-Nb_Simplified.tdb"])
+@pytest.mark.parametrize("name", ["Ni-Cu-Al_Ideal.tdb", "Ni-Nb_Simplified.tdb", "mc_simplified.tdb"])
 def test_ncomp_fp64_on_reference_tdb(gpu, name):
     sim = _tdb_sim(name, "FP64")
     sim.just_before_simulating()

@@ -107,14 +107,133 @@ def test_engine_parameters_and_defaults():
     assert model.solid.R == thermo.R_PYCALPHAD
 
 
-def test_unsupported_features_raise():
+def test_unsupported_features_are_recorded_then_refused_when_used(tmp_path):
+    """A large database is read leniently; what the model cannot represent is
+    refused when a model for components that touch it is built."""
+    db = tdb.Database("PHASE X % 2 1 1 ! CONSTITUENT X :A:B: ! PARAMETER G(X,A:B;0) 298.15 1; 6000 N ! "
+                      "PARAMETER TC(X,A:B;0) 298.15 1; 6000 N ! FUNCTION F 298.15 1+G; 6000 N !")
+    assert db.multi_sublattice == {"X": 2} and "X" not in db.phases
+    assert db.ignored == {"TC": 1}
     with pytest.raises(tdb.TDBError):
-        tdb.Database("PHASE X % 2 1 1 ! CONSTITUENT X :A:B: !")
-    with pytest.raises(tdb.TDBError):
-        tdb.Database("ELEMENT A X 1 1 1 ! ELEMENT B X 1 1 1 ! ELEMENT C X 1 1 1 ! PHASE P % 1 1 ! "
-                     "CONSTITUENT P :A,B,C: ! PARAMETER L(P,A,B,C;0) 298.15 1; 6000 N !")
+        db._function("F")
     db = tdb.Database("FUNCTION F 298.15 1+T**4; 6000 N !")
     with pytest.raises(tdb.TDBError):
         db.functions["F"].device_table(6)
-    with pytest.raises(tdb.TDBError):
-        tdb.Database("FUNCTION F 298.15 1+G; 6000 N !")
+
+    def model(params, comps=("A", "B", "C"), phases=None, extra=""):
+        text = ("ELEMENT A X 1 1 1 ! ELEMENT B X 1 1 1 ! ELEMENT C X 1 1 1 ! ELEMENT D X 1 1 1 ! "
+                "PHASE LIQUID % 1 1 ! CONSTITUENT LIQUID :A,B,C,D: ! PHASE FCC_A1 % 1 1 ! "
+                "CONSTITUENT FCC_A1 :A,B,C,D: ! " + extra)
+        for ph in ("LIQUID", "FCC_A1"):
+            for el in "ABCD":
+                text += "PARAMETER G(%s,%s;0) 298.15 -1000*T; 6000 N ! " % (ph, el)
+        path = tmp_path / "m.tdb"
+        path.write_text(text + params)
+        return thermo.load_tdb_model(str(path), components=list(comps), phases=phases)
+
+    quaternary = "PARAMETER L(LIQUID,A,B,C,D;0) 298.15 5; 6000 N !"
+    with pytest.raises(ValueError, match="4 species"):
+        model(quaternary, comps="ABCD")
+    model(quaternary)  # D not selected: the parameter does not enter
+    with pytest.raises(ValueError, match="0..4"):
+        model("PARAMETER L(LIQUID,A,B;5) 298.15 5; 6000 N !")
+    with pytest.raises(ValueError, match="undefined"):
+        model("PARAMETER L(LIQUID,A,B;0) 298.15 5+UNDEF#; 6000 N !")
+    model("PARAMETER L(LIQUID,A,D;0) 298.15 5+UNDEF#; 6000 N !")  # D not selected
+    with pytest.raises(ValueError, match="sublattices"):
+        model("", phases=["LIQUID", "X"], extra="PHASE X % 2 1 1 ! CONSTITUENT X :A,B,C,D:VA: ! ")
+
+
+def _excess(model, which, cs, T):
+    g0, L, R = model.oracle_phase(which)
+    lnT = ot.plog(T)
+    unary = sum(c * ot.tpoly(g, T, lnT) for c, g in zip(cs, g0))
+    ideal = R * T * sum(np.maximum(c, ot.CMIN) * ot.plog(np.maximum(c, ot.CMIN)) for c in cs)
+    return ot.gibbs(cs, T, g0, L, R) - unary - ideal
+
+
+def _ab_model(tmp_path, params):
+    text = ("ELEMENT A X 1 1 1 ! ELEMENT B X 1 1 1 ! ELEMENT C X 1 1 1 ! "
+            "PHASE LIQUID % 1 1 ! CONSTITUENT LIQUID :A,B,C: ! PHASE FCC_A1 % 1 1 ! CONSTITUENT FCC_A1 :A,B,C: ! ")
+    for ph in ("LIQUID", "FCC_A1"):
+        for el in "ABC":
+            text += "PARAMETER G(%s,%s;0) 298.15 -1000*T; 6000 N ! " % (ph, el)
+    path = tmp_path / "m.tdb"
+    path.write_text(text + params)
+    return thermo.load_tdb_model(str(path))
+
+
+def _comps(seed=0, n=400):
+    c = np.random.default_rng(seed).dirichlet([1.5, 1.5, 1.5], n)
+    return [c[:, 0], c[:, 1], c[:, 2]], np.random.default_rng(seed + 1).uniform(400, 2500, n)
+
+
+def test_binary_orders_up_to_four_with_reversed_pair(tmp_path):
+    """L(LIQUID,B,A;k) for k = 0..4 written with the pair reversed: the excess
+    is c_A c_B sum_k L^k_BA (c_B - c_A)^k, computed here with explicit powers
+    (the oracle and device use Horner in (c_A - c_B) with (-1)^k L^k)."""
+    Lk = [-12000.0, 3100.0, -2200.0, 1700.0, -900.0]
+    params = "".join("PARAMETER L(LIQUID,B,A;%d) 298.15 %r+%r*T; 6000 N ! " % (k, v, 0.5 * k)
+                     for k, v in enumerate(Lk))
+    m = _ab_model(tmp_path, params)
+    cs, T = _comps()
+    want = sum(cs[0] * cs[1] * (v + 0.5 * k * T) * np.power(cs[1] - cs[0], k) for k, v in enumerate(Lk))
+    np.testing.assert_allclose(_excess(m, "liquid", cs, T), want, rtol=1e-11, atol=1e-7)
+    np.testing.assert_allclose(_excess(m, "solid", cs, T), 0.0, atol=1e-7)
+
+
+def test_ternary_terms(tmp_path):
+    """Three-species interactions (pycalphad's Muggianu form):
+    * only L^0 given: the ternary excess is c_A c_B c_C L^0 (the three
+      weights sum to one);
+    * L^0, L^1, L^2 distinct and written in the order (C, A, B): order k
+      names the k-th written constituent, so the model equals the one
+      written (A, B, C) with the values permuted; at c_A = c_B = c_C every
+      weight is 1/3;
+    * the analytic derivatives along e_l - e_N agree with central
+      differences of G (orders 3..4 and ternary terms included)."""
+    cs, T = _comps(3)
+    m = _ab_model(tmp_path, "PARAMETER L(LIQUID,A,B,C;0) 298.15 7000-2*T; 6000 N ! ")
+    np.testing.assert_allclose(_excess(m, "liquid", cs, T), cs[0] * cs[1] * cs[2] * (7000 - 2 * T),
+                               rtol=1e-12, atol=1e-9)
+    V = {"A": 4000.0, "B": -2500.0, "C": 1200.0}
+    written = "".join("PARAMETER L(LIQUID,C,A,B;%d) 298.15 %r; 6000 N ! " % (k, V[x]) for k, x in enumerate("CAB"))
+    canon = "".join("PARAMETER L(LIQUID,A,B,C;%d) 298.15 %r; 6000 N ! " % (k, V[x]) for k, x in enumerate("ABC"))
+    m1, m2 = _ab_model(tmp_path, written), _ab_model(tmp_path, canon)
+    np.testing.assert_array_equal(_excess(m1, "liquid", cs, T), _excess(m2, "liquid", cs, T))
+    third = [np.full(3, 1 / 3.0)] * 3
+    Tt = np.full(3, 1000.0)
+    np.testing.assert_allclose(_excess(m2, "liquid", third, Tt), (1 / 27.0) * sum(V.values()) / 3.0, rtol=1e-12,
+                               atol=1e-9)  # G holds -1e6 J/mol of unary terms
+    # derivatives: binary orders 3..4 and a ternary together
+    extra = ("PARAMETER L(LIQUID,A,B;3) 298.15 1500; 6000 N ! PARAMETER L(LIQUID,B,C;4) 298.15 -800; 6000 N ! ")
+    m3 = _ab_model(tmp_path, canon + extra)
+    g0, L, R = m3.oracle_phase("liquid")
+    d = ot.analytic_derivatives(cs, T, g0, L, R)
+    h = 1e-6
+    for l in range(2):
+        up = [c.copy() for c in cs]
+        dn = [c.copy() for c in cs]
+        up[l] = up[l] + h
+        up[2] = up[2] - h
+        dn[l] = dn[l] - h
+        dn[2] = dn[2] + h
+        fd = (ot.gibbs(up, T, g0, L, R) - ot.gibbs(dn, T, g0, L, R)) / (2 * h)
+        np.testing.assert_allclose(d[l], fd, rtol=1e-6, atol=1e-3)
+
+
+def test_reference_multicomponent_database():
+    """mc_simplified.tdb (the reference's tests): 22 elements, LIQUID and a
+    magnetic FCC_A1 whose TC / BMAGN parameters the reference's model leaves
+    out (it sums G and L only, simulation.py:206-214).  For Al-Cr-Ni: the
+    liquid's Al-Ni orders reach 4, and the one ternary L(LIQUID,AL,CR,NI;0) =
+    16000 stands for all three orders."""
+    db = _db("mc_simplified.tdb")
+    assert db.ignored == {"TC": 30, "BMAGN": 20} and not db.multi_sublattice
+    m = thermo.load_tdb_model(os.path.join(GOLD, "mc_simplified.tdb"), components=["AL", "CR", "NI"])
+    assert m.solid.name == "FCC_A1" and m.liquid.name == "LIQUID"
+    assert (1, 4) in m.liquid.L  # pair (AL, NI), order 4
+    T = np.array([1000.0, 1700.0])
+    for k in range(3):
+        np.testing.assert_array_equal(m.liquid.T3[(0, k)].evaluate(T), [16000.0, 16000.0])
+    assert m.value("NI_TM") > 1700
