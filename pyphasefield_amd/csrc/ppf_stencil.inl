@@ -1081,15 +1081,21 @@ __device__ __forceinline__ double block_sum(double v) {
 #ifndef PPF_REDUCE_ROWS
 #define PPF_REDUCE_ROWS 4
 #endif
+// the energy kernel's rows b, b + G, ... start from the XCD-swizzled block
+// index, so rows r and r + 1 are read on one XCD and the y neighbour comes
+// from its L2 (512^3: 251 -> 205 us)
+#ifndef PPF_REDUCE_ESWZ
+#define PPF_REDUCE_ESWZ 1
+#endif
 constexpr int kRRows = PPF_REDUCE_ROWS;
 
 struct RowWalk {
   int64_t G, ny, dy, dz, cy, cz;
-  __device__ RowWalk(int64_t ny_) : G(gridDim.x), ny(ny_) {
+  __device__ RowWalk(int64_t ny_, int64_t start) : G(gridDim.x), ny(ny_) {
     dz = G / ny;
     dy = G - dz * ny;
-    cz = int64_t(blockIdx.x) / ny;
-    cy = int64_t(blockIdx.x) - cz * ny;
+    cz = start / ny;
+    cy = start - cz * ny;
   }
   // interior (y, z) (1-based) of the current row, then advance by G rows
   __device__ __forceinline__ void next(int64_t& y, int64_t& z) {
@@ -1112,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_reduce_sum(const R* __restrict__ in, Do
   const int64_t nx = d.ax.n, ny = d.ndim >= 2 ? d.ay.n : 1, nz = d.ndim >= 3 ? d.az.n : 1;
   const int64_t rows = ny * nz;
   const int64_t sy = d.ndim >= 2 ? d.ay.stride : 0, sz = d.ndim >= 3 ? d.az.stride : 0;
-  RowWalk w(ny);
+  RowWalk w(ny, blockIdx.x);
   double acc = 0.0;
   for (int64_t r = blockIdx.x; r < rows; r += kRRows * w.G) {
     const R* rowp[kRRows];
@@ -1175,9 +1181,15 @@ __global__ __launch_bounds__(256) void k_reduce_energy(const R* __restrict__ in,
   const double idx = 1.0 / double(dx);
   const double ke = 0.5 * eps * eps;
   auto last_layer = [&](int64_t y, int64_t z) { return (ND >= 2 && y == ny) || (ND >= 3 && z == nz); };
-  RowWalk w(ny);
+#if PPF_REDUCE_ESWZ
+  // rows r and r + 1 on one XCD (the y neighbour from the same L2)
+  const int64_t r0 = xcd_swizzle(blockIdx.x, gridDim.x);
+#else
+  const int64_t r0 = blockIdx.x;
+#endif
+  RowWalk w(ny, r0);
   double acc = 0.0;
-  for (int64_t r = blockIdx.x; r < rows; r += kE * w.G) {
+  for (int64_t r = r0; r < rows; r += kE * w.G) {
     const R* rowp[kE];
     int64_t yy[kE], zz[kE];
     bool keep[kE];
