@@ -435,15 +435,17 @@ int ppf_cahnhilliard_line_solve(const ppf_grid* g, const void* in, void* out, do
 int ppf_thermo_eval(const ppf_thermo* th, int64_t n, const double* T, const double* c,
                     double* out, ppf_stream_t s);
 
-/* ---- surrogate thermodynamics (BF16 MFMA, FP32 accumulate) -------------
+/* ---- surrogate thermodynamics (FP16 / BF16 MFMA, FP32 accumulate) -------
    No counterpart in the reference: a learned replacement for the per-cell
    Gibbs-energy evaluation of NComponent_helper_kernel (NCGPU_new.py:295-305),
    trained by the kernels below on samples of the plain evaluation. */
 
 /* C[M x N] = alpha A B + beta C with A(i,k) = A[i*sai + k*sak],
-   B(k,j) = B[k*sbk + j*sbj], C row-major (ldc); FP32 in memory, BF16 MFMA
-   products with FP32 accumulation.  A is rounded to BF16; split_b = 1 feeds B
-   as the BF16 pair hi + lo (hi = bf16(b), lo = bf16(b - hi)).  ksplit > 1
+   B(k,j) = B[k*sbk + j*sbj], C row-major (ldc); FP32 in memory, MFMA
+   products with FP32 accumulation.  split_b selects the operand formats:
+   0 = A and B rounded to BF16; 1 = A in BF16, B as the BF16 pair hi + lo
+   (hi = bf16(b), lo = bf16(b - hi)); 2 = A in FP16, B as the FP16 pair
+   hi + lo (the inference kernel's formats); other values are rejected.  ksplit > 1
    splits K across blocks with atomic accumulation (beta must be 0 and C
    zeroed by the caller). */
 int ppf_mlp_gemm(int M, int N, int K, const float* A, int64_t sai, int64_t sak, const float* B,
@@ -464,19 +466,21 @@ int ppf_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float 
                  float eps, int64_t t, ppf_stream_t s);
 
 /* Two trained 3-64-64-3 ReLU networks (liquid, solid), inputs (T, c1, c2),
-   BF16 weights, FP32 accumulation; inputs as BF16 hi + lo pairs, hidden
-   activations as pairs or BF16 (flags).
+   FP16 weights, FP32 accumulation; inputs as FP16 hi + lo pairs, hidden
+   activations as pairs or FP16 (flags).  The packed layers carry power-of-two
+   scales that keep every hidden value of an accepted cell in [0, 1]
+   (pyphasefield_amd.surrogate.pack_phase); b2 is stored divided by its scale.
    Outputs (normalised): liquid (G_ni, dG_ni/dc1, dG_ni/dc2); solid
    (G_S,ni - G_L,ni, dG_S,ni/dc1, dG_S,ni/dc2), where "ni" is G minus the ideal
    mixing term R T sum c ln c (evaluated analytically in the kernel). */
 typedef struct ppf_mlp_net {
-  const void* weights[2];  /* per phase: packed BF16 MFMA operands (see pyphasefield_amd.surrogate) */
+  const void* weights[2];  /* per phase: packed FP16 MFMA operands (see pyphasefield_amd.surrogate) */
   const float* bias[2];    /* per phase: b1[64], b2[64], b3[4] (FP32) */
   float in_lo[3], in_scale[3];          /* x_hat = (x - lo) * scale - 1 */
   float out_mu[2][3], out_sigma[2][3];  /* y = y_hat * sigma + mu */
   float T_lo, T_hi, c_lo;  /* range gate: cells outside use the plain FP64 evaluation */
   int32_t flags;           /* PPF_MLP_SPLIT2 / PPF_MLP_SPLIT3: the inputs of layer 2 / 3 enter as
-                              BF16 hi + lo pairs (else BF16 only) */
+                              FP16 hi + lo pairs (else FP16 only) */
   int32_t* marks;          /* optional device int32[2], zero-initialised once: counts of cells
                               handed to the FP64 fallback (lets it skip its scan when none are);
                               NULL: the fallback always scans */

@@ -60,6 +60,7 @@ def build(verbose=False, defines=(), lib=LIB, objdir=OBJ):
     -D options (tile and occupancy knobs of the kernels) for a variant build
     into another `lib` / `objdir`."""
     os.makedirs(objdir, exist_ok=True)
+    os.makedirs(os.path.dirname(os.path.abspath(lib)), exist_ok=True)
     missing = [k for k in UNITS if not os.path.exists(os.path.join(CSRC, k))]
     if missing:
         raise FileNotFoundError("missing HIP sources: %s" % ", ".join(missing))

@@ -1,9 +1,9 @@
-// Surrogate thermodynamics: BF16-MFMA kernels for a small MLP (forward and
+// Surrogate thermodynamics: MFMA kernels for a small MLP (forward and
 // backward GEMMs for training, elementwise ops, Adam) and the N-component
 // thermodynamic pass that evaluates the trained networks per cell.
 //
-// All matrix products use v_mfma_f32_32x32x8_bf16 (FP32 accumulate).  Operand
-// layout of that instruction, lane l (0..63), element t (0..3):
+// The training GEMMs use v_mfma_f32_32x32x8_bf16 / _f16 (FP32 accumulate).
+// Operand layout of those instructions, lane l (0..63), element t (0..3):
 //   A (32 x 8):  A[l % 32][4 * (l / 32) + t]
 //   B (8 x 32):  B[4 * (l / 32) + t][l % 32]
 //   D (32 x 32): item v (0..15) holds D[8 * (v / 4) + 4 * (l / 32) + v % 4][l % 32]
@@ -37,15 +37,21 @@ __device__ __forceinline__ f32x16 mfma32(bf16x4 a, bf16x4 b, f32x16 c) {
 // C[M x N] = alpha * A[M x K] B[K x N] (+ beta C); A(i,k) = A[i*sai + k*sak],
 // B(k,j) = B[k*sbk + j*sbj], C row-major with ldc.  One wave per 32x32 tile;
 // gridDim.z splits K, partial tiles are atomically added (C must be zeroed by
-// the caller when gridDim.z > 1).  A is rounded to BF16 (weights); with
-// split_b, B enters as the BF16 pair hi + lo, hi = bf16(b), lo = bf16(b - hi)
-// (about 16 significant bits: activations without BF16 jitter).
+// the caller when gridDim.z > 1).  Operand formats (MODE):
+//   0: A and B rounded to BF16;
+//   1: A in BF16, B as the BF16 pair hi + lo, hi = bf16(b), lo = bf16(b - hi)
+//      (about 16 significant bits: activations without BF16 jitter);
+//   2: A in FP16, B as the FP16 pair hi + lo -- the formats of the inference
+//      kernel's weights and inputs (forward products, weights as A).
 __device__ __forceinline__ float bf2f(short h) { return __uint_as_float(uint32_t(uint16_t(h)) << 16); }
 
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+
+template <int MODE>
 __global__ __launch_bounds__(64) void k_gemm(int M, int N, int K, const float* __restrict__ A, int64_t sai,
                                              int64_t sak, const float* __restrict__ B, int64_t sbk, int64_t sbj,
                                              float* __restrict__ C, int64_t ldc, float alpha, float beta,
-                                             int kchunk, int split_b) {
+                                             int kchunk) {
   const int lane = threadIdx.x;
   const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
   const int kb = blockIdx.z * kchunk;
@@ -56,18 +62,33 @@ __global__ __launch_bounds__(64) void k_gemm(int M, int N, int K, const float* _
   for (int v = 0; v < 16; ++v) acc[v] = 0.f;
   const int i = i0 + r, j = j0 + r;
   for (int k0 = kb; k0 < ke; k0 += 8) {
-    bf16x4 a, b, blo;
+    if constexpr (MODE == 2) {
+      f16x4 a, b, blo;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int k = k0 + 4 * u + t;
-      const bool kin = k < ke;
-      a[t] = (kin && i < M) ? f2bf(A[i * sai + k * sak]) : short(0);
-      const float bv = (kin && j < N) ? B[k * sbk + j * sbj] : 0.f;
-      b[t] = f2bf(bv);
-      blo[t] = f2bf(bv - bf2f(b[t]));
+      for (int t = 0; t < 4; ++t) {
+        const int k = k0 + 4 * u + t;
+        const bool kin = k < ke;
+        a[t] = (kin && i < M) ? _Float16(A[i * sai + k * sak]) : _Float16(0.f);
+        const float bv = (kin && j < N) ? B[k * sbk + j * sbj] : 0.f;
+        b[t] = _Float16(bv);
+        blo[t] = _Float16(bv - float(b[t]));
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, blo, acc, 0, 0, 0);
+    } else {
+      bf16x4 a, b, blo;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int k = k0 + 4 * u + t;
+        const bool kin = k < ke;
+        a[t] = (kin && i < M) ? f2bf(A[i * sai + k * sak]) : short(0);
+        const float bv = (kin && j < N) ? B[k * sbk + j * sbj] : 0.f;
+        b[t] = f2bf(bv);
+        blo[t] = f2bf(bv - bf2f(b[t]));
+      }
+      acc = mfma32(a, b, acc);
+      if (MODE == 1) acc = mfma32(a, blo, acc);
     }
-    acc = mfma32(a, b, acc);
-    if (split_b) acc = mfma32(a, blo, acc);
   }
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
@@ -147,34 +168,42 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float
 }
 
 // ----------------------------------------------- N-component inference --
-// Inference uses v_mfma_f32_32x32x16_bf16: lane l, element t (0..7) of the A
+// Inference uses v_mfma_f32_32x32x16_f16: lane l, element t (0..7) of the A
 // and B operands pair up along K in the same (l / 32, t) slot, so any slot ->
 // k assignment shared by A and B is valid.  The slot of hidden row k used here
 // lets the D tile of one layer feed the next layer without data movement:
 // D item v of lane l holds row 32h + 8(v/4) + 4(l/32) + v%4 of hidden half h,
 // so items 8p..8p+7 form the B operand of K block kb = 2h + p with
 //   slot (l/32, t) -> row 32h + 16p + 8(t/4) + 4(l/32) + t%4.
-// Packed weights of one phase network (bf16x8 words, 64 lanes each; host
+// Packed weights of one phase network (FP16 x 8 words, 64 lanes each; host
 // packing in pyphasefield_amd.surrogate.pack_phase):
 //   set 0..1    layer 1, hidden half h: slot (0, t) = weight of input t of
 //               (T_hi, c1_hi, c2_hi, T_lo, c1_lo, c2_lo, 1, 1): slots 6 and 7
-//               hold the layer-1 bias as a BF16 hi + lo pair; lanes >= 32 zero
+//               hold the layer-1 bias as an FP16 hi + lo pair; lanes >= 32 zero
 //   set 2..9    layer 2, output half m = (set-2)/4, K block kb = (set-2)%4
 //   set 10..13  layer 3, K block kb, output rows 0..2 (rows 3..31 zero)
-// followed by FP32 b1[64] (unused: packed above), b2[64], b3[4] (b3[3] unused).  The inputs enter
-// layer 1 as BF16 pairs hi + lo; the activations entering layers 2 and 3 are
-// either such pairs (two MFMAs with the same weights: the only BF16 rounding
-// is then that of the weights, which training sees) or BF16 only.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// followed by FP32 b1[64] (unused: packed above), b2[64], b3[4] (b3[3] unused).
+// Activation scaling: the host divides layer 1's weights and bias by a power
+// of two s1 and layer 2's bias by s2, multiplies layer 2's weights by s1/s2
+// and layer 3's by s2, with s1, s2 at least the interval bounds of the
+// layers' pre-activations over the trained input box.  Powers of two are
+// exact in FP16 and FP32, so the network is unchanged, and every hidden value
+// of an accepted cell lies in [0, 1]: ReLU(x) = clamp(x, 0, 1) there, and the
+// clamp rides on the FP16 conversion for free (v_cvt_pk_f16_f32 ... clamp).
+// The inputs enter layer 1 as FP16 pairs hi + lo; the activations entering
+// layers 2 and 3 are either such pairs (two MFMAs with the same weights) or
+// FP16 only (the default).
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 constexpr int kMlpSets = 14;
-constexpr int kMlpWords = kMlpSets * 64;  // bf16x8 words per phase
+constexpr int kMlpWords = kMlpSets * 64;  // FP16 x 8 words per phase
 
-__device__ __forceinline__ f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c) {
-  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f32x16 mfma16(f16x8 a, f16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
 struct MlpK {
-  const bf16x8* w[2];   // liquid, solid packed weights (device)
+  const f16x8* w[2];    // liquid, solid packed weights (device)
   const float* bias[2]; // b1[64], b2[64], b3[4] per phase (device)
   float in_lo[3], in_scale[3];        // x_hat = (x - lo) * scale - 1 over (T, c1, c2)
   float out_mu[2][3], out_sigma[2][3];
@@ -183,58 +212,61 @@ struct MlpK {
   int32_t* marks;                     // [2] cells marked for the fallback, by step parity (may be null)
 };
 
-typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-// two FP32 values to one register of BF16 (round to nearest even): low half
-// from a, high half from b (a single v_cvt_pk_bf16_f32)
-__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
-  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
+// two FP32 values to one register of FP16 (round to nearest even): low half
+// from a, high half from b (one v_cvt_pk_f16_f32)
+__device__ __forceinline__ uint32_t cvt_pk_f16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, f16x2));
 }
+
+// the same with the result clamped to [0, 1]: the compiler folds the clamp
+// into the conversion's clamp bit (one instruction)
+__device__ __forceinline__ uint32_t act_pk_f16(float a, float b) {
+  f16x2 p = __builtin_convertvector(f32x2{a, b}, f16x2);
+  p = __builtin_elementwise_min(__builtin_elementwise_max(p, f16x2{0, 0}), f16x2{1, 1});
+  return __builtin_bit_cast(uint32_t, p);
+}
+
+__device__ __forceinline__ float f16_lo(uint32_t w) { return float(__builtin_bit_cast(f16x2, w)[0]); }
+__device__ __forceinline__ float f16_hi(uint32_t w) { return float(__builtin_bit_cast(f16x2, w)[1]); }
 
 // max(x, 0) as a signed-integer max of the bits (one instruction, no NaN
 // canonicalisation; negative floats are negative integers)
 __device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
-// ReLU of items 8p..8p+7 of a D tile as the BF16 pair (hi, lo)
-__device__ __forceinline__ void relu_split(const f32x16& d, int p, bf16x8& hi, bf16x8& lo) {
+// ReLU of items 8p..8p+7 of a D tile as the FP16 pair (hi, lo)
+__device__ __forceinline__ void relu_split(const f32x16& d, int p, f16x8& hi, f16x8& lo) {
   u32x4 wh, wl;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const float v0 = relu(d[8 * p + 2 * t]), v1 = relu(d[8 * p + 2 * t + 1]);
-    const uint32_t h = cvt_pk_bf16(v0, v1);
+    const uint32_t h = cvt_pk_f16(v0, v1);
     wh[t] = h;
-    wl[t] = cvt_pk_bf16(v0 - __uint_as_float(h << 16), v1 - __uint_as_float(h & 0xffff0000u));
+    wl[t] = cvt_pk_f16(v0 - f16_lo(h), v1 - f16_hi(h));
   }
-  hi = __builtin_bit_cast(bf16x8, wh);
-  lo = __builtin_bit_cast(bf16x8, wl);
+  hi = __builtin_bit_cast(f16x8, wh);
+  lo = __builtin_bit_cast(f16x8, wl);
 }
 
-// ReLU of items 8p..8p+7 of a D tile rounded to BF16: pairs are converted
-// first and the ReLU is taken on the BF16 bits as signed 16-bit integers
-// (every negative BF16 value, -0 included, is a negative integer)
-__device__ __forceinline__ void relu_bf16(const f32x16& d, int p, bf16x8& hi) {
+// ReLU of items 8p..8p+7 of a D tile in FP16 (clamp to [0, 1], see above)
+__device__ __forceinline__ void relu_f16(const f32x16& d, int p, f16x8& hi) {
   u32x4 w;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const uint32_t pr = cvt_pk_bf16(d[8 * p + 2 * t], d[8 * p + 2 * t + 1]);
-    w[t] = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, pr), s16x2{0, 0}));
-  }
-  hi = __builtin_bit_cast(bf16x8, w);
+  for (int t = 0; t < 4; ++t) w[t] = act_pk_f16(d[8 * p + 2 * t], d[8 * p + 2 * t + 1]);
+  hi = __builtin_bit_cast(f16x8, w);
 }
 
 // next-layer B operands (K blocks kb = 0..3) from the two D tiles of a layer
 template <bool SPLIT>
-__device__ __forceinline__ void pack_layer(const f32x16 (&h)[2], bf16x8 (&bh)[4], bf16x8 (&bl)[4]) {
+__device__ __forceinline__ void pack_layer(const f32x16 (&h)[2], f16x8 (&bh)[4], f16x8 (&bl)[4]) {
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
     if (SPLIT) {
       relu_split(h[kb >> 1], kb & 1, bh[kb], bl[kb]);
     } else {
-      relu_bf16(h[kb >> 1], kb & 1, bh[kb]);
+      relu_f16(h[kb >> 1], kb & 1, bh[kb]);
     }
   }
 }
@@ -253,22 +285,22 @@ __device__ __forceinline__ f32x16 bias_tile(const float* b, int row0, int u) {
 // One phase network on G groups of 32 cells; the weight operands read from
 // LDS are shared by the groups.  x[g]: layer-1 B operand of group g.
 // out[g][q] = normalised output q of cell (lane % 32) of group g, valid in
-// lanes 0..31.  S2 / S3: the inputs of layer 2 / layer 3 enter as BF16 hi + lo
-// pairs (two MFMAs per K block) instead of BF16 only.
+// lanes 0..31.  S2 / S3: the inputs of layer 2 / layer 3 enter as FP16 hi + lo
+// pairs (two MFMAs per K block) instead of FP16 only.
 template <int G, bool S2, bool S3>
-__device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const float* __restrict__ b,
-                                         const bf16x8 (&x)[G], int lane, float (&out)[G][3]) {
+__device__ __forceinline__ void mlp_eval(const f16x8* __restrict__ w, const float* __restrict__ b,
+                                         const f16x8 (&x)[G], int lane, float (&out)[G][3]) {
   const int u = lane >> 5;
   f32x16 h[G][2];
   const f32x16 zero = {};
 #pragma unroll
   for (int hh = 0; hh < 2; ++hh) {
     // the layer-1 bias comes in through the packed words (constant-1 inputs)
-    const bf16x8 a = w[hh * 64 + lane];
+    const f16x8 a = w[hh * 64 + lane];
 #pragma unroll
     for (int g = 0; g < G; ++g) h[g][hh] = mfma16(a, x[g], zero);
   }
-  bf16x8 bh[G][4], bl[G][4];
+  f16x8 bh[G][4], bl[G][4];
 #pragma unroll
   for (int g = 0; g < G; ++g) pack_layer<S2>(h[g], bh[g], bl[g]);
 #pragma unroll
@@ -276,7 +308,7 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
     const f32x16 bt = bias_tile(b + 64, 32 * m, u);
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
-      const bf16x8 a = w[(2 + 4 * m + kb) * 64 + lane];
+      const f16x8 a = w[(2 + 4 * m + kb) * 64 + lane];
       // the bias tile enters as the first accumulator operand (no copy)
 #pragma unroll
       for (int g = 0; g < G; ++g) h[g][m] = mfma16(a, bh[g][kb], kb == 0 ? bt : h[g][m]);
@@ -291,7 +323,7 @@ __device__ __forceinline__ void mlp_eval(const bf16x8* __restrict__ w, const flo
   f32x16 y[G];
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
-    const bf16x8 a = w[(10 + kb) * 64 + lane];
+    const f16x8 a = w[(10 + kb) * 64 + lane];
 #pragma unroll
     for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], kb == 0 ? zero : y[g]);
     if (S3) {
@@ -366,45 +398,90 @@ __device__ __forceinline__ void write_transfer(float* __restrict__ tr, uint32_t 
   }
 }
 
-// Persistent: each block stages both networks in LDS once and its waves walk
-// 64-cell tiles of the padded domain in row-major order (wave w of block b
-// takes tiles 4b + w, 4b + w + 4 gridDim.x, ...).  Lane l owns cell 64 tile + l;
-// MFMA group 0 is the tile's first 32 cells, group 1 the next 32 (their inputs
-// moved to lanes 0..31 by a shuffle), so the weight operands are read once per
-// 64 cells.
-// waves per SIMD the register budget is set for (3: no spills at 153 VGPRs)
+// waves per SIMD the register budget is set for (3: no spills)
 #ifndef PPF_MLP_WAVES
 #define PPF_MLP_WAVES 3
 #endif
-template <bool S2, bool S3>
-__global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(const float* __restrict__ in,
-                                                             const float* __restrict__ b, Dom d, float dx,
-                                                             ppf_temperature t, float* __restrict__ tr, int64_t tsy,
-                                                             int64_t tsf, NcMlpK k, MlpK net, uint64_t seed,
-                                                             StepRef sr, uint32_t ntiles) {
+// persistent blocks per CU of the inference kernel
+#ifndef PPF_MLP_BLOCKS_PER_CU
+#define PPF_MLP_BLOCKS_PER_CU PPF_MLP_WAVES
+#endif
+
+// The surrogate's thermodynamic pass.  Persistent: each block stages both
+// networks in LDS once and its waves walk 64-cell tiles of the padded domain
+// in row-major order (wave w of block b takes tiles 4b + w, 4b + w + 4
+// gridDim.x, ...).  Lane l owns cell 64 tile + l; MFMA group 0 is the tile's
+// first 32 cells, group 1 the next 32, so the weight operands are read once
+// per 64 cells.
+//  * Tile bookkeeping on the scalar unit: the row and column of a tile's
+//    first cell advance by the host-computed (stride / px, stride % px); a
+//    lane adds its index and wraps at most once (px >= 64; narrower grids
+//    divide).
+//  * Fields and transfer arrays through buffer instructions with 32-bit byte
+//    offsets (the array offsets ride in scalar registers).
+//  * One v_permlane32_swap per input gives both MFMA groups their cells:
+//    swapping a register with itself broadcasts its low half (cells 0..31,
+//    group 0) into one result and its high half (cells 32..63, group 1) into
+//    the other.  The K slots of lanes 32..63 then hold a copy of the same
+//    cell's inputs, which the zero weights of those slots cancel exactly.
+//  * One v_permlane32_swap per output moves group 1's results (rows 0..2 of
+//    the layer-3 tile, lanes 0..31) to lanes 32..63, beside group 0's.
+//  * NOISE: composition noise on (a separate instantiation, so the noise
+//    path's arithmetic is never speculated into the common one).
+struct MlpLean {
+  uint32_t px, py, nx, ny, sy, sf4, tsy, tsf4;  // padded extents, strides (elements; *4: bytes)
+  uint32_t ntiles, srow, scol;                  // tiles; wave stride of cells as rows and columns
+  float T_lo, T_hi, c_lo, R;
+  int32_t* marks;
+};
+
+__device__ __forceinline__ void lean_store(float v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, voff, int32_t(soff), 0);
+}
+
+template <bool S2, bool S3, bool NOISE>
+__global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(
+    const float* __restrict__ in, const float* __restrict__ b, Dom d, float dx, ppf_temperature t,
+    float* __restrict__ tr, NcMlpK k, MlpK net, MlpLean g, uint64_t seed, StepRef sr) {
   constexpr int N = 3;
   const int64_t step = sr.get();
-  __shared__ bf16x8 sw[2][kMlpWords];
+  __shared__ f16x8 sw[2][kMlpWords];
   __shared__ float sb[2][132];
   for (int i = threadIdx.x; i < 2 * kMlpWords; i += 256) sw[i / kMlpWords][i % kMlpWords] = net.w[i / kMlpWords][i % kMlpWords];
   for (int i = threadIdx.x; i < 2 * 132; i += 256) sb[i / 132][i % 132] = net.bias[i / 132][i % 132];
   __syncthreads();
-  const int lane = threadIdx.x & 63, u = lane >> 5;
-  const uint32_t px = uint32_t(d.ax.n + 2), py = uint32_t(d.ay.n + 2), ncell = px * py;
-  const uint32_t nx = uint32_t(d.ax.n), ny = uint32_t(d.ay.n);
-  const uint32_t sy = uint32_t(d.ay.stride), sf = uint32_t(d.sf), tsy32 = uint32_t(tsy), tsf32 = uint32_t(tsf);
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t tstride = gridDim.x * 4;
+  uint32_t tile = blockIdx.x * 4 + wave;
+  // row and column of the tile's first cell (wave-uniform)
+  uint32_t y0 = __builtin_amdgcn_readfirstlane((tile * 64) / g.px);
+  uint32_t x0 = __builtin_amdgcn_readfirstlane(tile * 64 - y0 * g.px);
+  const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(in), int16_t(0),
+                                                                        int32_t(-1), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rtr = __builtin_amdgcn_make_buffer_rsrc(tr, int16_t(0), int32_t(-1), 0x00020000);
   const int64_t n3[3] = {d.ax.n, d.ay.n, 1};
   const int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
-  for (uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += gridDim.x * 4) {
-    const uint32_t cell = tile * 64 + lane;
-    const bool valid = cell < ncell;
-    const uint32_t y = valid ? cell / px : 1, x = valid ? cell - y * px : 1;
+  const uint64_t ncell = uint64_t(g.px) * g.py;
+  for (; tile < g.ntiles; tile += tstride) {
+    uint32_t x = x0 + lane, y = y0;
+    if (g.px >= 64) {
+      if (x >= g.px) {
+        x -= g.px;
+        ++y;
+      }
+    } else {
+      y += x / g.px;
+      x = x % g.px;
+    }
+    const bool valid = y < g.py;
+    if (!valid) x = y = 1;
     float phi, c1, c2;
-    if (x >= 1 && x <= nx && y >= 1 && y <= ny) {
-      const uint32_t o = y * sy + x;
-      phi = in[o];
-      c1 = in[o + 3 * sf];
-      c2 = in[o + 4 * sf];
+    if (x - 1 < g.nx && y - 1 < g.ny) {
+      const uint32_t o = (y * g.sy + x) * 4u;
+      phi = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, o, 0, 0));
+      c1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, o, int32_t(3 * g.sf4), 0));
+      c2 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, o, int32_t(4 * g.sf4), 0));
     } else {
       phi = fold(in, b, d, int64_t(x), int64_t(y), int64_t(0), dx);
       c1 = fold(in + 3 * d.sf, b ? b + 3 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
@@ -412,24 +489,28 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(const f
     }
     const float T = temperature_at<float>(t, n3, bc3, int64_t(x), int64_t(y), 1, double(dx), step);
     const float cN = 1.f - c1 - c2;
-    const bool gate = T < net.T_lo || T > net.T_hi || c1 < net.c_lo || c2 < net.c_lo || cN < net.c_lo;
+    const bool gate = T < g.T_lo || T > g.T_hi || c1 < g.c_lo || c2 < g.c_lo || cN < g.c_lo;
     const float xv[3] = {(T - net.in_lo[0]) * net.in_scale[0] - 1.f, (c1 - net.in_lo[1]) * net.in_scale[1] - 1.f,
                          (c2 - net.in_lo[2]) * net.in_scale[2] - 1.f};
-    bf16x8 xin[2];
+    // xs[0]: cell lane % 32 (group 0), xs[1]: cell 32 + lane % 32 (group 1)
+    float xs[2][3];
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
+    for (int q = 0; q < 3; ++q) {
+      const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(xv[q]), __float_as_uint(xv[q]), false, false);
+      xs[0][q] = __uint_as_float(r[0]);
+      xs[1][q] = __uint_as_float(r[1]);
+    }
+    // slots (T_hi, c1_hi, c2_hi, T_lo, c1_lo, c2_lo, 1, 1), hi = f16(x), lo = f16(x - hi)
+    f16x8 xin[2];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const float other = __shfl(xv[q], (lane & 31) + 32, 64);
-        const float xs = g == 0 ? xv[q] : other;
-        const __bf16 hq = static_cast<__bf16>(xs);
-        const __bf16 lq = static_cast<__bf16>(xs - static_cast<float>(hq));
-        xin[g][q] = u == 0 ? hq : static_cast<__bf16>(0.f);
-        xin[g][3 + q] = u == 0 ? lq : static_cast<__bf16>(0.f);
-      }
-      // constant inputs of the packed layer-1 bias (hi, lo)
-      xin[g][6] = u == 0 ? static_cast<__bf16>(1.f) : static_cast<__bf16>(0.f);
-      xin[g][7] = xin[g][6];
+    for (int gg = 0; gg < 2; ++gg) {
+      const uint32_t p01 = cvt_pk_f16(xs[gg][0], xs[gg][1]);
+      const float t_lo = xs[gg][0] - f16_lo(p01);
+      const float c1_lo = xs[gg][1] - f16_hi(p01);
+      const uint32_t p23 = cvt_pk_f16(xs[gg][2], t_lo);
+      const float c2_lo = xs[gg][2] - f16_lo(p23);
+      const uint32_t p45 = cvt_pk_f16(c1_lo, c2_lo);
+      xin[gg] = __builtin_bit_cast(f16x8, u32x4{p01, p23, p45, 0x3c003c00u});
     }
     // the two networks one after the other (not unrolled: one network's
     // operands and activations live at a time)
@@ -438,36 +519,52 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(const f
     for (int ph = 0; ph < 2; ++ph) {
       float o[2][3];
       mlp_eval<2, S2, S3>(sw[ph], sb[ph], xin, lane, o);
-      // own cell: lanes 0..31 -> group 0 (own lane), lanes 32..63 -> group 1 (lane - 32)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const float o1 = __shfl(o[1][q], lane & 31, 64);
-        const float v = (u == 0 ? o[0][q] : o1) * net.out_sigma[ph][q] + net.out_mu[ph][q];
+        // lanes 0..31: group 0 (own cell); lanes 32..63: group 1 from lane - 32
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(o[0][q]), __float_as_uint(o[1][q]), false,
+                                                        false);
+        const float v = __uint_as_float(r[0]) * net.out_sigma[ph][q] + net.out_mu[ph][q];
         GL[q] = ph == 0 ? v : GL[q];
         GS[q] = ph == 1 ? v : GS[q];
       }
     }
+    const uint32_t cell = y * g.px + x;
+    const uint32_t ob = (y * g.tsy + x) * 4u;
+    x0 += g.scol;
+    y0 += g.srow;
+    if (x0 >= g.px) {
+      x0 -= g.px;
+      ++y0;
+    }
     if (!valid) continue;
-    const uint32_t o = y * tsy32 + x;
     if (gate) {  // outside the trained box: marked for k_ncomp_thermo_fallback
-      tr[o] = __builtin_nanf("");
-      if (net.marks) atomicAdd(&net.marks[step & 1], 1);
+      lean_store(__builtin_nanf(""), rtr, ob, 0);
+      if (g.marks) atomicAdd(&g.marks[step & 1], 1);
       continue;
     }
-    float dG, dGL[N - 1], dGS[N - 1];
     // ideal mixing R T sum c ln c is analytic: it cancels in G_S - G_L and
     // adds R T (ln c_l - ln c_N) to each derivative
-    const float RT = net.R * T;
+    const float RT = g.R * T;
     const float lnN = fast_ln(fmaxf(cN, float(kCMin)));
     const float cl[N - 1] = {c1, c2};
-    dG = GS[0];
+    const float g_ = (phi * phi) * ((1.f - phi) * (1.f - phi));
+    const float h_ = (phi * phi * phi) * (6.f * (phi * phi) - 15.f * phi + 10.f);
+    lean_store(GS[0], rtr, ob, 0);
+    const float mob = k.v_m * (k.D_L + h_ * k.dD) * __builtin_amdgcn_rcpf(8.314f * T);
+    const float Wd[N - 1] = {k.W0, k.W1};
 #pragma unroll
     for (int l = 0; l < N - 1; ++l) {
       const float idl = RT * (fast_ln(fmaxf(cl[l], float(kCMin))) - lnN);
-      dGL[l] = GL[1 + l] + idl;
-      dGS[l] = GS[1 + l] + idl;
+      const float dGL = GL[1 + l] + idl, dGS = GS[1 + l] + idl;
+      lean_store(mob * cl[l], rtr, ob, (1 + l) * g.tsf4);
+      float noise = 0.f;
+      if constexpr (NOISE) {
+        float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + l, ncell, uint64_t(cell))));
+        noise = k.noise_c * sqrtf(2.f * 8.314f * T * k.inv_v_m) * xi;
+      }
+      lean_store((dGL + h_ * (dGS - dGL)) * k.inv_v_m + Wd[l] * g_ * T + noise, rtr, ob, (N + l) * g.tsf4);
     }
-    write_transfer(tr, o, tsf32, k, phi, T, cl, dG, dGL, dGS, seed, step, ncell, cell);
   }
 }
 
@@ -515,13 +612,15 @@ int ppf_mlp_gemm(int M, int N, int K, const float* A, int64_t sai, int64_t sak, 
                  int64_t sbj, float* C, int64_t ldc, float alpha, float beta, int ksplit, int split_b,
                  ppf_stream_t s) {
   if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return fail(PPF_EINVAL, "mlp_gemm: bad shape or null pointer");
+  if (split_b < 0 || split_b > 2) return fail(PPF_EINVAL, "mlp_gemm: operand format must be 0, 1 or 2");
   if (ksplit < 1) ksplit = 1;
   int kchunk = ((K + ksplit - 1) / ksplit + 7) / 8 * 8;
   ksplit = (K + kchunk - 1) / kchunk;
   if (ksplit > 1 && beta != 0.f) return fail(PPF_EINVAL, "mlp_gemm: split K needs beta = 0 and a zeroed C");
   dim3 grid(unsigned((N + 31) / 32), unsigned((M + 31) / 32), unsigned(ksplit));
-  hipLaunchKernelGGL(k_gemm, grid, dim3(64), 0, stream_of(s), M, N, K, A, sai, sak, B, sbk, sbj, C, ldc, alpha, beta,
-                     kchunk, split_b);
+  auto kern = split_b == 2 ? k_gemm<2> : (split_b == 1 ? k_gemm<1> : k_gemm<0>);
+  hipLaunchKernelGGL(kern, grid, dim3(64), 0, stream_of(s), M, N, K, A, sai, sak, B, sbk, sbj, C, ldc, alpha, beta,
+                     kchunk);
   return check_launch("mlp_gemm");
 }
 
@@ -588,7 +687,7 @@ static int ncomp_thermo_mlp_dispatch(const ppf_grid* g, const void* in, const vo
   k.W1 = float(p->W[1]) - float(p->W[2]);
   MlpK m;
   for (int q = 0; q < 2; ++q) {
-    m.w[q] = static_cast<const bf16x8*>(net->weights[q]);
+    m.w[q] = static_cast<const f16x8*>(net->weights[q]);
     m.bias[q] = net->bias[q];
     for (int j = 0; j < 3; ++j) {
       m.out_mu[q][j] = net->out_mu[q][j];
@@ -605,16 +704,40 @@ static int ncomp_thermo_mlp_dispatch(const ppf_grid* g, const void* in, const vo
   m.R = float(th->liquid.R);
   m.marks = net->marks;
   const int64_t ncell = (g->n[0] + 2) * (g->n[1] + 2);
-  if (ncell >= (int64_t(1) << 31) || tg->sf * 5 >= (int64_t(1) << 32) || g->sf * 5 >= (int64_t(1) << 32))
+  // byte offsets of every element the kernel addresses (field stack and
+  // transfer array) stay below 2^32
+  const int64_t fbytes = (4 * g->sf + (g->n[1] + 2) * g->sy) * 4, tbytes = (4 * tg->sf + (g->n[1] + 2) * tg->sy) * 4;
+  if (ncell >= (int64_t(1) << 31) || fbytes >= (int64_t(1) << 32) || tbytes >= (int64_t(1) << 32))
     return fail(PPF_EINVAL, "ncomp_thermo_mlp: grid too large for 32-bit offsets");
   const uint32_t ntiles = uint32_t((ncell + 63) / 64);
-  const unsigned blocks = unsigned(std::min<int64_t>((ntiles + 3) / 4, int64_t(PPF_MLP_WAVES) * device_cus()));
+  const unsigned blocks = unsigned(std::min<int64_t>((ntiles + 3) / 4, int64_t(PPF_MLP_BLOCKS_PER_CU) * device_cus()));
   const bool s2 = (net->flags & PPF_MLP_SPLIT2) != 0, s3 = (net->flags & PPF_MLP_SPLIT3) != 0;
-  auto kern = s2 ? (s3 ? k_ncomp_thermo_mlp<true, true> : k_ncomp_thermo_mlp<true, false>)
-                 : (s3 ? k_ncomp_thermo_mlp<false, true> : k_ncomp_thermo_mlp<false, false>);
+  MlpLean lg;
+  lg.px = uint32_t(g->n[0] + 2);
+  lg.py = uint32_t(g->n[1] + 2);
+  lg.nx = uint32_t(g->n[0]);
+  lg.ny = uint32_t(g->n[1]);
+  lg.sy = uint32_t(g->sy);
+  lg.sf4 = uint32_t(g->sf * 4);
+  lg.tsy = uint32_t(tg->sy);
+  lg.tsf4 = uint32_t(tg->sf * 4);
+  lg.ntiles = ntiles;
+  const uint64_t stride_cells = uint64_t(blocks) * 4 * 64;
+  lg.srow = uint32_t(stride_cells / lg.px);
+  lg.scol = uint32_t(stride_cells % lg.px);
+  lg.T_lo = m.T_lo;
+  lg.T_hi = m.T_hi;
+  lg.c_lo = m.c_lo;
+  lg.R = m.R;
+  lg.marks = m.marks;
+  const bool nz = k.noise_c != 0.f;
+  auto kern = nz ? (s2 ? (s3 ? k_ncomp_thermo_mlp<true, true, true> : k_ncomp_thermo_mlp<true, false, true>)
+                       : (s3 ? k_ncomp_thermo_mlp<false, true, true> : k_ncomp_thermo_mlp<false, false, true>))
+                 : (s2 ? (s3 ? k_ncomp_thermo_mlp<true, true, false> : k_ncomp_thermo_mlp<true, false, false>)
+                       : (s3 ? k_ncomp_thermo_mlp<false, true, false> : k_ncomp_thermo_mlp<false, false, false>));
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, stream_of(s), static_cast<const float*>(in),
-                     static_cast<const float*>(bcvals), d, float(g->dx), *t, static_cast<float*>(transfer), tg->sy,
-                     tg->sf, k, m, seed, step, ntiles);
+                     static_cast<const float*>(bcvals), d, float(g->dx), *t, static_cast<float*>(transfer), k, m, lg,
+                     seed, step);
   if (int rc = check_launch("ncomp_thermo_mlp")) return rc;
   const unsigned fblocks = unsigned(std::min<int64_t>((ncell + 255) / 256, int64_t(4) * device_cus()));
   hipLaunchKernelGGL(k_ncomp_thermo_fallback, dim3(fblocks), dim3(256), 0, stream_of(s),

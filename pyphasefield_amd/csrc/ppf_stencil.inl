@@ -1028,8 +1028,9 @@ int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* 
   if (g.ndim == 2) {
     hipLaunchKernelGGL(k_ch2d<R>, dim3(tiles2d(g)), dim3(kThreads), 0, s, static_cast<const R*>(in),
                        static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), R(eps), dtM);
-  } else if (g.ndim == 3 && g.sz < (int64_t(1) << 31) && PPF_CH3D_WAVE && g.bc[0] == PPF_PERIODIC &&
-             g.bc[1] == PPF_PERIODIC && g.bc[2] == PPF_PERIODIC) {  // in-plane offsets are 32-bit
+  } else if (g.ndim == 3 && g.sz * int64_t(sizeof(R)) < (int64_t(1) << 31) && PPF_CH3D_WAVE &&
+             g.bc[0] == PPF_PERIODIC && g.bc[1] == PPF_PERIODIC &&
+             g.bc[2] == PPF_PERIODIC) {  // in-plane byte offsets (row offsets, buffer offsets) are 32-bit
     constexpr int RPW = sizeof(R) == 8 ? 8 : PPF_CH3W_RPW;
     const int64_t xy = ((g.n[0] + kCh3wTx - 1) / kCh3wTx) * ((g.n[1] + RPW - 5) / (RPW - 4));
     const int32_t zc = ch3w_zchunk<R, RPW>(g, xy);

@@ -9,10 +9,12 @@ the driving force is not the small difference of two large predictions.
 
 Everything runs on the GPU through libppf_hip.so: training samples come from
 the plain evaluation (``ppf_thermo_eval``), forward and backward passes are
-BF16-MFMA GEMMs with FP32 accumulation (``ppf_mlp_gemm``) plus the
-project's elementwise and Adam kernels; inference is the fused MFMA kernel
-``ppf_ncomp_thermo_mlp``, which falls back to the plain FP64 evaluation for
-cells outside the trained range.  torch only provides device memory.
+MFMA GEMMs with FP32 accumulation (``ppf_mlp_gemm``: FP16 weights and
+activation pairs forward, as inference runs them; BF16 backward) plus the
+project's elementwise and Adam kernels; inference is the fused FP16-MFMA
+kernel ``ppf_ncomp_thermo_mlp``, which falls back to the plain FP64
+evaluation for cells outside the trained range.  torch only provides device
+memory.
 """
 
 import numpy as np
@@ -31,6 +33,11 @@ def _f2bf_bits(a):
     return u.astype(np.uint16)
 
 
+def _f2h_bits(a):
+    """FP32 -> FP16 bits, round to nearest even (as the device does)."""
+    return np.ascontiguousarray(np.asarray(a, np.float64).astype(np.float32).astype(np.float16)).view(np.uint16)
+
+
 def _slot_rows(h, p):
     """Hidden row held by slot (u, t) of K block kb = 2h + p: an array [2, 8]."""
     u = np.arange(2)[:, None]
@@ -38,28 +45,51 @@ def _slot_rows(h, p):
     return 32 * h + 16 * p + 8 * (t // 4) + 4 * u + t % 4
 
 
-def pack_phase(W1, W2, W3, b1):
-    """BF16 A-operand lane layout of ``ppf_ncomp_thermo_mlp`` for one network.
+def activation_scales(W1, b1, W2, b2, x_bound=1.001, margin=1.01):
+    """Powers of two s1, s2 at least the interval upper bounds of the layer-1
+    and layer-2 pre-activations over inputs |x_i| <= x_bound (the trained box
+    is normalised to [-1, 1]; the bound leaves room for FP32 rounding of the
+    normalisation and the margin for FP16 rounding of the weights).  With
+    layer 1 scaled by 1/s1 and layer 2 by 1/s2, every hidden value of an
+    accepted cell lies in [0, 1], where ReLU equals a clamp to [0, 1]."""
+    W1, b1, W2, b2 = (np.asarray(a, np.float64) for a in (W1, b1, W2, b2))
+    u1 = np.maximum(b1 + x_bound * np.abs(W1).sum(axis=1), 0.0)   # a1_j in [0, u1_j]
+    u2 = np.maximum(b2 + np.maximum(W2, 0.0) @ u1, 0.0)          # z2_i <= u2_i
+    pow2 = lambda u: float(2.0 ** np.ceil(np.log2(max(margin * float(u.max()), 2.0 ** -10))))
+    return pow2(u1), pow2(u2)
+
+
+def pack_phase(W1, W2, W3, b1, b2, b3):
+    """FP16 A-operand lane layout and FP32 biases of ``ppf_ncomp_thermo_mlp``
+    for one network: (words [14*64, 8] FP16 bits, bias [132] FP32).
 
     32x32x16 MFMA A operand: lane l, element t holds A[l % 32][slot (l // 32, t)];
     slot (u, t) of K block kb = 2h + p is hidden row 32h + 16p + 8(t//4) + 4u + t%4
     (the row the previous layer's D tile leaves in that lane and element).
     Sets of 64 lanes x 8: layer 1 (2 hidden halves; slots (0, 0..5) carry the
     inputs T, c1, c2, T, c1, c2 for the hi and lo parts, slots (0, 6..7) the
-    layer-1 bias as a BF16 hi + lo pair against two constant-1 inputs, so the
+    layer-1 bias as an FP16 hi + lo pair against two constant-1 inputs, so the
     bias needs no accumulator tile), layer 2 (2 output halves x 4 K blocks),
-    layer 3 (4 K blocks, output rows 0..2, rest zero).
+    layer 3 (4 K blocks, output rows 0..2, rest zero).  Scaled by the powers
+    of two of ``activation_scales``: layer 1 by 1/s1, layer 2's weights by
+    s1/s2 and its bias by 1/s2, layer 3's weights by s2 (the network's outputs
+    are unchanged; its hidden values become clamp-able to [0, 1]).
+    bias: b1 (unused by the kernel), b2 / s2, b3, 0.
     """
+    s1, s2 = activation_scales(W1, b1, W2, b2)
+    W1 = np.asarray(W1, np.float64) / s1
+    W2 = np.asarray(W2, np.float64) * (s1 / s2)
+    W3 = np.asarray(W3, np.float64) * s2
     lanes = np.arange(64)
     r = lanes % 32
     u = lanes // 32
     sets = []
     col_of_t = [0, 1, 2, 0, 1, 2]
-    b1f = np.asarray(b1, np.float64).astype(np.float32).astype(np.float64)
-    b1_hi = _bf16(b1f)
-    b1_lo = b1f - b1_hi  # rounded to BF16 with the rest of the set
+    b1f = (np.asarray(b1, np.float64) / s1).astype(np.float32).astype(np.float64)
+    b1_hi = _f16(b1f)
+    b1_lo = b1f - b1_hi  # rounded to FP16 with the rest of the set
     for h in range(2):
-        blk = np.zeros((64, 8), np.float32)
+        blk = np.zeros((64, 8), np.float64)
         for t, col in enumerate(col_of_t):
             blk[u == 0, t] = W1[32 * h + r[u == 0], col]
         blk[u == 0, 6] = b1_hi[32 * h + r[u == 0]]
@@ -68,15 +98,18 @@ def pack_phase(W1, W2, W3, b1):
     for m in range(2):
         for kb in range(4):
             rows = _slot_rows(kb // 2, kb % 2)      # [2, 8]
-            blk = W2[32 * m + r[:, None], rows[u]]  # [64, 8]
-            sets.append(blk.astype(np.float32))
+            sets.append(W2[32 * m + r[:, None], rows[u]])  # [64, 8]
     for kb in range(4):
         rows = _slot_rows(kb // 2, kb % 2)
-        blk = np.zeros((64, 8), np.float32)
+        blk = np.zeros((64, 8), np.float64)
         ok = r < 3
         blk[ok] = W3[r[ok][:, None], rows[u[ok]]]
         sets.append(blk)
-    return _f2bf_bits(np.concatenate(sets, axis=0))  # [14*64, 8] uint16
+    bias = np.zeros(132, np.float32)
+    bias[:64] = b1
+    bias[64:128] = np.asarray(b2, np.float64) / s2
+    bias[128:131] = np.asarray(b3, np.float64)[:3]
+    return _f2h_bits(np.concatenate(sets, axis=0)), bias
 
 
 class PhaseNet:
@@ -134,8 +167,8 @@ class SurrogateThermo:
         self.out_sigma = np.ones((2, 3))
         self.history = []
         self._packed = None
-        # inference: the inputs of layers 2 / 3 as BF16 hi + lo pairs (True)
-        # or BF16 only (default: BF16 activations add ~10 J/mol of rounding
+        # inference: the inputs of layers 2 / 3 as FP16 hi + lo pairs (True)
+        # or FP16 only (default: FP16 activations add a few J/mol of rounding
         # scatter and halve the MFMA and conversion work; the training GEMMs
         # always use pairs)
         self.split2 = bool(split2)
@@ -188,15 +221,17 @@ class SurrogateThermo:
         B = X.shape[1]
         v = net.view
         Z1 = torch.empty(HIDDEN, B, device=self.device)
-        self._gemm(HIDDEN, B, 3, v["W1"], 3, 1, X, B, 1, Z1)
+        # forward products in the inference kernel's formats: FP16 weights,
+        # FP16 hi + lo activations (training sees the weights it will run with)
+        self._gemm(HIDDEN, B, 3, v["W1"], 3, 1, X, B, 1, Z1, split_b=2)
         H1 = torch.empty_like(Z1)
         _lib.check(L.ppf_mlp_bias_act(_lib.ptr(Z1), _lib.ptr(v["b1"]), _lib.ptr(H1), HIDDEN, B, 1, s), "bias_act")
         Z2 = torch.empty_like(Z1)
-        self._gemm(HIDDEN, B, HIDDEN, v["W2"], HIDDEN, 1, H1, B, 1, Z2)
+        self._gemm(HIDDEN, B, HIDDEN, v["W2"], HIDDEN, 1, H1, B, 1, Z2, split_b=2)
         H2 = torch.empty_like(Z1)
         _lib.check(L.ppf_mlp_bias_act(_lib.ptr(Z2), _lib.ptr(v["b2"]), _lib.ptr(H2), HIDDEN, B, 1, s), "bias_act")
         Z3 = torch.empty(3, B, device=self.device)
-        self._gemm(3, B, HIDDEN, v["W3"], HIDDEN, 1, H2, B, 1, Z3)
+        self._gemm(3, B, HIDDEN, v["W3"], HIDDEN, 1, H2, B, 1, Z3, split_b=2)
         Y = torch.empty_like(Z3)
         _lib.check(L.ppf_mlp_bias_act(_lib.ptr(Z3), _lib.ptr(v["b3"]), _lib.ptr(Y), 3, B, 0, s), "bias_act")
         if keep is not None:
@@ -329,12 +364,8 @@ class SurrogateThermo:
         keep = []
         for p, pn in enumerate(self.nets):
             h = pn.numpy()
-            words = pack_phase(h["W1"], h["W2"], h["W3"], h["b1"])
+            words, bias = pack_phase(h["W1"], h["W2"], h["W3"], h["b1"], h["b2"], h["b3"])
             wdev = torch.from_numpy(words.view(np.int16).copy()).to(self.device)
-            bias = np.zeros(132, np.float32)
-            bias[:64] = h["b1"]  # unused by the kernel: layer 1 takes it from the packed words
-            bias[64:128] = h["b2"]
-            bias[128:131] = h["b3"]
             bdev = torch.from_numpy(bias).to(self.device)
             keep += [wdev, bdev]
             net.weights[p] = wdev.data_ptr()
@@ -361,6 +392,10 @@ def _bf16(a):
     return bits.view(np.float32).astype(np.float64)
 
 
+def _f16(a):
+    return np.asarray(a, np.float64).astype(np.float32).astype(np.float16).astype(np.float64)
+
+
 def _split(a):
     """the BF16 pair hi + lo the device feeds for an FP32 operand"""
     a32 = np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
@@ -368,19 +403,30 @@ def _split(a):
     return hi + _bf16(a32 - hi)
 
 
-def numpy_forward_backward(params, X, Yt, bf16=False):
-    """FP64 numpy MLP (checker for the GPU gradients); bf16=True reproduces the
-    device operand rounding: the A operand (weights, or the left factor of a
-    gradient product) in BF16, the B operand as a BF16 hi + lo pair."""
-    ra = _bf16 if bf16 else (lambda a: a)
-    rb = _split if bf16 else (lambda a: a)
+def _split16(a):
+    """the FP16 pair hi + lo the device feeds for an FP32 operand"""
+    a32 = np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+    hi = _f16(a32)
+    return hi + _f16(a32 - hi)
+
+
+def numpy_forward_backward(params, X, Yt, rounded=False):
+    """FP64 numpy MLP (checker for the GPU gradients); rounded=True reproduces
+    the device operand rounding: forward products with FP16 weights and FP16
+    hi + lo inputs, backward products with the A operand (a weight or the
+    left factor of a gradient product) in BF16 and the B operand as a BF16
+    hi + lo pair."""
+    fa = _f16 if rounded else (lambda a: a)
+    fb = _split16 if rounded else (lambda a: a)
+    ra = _bf16 if rounded else (lambda a: a)
+    rb = _split if rounded else (lambda a: a)
 
     W1, b1, W2, b2, W3, b3 = (params[k] for k in ("W1", "b1", "W2", "b2", "W3", "b3"))
-    Z1 = ra(W1) @ rb(X) + b1[:, None]
+    Z1 = fa(W1) @ fb(X) + b1[:, None]
     H1 = np.maximum(Z1, 0)
-    Z2 = ra(W2) @ rb(H1) + b2[:, None]
+    Z2 = fa(W2) @ fb(H1) + b2[:, None]
     H2 = np.maximum(Z2, 0)
-    Y = ra(W3) @ rb(H2) + b3[:, None]
+    Y = fa(W3) @ fb(H2) + b3[:, None]
     n = Y.size
     dY = 2 * (Y - Yt) / n
     g = {"W3": ra(dY) @ rb(H2).T, "b3": dY.sum(1)}

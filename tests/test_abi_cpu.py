@@ -137,6 +137,7 @@ def test_edge_case_arguments_are_rejected():
     short = (ctypes.c_int32 * 9)(*([0] * 8 + [3]))
     rejected(L.ppf_ensemble_step(a, 1, a, 4, short, 0, 8, a, b, None), "cover")
     rejected(L.ppf_mlp_gemm(0, 4, 4, a, 1, 1, a, 1, 1, b, 1, 1.0, 0.0, 1, 0, None), "bad shape")
+    rejected(L.ppf_mlp_gemm(4, 4, 4, a, 1, 1, a, 1, 1, b, 1, 1.0, 0.0, 1, 3, None), "operand format")
 
 
 def test_ncomp_flags_and_cahnhilliard_line_arguments_are_rejected():

@@ -2,7 +2,7 @@
 radius 6, synthetic Redlich-Kister model, T = 1440 K + 1 K/cell, 10,000 steps
 (the benchmark's workload, bench.ncomp_sim).
 
-The surrogate engine (FP32 stencil, BF16 networks trained on the GPU for this
+The surrogate engine (FP32 stencil, FP16-inference networks trained on the GPU for this
 run's temperature box) and the plain FP32 engine are compared with the plain
 FP64 engine -- the reference's method and precision, pinned to the FP64
 oracle at 1e-10 (tests/test_gpu_parity.py) -- at every 2,000 steps.

@@ -1,8 +1,8 @@
 """Surrogate thermodynamics on the GPU: MFMA GEMM, training gradients,
 packed inference kernel, accuracy, and the N-component engine driven by it.
 
-Checkers: numpy FP64 (and BF16-emulating) forward/backward of the same
-network; the plain evaluation (ppf_thermo_eval / oracle.thermo); the FP64
+Checkers: numpy FP64 (and device-rounding-emulating: FP16 forward, BF16
+backward) forward/backward of the same network; the plain evaluation (ppf_thermo_eval / oracle.thermo); the FP64
 oracle of the N-component engine."""
 
 import numpy as np
@@ -46,6 +46,16 @@ def test_mfma_gemm_matches_bf16_numpy(gpu, M, N, K, ks):
                                        1, _lib.stream_handle()), "gemm")
     want3 = _bf(A) @ _split(B)
     assert np.max(np.abs(C3.cpu().numpy() - want3)) <= 1e-4 * max(1.0, np.abs(want3).max()) * np.sqrt(K)
+    # FP16 operands: FP16 A times the FP16 pair (hi + lo) of B (the forward
+    # products of training, in the inference kernel's formats)
+    from pyphasefield_amd.surrogate import _f16, _split16
+    C4 = torch.zeros(M, N, device=gpu)
+    _lib.check(_lib.lib().ppf_mlp_gemm(M, N, K, _lib.ptr(Ad), K, 1, _lib.ptr(Bd), N, 1, _lib.ptr(C4), N, 1.0, 0.0, ks,
+                                       2, _lib.stream_handle()), "gemm")
+    want4 = _f16(A) @ _split16(B)
+    assert np.max(np.abs(C4.cpu().numpy() - want4)) <= 1e-5 * max(1.0, np.abs(want4).max()) * np.sqrt(K)
+    # and it is not the BF16 product
+    assert np.max(np.abs(want4 - want3)) > 1e-3 * np.abs(want3).max()
 
 
 def _small_surrogate(gpu, seed=0):
@@ -70,17 +80,17 @@ def test_training_gradients_match_numpy(gpu):
     sur.backward(net, dY, keep)
     params = net.numpy()
     Yb, lb, gb = __import__("pyphasefield_amd.surrogate", fromlist=["x"]).numpy_forward_backward(
-        params, X.astype(np.float64), Yt.astype(np.float64), bf16=True)
+        params, X.astype(np.float64), Yt.astype(np.float64), rounded=True)
     Yf, lf, gf = __import__("pyphasefield_amd.surrogate", fromlist=["x"]).numpy_forward_backward(
-        params, X.astype(np.float64), Yt.astype(np.float64), bf16=False)
+        params, X.astype(np.float64), Yt.astype(np.float64), rounded=False)
     np.testing.assert_allclose(Y.cpu().numpy(), Yb, rtol=0, atol=2e-4 * np.abs(Yb).max())
     assert abs(float(part.sum()) / Y.numel() - lb) <= 1e-4 * lb
     for k in ("W1", "b1", "W2", "b2", "W3", "b3"):
         g = net.gview[k].cpu().numpy()
         scale = np.abs(gb[k]).max()
-        # against the BF16-emulating reference: FP32 accumulation-order level
+        # against the rounding-emulating reference: FP32 accumulation-order level
         assert np.max(np.abs(g - gb[k])) <= 2e-3 * scale, k
-        # against exact FP64 arithmetic: BF16 operand rounding level
+        # against exact FP64 arithmetic: FP16 / BF16 operand rounding level
         assert np.max(np.abs(g - gf[k])) <= 3e-2 * np.abs(gf[k]).max(), k
 
 
@@ -157,17 +167,17 @@ def test_packed_inference_kernel_matches_training_forward(gpu, trained):
     sim.fields[0].data[1:-1, 1:-1] = rng.random((40, 70))
     sim.simulate(0)
     sim.just_before_simulating()
-    # exact layout check: every activation as a BF16 hi + lo pair, as in the
+    # exact layout check: every activation as an FP16 hi + lo pair, as in the
     # GEMM forward pass
     trained.split2 = trained.split3 = True
     trained._packed = None
     sim.thermo_pass()
     tr = sim._fields_transfer_gpu_device.cpu().numpy().astype(np.float64)
-    # default inference (BF16 activations): within the rounding scatter
+    # default inference (FP16 activations): within the rounding scatter
     trained.split2 = trained.split3 = False
     trained._packed = None
     sim.thermo_pass()
-    tr_bf16 = sim._fields_transfer_gpu_device.cpu().numpy().astype(np.float64)
+    tr_16 = sim._fields_transfer_gpu_device.cpu().numpy().astype(np.float64)
     Tpad = sim.temperature._data
     c1 = sim.fields[3].data
     c2 = sim.fields[4].data
@@ -183,7 +193,7 @@ def test_packed_inference_kernel_matches_training_forward(gpu, trained):
     pL[gated], pS[gated] = yL[gated], yS[gated]
     dg = tr[0][inner].ravel()
     assert np.max(np.abs(dg - pS[:, 0])) <= 1e-3 * np.abs(pS[:, 0]).max() + 0.5
-    d16 = tr_bf16[0][inner].ravel() - dg
+    d16 = tr_16[0][inner].ravel() - dg
     assert np.sqrt(np.mean(d16 ** 2)) < 20.0 and np.abs(d16).max() < 100.0  # J/mol
     # mu_l = (dG_L,l + h (dG_S,l - dG_L,l)) / v_m + (W_l - W_N) g T, ideal
     # mixing added analytically to both phase derivatives
@@ -256,9 +266,9 @@ def growth_case(gpu, trained):
     steps = 200
     want = None
     results = {}
-    variants = [("surrogate, hidden inputs as BF16 pairs", (True, True)),
-                ("surrogate, layer-3 inputs BF16", (True, False)),
-                ("surrogate, hidden inputs BF16 (default)", (False, False)), ("fp32", None)]
+    variants = [("surrogate, hidden inputs as FP16 pairs", (True, True)),
+                ("surrogate, layer-3 inputs FP16", (True, False)),
+                ("surrogate, hidden inputs FP16 (default)", (False, False)), ("fp32", None)]
     for name, split in variants:
         extra = {}
         if split is not None:
@@ -283,8 +293,8 @@ def growth_case(gpu, trained):
     return results
 
 
-SURROGATE_VARIANTS = ("surrogate, hidden inputs as BF16 pairs", "surrogate, layer-3 inputs BF16",
-                      "surrogate, hidden inputs BF16 (default)")
+SURROGATE_VARIANTS = ("surrogate, hidden inputs as FP16 pairs", "surrogate, layer-3 inputs FP16",
+                      "surrogate, hidden inputs FP16 (default)")
 
 
 @pytest.mark.xfail(strict=False, reason="unmet bar: full-domain composition L-inf of the surrogate engine is "
