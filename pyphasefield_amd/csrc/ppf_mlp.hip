@@ -181,7 +181,9 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float
 //               (T_hi, c1_hi, c2_hi, T_lo, c1_lo, c2_lo, 1, 1): slots 6 and 7
 //               hold the layer-1 bias as an FP16 hi + lo pair; lanes >= 32 zero
 //   set 2..9    layer 2, output half m = (set-2)/4, K block kb = (set-2)%4
-//   set 10..13  layer 3, K block kb, output rows 0..2 (rows 3..31 zero)
+//   set 10..13  layer 3, K block kb, as the A operands of the 4x4x4 MFMA
+//               (mlp_eval): lane l element t = weight of output row l % 4
+//               (row 3 zero) for the hidden row of slot (l / 32, t)
 // followed by FP32 b1[64] (unused: packed above), b2[64], b3[4] (b3[3] unused).
 // Activation scaling: the host divides layer 1's weights and bias by a power
 // of two s1 and layer 2's bias by s2, multiplies layer 2's weights by s1/s2
@@ -282,14 +284,23 @@ __device__ __forceinline__ f32x16 bias_tile(const float* b, int row0, int u) {
   return acc;
 }
 
-// One phase network on G groups of 32 cells; the weight operands read from
-// LDS are shared by the groups.  x[g]: layer-1 B operand of group g.
-// out[g][q] = normalised output q of cell (lane % 32) of group g, valid in
-// lanes 0..31.  S2 / S3: the inputs of layer 2 / layer 3 enter as FP16 hi + lo
-// pairs (two MFMAs per K block) instead of FP16 only.
-template <int G, bool S2, bool S3>
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// One phase network on the two groups of 32 cells of a tile; the weight
+// operands read from LDS are shared by the groups.  x[g]: layer-1 B operand
+// of group g.  out[q] = normalised output q of the lane's own cell: cell
+// lane % 32 of group lane / 32.  S2 / S3: the inputs of layer 2 / layer 3
+// enter as FP16 hi + lo pairs (two MFMAs per K block) instead of FP16 only.
+// Layer 3 (three outputs) runs on the 16-block 4x4x4 MFMA: block b is lanes
+// 4b..4b+3; lane 4b + r supplies weight row r (row 3 is zero padding) and
+// lane 4b + j its own cell's hidden values, so item r of lane l ends as
+// output r of cell lane % 32 summed over the hidden rows lane l holds (half
+// l / 32 of them); one lane-half swap adds the two halves.  A quarter of the
+// 32x32x16 MFMA's work for 3 of its 32 rows.
+template <bool S2, bool S3>
 __device__ __forceinline__ void mlp_eval(const f16x8* __restrict__ w, const float* __restrict__ b,
-                                         const f16x8 (&x)[G], int lane, float (&out)[G][3]) {
+                                         const f16x8 (&x)[2], int lane, float (&out)[3]) {
+  constexpr int G = 2;
   const int u = lane >> 5;
   f32x16 h[G][2];
   const f32x16 zero = {};
@@ -320,22 +331,33 @@ __device__ __forceinline__ void mlp_eval(const f16x8* __restrict__ w, const floa
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) pack_layer<S3>(h[g], bh[g], bl[g]);
-  f32x16 y[G];
+  f32x4 y[G] = {};
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
     const f16x8 a = w[(10 + kb) * 64 + lane];
+    const f16x4 a0 = __builtin_shufflevector(a, a, 0, 1, 2, 3), a1 = __builtin_shufflevector(a, a, 4, 5, 6, 7);
 #pragma unroll
-    for (int g = 0; g < G; ++g) y[g] = mfma16(a, bh[g][kb], kb == 0 ? zero : y[g]);
-    if (S3) {
-#pragma unroll
-      for (int g = 0; g < G; ++g) y[g] = mfma16(a, bl[g][kb], y[g]);
+    for (int g = 0; g < G; ++g) {
+      y[g] = __builtin_amdgcn_mfma_f32_4x4x4f16(a0, __builtin_shufflevector(bh[g][kb], bh[g][kb], 0, 1, 2, 3), y[g],
+                                                0, 0, 0);
+      y[g] = __builtin_amdgcn_mfma_f32_4x4x4f16(a1, __builtin_shufflevector(bh[g][kb], bh[g][kb], 4, 5, 6, 7), y[g],
+                                                0, 0, 0);
+      if (S3) {
+        y[g] = __builtin_amdgcn_mfma_f32_4x4x4f16(a0, __builtin_shufflevector(bl[g][kb], bl[g][kb], 0, 1, 2, 3),
+                                                  y[g], 0, 0, 0);
+        y[g] = __builtin_amdgcn_mfma_f32_4x4x4f16(a1, __builtin_shufflevector(bl[g][kb], bl[g][kb], 4, 5, 6, 7),
+                                                  y[g], 0, 0, 0);
+      }
     }
   }
-  // output bias on the three used rows (valid in lanes 0..31)
+  // swap the lane halves between the groups' partial sums: r[0] = (group 0
+  // low, group 1 low), r[1] = (group 0 high, group 1 high); their sum is
+  // group 0's output in lanes 0..31 and group 1's in lanes 32..63
 #pragma unroll
-  for (int g = 0; g < G; ++g)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) out[g][q] = y[g][q] + b[128 + q];
+  for (int q = 0; q < 3; ++q) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(y[0][q]), __float_as_uint(y[1][q]), false, false);
+    out[q] = (__uint_as_float(r[0]) + __uint_as_float(r[1])) + b[128 + q];
+  }
 }
 
 struct Plain5 {
@@ -424,8 +446,8 @@ __device__ __forceinline__ void write_transfer(float* __restrict__ tr, uint32_t 
 //    group 0) into one result and its high half (cells 32..63, group 1) into
 //    the other.  The K slots of lanes 32..63 then hold a copy of the same
 //    cell's inputs, which the zero weights of those slots cancel exactly.
-//  * One v_permlane32_swap per output moves group 1's results (rows 0..2 of
-//    the layer-3 tile, lanes 0..31) to lanes 32..63, beside group 0's.
+//  * Layer 3's lane-half sum (mlp_eval) leaves group 0's outputs in lanes
+//    0..31 and group 1's in lanes 32..63, each lane's own cell.
 //  * NOISE: composition noise on (a separate instantiation, so the noise
 //    path's arithmetic is never speculated into the common one).
 struct MlpLean {
@@ -517,14 +539,11 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(
     float GL[3], GS[3];
 #pragma unroll 1
     for (int ph = 0; ph < 2; ++ph) {
-      float o[2][3];
-      mlp_eval<2, S2, S3>(sw[ph], sb[ph], xin, lane, o);
+      float o[3];
+      mlp_eval<S2, S3>(sw[ph], sb[ph], xin, lane, o);
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        // lanes 0..31: group 0 (own cell); lanes 32..63: group 1 from lane - 32
-        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(o[0][q]), __float_as_uint(o[1][q]), false,
-                                                        false);
-        const float v = __uint_as_float(r[0]) * net.out_sigma[ph][q] + net.out_mu[ph][q];
+        const float v = o[q] * net.out_sigma[ph][q] + net.out_mu[ph][q];
         GL[q] = ph == 0 ? v : GL[q];
         GS[q] = ph == 1 ? v : GS[q];
       }

@@ -732,7 +732,7 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
 
   const ppf_ncomp_params& p = k.p;
   const R idx = R(1) / dx;
-  const R v_m = R(p.v_m), M_qmax = R(p.M_qmax), Hh = R(p.H), y_e = R(p.y_e), beta = R(p.beta);
+  const R v_m = R(p.v_m), inv_v_m = R(1.0 / p.v_m), M_qmax = R(p.M_qmax), Hh = R(p.H), y_e = R(p.y_e), beta = R(p.beta);
   const R dt = R(p.dt), ebar = R(p.ebar), eqbar = R(p.eqbar);
   R ebar2 = ebar * ebar, eqbar2 = eqbar * eqbar;
   if (p.flags & PPF_NC_EBAR_FROM_S) {
@@ -741,7 +741,10 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
     eqbar2 = R(0.25) * ebar2;
   }
   const uint64_t ncells = uint64_t(d.ax.n + 2) * uint64_t(d.ay.n + 2);
-  int tx = threadIdx.x % kTX, tyb = threadIdx.x / kTX;
+  // kTX = 64: each wave owns one row of the tile per pass, so the row (and
+  // every row base address) is wave-uniform
+  static_assert(kTX == 64, "one wave per tile row");
+  int tx = threadIdx.x % kTX, tyb = __builtin_amdgcn_readfirstlane(threadIdx.x / kTX);
   int64_t x = x0 + tx;
   if (x > d.ax.n) return;
   const int lx = tx + 1;
@@ -855,8 +858,8 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
       // clamped |grad phi|^2 (every term is homogeneous in that scale), so no
       // |grad phi|^8 is formed; with dx ~ 5e-6 cm the reference form exceeds
       // the FP32 range.
-      R m = fsqrt(mag_grad_phi2);
-      R im = R(1) / m;
+      const R im = __builtin_amdgcn_rsqf(mag_grad_phi2);
+      const R m = mag_grad_phi2 * im;
       R X = dphidx * im, Yg = dphidy * im, Px = psix * im, Py = psiy * im;
       R Px2 = Px * Px, Py2 = Py * Py, Px3 = Px2 * Px, Py3 = Py2 * Py;
       R P4 = Px2 * Px2 + Py2 * Py2;
@@ -898,20 +901,24 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
     if constexpr (FUSED) {
       dG = sG[ly - 1][lx - 1];
     } else {
-      dG = tr[y * tsy + x];
+      dG = (tr + y * tsy)[x];
     }
     R dphidt = ebar2 * (R(1) - R(3) * y_e) * (Tc * lphi + dTdx * dphidx + dTdy * dphidy);
     dphidt = dphidt + d_term_dx + d_term_dy;
-    dphidt = dphidt - hprime * dG / v_m;
+    if constexpr (sizeof(R) == 8) {
+      dphidt = dphidt - hprime * dG / v_m;
+    } else {
+      dphidt = dphidt - hprime * dG * inv_v_m;
+    }
     dphidt = dphidt - gprime * Tc * cW;
     dphidt = dphidt - R(4) * Hh * Tc * pc * mag_grad_q;
     dphidt = dphidt * M_phi;
-    uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
     if (p.noise_phi != 0.0) {
+      const uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
       R xi = normal01<R>(hash64(seed, rng_counter(step, S_NC_PHI, ncells, cell)));
       dphidt = dphidt + sqrt(R(2.) * R(8.314) * Tc * M_phi / v_m) * xi * R(p.noise_phi);
     }
-    int64_t o = y * d.ay.stride + x;
+    R* const orow = out + y * d.ay.stride;  // wave-uniform row base
     // dc_i/dt (:191-203) and update (:243-249)
 #pragma unroll
     for (int l = 0; l < N - 1; ++l) {
@@ -923,7 +930,7 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
         acc = acc - divagradb(Mc.c * cj.c, Mc.xp * cj.xp, Mc.xm * cj.xm, Mc.yp * cj.yp, Mc.ym * cj.ym, dFj.c,
                               dFj.xp, dFj.xm, dFj.yp, dFj.ym, idx);
       }
-      out[o + (3 + l) * d.sf] = acc * dt + sf[3 + l][ly][lx];
+      orow[x + (3 + l) * d.sf] = acc * dt + sf[3 + l][ly][lx];
     }
     // dq/dt (:205-232)
     R D_q = R(2) * Hh * Tc * (pc * pc);
@@ -949,6 +956,7 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
     R lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx * idx;
     R noise_q1 = R(0), noise_q4 = R(0);
     if (p.noise_q != 0.0) {
+      const uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
       R amp = R(p.noise_q) * sqrt(R(0.0000000001) * R(8.314) * Tc / v_m);
       noise_q1 = amp * normal01<R>(hash64(seed, rng_counter(step, S_NC_Q1, ncells, cell)));
       noise_q4 = amp * normal01<R>(hash64(seed, rng_counter(step, S_NC_Q4, ncells, cell)));
@@ -965,15 +973,15 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
     if (phin > phi_) phin = phi_;
     R q1n = q1c + dt * dq1dt;
     R q4n = q4c + dt * dq4dt;
-    out[o] = phin;
+    orow[x] = phin;
     if constexpr (sizeof(R) == 8) {
       R renorm = fsqrt(q1n * q1n + q4n * q4n);
-      out[o + d.sf] = q1n / renorm;
-      out[o + 2 * d.sf] = q4n / renorm;
+      orow[x + d.sf] = q1n / renorm;
+      orow[x + 2 * d.sf] = q4n / renorm;
     } else {
       const R irn = __builtin_amdgcn_rsqf(q1n * q1n + q4n * q4n);
-      out[o + d.sf] = q1n * irn;
-      out[o + 2 * d.sf] = q4n * irn;
+      orow[x + d.sf] = q1n * irn;
+      orow[x + 2 * d.sf] = q4n * irn;
     }
   }
 }

@@ -70,7 +70,9 @@ def pack_phase(W1, W2, W3, b1, b2, b3):
     inputs T, c1, c2, T, c1, c2 for the hi and lo parts, slots (0, 6..7) the
     layer-1 bias as an FP16 hi + lo pair against two constant-1 inputs, so the
     bias needs no accumulator tile), layer 2 (2 output halves x 4 K blocks),
-    layer 3 (4 K blocks, output rows 0..2, rest zero).  Scaled by the powers
+    layer 3 (4 K blocks as 4x4x4-MFMA A operands: lane l, element t holds the
+    weight of output row l % 4, zero for row 3, from the hidden row of slot
+    (l // 32, t)).  Scaled by the powers
     of two of ``activation_scales``: layer 1 by 1/s1, layer 2's weights by
     s1/s2 and its bias by 1/s2, layer 3's weights by s2 (the network's outputs
     are unchanged; its hidden values become clamp-able to [0, 1]).
@@ -99,11 +101,15 @@ def pack_phase(W1, W2, W3, b1, b2, b3):
         for kb in range(4):
             rows = _slot_rows(kb // 2, kb % 2)      # [2, 8]
             sets.append(W2[32 * m + r[:, None], rows[u]])  # [64, 8]
+    # layer 3 as the A operands of the 16-block 4x4x4 MFMA: lane l supplies
+    # output row l % 4 (row 3 zero) for the hidden rows its block's B lanes
+    # hold, those of half l // 32
+    orow = lanes % 4
     for kb in range(4):
         rows = _slot_rows(kb // 2, kb % 2)
         blk = np.zeros((64, 8), np.float64)
-        ok = r < 3
-        blk[ok] = W3[r[ok][:, None], rows[u[ok]]]
+        ok = orow < 3
+        blk[ok] = W3[orow[ok][:, None], rows[u[ok]]]
         sets.append(blk)
     bias = np.zeros(132, np.float32)
     bias[:64] = b1

@@ -4,9 +4,10 @@ kernel, checked by emulating the kernel's operand flow in numpy.
 The emulation follows the documented v_mfma_f32_32x32x16_f16 lane layout:
 A lane l element t = A[l % 32][slot (l // 32, t)], B lane l element t =
 B[slot (l // 32, t)][l % 32], D lane l item v = D[8 (v // 4) + 4 (l // 32) +
-v % 4][l % 32]; the kernel turns items 8p..8p+7 of a D tile into the next B
-operand, either clamped to [0, 1] and rounded to FP16 (default) or as the
-FP16 pair hi + lo of the ReLU.  Reference: the plain numpy forward pass of
+v % 4][l % 32], and for layer 3 that of the 16-block 4x4x4 MFMA (lane 4b + j,
+item r += sum_k A[lane 4b + r][k] B[lane 4b + j][k]); the kernel turns items
+8p..8p+7 of a D tile into the next B operand, either clamped to [0, 1] and
+rounded to FP16 (default) or as the FP16 pair hi + lo of the ReLU.  Reference: the plain numpy forward pass of
 the unscaled network with FP16 weights."""
 
 import numpy as np
@@ -30,6 +31,18 @@ def _mfma(A, B, D):
     for l in range(64):
         for v in range(16):
             out[l, v] += full[8 * (v // 4) + 4 * (l // 32) + v % 4, l % 32]
+    return out
+
+
+def _mfma4(A, B, D):
+    """16-block 4x4x4 MFMA on [64, 4] lane operands and [64, 4] accumulators:
+    lane 4b + j, item r += sum_k A[4b + r][k] B[4b + j][k] (the layout
+    measured on gfx950)."""
+    out = D.copy()
+    for l in range(64):
+        b0 = l - l % 4
+        for r in range(4):
+            out[l, r] += float(np.dot(A[b0 + r], B[l]))
     return out
 
 
@@ -76,12 +89,14 @@ def _emulate(words, bias, X, split=False):
         z.append(acc)
     peak = max(peak, max(float(np.max(x)) for x in z))
     packs = [_act(z[kb // 2], kb % 2, split) for kb in range(4)]
-    y = np.zeros((64, 16))
-    y[:32, 0:3] = b3
+    y = np.zeros((64, 4))
     for kb in range(4):
         for op in packs[kb]:
-            y = _mfma(W[10 + kb], op, y)
-    return y[:32, 0:3].T, peak
+            for e in range(2):
+                y = _mfma4(W[10 + kb][:, 4 * e:4 * e + 4], op[:, 4 * e:4 * e + 4], y)
+    # the two lane halves hold the two halves of the hidden rows of each cell
+    out = y[:32, 0:3] + y[32:, 0:3] + b3
+    return out.T, peak
 
 
 def _net(seed=0):
