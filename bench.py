@@ -163,15 +163,18 @@ def copy_peak(steps=20):
 
 # per network 3-64-64-3: (3*64 + 64*64 + 64*3) multiply-adds; two networks per cell
 MLP_FLOP_PER_CELL = 2 * 2 * (3 * 64 + 64 * 64 + 64 * 3)
-BF16_DENSE_PEAK_TFLOPS = 2500.0
+FP16_DENSE_PEAK_TFLOPS = 2500.0  # dense FP16 / BF16 MFMA peak of the MI355X
 
 
-def ncomp_same_run(steps=60, warmup=20):
+def ncomp_same_run(steps=100, warmup=20, rounds=3):
     """The N-component workload (2048^2, T0 = 1440 K + 1 K/cell) with the
     surrogate thermodynamics, timed in this process next to the plain FP64
     engine (forward differences: the reference's method and precision) and
     the plain FP32 engine (analytic derivatives).  The networks are trained
-    here on 1e6 samples of the plain evaluation (time reported)."""
+    here on 1e6 samples of the plain evaluation (time reported).  The three
+    engines are built and warmed first, then timed in `rounds` interleaved
+    rounds of `steps` steps; each engine's median round is reported, so all
+    three are measured under the same GPU conditions."""
     import torch
     out = {"workload": "N-component 2D 2048x2048, 3 components, 16 seeds r=6, T = %g K + 1 K/cell" % NC_T0}
     t0 = time.perf_counter()
@@ -179,8 +182,17 @@ def ncomp_same_run(steps=60, warmup=20):
     sur.simulate(1)  # uploads, trains the networks, captures nothing yet
     torch.cuda.synchronize()
     out["training_seconds"] = time.perf_counter() - t0
-    _warm(sur, warmup)
-    t_sur = _timed(sur.simulate, steps) / steps
+    sims = {"surrogate": sur,
+            "plain_fp64_forward_difference": ncomp_sim(2048, "FP64", "forward_difference"),
+            "plain_fp32_analytic": ncomp_sim(2048, "FP32", "analytic")}
+    for sim in sims.values():
+        _warm(sim, warmup)
+    times = {k: [] for k in sims}
+    for _ in range(rounds):
+        for k, sim in sims.items():
+            times[k].append(_timed(sim.simulate, steps) / steps)
+    med = {k: sorted(v)[len(v) // 2] for k, v in times.items()}
+    t_sur = med["surrogate"]
     # cells of the last step handed to the FP64 fallback (outside the trained box)
     padded = 2050 * 2050
     marks = sur._surrogate.packed()[1][-1]
@@ -188,20 +200,17 @@ def ncomp_same_run(steps=60, warmup=20):
     k = 20
     t_mlp = _timed(lambda m: [sur.thermo_pass() for _ in range(m)], k) / k
     tflops = MLP_FLOP_PER_CELL * padded / t_mlp / 1e12
-    out["surrogate"] = {"ms_per_step": t_sur * 1e3, "network_pass_ms": t_mlp * 1e3,
-                        "network_tflops": tflops, "network_frac_of_bf16_dense_peak": tflops / BF16_DENSE_PEAK_TFLOPS,
+    out["surrogate"] = {"ms_per_step": t_sur * 1e3, "rounds_ms": [t * 1e3 for t in times["surrogate"]],
+                        "network_pass_ms": t_mlp * 1e3, "network_tflops": tflops,
+                        "network_frac_of_fp16_dense_peak": tflops / FP16_DENSE_PEAK_TFLOPS,
                         "fallback_cell_fraction": fallback / padded}
-    del sur
-    torch.cuda.empty_cache()
-    for name, prec, dg in (("plain_fp64_forward_difference", "FP64", "forward_difference"),
-                           ("plain_fp32_analytic", "FP32", "analytic")):
-        sim = ncomp_sim(2048, prec, dg)
-        _warm(sim, warmup)
-        tt = _timed(sim.simulate, steps) / steps
-        out[name] = {"ms_per_step": tt * 1e3, "surrogate_speedup": tt / t_sur}
-        del sim
-        torch.cuda.empty_cache()
+    for name in ("plain_fp64_forward_difference", "plain_fp32_analytic"):
+        out[name] = {"ms_per_step": med[name] * 1e3, "rounds_ms": [t * 1e3 for t in times[name]],
+                     "surrogate_speedup": med[name] / t_sur}
     out["steps"] = steps
+    out["rounds"] = rounds
+    del sims, sur
+    torch.cuda.empty_cache()
     return out
 
 

@@ -484,6 +484,9 @@ typedef struct ppf_mlp_net {
   int32_t* marks;          /* optional device int32[2], zero-initialised once: counts of cells
                               handed to the FP64 fallback (lets it skip its scan when none are);
                               NULL: the fallback always scans */
+  const ppf_thermo* thermo; /* optional device copy of the ppf_thermo passed with the call: when
+                              set, cells outside the trained box are evaluated in the network
+                              kernel itself (FP64 plain evaluation) and no second kernel runs */
 } ppf_mlp_net;
 #define PPF_MLP_SPLIT3 1
 #define PPF_MLP_SPLIT2 2

@@ -139,8 +139,19 @@ class NCGPU_new(Simulation):
                                                             self._torch_dtype(), dev)
         self._tgrid = _lib.make_grid(self._fields_transfer_gpu_device, 2, self._boundary_conditions_type, self.dx)
         self._surrogate = None
+        self._thermo_dev = None
         if u.get("thermo_eval", "plain") == "surrogate":
             self._setup_surrogate()
+            # cells outside the trained box: the plain FP64 evaluation inside
+            # the network kernel ("inline", reading this device copy of the
+            # model) or in a second kernel ("kernel")
+            fb = u.get("surrogate_fallback", "inline")
+            if fb not in ("inline", "kernel"):
+                raise ValueError("surrogate_fallback must be 'inline' or 'kernel'")
+            if fb == "inline":
+                import torch
+                raw = bytes(self._thermo)
+                self._thermo_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev)
 
     def _temperature_range(self):
         if self._thermal is not None:
@@ -192,6 +203,8 @@ class NCGPU_new(Simulation):
 
     def _graph_key(self):
         sur = getattr(self, "_surrogate", None)
+        if sur is not None:
+            sur.packed()[0].thermo = self._thermo_dev.data_ptr() if self._thermo_dev is not None else None
         net = bytes(sur.packed()[0]) if sur is not None else b""
         return (self._kernel_mode(), bytes(self._ncparams), bytes(self._thermo), bytes(self.temperature_descriptor()),
                 int(self.seed), net, self._fields_transfer_gpu_device.data_ptr())
@@ -215,6 +228,7 @@ class NCGPU_new(Simulation):
                 self._ncparams, self._thermo)
         if self._surrogate is not None:
             net, _keep = self._surrogate.packed()
+            net.thermo = self._thermo_dev.data_ptr() if self._thermo_dev is not None else None
             if src[0] == "host":
                 rc = L.ppf_ncomp_thermo_mlp(*head, net, self.seed, src[1], _lib.stream_handle())
             else:

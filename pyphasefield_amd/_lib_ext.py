@@ -14,7 +14,7 @@ PPF_MLP_SPLIT2 = 2
 class MlpNet(ctypes.Structure):
     _fields_ = [("weights", c_vp * 2), ("bias", c_vp * 2), ("in_lo", c_f32 * 3), ("in_scale", c_f32 * 3),
                 ("out_mu", (c_f32 * 3) * 2), ("out_sigma", (c_f32 * 3) * 2), ("T_lo", c_f32), ("T_hi", c_f32),
-                ("c_lo", c_f32), ("flags", c_i32), ("marks", c_vp)]
+                ("c_lo", c_f32), ("flags", c_i32), ("marks", c_vp), ("thermo", c_vp)]
 
 
 P = ctypes.POINTER

@@ -157,6 +157,23 @@ __device__ __forceinline__ R temperature_at(const ppf_temperature& t, const int6
   return R(T);
 }
 
+// temperature_at for an interior cell (x in [1, n_x], y in [1, n_y], 2D):
+// the same operations in the same order, without the boundary resolution
+template <typename R>
+__device__ __forceinline__ R temperature_interior(const ppf_temperature& t, const int64_t n[3], const int bc[3],
+                                                  int64_t x, int64_t y, double dx, int64_t step) {
+  if (t.kind == PPF_T_CONST) return R(t.T0);
+  if (t.kind == PPF_T_LINEAR) {
+    double T = t.T0;
+    if (t.dTdx != 0.0) T = T + double(x - 1) * (dx * t.dTdx);
+    if (t.dTdy != 0.0) T = T + double(y - 1) * (dx * t.dTdy);
+    if (t.dTdz != 0.0) T = T + double(1 - 1) * (dx * t.dTdz);
+    if (t.dTdt != 0.0) T = T + double(step - 1) * (t.dTdt * t.dt);
+    return R(T);
+  }
+  return temperature_at<R>(t, n, bc, x, y, 1, dx, step);
+}
+
 // ------------------------------------------------------------ step counter --
 // Step number seen by a step kernel: `off` alone, or *base + off when `base`
 // (a device int64) is set -- the form a captured HIP graph replays, with the

@@ -364,7 +364,9 @@ struct Plain5 {
   float dG, dGL[2], dGS[2];
 };
 
-// plain FP64 evaluation of G_S - G_L and both phases' derivatives
+// plain FP64 evaluation of G_S - G_L and both phases' derivatives; the
+// phases one after the other (each value depends on its own phase only, so
+// the order changes no result and halves the coefficients held at a time)
 __device__ __forceinline__ Plain5 plain_outputs(float c1, float c2, float T, const ppf_thermo& th) {
   constexpr int N = 3;
   double c[N] = {double(c1), double(c2), 1.0 - double(c1) - double(c2)};
@@ -373,12 +375,33 @@ __device__ __forceinline__ Plain5 plain_outputs(float c1, float c2, float T, con
   double id = ideal_sum<double, N>(c);
   double gl = 0.0, gs = 0.0;
   double dl[N], ds[N];
-  with_phases<double, N>(th, t, lnT, [&](const auto& PL, const auto& PS) {
-    gl = gibbs(c, PL, id);
-    gs = gibbs(c, PS, id);
-    gibbs_grad(c, PL, dl);
-    gibbs_grad(c, PS, ds);
-  });
+  if (th.tables) {
+    {
+      PhaseAtT<double, N, true> P;
+      phase_at_tables<double, N>(&th.tables->liquid, th.liquid.R, t, lnT, P);
+      gl = gibbs(c, P, id);
+      gibbs_grad(c, P, dl);
+    }
+    {
+      PhaseAtT<double, N, true> P;
+      phase_at_tables<double, N>(&th.tables->solid, th.solid.R, t, lnT, P);
+      gs = gibbs(c, P, id);
+      gibbs_grad(c, P, ds);
+    }
+  } else {
+    {
+      PhaseAtT<double, N, false> P;
+      phase_at<double, N>(th.liquid, t, lnT, P);
+      gl = gibbs(c, P, id);
+      gibbs_grad(c, P, dl);
+    }
+    {
+      PhaseAtT<double, N, false> P;
+      phase_at<double, N>(th.solid, t, lnT, P);
+      gs = gibbs(c, P, id);
+      gibbs_grad(c, P, ds);
+    }
+  }
   Plain5 r;
   r.dG = float(gs - gl);
   for (int l = 0; l < N - 1; ++l) {
@@ -386,6 +409,13 @@ __device__ __forceinline__ Plain5 plain_outputs(float c1, float c2, float T, con
     r.dGS[l] = float(ds[l]);
   }
   return r;
+}
+
+// the same, out of line, for the surrogate kernel's cells outside the trained
+// box: a call keeps the FP64 evaluation's registers out of the tile loop's
+// allocation (the model is read through a device pointer)
+__device__ __attribute__((noinline)) Plain5 plain_outputs_call(float c1, float c2, float T, const ppf_thermo* th) {
+  return plain_outputs(c1, c2, T, *th);
 }
 
 __device__ __forceinline__ float fast_ln(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
@@ -450,12 +480,19 @@ __device__ __forceinline__ void write_transfer(float* __restrict__ tr, uint32_t 
 //    0..31 and group 1's in lanes 32..63, each lane's own cell.
 //  * NOISE: composition noise on (a separate instantiation, so the noise
 //    path's arithmetic is never speculated into the common one).
+//  * Cells outside the trained box: with a device copy of the thermodynamic
+//    model (thermo), the plain FP64 evaluation runs here, in the lanes of
+//    those cells (its coefficients are read through the pointer inside the
+//    branch, so none of them is held in registers across the tile loop);
+//    without one they are marked for k_ncomp_thermo_fallback.
 struct MlpLean {
   uint32_t px, py, nx, ny, sy, sf4, tsy, tsf4;  // padded extents, strides (elements; *4: bytes)
   uint32_t ntiles, srow, scol;                  // tiles; wave stride of cells as rows and columns
   float T_lo, T_hi, c_lo, R;
   int32_t* marks;
+  const ppf_thermo* thermo;  // device copy of the model, or null
 };
+static_assert(sizeof(ppf_mlp_net) == 136, "ppf_mlp_net layout (mirrored by _lib_ext.MlpNet)");
 
 __device__ __forceinline__ void lean_store(float v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, voff, int32_t(soff), 0);
@@ -472,6 +509,9 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(
   for (int i = threadIdx.x; i < 2 * kMlpWords; i += 256) sw[i / kMlpWords][i % kMlpWords] = net.w[i / kMlpWords][i % kMlpWords];
   for (int i = threadIdx.x; i < 2 * 132; i += 256) sb[i / 132][i % 132] = net.bias[i / 132][i % 132];
   __syncthreads();
+  // this step's cells outside the box are counted in marks[step & 1]; the
+  // next step's counter starts at zero (cleared here when no second kernel runs)
+  if (g.thermo && g.marks && blockIdx.x == 0 && threadIdx.x == 0) g.marks[(step + 1) & 1] = 0;
   const int lane = threadIdx.x & 63;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t tstride = gridDim.x * 4;
@@ -498,18 +538,19 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(
     }
     const bool valid = y < g.py;
     if (!valid) x = y = 1;
-    float phi, c1, c2;
+    float phi, c1, c2, T;
     if (x - 1 < g.nx && y - 1 < g.ny) {
       const uint32_t o = (y * g.sy + x) * 4u;
       phi = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, o, 0, 0));
       c1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, o, int32_t(3 * g.sf4), 0));
       c2 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, o, int32_t(4 * g.sf4), 0));
+      T = temperature_interior<float>(t, n3, bc3, int64_t(x), int64_t(y), double(dx), step);
     } else {
       phi = fold(in, b, d, int64_t(x), int64_t(y), int64_t(0), dx);
       c1 = fold(in + 3 * d.sf, b ? b + 3 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
       c2 = fold(in + 4 * d.sf, b ? b + 4 * d.sf : nullptr, d, int64_t(x), int64_t(y), int64_t(0), dx);
+      T = temperature_at<float>(t, n3, bc3, int64_t(x), int64_t(y), 1, double(dx), step);
     }
-    const float T = temperature_at<float>(t, n3, bc3, int64_t(x), int64_t(y), 1, double(dx), step);
     const float cN = 1.f - c1 - c2;
     const bool gate = T < g.T_lo || T > g.T_hi || c1 < g.c_lo || c2 < g.c_lo || cN < g.c_lo;
     const float xv[3] = {(T - net.in_lo[0]) * net.in_scale[0] - 1.f, (c1 - net.in_lo[1]) * net.in_scale[1] - 1.f,
@@ -557,32 +598,48 @@ __global__ __launch_bounds__(256, PPF_MLP_WAVES) void k_ncomp_thermo_mlp(
       ++y0;
     }
     if (!valid) continue;
-    if (gate) {  // outside the trained box: marked for k_ncomp_thermo_fallback
-      lean_store(__builtin_nanf(""), rtr, ob, 0);
-      if (g.marks) atomicAdd(&g.marks[step & 1], 1);
-      continue;
-    }
-    // ideal mixing R T sum c ln c is analytic: it cancels in G_S - G_L and
-    // adds R T (ln c_l - ln c_N) to each derivative
-    const float RT = g.R * T;
-    const float lnN = fast_ln(fmaxf(cN, float(kCMin)));
     const float cl[N - 1] = {c1, c2};
+    float dG, dGL[N - 1], dGS[N - 1];
+    if (gate) {  // outside the trained box
+      if (g.marks) atomicAdd(&g.marks[step & 1], 1);
+      if (!g.thermo) {  // marked for k_ncomp_thermo_fallback
+        lean_store(__builtin_nanf(""), rtr, ob, 0);
+        continue;
+      }
+      const Plain5 r = plain_outputs_call(c1, c2, T, g.thermo);
+      dG = r.dG;
+#pragma unroll
+      for (int l = 0; l < N - 1; ++l) {
+        dGL[l] = r.dGL[l];
+        dGS[l] = r.dGS[l];
+      }
+    } else {
+      // ideal mixing R T sum c ln c is analytic: it cancels in G_S - G_L and
+      // adds R T (ln c_l - ln c_N) to each derivative
+      const float RT = g.R * T;
+      const float lnN = fast_ln(fmaxf(cN, float(kCMin)));
+      dG = GS[0];
+#pragma unroll
+      for (int l = 0; l < N - 1; ++l) {
+        const float idl = RT * (fast_ln(fmaxf(cl[l], float(kCMin))) - lnN);
+        dGL[l] = GL[1 + l] + idl;
+        dGS[l] = GS[1 + l] + idl;
+      }
+    }
     const float g_ = (phi * phi) * ((1.f - phi) * (1.f - phi));
     const float h_ = (phi * phi * phi) * (6.f * (phi * phi) - 15.f * phi + 10.f);
-    lean_store(GS[0], rtr, ob, 0);
+    lean_store(dG, rtr, ob, 0);
     const float mob = k.v_m * (k.D_L + h_ * k.dD) * __builtin_amdgcn_rcpf(8.314f * T);
     const float Wd[N - 1] = {k.W0, k.W1};
 #pragma unroll
     for (int l = 0; l < N - 1; ++l) {
-      const float idl = RT * (fast_ln(fmaxf(cl[l], float(kCMin))) - lnN);
-      const float dGL = GL[1 + l] + idl, dGS = GS[1 + l] + idl;
       lean_store(mob * cl[l], rtr, ob, (1 + l) * g.tsf4);
       float noise = 0.f;
       if constexpr (NOISE) {
         float xi = normal01<float>(hash64(seed, rng_counter(step, S_NC_C0 + l, ncell, uint64_t(cell))));
         noise = k.noise_c * sqrtf(2.f * 8.314f * T * k.inv_v_m) * xi;
       }
-      lean_store((dGL + h_ * (dGS - dGL)) * k.inv_v_m + Wd[l] * g_ * T + noise, rtr, ob, (N + l) * g.tsf4);
+      lean_store((dGL[l] + h_ * (dGS[l] - dGL[l])) * k.inv_v_m + Wd[l] * g_ * T + noise, rtr, ob, (N + l) * g.tsf4);
     }
   }
 }
@@ -749,6 +806,7 @@ static int ncomp_thermo_mlp_dispatch(const ppf_grid* g, const void* in, const vo
   lg.c_lo = m.c_lo;
   lg.R = m.R;
   lg.marks = m.marks;
+  lg.thermo = net->thermo;
   const bool nz = k.noise_c != 0.f;
   auto kern = nz ? (s2 ? (s3 ? k_ncomp_thermo_mlp<true, true, true> : k_ncomp_thermo_mlp<true, false, true>)
                        : (s3 ? k_ncomp_thermo_mlp<false, true, true> : k_ncomp_thermo_mlp<false, false, true>))
@@ -758,6 +816,7 @@ static int ncomp_thermo_mlp_dispatch(const ppf_grid* g, const void* in, const vo
                      static_cast<const float*>(bcvals), d, float(g->dx), *t, static_cast<float*>(transfer), k, m, lg,
                      seed, step);
   if (int rc = check_launch("ncomp_thermo_mlp")) return rc;
+  if (net->thermo) return 0;  // cells outside the box already evaluated in the kernel
   const unsigned fblocks = unsigned(std::min<int64_t>((ncell + 255) / 256, int64_t(4) * device_cus()));
   hipLaunchKernelGGL(k_ncomp_thermo_fallback, dim3(fblocks), dim3(256), 0, stream_of(s),
                      static_cast<const float*>(in), static_cast<const float*>(bcvals), d, float(g->dx), *t,

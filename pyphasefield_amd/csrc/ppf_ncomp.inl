@@ -688,7 +688,7 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
     for (int j = 0; j < NE; ++j) {
       const int e = threadIdx.x + j * kThreads;
       const int ly = e / W, lx = e - ly * W;
-      if (e < W * HH) sT[ly][lx] = temperature_at<R>(t, n3, bc3, x0 - 1 + lx, y0 - 1 + ly, 1, double(dx), step);
+      if (e < W * HH) sT[ly][lx] = temperature_interior<R>(t, n3, bc3, x0 - 1 + lx, y0 - 1 + ly, double(dx), step);
     }
   } else {
 #pragma unroll

@@ -50,7 +50,7 @@ def test_struct_layouts_match_header():
     from pyphasefield_amd.ensemble import EnsInstance
     assert ctypes.sizeof(EnsInstance) == 88
     from pyphasefield_amd._lib_ext import MlpNet
-    assert ctypes.sizeof(MlpNet) == 2 * 8 + 2 * 8 + 6 * 4 + 12 * 4 + 4 * 4 + 8
+    assert ctypes.sizeof(MlpNet) == 2 * 8 + 2 * 8 + 6 * 4 + 12 * 4 + 4 * 4 + 8 + 8
 
 
 def test_invalid_arguments_are_rejected_without_gpu_work():

@@ -213,6 +213,49 @@ def test_packed_inference_kernel_matches_training_forward(gpu, trained):
         assert np.max(np.abs(got - mu)) <= 1e-3 * np.abs(mu).max()
 
 
+def test_inline_fallback_equals_fallback_kernel(gpu, trained):
+    """Cells outside the trained box: the plain FP64 evaluation inside the
+    network kernel gives the transfer values of the separate fallback
+    kernel, with noise on and off."""
+    import pyphasefield_amd.Engines as engines
+    for noise in (0.0, 1.0):
+        out = {}
+        for fb in ("inline", "kernel"):
+            sim = engines.NCGPU_new(dimensions=[50, 90], seed=9)
+            sim.set_dx(0.0000046)
+            sim.set_temperature_type("LINEAR_GRADIENT")
+            sim.set_temperature_initial_T(1200.)
+            sim.set_temperature_dTdx(25.0 / 0.0000046)
+            sim.set_temperature_dTdy(0.0)
+            sim.set_temperature_dTdt(0.0)
+            sim.set_tdb_container(trained.model)
+            sim.set_boundary_conditions("NEUMANN")
+            sim.set_precision("FP32")
+            sim.set_user_data({"d_ratio": 1.1, "sim_type": "seeds", "number_of_seeds": 4, "seed_radius": 5,
+                               "initial_concentration_array": [0.3, 0.3], "noise_phi": 0., "noise_c": noise,
+                               "noise_q": 0., "thermo_eval": "surrogate", "surrogate": trained,
+                               "surrogate_fallback": fb})
+            sim.initialize_fields_and_imported_data()
+            rng = np.random.default_rng(8)
+            sim.fields[3].data[1:-1, 1:-1] = 0.005 + 0.45 * rng.random((50, 90))
+            sim.fields[4].data[1:-1, 1:-1] = 0.005 + 0.45 * rng.random((50, 90))
+            sim.fields[0].data[1:-1, 1:-1] = rng.random((50, 90))
+            sim.simulate(0)
+            sim.just_before_simulating()
+            marks = trained.packed()[1][-1]  # shared by every engine using this surrogate
+            marks.zero_()
+            sim.thermo_pass()
+            out[fb] = (sim._fields_transfer_gpu_device.cpu().numpy().astype(np.float64),
+                       int(marks[int(sim.time_step_counter) & 1].item()))
+        (a, na), (b, nb) = out["inline"], out["kernel"]
+        # cells outside the box in T (x beyond the trained range) and in c
+        assert na == nb and 0.1 * 50 * 90 < na < 50 * 90, (na, nb)
+        assert np.all(np.isfinite(a)) and np.all(np.isfinite(b))
+        for f in range(5):
+            scale = np.abs(b[f]).max()
+            assert np.max(np.abs(a[f] - b[f])) <= 1e-6 * scale, (noise, f)
+
+
 def _nc_growth_case(n, seed, **extra):
     """48^2 growth case near equilibrium: T = 1440 K + 1 K/cell along x
     (the synthetic model's liquidus at c = (0.3, 0.3) is 1471 K), three seeds."""
