@@ -548,7 +548,7 @@ def bench_ensemble(args):
         gs = 50 if args.steps % 50 == 0 else 2
     ens.simulate(args.warmup - args.warmup % 2 if args.warmup > 1 else 0, graph_steps=gs)
     t = _timed(lambda k: ens.simulate(k, graph_steps=gs), args.steps)
-    roof = _roofline(8.0 * ens.cells, t / args.steps, "k_ensemble")
+    roof = _roofline(8.0 * ens.cells, t / args.steps, Ensemble.kernel(best["config"]))
     cpu = None if args.no_cpu else cpu_ensemble(specs)
     nac = sum(s.kind == "AC" for s in specs)
     return _line("cell-updates/s", ens.cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f32",

@@ -521,7 +521,9 @@ typedef struct ppf_ens_instance {
 } ppf_ens_instance;        /* 88 bytes */
 
 /* number of tile configurations, and the tile shapes of one:
-   shape = {Allen-Cahn TX, TY, Cahn-Hilliard TX, TY} */
+   shape = {Allen-Cahn TX, TY, Cahn-Hilliard TX, TY}.  Configurations with
+   TX = {62, 60} (64 lanes less the halo) are wave tiles held in registers,
+   the others block tiles staged through LDS; both give identical results. */
 int ppf_ensemble_num_configs(void);
 int ppf_ensemble_config(int config, int32_t* shape);
 /* One explicit step of every instance: out = step(in).  instances: device

@@ -31,6 +31,8 @@ BCS = {"PERIODIC": 0, "NEUMANN": 1}
 XCDS = 8
 # relative cost of one Cahn-Hilliard cell-update (mu ring + second Laplacian)
 CH_WEIGHT = 2.0
+# output tile widths (Allen-Cahn, Cahn-Hilliard) of the wave-tile configurations
+WAVE_TILE_WIDTHS = (62, 60)
 
 
 class EnsInstance(ctypes.Structure):
@@ -282,7 +284,7 @@ class Ensemble:
             self.step(n)
 
     # ---------------------------------------------------------- autotune --
-    def autotune(self, steps=20, blocks_per_cu=(2, 4, 8), configs=None, cache=True, verbose=False):
+    def autotune(self, steps=20, blocks_per_cu=(4, 8, 16, 32), configs=None, cache=True, verbose=False):
         """Time every tile configuration x persistent grid size on this
         ensemble (graph of `steps` steps, device state restored afterwards);
         keep the fastest.  Results are cached per device and ensemble."""
@@ -347,6 +349,13 @@ class Ensemble:
         shape = (ctypes.c_int32 * 4)()
         _lib.check(_lib.lib().ppf_ensemble_config(config, shape), "ensemble_config")
         return tuple(shape)
+
+    @staticmethod
+    def kernel(config):
+        """The kernel a configuration runs: wave tiles in registers
+        (k_ensemble_w, output widths 62 / 60 = 64 lanes less the halo) or
+        block tiles through LDS (k_ensemble)."""
+        return "k_ensemble_w" if Ensemble.shape(config)[::2] == WAVE_TILE_WIDTHS else "k_ensemble"
 
     # ------------------------------------------- Simulation interoperation --
     @classmethod

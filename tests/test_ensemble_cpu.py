@@ -65,7 +65,7 @@ def test_layout_alignment_and_disjoint():
     assert total > end
 
 
-@pytest.mark.parametrize("shape", [(64, 8, 64, 8), (128, 16, 32, 16), (256, 4, 128, 4)])
+@pytest.mark.parametrize("shape", [(64, 8, 64, 8), (128, 16, 32, 16), (256, 4, 128, 4), (62, 32, 60, 24)])
 def test_tile_list_covers_every_cell_once(shape):
     specs = [InstanceSpec("AC", 70, 40), InstanceSpec("CH", 33, 65), InstanceSpec("AC", 1, 17),
              InstanceSpec("CH", 200, 3), InstanceSpec("AC", 257, 31)]
@@ -82,6 +82,23 @@ def test_tile_list_covers_every_cell_once(shape):
     # instance order, row-major inside an instance
     keys = [(i, y0, x0) for i, x0, y0, _ in tiles]
     assert keys == sorted(keys)
+
+
+def test_configurations_name_their_kernel():
+    """Block tiles (LDS) first, then wave tiles whose output widths are the
+    64 lanes less the halo (1 Allen-Cahn, 2 Cahn-Hilliard columns a side)."""
+    from pyphasefield_amd import _lib
+    from pyphasefield_amd.ensemble import WAVE_TILE_WIDTHS, Ensemble
+    n = _lib.lib().ppf_ensemble_num_configs()
+    kinds = [Ensemble.kernel(c) for c in range(n)]
+    assert kinds == sorted(kinds) and "k_ensemble" in kinds and "k_ensemble_w" in kinds
+    assert WAVE_TILE_WIDTHS == (64 - 2, 64 - 4)
+    for c in range(n):
+        shape = Ensemble.shape(c)
+        if kinds[c] == "k_ensemble_w":
+            assert shape[::2] == WAVE_TILE_WIDTHS and shape[1] > 0 and shape[3] > 0
+        else:
+            assert shape[0] % 32 == 0 and shape[2] % 32 == 0
 
 
 def test_segments_balance_cost():
