@@ -223,7 +223,7 @@ def bench_warren(args):
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     value = cells * args.steps / t
-    roof = _roofline(16.0 * cells, t / args.steps, "k_warren<float>")
+    roof = _roofline(16.0 * cells, t / args.steps, "k_warren_w<float>")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_warren(n)

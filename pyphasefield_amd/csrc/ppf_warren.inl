@@ -47,7 +47,7 @@ struct WarrenC {
   R y_e, M_A, M_B, dt, A, dtc, alpha;
   R dD, dW, dk, dM;  // D_L - D_S, W_B - W_A, kB - kA, M_B - M_A (FP32 forms)
   bool noise;
-  __device__ WarrenC(const WarrenK& p, R dx) {
+  __host__ __device__ WarrenC(const WarrenK& p, R dx) {
     D_L = R(p.D_L); D_S = R(p.D_S); v_m = R(p.v_m); Rg = R(p.R); vmR = v_m / Rg;
     W_A = R(p.W_A); W_B = R(p.W_B); kA = R(p.kA); kB = R(p.kB);
     dD = R(p.D_L - p.D_S); dW = R(p.W_B - p.W_A); dk = R(p.kB - p.kA); dM = R(p.M_B - p.M_A);
@@ -84,17 +84,15 @@ __device__ __forceinline__ void warren_point(const WarrenC<R>& K, R ph, R c, R& 
   }
 }
 
-// One cell of the step (Warren1995.py:69-86) from its 3x3 phi neighbourhood
-// (rows "m" = i-1, "c" = i, "p" = i+1 along array axis 0; columns 0/1/2 =
-// x-1/x/x+1) and the 5-point c, D_C, temp neighbourhoods.  Derivatives are
-// raw grid differences with the dx scalings folded into the constants:
-// phixx = Dxx/dx^2, phix = gx/(2dx), phixy = gxy/(4dx^2), and products of two
-// first differences carry 1/(4dx^2).
+// The cell update (Warren1995.py:69-86) from its stencil terms: raw grid
+// differences with the dx scalings folded into the constants, phixx =
+// Dxx/dx^2, phix = gx/(2dx), phixy = gxy/(4dx^2); products of two first
+// differences carry 1/(4dx^2).  clap is the 5-point sum of c less 4c, cross
+// the sum of the four products of first differences (D_C and c, temp and phi).
 template <typename R>
-__device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R pm2, R pc0, R ph, R pc2, R pp0,
-                                            R pp1, R pp2, R cm, R c0, R c, R c2v, R cp, R Dm, R D0, R D1, R D2,
-                                            R Dp, R tm, R t0, R t1, R t2, R tp, uint64_t seed, int64_t step,
-                                            uint64_t ncells, uint64_t cell, R& nph, R& nc) {
+__device__ __forceinline__ void warren_update(const WarrenC<R>& K, R ph, R c, R Dxx, R Dyy, R gx, R gy, R gxy,
+                                              R clap, R cross, R D1, R t1, uint64_t seed, int64_t step,
+                                              uint64_t ncells, uint64_t cell, R& nph, R& nc) {
   R g, gp, Hmix, M_phi;
   if constexpr (sizeof(R) == 8) {
     g = (ph * ph * (R(1) - ph) * (R(1) - ph));
@@ -109,16 +107,9 @@ __device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R
     Hmix = __builtin_fmaf(gp, __builtin_fmaf(c, K.dW, K.W_A), g * __builtin_fmaf(c, K.dk, K.kA));
     M_phi = __builtin_fmaf(c, K.dM, K.M_A);
   }
-  const R Dxx = pm1 + pp1 - R(2) * ph;
-  const R Dyy = pc0 + pc2 - R(2) * ph;
   const R lap = Dxx + Dyy;
-  const R gx = pm1 - pp1;
-  const R gy = pc0 - pc2;
-  const R gxy = (pm0 - pp0) - (pm2 - pp2);
   // change in c (Warren1995.py:69-71):
   // D_C lap c + grad D_C . grad c + temp lap phi + grad temp . grad phi
-  const R clap = cm + cp + c0 + c2v - R(4) * c;
-  const R cross = (Dm - Dp) * (cm - cp) + (D0 - D2) * (c0 - c2v) + (tm - tp) * gx + (t0 - t2) * gy;
   const R deltac = D1 * clap + t1 * lap + R(0.25) * cross;
   // change in phi (Warren1995.py:76-86); theta = atan2(phiy, phix) evaluated
   // algebraically from the gradient, with the arctan2(0, 0) = 0 case giving
@@ -152,6 +143,24 @@ __device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R
   }
   nph = ph + deltaphi * K.dt;
   nc = c + deltac * K.dtc;
+}
+
+// One cell from its 3x3 phi neighbourhood (rows "m" = i-1, "c" = i, "p" =
+// i+1 along array axis 0; columns 0/1/2 = x-1/x/x+1) and the 5-point c, D_C,
+// temp neighbourhoods.
+template <typename R>
+__device__ __forceinline__ void warren_cell(const WarrenC<R>& K, R pm0, R pm1, R pm2, R pc0, R ph, R pc2, R pp0,
+                                            R pp1, R pp2, R cm, R c0, R c, R c2v, R cp, R Dm, R D0, R D1, R D2,
+                                            R Dp, R tm, R t0, R t1, R t2, R tp, uint64_t seed, int64_t step,
+                                            uint64_t ncells, uint64_t cell, R& nph, R& nc) {
+  const R Dxx = pm1 + pp1 - R(2) * ph;
+  const R Dyy = pc0 + pc2 - R(2) * ph;
+  const R gx = pm1 - pp1;
+  const R gy = pc0 - pc2;
+  const R gxy = (pm0 - pp0) - (pm2 - pp2);
+  const R clap = cm + cp + c0 + c2v - R(4) * c;
+  const R cross = (Dm - Dp) * (cm - cp) + (D0 - D2) * (c0 - c2v) + (tm - tp) * gx + (t0 - t2) * gy;
+  warren_update(K, ph, c, Dxx, Dyy, gx, gy, gxy, clap, cross, D1, t1, seed, step, ncells, cell, nph, nc);
 }
 
 // ---- pieces of the tile step
@@ -316,6 +325,162 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
   warren_tile_step<R>(in, out, b, d, dx, p, seed, sr);
 }
 
+// ------------------------------------------------------------ wave tiles --
+// Each wave of a 256-thread block owns a tile of its own: lane l holds column
+// x0 - 1 + l of phi and c on the rows y0 - 1 .. y0 + kWvTY in registers (all
+// its loads issued together), so the 64 lanes give 62 output columns.  The
+// wave walks its rows computing D_C and temp one row ahead; x neighbours come
+// from the adjacent lanes (DPP lane shifts), y neighbours from the lane's own
+// rows.  No LDS and no barrier.  Halo rows and columns are a coordinate remap
+// (periodic wrap, nearest interior cell for zero gradient) when there are no
+// bc values and no Dirichlet side; otherwise each value is folded.  Stencil
+// terms: the x differences of a row are differences of shifted copies, and
+// gxy = (pm0 - pp0) - (pm2 - pp2) is the difference of the x neighbours' gx,
+// with the same operands and operations as warren_cell.
+constexpr int kWvTX = 62;
+#ifndef PPF_WARREN_WTY
+#define PPF_WARREN_WTY 4
+#endif
+constexpr int kWvTY = PPF_WARREN_WTY;
+// rows fetched ahead of the row being updated (kWvTY or more: the whole tile
+// is fetched before the first update)
+#ifndef PPF_WARREN_AHEAD
+#define PPF_WARREN_AHEAD 64
+#endif
+constexpr int kWvAhead = PPF_WARREN_AHEAD;
+
+// f(0), f(1), ... (compile-time row indices)
+template <typename F, int... P>
+__device__ __forceinline__ void warren_unrolled(F& f, std::integer_sequence<int, P...>) {
+  (f(std::integral_constant<int, P>{}), ...);
+}
+
+template <typename R>
+__device__ __forceinline__ void warren_wave_cell(const WarrenC<R>& K, R pm, R pc, R pp, R cm, R cc, R cp, R Dm,
+                                                 R D1, R Dp, R tm, R t1, R tp, uint64_t seed, int64_t step,
+                                                 uint64_t ncells, uint64_t cell, R& nph, R& nc) {
+  const R pc0 = lane_shift<kDppWaveShr1>(pc), pc2 = lane_shift<kDppWaveShl1>(pc);
+  const R Dxx = pm + pp - R(2) * pc;
+  const R Dyy = pc0 + pc2 - R(2) * pc;
+  const R gx = pm - pp;
+  const R gy = pc0 - pc2;
+  const R gxy = lane_shift<kDppWaveShr1>(gx) - lane_shift<kDppWaveShl1>(gx);
+  const R c0 = lane_shift<kDppWaveShr1>(cc), c2v = lane_shift<kDppWaveShl1>(cc);
+  const R clap = cm + cp + c0 + c2v - R(4) * cc;
+  const R D0 = lane_shift<kDppWaveShr1>(D1), D2 = lane_shift<kDppWaveShl1>(D1);
+  const R t0 = lane_shift<kDppWaveShr1>(t1), t2 = lane_shift<kDppWaveShl1>(t1);
+  const R cross = (Dm - Dp) * (cm - cp) + (D0 - D2) * (c0 - c2v) + (tm - tp) * gx + (t0 - t2) * gy;
+  warren_update(K, pc, cc, Dxx, Dyy, gx, gy, gxy, clap, cross, D1, t1, seed, step, ncells, cell, nph, nc);
+}
+
+// occupancy hint of the wave-tile kernel (waves per SIMD; 0 = compiler's choice)
+#ifndef PPF_WARREN_W_WAVES
+#define PPF_WARREN_W_WAVES 0
+#endif
+#if PPF_WARREN_W_WAVES
+#define PPF_WARREN_W_ATTR __attribute__((amdgpu_waves_per_eu(PPF_WARREN_W_WAVES, 8)))
+#else
+#define PPF_WARREN_W_ATTR
+#endif
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) PPF_WARREN_W_ATTR void k_warren_w(const R* __restrict__ in, R* __restrict__ out,
+                                                       const R* __restrict__ b, Dom d, R dx, WarrenC<R> K,
+                                                       uint64_t seed, StepRef sr, uint32_t nstrips,
+                                                       uint32_t ntiles) {
+  constexpr int kWaves = kThreads / 64;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
+  const int lane = int(threadIdx.x) % 64;
+  const uint32_t t = xcd_swizzle(blockIdx.x, gridDim.x) * uint32_t(kWaves) + uint32_t(wave);
+  if (t >= ntiles) return;  // uniform per wave
+  const int32_t x0 = 1 + int32_t(t % nstrips) * kWvTX, y0 = 1 + int32_t(t / nstrips) * kWvTY;
+  const int32_t nx = int32_t(d.ax.n), ny = int32_t(d.ay.n);
+  const int32_t xr = x0 - 1 + lane;
+  const int32_t sy = int32_t(d.ay.stride * int64_t(sizeof(R))), sfb = int32_t(d.sf * int64_t(sizeof(R)));
+  const int64_t step = sr.get();
+  const uint64_t ncells = uint64_t(nx + 2) * uint64_t(ny + 2);
+  const bool out_lane = lane >= 1 && lane < 63 && xr <= nx;
+  const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in), ws = plane_rsrc(out);
+  const uint32_t oo = uint32_t(int64_t(y0) * d.ay.stride + xr) * uint32_t(sizeof(R));
+  // the tile's march; REMAP: halo rows and columns are a coordinate remap and
+  // rows are fetched kWvAhead ahead of the update (unconditional loads: a
+  // conditional one makes the compiler wait for it at once), else every value
+  // is folded before the first update
+  auto march = [&](auto remap_c) {
+    constexpr bool REMAP = decltype(remap_c)::value;
+    R ph[kWvTY + 2], cc[kWvTY + 2];  // rows y0 - 1 .. y0 + kWvTY
+    const uint32_t xo = uint32_t(remap_coord32(xr, nx, d.ax.bc)) * uint32_t(sizeof(R));
+    auto fetch = [&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if constexpr (REMAP && k < kWvTY + 2) {
+        const int32_t yo = remap_coord32(y0 - 1 + k, ny, d.ay.bc) * sy;
+        ph[k] = buf_load<R>(rs, xo, yo);
+        cc[k] = buf_load<R>(rs, xo, yo + sfb);
+      }
+    };
+    if constexpr (REMAP) {
+      warren_unrolled(fetch,
+                      std::make_integer_sequence<int, (kWvAhead + 3 < kWvTY + 2 ? kWvAhead + 3 : kWvTY + 2)>{});
+    } else {
+#pragma unroll
+      for (int k = 0; k < kWvTY + 2; ++k) {
+        const int64_t y = y0 - 1 + k;
+        ph[k] = R(1);
+        cc[k] = R(0);
+        if (xr <= nx + 1 && y <= ny + 1) {  // beyond: never used
+          ph[k] = fold(in, b, d, int64_t(xr), y, int64_t(0), dx);
+          cc[k] = fold(in + d.sf, b ? b + d.sf : nullptr, d, int64_t(xr), y, int64_t(0), dx);
+        }
+      }
+    }
+    R Dm, tm, D1, t1;
+    warren_point(K, ph[0], cc[0], Dm, tm);
+    warren_point(K, ph[1], cc[1], D1, t1);
+    auto row = [&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      // rows up to r + 2 are in flight; start row r + kWvAhead + 3
+      if constexpr (REMAP && kWvAhead + 3 < kWvTY + 2) {
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(std::integral_constant<int, r + kWvAhead + 3>{});
+      }
+      R Dp, tp;
+      warren_point(K, ph[r + 2], cc[r + 2], Dp, tp);
+      const int32_t y = y0 + r;
+      R nph, nc;
+      warren_wave_cell(K, ph[r], ph[r + 1], ph[r + 2], cc[r], cc[r + 1], cc[r + 2], Dm, D1, Dp, tm, t1, tp,
+                       seed, step, ncells, uint64_t(y) * uint64_t(nx + 2) + uint64_t(xr), nph, nc);
+      if (out_lane && y <= ny) {
+        buf_store<R>(nph, ws, oo, r * sy);
+        buf_store<R>(nc, ws, oo, r * sy + sfb);
+      }
+      Dm = D1;
+      tm = t1;
+      D1 = Dp;
+      t1 = tp;
+    };
+    warren_unrolled(row, std::make_integer_sequence<int, kWvTY>{});
+  };
+  if (!b && d.ax.bc != PPF_DIRICHLET && d.ay.bc != PPF_DIRICHLET)
+    march(std::true_type{});
+  else
+    march(std::false_type{});
+}
+
+// whether the wave-tile kernel runs the grid: FP32 (FP64 keeps the LDS tiles,
+// which are faster there), 2D, enough wave tiles to fill the GPU (smaller
+// grids keep the 256-cell LDS tiles, four times as many work units), and both
+// fields' bytes within the 32-bit buffer offsets.  PPF_WARREN_WAVE_TILES=0
+// keeps the LDS tiles everywhere.
+#ifndef PPF_WARREN_WAVE_TILES
+#define PPF_WARREN_WAVE_TILES 1
+#endif
+constexpr int64_t kWvMinTiles = 1024;
+inline bool warren_wave_fits(const ppf_grid& g, size_t elem) {
+  if (!PPF_WARREN_WAVE_TILES || elem != sizeof(float) || g.ndim != 2) return false;
+  const int64_t tiles = ((g.n[0] + kWvTX - 1) / kWvTX) * ((g.n[1] + kWvTY - 1) / kWvTY);
+  return tiles >= kWvMinTiles && (g.sf + (g.n[1] + 2) * g.sy) * int64_t(elem) < (int64_t(1) << 31);
+}
+
 // FP32 occupancy hint (waves per SIMD the allocation must allow; 0 =
 // compiler's choice).  Left to itself the compiler takes 102 SGPRs, which
 // caps the kernel at 7 waves per SIMD (7 blocks per CU: a 2048^2 grid's 4096
@@ -362,6 +527,16 @@ int warren_step(const ppf_grid& g, const void* in, void* out, const void* bcvals
   k.M_A = p.M_A;
   k.M_B = p.M_B;
   k.dt = p.dt;
+  if (sizeof(R) == 4 && warren_wave_fits(g, sizeof(R))) {
+    const WarrenC<R> K(k, R(g.dx));
+    const uint32_t nstrips = uint32_t((g.n[0] + kWvTX - 1) / kWvTX);
+    const uint32_t ntiles = nstrips * uint32_t((g.n[1] + kWvTY - 1) / kWvTY);
+    const unsigned blocks = (ntiles + kThreads / 64 - 1) / (kThreads / 64);
+    hipLaunchKernelGGL(k_warren_w<R>, dim3(blocks), dim3(kThreads), 0, s, static_cast<const R*>(in),
+                       static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), K, seed, step, nstrips,
+                       ntiles);
+    return check_launch("warren_step");
+  }
   const unsigned tiles = unsigned(((g.n[0] + kTX - 1) / kTX) * ((g.n[1] + kWTY - 1) / kWTY));
   hipLaunchKernelGGL(k_warren<R>, dim3(tiles), dim3(kThreads), 0, s, static_cast<const R*>(in),
                      static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), k, seed, step);
