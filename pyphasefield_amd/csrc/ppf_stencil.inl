@@ -815,14 +815,16 @@ __device__ __forceinline__ R lane_shift(R v) {
   }
 }
 
-// 3D Cahn-Hilliard on an all-periodic box, one wave per block, marching in z
+// 3D Cahn-Hilliard on a box whose halos are a coordinate remap (every side
+// periodic, or zero-gradient Neumann without bc values), one wave per block, marching in z
 // with the cells in registers.  Lane l owns column x = x0 - 2 + l of a 64-wide
 // window (60 outputs and a 2-cell halo on each side) and the RPW rows
 // y0 - 2 .. y0 + RPW - 3 (RPW - 4 output rows).  x neighbours come from the
 // adjacent lanes (lane shifts), y neighbours from the lane's own rows, z
 // neighbours from the register sets of the neighbouring planes, so the march
-// reads no LDS and needs no barrier; halo coordinates are wrapped once into
-// the row offsets and the lane's column, and planes are read and written
+// reads no LDS and needs no barrier; halo coordinates are remapped once into
+// the row offsets and the lane's column (the zero-flux mu ring of Neumann
+// sides: lane shifts, row and plane copies), and planes are read and written
 // with buffer instructions (plane base in the descriptor, row offset scalar,
 // lane offset in one VGPR).  Iteration k: mu(k) on rows 1..RPW-2 from
 // c(k-1..k+1); output plane k-1 on rows 2..RPW-3 from mu(k-2..k) and c(k-1);
@@ -862,10 +864,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), int16_t(0), int32_t(-1), 0x00020000);
 }
 
+// Interior coordinate a halo coordinate reads: periodic wrap, or the nearest
+// interior cell (zero gradient without bc values)
+__device__ __forceinline__ int64_t remap_halo(int64_t p, const Axis& a) {
+  if (a.bc == PPF_PERIODIC) return wrap_periodic(p, a);
+  return p < 1 ? 1 : (p > a.n ? a.n : p);
+}
+
 template <typename R, int RPW>
 __device__ __forceinline__ void ch3w_fetch(R (&v)[RPW], const R* __restrict__ in, const Dom& d,
                                            const int32_t (&yoff)[RPW], uint32_t xoff, int64_t z) {
-  const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in + wrap_periodic(z, d.az) * d.az.stride);
+  const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in + remap_halo(z, d.az) * d.az.stride);
 #pragma unroll
   for (int r = 0; r < RPW; ++r) v[r] = buf_load<R>(rs, xoff, yoff[r]);
 }
@@ -893,11 +902,17 @@ __global__ __launch_bounds__(64) void k_ch3w(const R* __restrict__ in, R* __rest
   const R k_out = dtM * idx2;   // FP32 form: out = c + k_out * (sum of mu neighbours - 6 mu)
   const int lane = threadIdx.x;
   const int64_t xr = x0 - 2 + lane;
-  const uint32_t xoff = uint32_t(wrap_periodic(xr, d.ax)) * uint32_t(sizeof(R));
+  const uint32_t xoff = uint32_t(remap_halo(xr, d.ax)) * uint32_t(sizeof(R));
   const uint32_t ooff = uint32_t(xr) * uint32_t(sizeof(R));  // used on output lanes only (xr >= 1 there)
-  int32_t yoff[RPW];  // byte offset of row r in its plane (wrapped)
+  int32_t yoff[RPW];  // byte offset of row r in its plane (remapped)
 #pragma unroll
-  for (int r = 0; r < RPW; ++r) yoff[r] = int32_t(wrap_periodic(y0 - 2 + r, d.ay) * d.ay.stride * int64_t(sizeof(R)));
+  for (int r = 0; r < RPW; ++r) yoff[r] = int32_t(remap_halo(y0 - 2 + r, d.ay) * d.ay.stride * int64_t(sizeof(R)));
+  // zero-flux sides: mu ring cells outside the domain take their interior
+  // neighbour's mu (lanes xr = 0 / n + 1, rows y = 0 / n + 1, planes 0 / n + 1)
+  const bool zfx = d.ax.bc != PPF_PERIODIC, zfy = d.ay.bc != PPF_PERIODIC, zfz = d.az.bc != PPF_PERIODIC;
+  const bool xlo = xr < 1, xhi = xr > d.ax.n;
+  const int32_t yrow0 = int32_t(y0 - 2);  // y of row 0
+  const int32_t nyp1 = int32_t(d.ay.n + 1), nzp1 = int32_t(d.az.n + 1);
   const bool out_lane = lane >= 2 && lane < 2 + kCh3wTx && xr <= d.ax.n;
   const int32_t zb32 = int32_t(zb), ze32 = int32_t(ze);
   const int32_t ny_out = int32_t(d.ay.n - (y0 - 2));  // rows r <= ny_out are inside the domain
@@ -931,6 +946,30 @@ __global__ __launch_bounds__(64) void k_ch3w(const R* __restrict__ in, R* __rest
         const R sum = ((cp[r] + cm[r]) + (c0[r + 1] + c0[r - 1])) + (cxp + cxm);
         const R well = c * __builtin_fmaf(c, __builtin_fmaf(c, R(4), R(-6)), R(2));
         mp[r] = __builtin_fmaf(k_mu, __builtin_fmaf(R(-6), c, sum), well);
+      }
+    }
+    if (zfx) {
+#pragma unroll
+      for (int r = 1; r < RPW - 1; ++r) {
+        const R from_right = lane_shift<kDppWaveShl1>(mp[r]), from_left = lane_shift<kDppWaveShr1>(mp[r]);
+        mp[r] = xlo ? from_right : (xhi ? from_left : mp[r]);
+      }
+    }
+    if (zfy) {
+#pragma unroll
+      for (int r = 1; r < RPW - 1; ++r) {
+        if (yrow0 + r == 0) mp[r] = mp[r + 1];
+        if (yrow0 + r == nyp1) mp[r] = mp[r - 1];
+      }
+    }
+    if (zfz) {
+      if (k == 1) {  // mu(0) := mu(1)
+#pragma unroll
+        for (int r = 1; r < RPW - 1; ++r) m0[r] = mp[r];
+      }
+      if (k == nzp1) {  // mu(n + 1) := mu(n)
+#pragma unroll
+        for (int r = 1; r < RPW - 1; ++r) mp[r] = m0[r];
       }
     }
     // output plane k - 1 from mu(k-2) = mm, mu(k-1) = m0, mu(k) = mp, c(k-1) = cm
@@ -1020,6 +1059,17 @@ inline int32_t ch3d_zchunk(const ppf_grid& g, int c3y) {
   return int32_t(zc);
 }
 
+// the register march reads halos as a coordinate remap: every side periodic,
+// or zero-gradient Neumann without bc values (no Dirichlet side)
+inline bool ch3w_remappable(const ppf_grid& g, const void* bcvals) {
+  bool periodic = true, dirichlet = false;
+  for (int a = 0; a < 3; ++a) {
+    periodic = periodic && g.bc[a] == PPF_PERIODIC;
+    dirichlet = dirichlet || g.bc[a] == PPF_DIRICHLET;
+  }
+  return periodic || (!bcvals && !dirichlet);
+}
+
 template <typename R>
 int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double eps,
                       double dt, hipStream_t s) {
@@ -1029,8 +1079,7 @@ int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* 
     hipLaunchKernelGGL(k_ch2d<R>, dim3(tiles2d(g)), dim3(kThreads), 0, s, static_cast<const R*>(in),
                        static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), R(eps), dtM);
   } else if (g.ndim == 3 && g.sz * int64_t(sizeof(R)) < (int64_t(1) << 31) && PPF_CH3D_WAVE &&
-             g.bc[0] == PPF_PERIODIC && g.bc[1] == PPF_PERIODIC &&
-             g.bc[2] == PPF_PERIODIC) {  // in-plane byte offsets (row offsets, buffer offsets) are 32-bit
+             ch3w_remappable(g, bcvals)) {  // in-plane byte offsets (row offsets, buffer offsets) are 32-bit
     constexpr int RPW = sizeof(R) == 8 ? 8 : PPF_CH3W_RPW;
     const int64_t xy = ((g.n[0] + kCh3wTx - 1) / kCh3wTx) * ((g.n[1] + RPW - 5) / (RPW - 4));
     const int32_t zc = ch3w_zchunk<R, RPW>(g, xy);

@@ -220,21 +220,24 @@ def test_cahnhilliard_3d_marching_kernel_fp64(gpu, bc):
         assert abs(sim.field_sums()[0] - c0.sum()) <= 1e-10 * c0.sum()
 
 
-def test_cahnhilliard_3d_fp32_periodic_wave_march(gpu):
-    """All-periodic 3D FP32 (the register-marching kernel): partial x and y
-    windows, three z chunks (the last short), against the FP64 oracle; mass is
+@pytest.mark.parametrize("bc", ["PERIODIC", "NEUMANN", ["NEUMANN", "PERIODIC", "NEUMANN"],
+                                ["PERIODIC", "NEUMANN", "PERIODIC"]])
+def test_cahnhilliard_3d_fp32_wave_march(gpu, bc):
+    """3D FP32 boxes whose halos are a coordinate remap (the register-marching
+    kernel; zero-flux mu ring on Neumann sides): partial x and y windows,
+    three z chunks (the last short), against the FP64 oracle; mass is
     conserved to FP32 rounding."""
     E = _engines()
     shape = [70, 37, 130]
     sim = E.CahnHilliard(dimensions=shape, seed=4)
     sim.set_user_data({"M": 1.0, "epsilon": 1.0})
-    _setup(sim, "PERIODIC", 1.0, 0.01, "FP32")
+    _setup(sim, bc, 1.0, 0.01, "FP32")
     rng = np.random.default_rng(12)
     c0 = 0.5 + 0.05 * (2 * rng.random(shape) - 1)
     sim.fields[0].data[1:-1, 1:-1, 1:-1] = c0
     m0 = sim.field_sums()[0]
     sim.simulate(20)
-    want = run_point("cahnhilliard", c0, 20, "PERIODIC", 1.0, 0.01, M=1.0, epsilon=1.0)
+    want = run_point("cahnhilliard", c0, 20, bc, 1.0, 0.01, M=1.0, epsilon=1.0)
     assert rel_l2(sim.fields[0].get_cells(), want[1:-1, 1:-1, 1:-1]) <= 1e-5
     assert abs(sim.field_sums()[0] - m0) <= 1e-6 * m0
 
