@@ -733,6 +733,10 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
   const ppf_ncomp_params& p = k.p;
   const R idx = R(1) / dx;
   const R v_m = R(p.v_m), inv_v_m = R(1.0 / p.v_m), M_qmax = R(p.M_qmax), Hh = R(p.H), y_e = R(p.y_e), beta = R(p.beta);
+  // FP32 scale factors: 0.5/dx, 1/dx^2, 0.25/dx^2, H/dx, and the clamp of the
+  // face |grad q| in grid units (beta dx); FP64 clamps |grad q| itself
+  const R hidx = R(0.5) * idx, idx2 = idx * idx, qidx2 = R(0.25) * idx2, Hidx = Hh * idx;
+  const R beta_g = sizeof(R) == 8 ? beta : R(p.beta * double(dx));
   const R dt = R(p.dt), ebar = R(p.ebar), eqbar = R(p.eqbar);
   R ebar2 = ebar * ebar, eqbar2 = eqbar * eqbar;
   if (p.flags & PPF_NC_EBAR_FROM_S) {
@@ -760,24 +764,40 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
     // interpolating functions (NCGPU_new.py:84-87)
     R pc3 = pc * pc * pc;
     R g = (pc * pc) * ((R(1) - pc) * (R(1) - pc));
-    R h = pc3 * (R(6) * (pc * pc) - R(15) * pc + R(10));
+    R h, gprime;
+    if constexpr (sizeof(R) == 8) {
+      h = pc3 * (R(6) * (pc * pc) - R(15) * pc + R(10));
+      gprime = R(4) * pc3 - R(6) * (pc * pc) + R(2) * pc;
+    } else {  // Horner forms
+      h = pc3 * (pc * (R(6) * pc - R(15)) + R(10));
+      gprime = pc * (pc * (R(4) * pc - R(6)) + R(2));
+    }
     R hprime = R(30) * g;
-    R gprime = R(4) * pc3 - R(6) * (pc * pc) + R(2) * pc;
-    // gradients (:89-107)
-    R dphidx = R(0.5) * (phi[ly][lx + 1] - phi[ly][lx - 1]) * idx;
+    // gradients (:89-107); FP32 takes the uniform scale factors as one
+    // product each (0.5/dx, 1/dx^2, 0.25/dx^2), FP64 the reference's order
+    R dphidx, dphidy, d2phidx2, d2phidy2, d2phidxy;
+    if constexpr (sizeof(R) == 8) {
+      dphidx = R(0.5) * (phi[ly][lx + 1] - phi[ly][lx - 1]) * idx;
+      dphidy = R(0.5) * (phi[ly + 1][lx] - phi[ly - 1][lx]) * idx;
+      d2phidx2 = (phi[ly][lx + 1] + phi[ly][lx - 1] - R(2) * pc) * idx * idx;
+      d2phidy2 = (phi[ly + 1][lx] + phi[ly - 1][lx] - R(2) * pc) * idx * idx;
+      d2phidxy = R(0.25) * (phi[ly + 1][lx + 1] - phi[ly + 1][lx - 1] - phi[ly - 1][lx + 1] + phi[ly - 1][lx - 1]) *
+                 idx * idx;
+    } else {
+      dphidx = hidx * (phi[ly][lx + 1] - phi[ly][lx - 1]);
+      dphidy = hidx * (phi[ly + 1][lx] - phi[ly - 1][lx]);
+      d2phidx2 = (phi[ly][lx + 1] + phi[ly][lx - 1] - R(2) * pc) * idx2;
+      d2phidy2 = (phi[ly + 1][lx] + phi[ly - 1][lx] - R(2) * pc) * idx2;
+      d2phidxy = qidx2 * (phi[ly + 1][lx + 1] - phi[ly + 1][lx - 1] - phi[ly - 1][lx + 1] + phi[ly - 1][lx - 1]);
+    }
     R dphidx2 = dphidx * dphidx;
     R dphidx3 = dphidx2 * dphidx;
-    R dphidy = R(0.5) * (phi[ly + 1][lx] - phi[ly - 1][lx]) * idx;
     R dphidy2 = dphidy * dphidy;
     R dphidy3 = dphidy2 * dphidy;
     R Tc = sT[ly][lx];
     R dTdx = R(0.5) * idx * (sT[ly][lx + 1] - sT[ly][lx - 1]);
     R dTdy = R(0.5) * idx * (sT[ly + 1][lx] - sT[ly - 1][lx]);
-    R d2phidx2 = (phi[ly][lx + 1] + phi[ly][lx - 1] - R(2) * pc) * idx * idx;
-    R d2phidy2 = (phi[ly + 1][lx] + phi[ly - 1][lx] - R(2) * pc) * idx * idx;
     R lphi = d2phidx2 + d2phidy2;
-    R d2phidxy = R(0.25) * (phi[ly + 1][lx + 1] - phi[ly + 1][lx - 1] - phi[ly - 1][lx + 1] + phi[ly - 1][lx - 1]) *
-                 idx * idx;
     R mag_grad_phi2 = dphidx * dphidx + dphidy * dphidy;
     if (mag_grad_phi2 < R(1e-6)) mag_grad_phi2 = R(1e-6);
     R mag_grad_phi4 = mag_grad_phi2 * mag_grad_phi2;
@@ -788,15 +808,26 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
     R dq1dy = R(0.5) * idx * (Q1.yp - Q1.ym);
     R dq4dx = R(0.5) * idx * (Q4.xp - Q4.xm);
     R dq4dy = R(0.5) * idx * (Q4.yp - Q4.ym);
-    R mgq_xp = idx * fsqrt((Q1.xp - Q1.c) * (Q1.xp - Q1.c) + (Q4.xp - Q4.c) * (Q4.xp - Q4.c));
-    R mgq_xm = idx * fsqrt((Q1.xm - Q1.c) * (Q1.xm - Q1.c) + (Q4.xm - Q4.c) * (Q4.xm - Q4.c));
-    R mgq_yp = idx * fsqrt((Q1.yp - Q1.c) * (Q1.yp - Q1.c) + (Q4.yp - Q4.c) * (Q4.yp - Q4.c));
-    R mgq_ym = idx * fsqrt((Q1.ym - Q1.c) * (Q1.ym - Q1.c) + (Q4.ym - Q4.c) * (Q4.ym - Q4.c));
-    R mag_grad_q = R(0.5) * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
-    if (mgq_xp < beta) mgq_xp = beta;
-    if (mgq_xm < beta) mgq_xm = beta;
-    if (mgq_yp < beta) mgq_yp = beta;
-    if (mgq_ym < beta) mgq_ym = beta;
+    // face |grad q|; FP32 keeps them in grid units (r = dx |grad q|): the
+    // 1/dx goes into mag_grad_q's factor and the face weights' common factor
+    R mgq_xp = fsqrt((Q1.xp - Q1.c) * (Q1.xp - Q1.c) + (Q4.xp - Q4.c) * (Q4.xp - Q4.c));
+    R mgq_xm = fsqrt((Q1.xm - Q1.c) * (Q1.xm - Q1.c) + (Q4.xm - Q4.c) * (Q4.xm - Q4.c));
+    R mgq_yp = fsqrt((Q1.yp - Q1.c) * (Q1.yp - Q1.c) + (Q4.yp - Q4.c) * (Q4.yp - Q4.c));
+    R mgq_ym = fsqrt((Q1.ym - Q1.c) * (Q1.ym - Q1.c) + (Q4.ym - Q4.c) * (Q4.ym - Q4.c));
+    R mag_grad_q;
+    if constexpr (sizeof(R) == 8) {
+      mgq_xp = idx * mgq_xp;
+      mgq_xm = idx * mgq_xm;
+      mgq_yp = idx * mgq_yp;
+      mgq_ym = idx * mgq_ym;
+      mag_grad_q = R(0.5) * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
+    } else {
+      mag_grad_q = hidx * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
+    }
+    if (mgq_xp < beta_g) mgq_xp = beta_g;
+    if (mgq_xm < beta_g) mgq_xm = beta_g;
+    if (mgq_yp < beta_g) mgq_yp = beta_g;
+    if (mgq_ym < beta_g) mgq_ym = beta_g;
     // psi terms (:127-146)
     R q1c = Q1.c, q4c = Q4.c;
     R q2q2 = (q1c * q1c - q4c * q4c);
@@ -860,27 +891,23 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
       // the FP32 range.
       const R im = __builtin_amdgcn_rsqf(mag_grad_phi2);
       const R m = mag_grad_phi2 * im;
+      // Common factors taken out: with s = X^2 + Y^2, u_x = X phi_xx + Y phi_xy
+      // and u_y = X phi_xy + Y phi_yy, the last term of each component,
+      // 4 X^3 phi_xx + 4 X phi_xx Y^2 + 4 Y phi_xy X^2 + 4 Y^3 phi_xy, is
+      // 4 s u_x (and 4 s u_y), and the third is 2 A u.
       R X = dphidx * im, Yg = dphidy * im, Px = psix * im, Py = psiy * im;
       R Px2 = Px * Px, Py2 = Py * Py, Px3 = Px2 * Px, Py3 = Py2 * Py;
       R P4 = Px2 * Px2 + Py2 * Py2;
-      R Ax = R(2) * Px3 * q2q2 + R(2) * Py3 * qq2;
-      R Ay = R(-2) * Px3 * qq2 + R(2) * Py3 * q2q2;
+      R Ax = R(2) * (Px3 * q2q2 + Py3 * qq2);
+      R Ay = R(2) * (Py3 * q2q2 - Px3 * qq2);
+      const R s4 = R(4) * P4 * (X * X + Yg * Yg);
+      const R ux = X * d2phidx2 + Yg * d2phidxy, uy = X * d2phidxy + Yg * d2phidy2;
       R tx = dTdx * m * (Ax - X * P4);
-      tx += Tc * (R(6) * Px2 * dpsixdx * q2q2 + R(2) * Px3 * m * dq2q2dx + R(6) * Py2 * dpsiydx * qq2 +
-                  R(2) * Py3 * m * dqq2dx);
-      tx -= Tc * Ax * (R(2) * X * d2phidx2 + R(2) * Yg * d2phidxy);
-      tx -= Tc * (d2phidx2 * P4 + X * (R(4) * Px3 * dpsixdx + R(4) * Py3 * dpsiydx));
-      tx += Tc * (X * P4 *
-                  (R(4) * X * X * X * d2phidx2 + R(4) * X * d2phidx2 * Yg * Yg + R(4) * Yg * d2phidxy * X * X +
-                   R(4) * Yg * Yg * Yg * d2phidxy));
+      tx += Tc * (R(6) * (Px2 * dpsixdx * q2q2 + Py2 * dpsiydx * qq2) + R(2) * m * (Px3 * dq2q2dx + Py3 * dqq2dx) -
+                  R(2) * Ax * ux - d2phidx2 * P4 - R(4) * X * (Px3 * dpsixdx + Py3 * dpsiydx) + X * s4 * ux);
       R ty = dTdy * m * (Ay - Yg * P4);
-      ty += Tc * (R(-6) * Px2 * dpsixdy * qq2 - R(2) * Px3 * m * dqq2dy + R(6) * Py2 * dpsiydy * q2q2 +
-                  R(2) * Py3 * m * dq2q2dy);
-      ty -= Tc * Ay * (R(2) * X * d2phidxy + R(2) * Yg * d2phidy2);
-      ty -= Tc * (d2phidy2 * P4 + Yg * (R(4) * Px3 * dpsixdy + R(4) * Py3 * dpsiydy));
-      ty += Tc * (Yg * P4 *
-                  (R(4) * X * X * X * d2phidxy + R(4) * X * d2phidxy * Yg * Yg + R(4) * Yg * d2phidy2 * X * X +
-                   R(4) * Yg * Yg * Yg * d2phidy2));
+      ty += Tc * (R(6) * (Py2 * dpsiydy * q2q2 - Px2 * dpsixdy * qq2) + R(2) * m * (Py3 * dq2q2dy - Px3 * dqq2dy) -
+                  R(2) * Ay * uy - d2phidy2 * P4 - R(4) * Yg * (Px3 * dpsixdy + Py3 * dpsiydy) + Yg * s4 * uy);
       d_term_dx = tx * (R(4) * y_e * ebar2);
       d_term_dy = ty * (R(4) * y_e * ebar2);
     }
@@ -933,27 +960,38 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
       orow[x + (3 + l) * d.sf] = acc * dt + sf[3 + l][ly][lx];
     }
     // dq/dt (:205-232)
-    R D_q = R(2) * Hh * Tc * (pc * pc);
-    R D_q_xp = R(2) * Hh * sT[ly][lx + 1] * (phi[ly][lx + 1] * phi[ly][lx + 1]);
-    R D_q_xm = R(2) * Hh * sT[ly][lx - 1] * (phi[ly][lx - 1] * phi[ly][lx - 1]);
-    R D_q_yp = R(2) * Hh * sT[ly + 1][lx] * (phi[ly + 1][lx] * phi[ly + 1][lx]);
-    R D_q_ym = R(2) * Hh * sT[ly - 1][lx] * (phi[ly - 1][lx] * phi[ly - 1][lx]);
     R f_ori_1, f_ori_4;
     if constexpr (sizeof(R) == 8) {
+      R D_q = R(2) * Hh * Tc * (pc * pc);
+      R D_q_xp = R(2) * Hh * sT[ly][lx + 1] * (phi[ly][lx + 1] * phi[ly][lx + 1]);
+      R D_q_xm = R(2) * Hh * sT[ly][lx - 1] * (phi[ly][lx - 1] * phi[ly][lx - 1]);
+      R D_q_yp = R(2) * Hh * sT[ly + 1][lx] * (phi[ly + 1][lx] * phi[ly + 1][lx]);
+      R D_q_ym = R(2) * Hh * sT[ly - 1][lx] * (phi[ly - 1][lx] * phi[ly - 1][lx]);
       f_ori_1 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q1.c, Q1.xp, Q1.xm,
                            Q1.yp, Q1.ym, idx);
       f_ori_4 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q4.c, Q4.xp, Q4.xm,
                            Q4.yp, Q4.ym, idx);
     } else {
-      // FP32: the four face weights (D_q + D_q,nb) / |grad q|_face once, shared by q1 and q4
-      const R wxp = (D_q + D_q_xp) * __builtin_amdgcn_rcpf(mgq_xp), wxm = (D_q + D_q_xm) * __builtin_amdgcn_rcpf(mgq_xm);
-      const R wyp = (D_q + D_q_yp) * __builtin_amdgcn_rcpf(mgq_yp), wym = (D_q + D_q_ym) * __builtin_amdgcn_rcpf(mgq_ym);
-      const R hi2 = R(0.5) * idx * idx;
-      f_ori_1 = hi2 * (wxp * (Q1.xp - Q1.c) - wxm * (Q1.c - Q1.xm) + wyp * (Q1.yp - Q1.c) - wym * (Q1.c - Q1.ym));
-      f_ori_4 = hi2 * (wxp * (Q4.xp - Q4.c) - wxm * (Q4.c - Q4.xm) + wyp * (Q4.yp - Q4.c) - wym * (Q4.c - Q4.ym));
+      // FP32: D_q = 2 H T phi^2 and |grad q|_face = r / dx, so the face weight
+      // (D_q + D_q,nb) / |grad q|_face is 2 H dx (e + e_nb) / r with e = T phi^2;
+      // 2 H dx goes into the common factor 0.5 / dx^2 (giving H / dx), and
+      // the four weights are shared by q1 and q4
+      const R e = Tc * (pc * pc);
+      const R wxp = (e + sT[ly][lx + 1] * (phi[ly][lx + 1] * phi[ly][lx + 1])) * __builtin_amdgcn_rcpf(mgq_xp);
+      const R wxm = (e + sT[ly][lx - 1] * (phi[ly][lx - 1] * phi[ly][lx - 1])) * __builtin_amdgcn_rcpf(mgq_xm);
+      const R wyp = (e + sT[ly + 1][lx] * (phi[ly + 1][lx] * phi[ly + 1][lx])) * __builtin_amdgcn_rcpf(mgq_yp);
+      const R wym = (e + sT[ly - 1][lx] * (phi[ly - 1][lx] * phi[ly - 1][lx])) * __builtin_amdgcn_rcpf(mgq_ym);
+      f_ori_1 = Hidx * (wxp * (Q1.xp - Q1.c) - wxm * (Q1.c - Q1.xm) + wyp * (Q1.yp - Q1.c) - wym * (Q1.c - Q1.ym));
+      f_ori_4 = Hidx * (wxp * (Q4.xp - Q4.c) - wxm * (Q4.c - Q4.xm) + wyp * (Q4.yp - Q4.c) - wym * (Q4.c - Q4.ym));
     }
-    R lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx * idx;
-    R lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx * idx;
+    R lq1, lq4;
+    if constexpr (sizeof(R) == 8) {
+      lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx * idx;
+      lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx * idx;
+    } else {
+      lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx2;
+      lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx2;
+    }
     R noise_q1 = R(0), noise_q4 = R(0);
     if (p.noise_q != 0.0) {
       const uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
