@@ -84,7 +84,11 @@ __device__ __forceinline__ void aniso_point(const AnisoK& k, R phi, R c, R& temp
   const R H_A = R(k.W_A) * gp - R(k.kA) * g;
   const R H_B = R(k.W_B) * gp - R(k.kB) * g;
   D_C = R(k.D_S) + m * R(k.dLS);
-  temp = D_C * R(k.v_m) * c * (R(1) - c) * (H_B - H_A) / R(k.Rg);
+  if constexpr (sizeof(R) == 8) {
+    temp = D_C * R(k.v_m) * c * (R(1) - c) * (H_B - H_A) / R(k.Rg);
+  } else {  // FP32: v_m / R as one constant instead of a division per point
+    temp = D_C * R(k.v_m / k.Rg) * c * (R(1) - c) * (H_B - H_A);
+  }
 }
 
 // Vertex terms at (x-1/2, y-1/2) (AnisoDorrGPU.py:346-375) from the cells
@@ -107,10 +111,20 @@ __device__ __forceinline__ void aniso_vertex(const AnisoK& k, R idx, R phi, R pl
   mg2 = mg2 > R(0.000000001) ? mg2 : R(0.000000001);
   const R sx3 = sx * sx * sx, sy3 = sy * sy * sy;
   const R ye4 = R(k.ye4);
-  pfx = ye4 * ((R(2) * a2_b2 * sx3 + R(2) * ab2 * sy3) / mg2 - gx * (sx3 * gx + sy3 * gy) / (mg2 * mg2));
-  pfy = ye4 * ((R(2) * a2_b2 * sy3 - R(2) * ab2 * sx3) / mg2 - gy * (sx3 * gx + sy3 * gy) / (mg2 * mg2));
+  R kk;
+  if constexpr (sizeof(R) == 8) {
+    pfx = ye4 * ((R(2) * a2_b2 * sx3 + R(2) * ab2 * sy3) / mg2 - gx * (sx3 * gx + sy3 * gy) / (mg2 * mg2));
+    pfy = ye4 * ((R(2) * a2_b2 * sy3 - R(2) * ab2 * sx3) / mg2 - gy * (sx3 * gx + sy3 * gy) / (mg2 * mg2));
+    kk = R(k.k16) / mg2;
+  } else {
+    // FP32: one reciprocal of the clamped |psi|^2 (>= 1e-9, a normal float)
+    const R img2 = __builtin_amdgcn_rcpf(mg2);
+    const R t = (sx3 * gx + sy3 * gy) * (img2 * img2);
+    pfx = ye4 * (R(2) * (a2_b2 * sx3 + ab2 * sy3) * img2 - gx * t);
+    pfy = ye4 * (R(2) * (a2_b2 * sy3 - ab2 * sx3) * img2 - gy * t);
+    kk = R(k.k16) * img2;
+  }
   const R q1px = q1m * gx, q1py = q1m * gy, q4px = q4m * gx, q4py = q4m * gy;
-  const R kk = R(k.k16) / mg2;
   t1t = kk * (psi_xyy * (q1px - q4py) + psi_xxy * (q1py + q4px));
   t4t = kk * (psi_xyy * (-q4px - q1py) + psi_xxy * (-q4py + q1px));
 }
@@ -154,13 +168,17 @@ __device__ __forceinline__ void aniso_cell(const AnisoK& k, R idx, const Ad5<R>&
   R sym = g1ym * g1ym + g4ym * g4ym;
   R sxp = g1xp * g1xp + g4xp * g4xp;
   R syp = g1yp * g1yp + g4yp * g4yp;
-  const R rg0 = sqrt(R(0.5) * (sxm + sym + sxp + syp));
   const R beta = R(k.beta);
+  R rg0;
+  if constexpr (sizeof(R) == 8) {
+    rg0 = sqrt(R(0.5) * (sxm + sym + sxp + syp));
+  } else {
+    rg0 = __builtin_amdgcn_sqrtf(R(0.5) * (sxm + sym + sxp + syp));
+  }
   sxm = sxm > beta ? sxm : beta;
   sym = sym > beta ? sym : beta;
   sxp = sxp > beta ? sxp : beta;
   syp = syp > beta ? syp : beta;
-  const R rxm = sqrt(sxm), rym = sqrt(sym), rxp = sqrt(sxp), ryp = sqrt(syp);
 
   // dc/dt: div(temp grad phi) + div(D_C grad c)
   R dc = hidx2 * ((T4.xp + T4.c) * (P.xp - p) - (T4.c + T4.xm) * (p - P.xm) + (T4.yp + T4.c) * (P.yp - p) -
@@ -181,13 +199,26 @@ __device__ __forceinline__ void aniso_cell(const AnisoK& k, R idx, const Ad5<R>&
   }
 
   // dq/dt
-  const R base = R(k.twoHT);
-  const R Dq = base * (p * p);
-  const R Dxp = base * P.xp * P.xp, Dxm = base * P.xm * P.xm, Dyp = base * P.yp * P.yp, Dym = base * P.ym * P.ym;
-  const R gaq1 = hidx2 * ((Dq + Dxp) * (A.xp - a) / rxp - (Dq + Dxm) * (a - A.xm) / rxm +
-                          (Dq + Dyp) * (A.yp - a) / ryp - (Dq + Dym) * (a - A.ym) / rym);
-  const R gaq4 = hidx2 * ((Dq + Dxp) * (B.xp - b) / rxp - (Dq + Dxm) * (b - B.xm) / rxm +
-                          (Dq + Dyp) * (B.yp - b) / ryp - (Dq + Dym) * (b - B.ym) / rym);
+  R gaq1, gaq4;
+  if constexpr (sizeof(R) == 8) {
+    const R rxm = sqrt(sxm), rym = sqrt(sym), rxp = sqrt(sxp), ryp = sqrt(syp);
+    const R base = R(k.twoHT);
+    const R Dq = base * (p * p);
+    const R Dxp = base * P.xp * P.xp, Dxm = base * P.xm * P.xm, Dyp = base * P.yp * P.yp, Dym = base * P.ym * P.ym;
+    gaq1 = hidx2 * ((Dq + Dxp) * (A.xp - a) / rxp - (Dq + Dxm) * (a - A.xm) / rxm +
+                    (Dq + Dyp) * (A.yp - a) / ryp - (Dq + Dym) * (a - A.ym) / rym);
+    gaq4 = hidx2 * ((Dq + Dxp) * (B.xp - b) / rxp - (Dq + Dxm) * (b - B.xm) / rxm +
+                    (Dq + Dyp) * (B.yp - b) / ryp - (Dq + Dym) * (b - B.ym) / rym);
+  } else {
+    // FP32: face weights (D + D_nb) / |grad q|_face = 2HT (p^2 + p_nb^2) / sqrt(s),
+    // one reciprocal square root per face (s >= beta), 2HT in the common
+    // factor, the weights shared by q1 and q4
+    const R p2 = p * p, hk = hidx2 * R(k.twoHT);
+    const R wxp = (p2 + P.xp * P.xp) * __builtin_amdgcn_rsqf(sxp), wxm = (p2 + P.xm * P.xm) * __builtin_amdgcn_rsqf(sxm);
+    const R wyp = (p2 + P.yp * P.yp) * __builtin_amdgcn_rsqf(syp), wym = (p2 + P.ym * P.ym) * __builtin_amdgcn_rsqf(sym);
+    gaq1 = hk * (wxp * (A.xp - a) - wxm * (a - A.xm) + wyp * (A.yp - a) - wym * (a - A.ym));
+    gaq4 = hk * (wxp * (B.xp - b) - wxm * (b - B.xm) + wyp * (B.yp - b) - wym * (b - B.ym));
+  }
   const R cc1 = R(0.25) * (T1.v00 + T1.v01 + T1.v10 + T1.v11);
   const R cc4 = R(0.25) * (T4v.v00 + T4v.v01 + T4v.v10 + T4v.v11);
   const R t1 = R(k.eqbar2) * lq1 + gaq1 + cc1;
@@ -201,9 +232,15 @@ __device__ __forceinline__ void aniso_cell(const AnisoK& k, R idx, const Ad5<R>&
   const R q1n = a + dq1 * dt;
   const R q4n = b + dq4 * dt;
   nc = c + dc * dt;
-  const R rn = sqrt(q1n * q1n + q4n * q4n);
-  nq1 = q1n / rn;
-  nq4 = q4n / rn;
+  if constexpr (sizeof(R) == 8) {
+    const R rn = sqrt(q1n * q1n + q4n * q4n);
+    nq1 = q1n / rn;
+    nq4 = q4n / rn;
+  } else {
+    const R irn = __builtin_amdgcn_rsqf(q1n * q1n + q4n * q4n);
+    nq1 = q1n * irn;
+    nq4 = q4n * irn;
+  }
 }
 
 // ------------------------------------------------------- two-pass path --
