@@ -411,6 +411,112 @@ __global__ __launch_bounds__(256) void k_aniso_fused(const R* __restrict__ in, R
   }
 }
 
+// ------------------------------------------------------------ wave tiles --
+// FP32 grids with enough tiles: each wave of a 256-thread block owns a 62 x
+// kAwTY tile.  Lane l holds column x0 - 1 + l of the four fields on the rows
+// y0 - 1 .. y0 + kAwTY in registers (all loads issued together; halos a
+// coordinate remap when there are no bc values and no Dirichlet side, folded
+// otherwise), computes temp / D_C on those rows and the vertex terms on the
+// vertex rows y0 .. y0 + kAwTY (vertex (x, y) at (x - 1/2, y - 1/2) from the
+// cells of lanes l and l - 1 on rows y and y - 1), then updates its rows.  x
+// neighbours come from DPP lane shifts, y neighbours from the lane's rows:
+// no LDS and no barrier.  The device functions are the block path's.
+#ifndef PPF_AD_WTY
+#define PPF_AD_WTY 4
+#endif
+constexpr int kAwTX = 62, kAwTY = PPF_AD_WTY;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_aniso_w(const R* __restrict__ in, R* __restrict__ out,
+                                                 const R* __restrict__ bv, Dom d, R dx, AnisoK k, uint64_t seed,
+                                                 StepRef sr, uint32_t nstrips, uint32_t ntiles) {
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
+  const int lane = int(threadIdx.x) % 64;
+  const uint32_t t = xcd_swizzle(blockIdx.x, gridDim.x) * 4u + uint32_t(wave);
+  if (t >= ntiles) return;  // uniform per wave
+  const int32_t x0 = 1 + int32_t(t % nstrips) * kAwTX, y0 = 1 + int32_t(t / nstrips) * kAwTY;
+  const int32_t nx = int32_t(d.ax.n), ny = int32_t(d.ay.n);
+  const int32_t xr = x0 - 1 + lane;
+  constexpr int NR = kAwTY + 2;  // cell rows y0 - 1 .. y0 + kAwTY
+  R F[4][NR];
+  if (!bv && d.ax.bc != PPF_DIRICHLET && d.ay.bc != PPF_DIRICHLET) {
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in);
+    const uint32_t xo = uint32_t(remap_coord32(xr, nx, d.ax.bc)) * uint32_t(sizeof(R));
+    const int32_t sy = int32_t(d.ay.stride * int64_t(sizeof(R))), sfb = int32_t(d.sf * int64_t(sizeof(R)));
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int32_t yo = remap_coord32(y0 - 1 + r, ny, d.ay.bc) * sy;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) F[i][r] = buf_load<R>(rs, xo, yo + i * sfb);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int64_t y = y0 - 1 + r;
+      const bool ok = xr <= nx + 1 && y <= ny + 1;  // beyond: never used
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        F[i][r] = ok ? fold(in + i * d.sf, bv ? bv + i * d.sf : nullptr, d, int64_t(xr), y, int64_t(0), dx) : R(0);
+    }
+  }
+  const R idx = R(1) / dx;
+  R Tp[NR], DC[NR];  // temp, D_C
+#pragma unroll
+  for (int r = 0; r < NR; ++r) aniso_point(k, F[0][r], F[3][r], Tp[r], DC[r]);
+  // vertex rows y0 .. y0 + kAwTY: row j from cell rows j (y - 1) and j + 1 (y)
+  R V[4][kAwTY + 1];
+#pragma unroll
+  for (int j = 0; j < kAwTY + 1; ++j) {
+    R L[3][2];  // lane l - 1 of rows y and y - 1
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      L[i][0] = lane_shift<kDppWaveShr1>(F[i][j + 1]);
+      L[i][1] = lane_shift<kDppWaveShr1>(F[i][j]);
+    }
+    aniso_vertex(k, idx, F[0][j + 1], L[0][0], F[0][j], L[0][1], F[1][j + 1], F[1][j], L[1][0], L[1][1], F[2][j + 1],
+                 F[2][j], L[2][0], L[2][1], V[0][j], V[1][j], V[2][j], V[3][j]);
+  }
+  const int64_t step = sr.get();
+  const uint64_t ncell = uint64_t(nx + 2) * uint64_t(ny + 2);
+  const bool out_lane = lane >= 1 && lane < 63 && xr <= nx;
+  const __amdgpu_buffer_rsrc_t ws = plane_rsrc(out);
+  const uint32_t oo = uint32_t(int64_t(y0) * d.ay.stride + xr) * uint32_t(sizeof(R));
+  const int32_t osy = int32_t(d.ay.stride * int64_t(sizeof(R))), osf = int32_t(d.sf * int64_t(sizeof(R)));
+#pragma unroll
+  for (int r = 0; r < kAwTY; ++r) {
+    const int c = r + 1;  // cell row of the output
+    auto five = [&](const R (&a)[NR]) {
+      return Ad5<R>{a[c], lane_shift<kDppWaveShl1>(a[c]), lane_shift<kDppWaveShr1>(a[c]), a[c + 1], a[c - 1]};
+    };
+    auto four = [&](const R (&v)[kAwTY + 1]) {
+      return Ad4<R>{v[r], lane_shift<kDppWaveShl1>(v[r]), v[r + 1], lane_shift<kDppWaveShl1>(v[r + 1])};
+    };
+    const int32_t y = y0 + r;
+    R nphi, nq1, nq4, nc;
+    aniso_cell(k, idx, five(F[0]), five(F[1]), five(F[2]), five(F[3]), five(Tp), five(DC), four(V[0]), four(V[1]),
+               four(V[2]), four(V[3]), seed, step, ncell, uint64_t(y) * uint64_t(nx + 2) + uint64_t(xr), nphi, nq1,
+               nq4, nc);
+    if (out_lane && y <= ny) {
+      buf_store<R>(nphi, ws, oo, r * osy);
+      buf_store<R>(nq1, ws, oo, r * osy + osf);
+      buf_store<R>(nq4, ws, oo, r * osy + 2 * osf);
+      buf_store<R>(nc, ws, oo, r * osy + 3 * osf);
+    }
+  }
+}
+
+// whether the wave-tile kernel runs the grid: FP32, enough wave tiles to fill
+// the GPU, and the four fields' bytes within the 32-bit buffer offsets
+// (PPF_AD_WAVE_TILES=0 keeps the block tiles everywhere)
+#ifndef PPF_AD_WAVE_TILES
+#define PPF_AD_WAVE_TILES 1
+#endif
+inline bool aniso_wave_fits(const ppf_grid& g, size_t elem) {
+  if (!PPF_AD_WAVE_TILES || elem != sizeof(float) || g.ndim != 2) return false;
+  const int64_t tiles = ((g.n[0] + kAwTX - 1) / kAwTX) * ((g.n[1] + kAwTY - 1) / kAwTY);
+  return tiles >= 1024 && (3 * g.sf + (g.n[1] + 2) * g.sy) * int64_t(elem) < (int64_t(1) << 31);
+}
+
 template <typename R>
 int aniso_helper(const ppf_grid& g, const void* in, const void* bcvals, const ppf_grid& tg, void* transfer,
                  const ppf_aniso_params& p, hipStream_t s) {
@@ -426,6 +532,14 @@ template <typename R>
 int aniso_fused_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_aniso_params& p,
                      uint64_t seed, StepRef step, hipStream_t s) {
   Dom d = make_dom(g);
+  if (sizeof(R) == 4 && aniso_wave_fits(g, sizeof(R))) {
+    const uint32_t nstrips = uint32_t((g.n[0] + kAwTX - 1) / kAwTX);
+    const uint32_t ntiles = nstrips * uint32_t((g.n[1] + kAwTY - 1) / kAwTY);
+    hipLaunchKernelGGL(k_aniso_w<R>, dim3((ntiles + 3) / 4), dim3(256), 0, s, static_cast<const R*>(in),
+                       static_cast<R*>(out), static_cast<const R*>(bcvals), d, R(g.dx), aniso_constants(p), seed,
+                       step, nstrips, ntiles);
+    return check_launch("aniso_fused_step");
+  }
   dim3 grid(unsigned(((g.n[0] + kAdTX - 1) / kAdTX) * ((g.n[1] + kAdTY - 1) / kAdTY)));
   hipLaunchKernelGGL(k_aniso_fused<R>, grid, dim3(256), 0, s, static_cast<const R*>(in), static_cast<R*>(out),
                      static_cast<const R*>(bcvals), d, R(g.dx), aniso_constants(p), seed, step);

@@ -98,3 +98,26 @@ def test_aniso_fused_equals_two_pass_fp64(gpu, bc):
     b.simulate(30)
     for i in range(4):
         assert np.array_equal(a.fields[i].data, b.fields[i].data), a.fields[i].name
+
+
+@pytest.mark.parametrize("bc", ["PERIODIC", "NEUMANN", ["PERIODIC", "NEUMANN"]])
+def test_aniso_fp32_wave_tiles_equal_two_pass(gpu, bc):
+    """FP32 grids of 1024 or more 62 x 4 tiles take the register-tile kernel
+    (lane shifts for x, the lane's rows for y); it evaluates the same device
+    functions as the reference-shaped two launches, one thread per cell, so
+    one step agrees to a few FP32 ulp (the two kernels' contractions differ),
+    and three steps to the orientation flux's amplification of those ulps
+    (200 x 1030: 4 strips, the last partial, x 258 tile rows; seeds so that
+    interfaces cross tile edges)."""
+    shape = (1030, 200)
+    a = _sim(shape, bc, precision="FP32", sim_type="seeds", number_of_seeds=12)
+    b = _sim(shape, bc, precision="FP32", sim_type="seeds", number_of_seeds=12)
+    b.user_data["aniso_two_pass"] = True
+    for steps, tol in ((1, 4e-7), (2, 1e-5)):
+        a.simulate(steps - a.time_step_counter)
+        b.simulate(steps - b.time_step_counter)
+        for i in range(4):
+            da, db = a.fields[i].get_cells(), b.fields[i].get_cells()
+            assert np.all(np.isfinite(da)), a.fields[i].name
+            err = float(np.abs(da - db).max())
+            assert err <= tol * np.abs(db).max(), (steps, a.fields[i].name, err)
