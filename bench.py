@@ -488,7 +488,9 @@ def bench_aniso(args):
     t = _timed(sim.simulate, args.steps)
     cells = n * n
     esize = 8 if precision == "FP64" else 4
-    kernel = "k_aniso_fused<%s>" % ("double" if esize == 8 else "float")
+    # FP32 grids of 1024 or more 62 x 4 wave tiles run the register-tile kernel
+    wave = esize == 4 and -(-n // 62) * -(-n // 4) >= 1024
+    kernel = "k_aniso_w<float>" if wave else "k_aniso_fused<%s>" % ("double" if esize == 8 else "float")
     roof = _roofline(2 * 4 * esize * cells, t / args.steps, kernel)
     return _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t,
                  "f64" if precision == "FP64" else "f32",
