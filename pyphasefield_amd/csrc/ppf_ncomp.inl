@@ -612,6 +612,346 @@ struct Nb5 {
 #define PPF_NC_STEP_ATTR
 #endif
 
+// Per-launch constants of the step's cell update, in the arithmetic type.
+template <typename R>
+struct NcStepC {
+  R idx, v_m, inv_v_m, M_qmax, Hh, y_e, beta, hidx, idx2, qidx2, Hidx, beta_g, dt, ebar2, eqbar2;
+  __device__ __forceinline__ NcStepC(const ppf_ncomp_params& p, R dx) {
+    idx = R(1) / dx;
+    v_m = R(p.v_m);
+    inv_v_m = R(1.0 / p.v_m);
+    M_qmax = R(p.M_qmax);
+    Hh = R(p.H);
+    y_e = R(p.y_e);
+    beta = R(p.beta);
+    // FP32 scale factors: 0.5/dx, 1/dx^2, 0.25/dx^2, H/dx, and the clamp of
+    // the face |grad q| in grid units (beta dx); FP64 clamps |grad q| itself
+    hidx = R(0.5) * idx;
+    idx2 = idx * idx;
+    qidx2 = R(0.25) * idx2;
+    Hidx = Hh * idx;
+    beta_g = sizeof(R) == 8 ? beta : R(p.beta * double(dx));
+    dt = R(p.dt);
+    const R ebar = R(p.ebar), eqbar = R(p.eqbar);
+    ebar2 = ebar * ebar;
+    eqbar2 = eqbar * eqbar;
+    if (p.flags & PPF_NC_EBAR_FROM_S) {
+      // the legacy engine's values, in its operation order (NCGPU.py:74-75)
+      ebar2 = R(6.0 * 1.4142135623730951 * p.S[1] * p.d / p.T_M[1]);
+      eqbar2 = R(0.25) * ebar2;
+    }
+  }
+};
+
+// The neighbourhood one cell update reads, through an accessor evaluated at
+// the point of use: ph(i, j) = phi at row y-1+i, column x-1+j; T(), q1(),
+// q4(), c(l), Mc(l), dF(l) the 5-point stencils; dG() = G_S - G_L at the
+// centre.
+template <typename R>
+struct Nb5v {
+  R c, xp, xm, yp, ym;
+};
+
+template <typename R, int W>
+__device__ __forceinline__ Nb5v<R> lds5(const R (*s)[W], int ly, int lx) {
+  return Nb5v<R>{s[ly][lx], s[ly][lx + 1], s[ly][lx - 1], s[ly + 1][lx], s[ly - 1][lx]};
+}
+
+// One cell of NComponent_kernel (NCGPU_new.py:84-249) from its neighbourhood:
+// new phi, q1, q4 (renormalised) and compositions.  `cell` keys the noise.
+template <typename R, int N, typename Acc>
+__device__ __forceinline__ void nc_cell(const Acc& v, const NcStepC<R>& K, const ppf_ncomp_params& p,
+                                        uint64_t seed, int64_t step, uint64_t ncells, uint64_t cell, R& phi_out,
+                                        R& q1_out, R& q4_out, R (&cn)[N - 1]) {
+  const R idx = K.idx, v_m = K.v_m, inv_v_m = K.inv_v_m, M_qmax = K.M_qmax, Hh = K.Hh, y_e = K.y_e;
+  const R hidx = K.hidx, idx2 = K.idx2, qidx2 = K.qidx2, Hidx = K.Hidx, beta_g = K.beta_g, dt = K.dt;
+  const R ebar2 = K.ebar2, eqbar2 = K.eqbar2;
+  R pc = v.ph(1, 1);
+  // interpolating functions (NCGPU_new.py:84-87)
+  R pc3 = pc * pc * pc;
+  R g = (pc * pc) * ((R(1) - pc) * (R(1) - pc));
+  R h, gprime;
+  if constexpr (sizeof(R) == 8) {
+    h = pc3 * (R(6) * (pc * pc) - R(15) * pc + R(10));
+    gprime = R(4) * pc3 - R(6) * (pc * pc) + R(2) * pc;
+  } else {  // Horner forms
+    h = pc3 * (pc * (R(6) * pc - R(15)) + R(10));
+    gprime = pc * (pc * (R(4) * pc - R(6)) + R(2));
+  }
+  R hprime = R(30) * g;
+  // gradients (:89-107); FP32 takes the uniform scale factors as one
+  // product each (0.5/dx, 1/dx^2, 0.25/dx^2), FP64 the reference's order
+  R dphidx, dphidy, d2phidx2, d2phidy2, d2phidxy;
+  if constexpr (sizeof(R) == 8) {
+    dphidx = R(0.5) * (v.ph(1, 2) - v.ph(1, 0)) * idx;
+    dphidy = R(0.5) * (v.ph(2, 1) - v.ph(0, 1)) * idx;
+    d2phidx2 = (v.ph(1, 2) + v.ph(1, 0) - R(2) * pc) * idx * idx;
+    d2phidy2 = (v.ph(2, 1) + v.ph(0, 1) - R(2) * pc) * idx * idx;
+    d2phidxy = R(0.25) * (v.ph(2, 2) - v.ph(2, 0) - v.ph(0, 2) + v.ph(0, 0)) *
+               idx * idx;
+  } else {
+    dphidx = hidx * (v.ph(1, 2) - v.ph(1, 0));
+    dphidy = hidx * (v.ph(2, 1) - v.ph(0, 1));
+    d2phidx2 = (v.ph(1, 2) + v.ph(1, 0) - R(2) * pc) * idx2;
+    d2phidy2 = (v.ph(2, 1) + v.ph(0, 1) - R(2) * pc) * idx2;
+    d2phidxy = qidx2 * (v.ph(2, 2) - v.ph(2, 0) - v.ph(0, 2) + v.ph(0, 0));
+  }
+  R dphidx2 = dphidx * dphidx;
+  R dphidx3 = dphidx2 * dphidx;
+  R dphidy2 = dphidy * dphidy;
+  R dphidy3 = dphidy2 * dphidy;
+  R Tc = v.T5().c;
+  R dTdx = R(0.5) * idx * (v.T5().xp - v.T5().xm);
+  R dTdy = R(0.5) * idx * (v.T5().yp - v.T5().ym);
+  R lphi = d2phidx2 + d2phidy2;
+  R mag_grad_phi2 = dphidx * dphidx + dphidy * dphidy;
+  if (mag_grad_phi2 < R(1e-6)) mag_grad_phi2 = R(1e-6);
+  R mag_grad_phi4 = mag_grad_phi2 * mag_grad_phi2;
+  R mag_grad_phi8 = mag_grad_phi4 * mag_grad_phi4;
+  // orientation gradients (:109-125)
+  const Nb5v<R> Q1 = v.q1();
+  const Nb5v<R> Q4 = v.q4();
+  R dq1dx = R(0.5) * idx * (Q1.xp - Q1.xm);
+  R dq1dy = R(0.5) * idx * (Q1.yp - Q1.ym);
+  R dq4dx = R(0.5) * idx * (Q4.xp - Q4.xm);
+  R dq4dy = R(0.5) * idx * (Q4.yp - Q4.ym);
+  // face |grad q|; FP32 keeps them in grid units (r = dx |grad q|): the
+  // 1/dx goes into mag_grad_q's factor and the face weights' common factor
+  R mgq_xp = fsqrt((Q1.xp - Q1.c) * (Q1.xp - Q1.c) + (Q4.xp - Q4.c) * (Q4.xp - Q4.c));
+  R mgq_xm = fsqrt((Q1.xm - Q1.c) * (Q1.xm - Q1.c) + (Q4.xm - Q4.c) * (Q4.xm - Q4.c));
+  R mgq_yp = fsqrt((Q1.yp - Q1.c) * (Q1.yp - Q1.c) + (Q4.yp - Q4.c) * (Q4.yp - Q4.c));
+  R mgq_ym = fsqrt((Q1.ym - Q1.c) * (Q1.ym - Q1.c) + (Q4.ym - Q4.c) * (Q4.ym - Q4.c));
+  R mag_grad_q;
+  if constexpr (sizeof(R) == 8) {
+    mgq_xp = idx * mgq_xp;
+    mgq_xm = idx * mgq_xm;
+    mgq_yp = idx * mgq_yp;
+    mgq_ym = idx * mgq_ym;
+    mag_grad_q = R(0.5) * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
+  } else {
+    mag_grad_q = hidx * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
+  }
+  if (mgq_xp < beta_g) mgq_xp = beta_g;
+  if (mgq_xm < beta_g) mgq_xm = beta_g;
+  if (mgq_yp < beta_g) mgq_yp = beta_g;
+  if (mgq_ym < beta_g) mgq_ym = beta_g;
+  // psi terms (:127-146)
+  R q1c = Q1.c, q4c = Q4.c;
+  R q2q2 = (q1c * q1c - q4c * q4c);
+  R qq2 = R(2) * q1c * q4c;
+  R psix = q2q2 * dphidx - qq2 * dphidy;
+  R psiy = qq2 * dphidx + q2q2 * dphidy;
+  R psix2 = psix * psix;
+  R psix3 = psix2 * psix;
+  R psix4 = psix2 * psix2;
+  R psiy2 = psiy * psiy;
+  R psiy3 = psiy2 * psiy;
+  R psiy4 = psiy2 * psiy2;
+  R dq2q2dx = (R(2) * q1c * dq1dx - R(2) * q4c * dq4dx);
+  R dqq2dx = (R(2) * q4c * dq1dx + R(2) * q1c * dq4dx);
+  R dq2q2dy = (R(2) * q1c * dq1dy - R(2) * q4c * dq4dy);
+  R dqq2dy = (R(2) * q4c * dq1dy + R(2) * q1c * dq4dy);
+  R dpsixdx = dq2q2dx * dphidx + q2q2 * d2phidx2 - dqq2dx * dphidy - qq2 * d2phidxy;
+  R dpsixdy = dq2q2dy * dphidx + q2q2 * d2phidxy - dqq2dy * dphidy - qq2 * d2phidy2;
+  R dpsiydx = dq2q2dx * dphidy + q2q2 * d2phidxy + dqq2dx * dphidx + qq2 * d2phidx2;
+  R dpsiydy = dq2q2dy * dphidy + q2q2 * d2phidy2 + dqq2dy * dphidx + qq2 * d2phidxy;
+  R d_term_dx, d_term_dy;
+  if constexpr (sizeof(R) == 8) {
+    // reference form (oracle parity)
+    R psi4 = psix4 + psiy4;
+    // anisotropic terms (:148-160)
+    d_term_dx = dTdx * ((R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) / mag_grad_phi2 - dphidx * psi4 / mag_grad_phi4);
+    d_term_dx = d_term_dx + Tc *
+                                (R(6) * psix2 * dpsixdx * q2q2 + R(2) * psix3 * dq2q2dx +
+                                 R(6) * psiy2 * dpsiydx * qq2 + R(2) * psiy3 * dqq2dx) /
+                                mag_grad_phi2;
+    d_term_dx = d_term_dx - Tc * (R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) *
+                                (R(2) * dphidx * d2phidx2 + R(2) * dphidy * d2phidxy) / mag_grad_phi4;
+    d_term_dx = d_term_dx - Tc * (d2phidx2 * psi4 + dphidx * (R(4) * psix3 * dpsixdx + R(4) * psiy3 * dpsiydx)) /
+                                mag_grad_phi4;
+    d_term_dx = d_term_dx + Tc *
+                                (dphidx * psi4 *
+                                 (R(4) * dphidx3 * d2phidx2 + R(4) * dphidx * d2phidx2 * dphidy2 +
+                                  R(4) * dphidy * d2phidxy * dphidx2 + R(4) * dphidy3 * d2phidxy)) /
+                                mag_grad_phi8;
+    d_term_dx = d_term_dx * (R(4) * y_e * ebar2);
+    d_term_dy =
+        dTdy * ((R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) / mag_grad_phi2 - dphidy * psi4 / mag_grad_phi4);
+    d_term_dy = d_term_dy + Tc *
+                                (R(-6) * psix2 * dpsixdy * qq2 - R(2) * psix3 * dqq2dy +
+                                 R(6) * psiy2 * dpsiydy * q2q2 + R(2) * psiy3 * dq2q2dy) /
+                                mag_grad_phi2;
+    d_term_dy = d_term_dy - Tc * (R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) *
+                                (R(2) * dphidx * d2phidxy + R(2) * dphidy * d2phidy2) / mag_grad_phi4;
+    d_term_dy = d_term_dy - Tc * (d2phidy2 * psi4 + dphidy * (R(4) * psix3 * dpsixdy + R(4) * psiy3 * dpsiydy)) /
+                                mag_grad_phi4;
+    d_term_dy = d_term_dy + Tc *
+                                (dphidy * psi4 *
+                                 (R(4) * dphidx3 * d2phidxy + R(4) * dphidx * d2phidxy * dphidy2 +
+                                  R(4) * dphidy * d2phidy2 * dphidx2 + R(4) * dphidy3 * d2phidy2)) /
+                                mag_grad_phi8;
+    d_term_dy = d_term_dy * (R(4) * y_e * ebar2);
+  } else {
+    // FP32: the same algebra with gradients and psi divided by m = sqrt of the
+    // clamped |grad phi|^2 (every term is homogeneous in that scale), so no
+    // |grad phi|^8 is formed; with dx ~ 5e-6 cm the reference form exceeds
+    // the FP32 range.
+    const R im = __builtin_amdgcn_rsqf(mag_grad_phi2);
+    const R m = mag_grad_phi2 * im;
+    // Common factors taken out: with s = X^2 + Y^2, u_x = X phi_xx + Y phi_xy
+    // and u_y = X phi_xy + Y phi_yy, the last term of each component,
+    // 4 X^3 phi_xx + 4 X phi_xx Y^2 + 4 Y phi_xy X^2 + 4 Y^3 phi_xy, is
+    // 4 s u_x (and 4 s u_y), and the third is 2 A u.
+    R X = dphidx * im, Yg = dphidy * im, Px = psix * im, Py = psiy * im;
+    R Px2 = Px * Px, Py2 = Py * Py, Px3 = Px2 * Px, Py3 = Py2 * Py;
+    R P4 = Px2 * Px2 + Py2 * Py2;
+    R Ax = R(2) * (Px3 * q2q2 + Py3 * qq2);
+    R Ay = R(2) * (Py3 * q2q2 - Px3 * qq2);
+    const R s4 = R(4) * P4 * (X * X + Yg * Yg);
+    const R ux = X * d2phidx2 + Yg * d2phidxy, uy = X * d2phidxy + Yg * d2phidy2;
+    R tx = dTdx * m * (Ax - X * P4);
+    tx += Tc * (R(6) * (Px2 * dpsixdx * q2q2 + Py2 * dpsiydx * qq2) + R(2) * m * (Px3 * dq2q2dx + Py3 * dqq2dx) -
+                R(2) * Ax * ux - d2phidx2 * P4 - R(4) * X * (Px3 * dpsixdx + Py3 * dpsiydx) + X * s4 * ux);
+    R ty = dTdy * m * (Ay - Yg * P4);
+    ty += Tc * (R(6) * (Py2 * dpsiydy * q2q2 - Px2 * dpsixdy * qq2) + R(2) * m * (Py3 * dq2q2dy - Px3 * dqq2dy) -
+                R(2) * Ay * uy - d2phidy2 * P4 - R(4) * Yg * (Px3 * dpsixdy + Py3 * dpsiydy) + Yg * s4 * uy);
+    d_term_dx = tx * (R(4) * y_e * ebar2);
+    d_term_dy = ty * (R(4) * y_e * ebar2);
+  }
+  // c_N, cW, M_phi (:162-173)
+  R cW = R(0), c_N = R(1), M_phi = R(0);
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) {
+    R ci = v.c(l).c;
+    cW = cW + ci * R(p.W[l]);
+    c_N = c_N - ci;
+    M_phi = M_phi + ci * R(p.M[l]);
+  }
+  cW = cW + c_N * R(p.W[N - 1]);
+  M_phi = M_phi + c_N * R(p.M[N - 1]);
+  R M_q = M_qmax + (R(1e-6) - M_qmax) * h;
+  // dphidt (:179-189)
+  const R dG = v.dG();  // G_S - G_L
+  R dphidt = ebar2 * (R(1) - R(3) * y_e) * (Tc * lphi + dTdx * dphidx + dTdy * dphidy);
+  dphidt = dphidt + d_term_dx + d_term_dy;
+  if constexpr (sizeof(R) == 8) {
+    dphidt = dphidt - hprime * dG / v_m;
+  } else {
+    dphidt = dphidt - hprime * dG * inv_v_m;
+  }
+  dphidt = dphidt - gprime * Tc * cW;
+  dphidt = dphidt - R(4) * Hh * Tc * pc * mag_grad_q;
+  dphidt = dphidt * M_phi;
+  if (p.noise_phi != 0.0) {
+    R xi = normal01<R>(hash64(seed, rng_counter(step, S_NC_PHI, ncells, cell)));
+    dphidt = dphidt + sqrt(R(2.) * R(8.314) * Tc * M_phi / v_m) * xi * R(p.noise_phi);
+  }
+  // dc_i/dt (:191-203) and update (:243-249)
+#pragma unroll
+  for (int l = 0; l < N - 1; ++l) {
+    const Nb5v<R> Mc = v.Mc(l);
+    const Nb5v<R> dF = v.dF(l);
+    R acc = divagradb(Mc.c, Mc.xp, Mc.xm, Mc.yp, Mc.ym, dF.c, dF.xp, dF.xm, dF.yp, dF.ym, idx);
+#pragma unroll
+    for (int m = 0; m < N - 1; ++m) {
+      const Nb5v<R> cj = v.c(m);
+      const Nb5v<R> dFj = v.dF(m);
+      acc = acc - divagradb(Mc.c * cj.c, Mc.xp * cj.xp, Mc.xm * cj.xm, Mc.yp * cj.yp, Mc.ym * cj.ym, dFj.c,
+                            dFj.xp, dFj.xm, dFj.yp, dFj.ym, idx);
+    }
+    cn[l] = acc * dt + v.c(l).c;
+  }
+  // dq/dt (:205-232)
+  R f_ori_1, f_ori_4;
+  if constexpr (sizeof(R) == 8) {
+    R D_q = R(2) * Hh * Tc * (pc * pc);
+    R D_q_xp = R(2) * Hh * v.T5().xp * (v.ph(1, 2) * v.ph(1, 2));
+    R D_q_xm = R(2) * Hh * v.T5().xm * (v.ph(1, 0) * v.ph(1, 0));
+    R D_q_yp = R(2) * Hh * v.T5().yp * (v.ph(2, 1) * v.ph(2, 1));
+    R D_q_ym = R(2) * Hh * v.T5().ym * (v.ph(0, 1) * v.ph(0, 1));
+    f_ori_1 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q1.c, Q1.xp, Q1.xm,
+                         Q1.yp, Q1.ym, idx);
+    f_ori_4 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q4.c, Q4.xp, Q4.xm,
+                         Q4.yp, Q4.ym, idx);
+  } else {
+    // FP32: D_q = 2 H T phi^2 and |grad q|_face = r / dx, so the face weight
+    // (D_q + D_q,nb) / |grad q|_face is 2 H dx (e + e_nb) / r with e = T phi^2;
+    // 2 H dx goes into the common factor 0.5 / dx^2 (giving H / dx), and
+    // the four weights are shared by q1 and q4
+    const R e = Tc * (pc * pc);
+    const R wxp = (e + v.T5().xp * (v.ph(1, 2) * v.ph(1, 2))) * __builtin_amdgcn_rcpf(mgq_xp);
+    const R wxm = (e + v.T5().xm * (v.ph(1, 0) * v.ph(1, 0))) * __builtin_amdgcn_rcpf(mgq_xm);
+    const R wyp = (e + v.T5().yp * (v.ph(2, 1) * v.ph(2, 1))) * __builtin_amdgcn_rcpf(mgq_yp);
+    const R wym = (e + v.T5().ym * (v.ph(0, 1) * v.ph(0, 1))) * __builtin_amdgcn_rcpf(mgq_ym);
+    f_ori_1 = Hidx * (wxp * (Q1.xp - Q1.c) - wxm * (Q1.c - Q1.xm) + wyp * (Q1.yp - Q1.c) - wym * (Q1.c - Q1.ym));
+    f_ori_4 = Hidx * (wxp * (Q4.xp - Q4.c) - wxm * (Q4.c - Q4.xm) + wyp * (Q4.yp - Q4.c) - wym * (Q4.c - Q4.ym));
+  }
+  R lq1, lq4;
+  if constexpr (sizeof(R) == 8) {
+    lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx * idx;
+    lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx * idx;
+  } else {
+    lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx2;
+    lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx2;
+  }
+  R noise_q1 = R(0), noise_q4 = R(0);
+  if (p.noise_q != 0.0) {
+    R amp = R(p.noise_q) * sqrt(R(0.0000000001) * R(8.314) * Tc / v_m);
+    noise_q1 = amp * normal01<R>(hash64(seed, rng_counter(step, S_NC_Q1, ncells, cell)));
+    noise_q4 = amp * normal01<R>(hash64(seed, rng_counter(step, S_NC_Q4, ncells, cell)));
+  }
+  R t1 = f_ori_1 + lq1 * eqbar2 + noise_q1;
+  R t4 = f_ori_4 + lq4 * eqbar2 + noise_q4;
+  R dq1dt = M_q * ((R(1) - q1c * q1c) * t1 - q1c * q4c * t4);
+  R dq4dt = M_q * ((R(1) - q4c * q4c) * t4 - q1c * q4c * t1);
+  R phin = pc + dt * dphidt;
+  // NCGPU_new.py:234-237; the legacy engine's wider bounds, NCGPU.py:229-232
+  const bool wide = (p.flags & PPF_NC_CLAMP_WIDE) != 0;
+  const R plo = wide ? R(0.000001) : R(0.0001), phi_ = wide ? R(0.999999) : R(0.9999);
+  if (phin < plo) phin = plo;
+  if (phin > phi_) phin = phi_;
+  R q1n = q1c + dt * dq1dt;
+  R q4n = q4c + dt * dq4dt;
+  phi_out = phin;
+  if constexpr (sizeof(R) == 8) {
+    R renorm = fsqrt(q1n * q1n + q4n * q4n);
+    q1_out = q1n / renorm;
+    q4_out = q4n / renorm;
+  } else {
+    const R irn = __builtin_amdgcn_rsqf(q1n * q1n + q4n * q4n);
+    q1_out = q1n * irn;
+    q4_out = q4n * irn;
+  }
+}
+
+// Accessor of the LDS tile of ncomp_step_body at (ly, lx): every value is
+// read where the update uses it.
+template <typename R, int N, int W, int HH, bool FUSED>
+struct NcLds {
+  const R (*sf)[HH][W];
+  const R (*sT)[W];
+  const R (*st)[HH][W];
+  const R* sG;  // FUSED: [kNcTY][kTX]
+  const R* tr;
+  int64_t tsy;
+  int ly, lx;
+  int64_t y, x;
+  __device__ __forceinline__ R ph(int i, int j) const { return sf[0][ly - 1 + i][lx - 1 + j]; }
+  __device__ __forceinline__ Nb5v<R> T5() const { return lds5(sT, ly, lx); }
+  __device__ __forceinline__ Nb5v<R> q1() const { return lds5(sf[1], ly, lx); }
+  __device__ __forceinline__ Nb5v<R> q4() const { return lds5(sf[2], ly, lx); }
+  __device__ __forceinline__ Nb5v<R> c(int l) const { return lds5(sf[3 + l], ly, lx); }
+  __device__ __forceinline__ Nb5v<R> Mc(int l) const { return lds5(st[l], ly, lx); }
+  __device__ __forceinline__ Nb5v<R> dF(int l) const { return lds5(st[N - 1 + l], ly, lx); }
+  __device__ __forceinline__ R dG() const {
+    if constexpr (FUSED) {
+      return sG[(ly - 1) * kTX + (lx - 1)];
+    } else {
+      return (tr + y * tsy)[x];
+    }
+  }
+};
+
 // The main step (NComponent_kernel) on a 64 x kNcTY tile.  The fields and T
 // of the tile plus its +1 ring sit in LDS.  The thermodynamic quantities the
 // step reads on that ring (M_c, dF/dc) and at its centre (G_S - G_L) come
@@ -731,19 +1071,7 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
   }
 
   const ppf_ncomp_params& p = k.p;
-  const R idx = R(1) / dx;
-  const R v_m = R(p.v_m), inv_v_m = R(1.0 / p.v_m), M_qmax = R(p.M_qmax), Hh = R(p.H), y_e = R(p.y_e), beta = R(p.beta);
-  // FP32 scale factors: 0.5/dx, 1/dx^2, 0.25/dx^2, H/dx, and the clamp of the
-  // face |grad q| in grid units (beta dx); FP64 clamps |grad q| itself
-  const R hidx = R(0.5) * idx, idx2 = idx * idx, qidx2 = R(0.25) * idx2, Hidx = Hh * idx;
-  const R beta_g = sizeof(R) == 8 ? beta : R(p.beta * double(dx));
-  const R dt = R(p.dt), ebar = R(p.ebar), eqbar = R(p.eqbar);
-  R ebar2 = ebar * ebar, eqbar2 = eqbar * eqbar;
-  if (p.flags & PPF_NC_EBAR_FROM_S) {
-    // the legacy engine's values, in its operation order (NCGPU.py:74-75)
-    ebar2 = R(6.0 * 1.4142135623730951 * p.S[1] * p.d / p.T_M[1]);
-    eqbar2 = R(0.25) * ebar2;
-  }
+  const NcStepC<R> K(p, dx);
   const uint64_t ncells = uint64_t(d.ax.n + 2) * uint64_t(d.ay.n + 2);
   // kTX = 64: each wave owns one row of the tile per pass, so the row (and
   // every row base address) is wave-uniform
@@ -757,270 +1085,15 @@ __device__ __forceinline__ void ncomp_step_body(const R* __restrict__ in, R* __r
     int ly = tyb + 4 * kk + 1;
     int64_t y = y0 + ly - 1;
     if (y > d.ay.n) break;
-    const R(*phi)[W] = sf[0];
-    const R(*q1)[W] = sf[1];
-    const R(*q4)[W] = sf[2];
-    R pc = phi[ly][lx];
-    // interpolating functions (NCGPU_new.py:84-87)
-    R pc3 = pc * pc * pc;
-    R g = (pc * pc) * ((R(1) - pc) * (R(1) - pc));
-    R h, gprime;
-    if constexpr (sizeof(R) == 8) {
-      h = pc3 * (R(6) * (pc * pc) - R(15) * pc + R(10));
-      gprime = R(4) * pc3 - R(6) * (pc * pc) + R(2) * pc;
-    } else {  // Horner forms
-      h = pc3 * (pc * (R(6) * pc - R(15)) + R(10));
-      gprime = pc * (pc * (R(4) * pc - R(6)) + R(2));
-    }
-    R hprime = R(30) * g;
-    // gradients (:89-107); FP32 takes the uniform scale factors as one
-    // product each (0.5/dx, 1/dx^2, 0.25/dx^2), FP64 the reference's order
-    R dphidx, dphidy, d2phidx2, d2phidy2, d2phidxy;
-    if constexpr (sizeof(R) == 8) {
-      dphidx = R(0.5) * (phi[ly][lx + 1] - phi[ly][lx - 1]) * idx;
-      dphidy = R(0.5) * (phi[ly + 1][lx] - phi[ly - 1][lx]) * idx;
-      d2phidx2 = (phi[ly][lx + 1] + phi[ly][lx - 1] - R(2) * pc) * idx * idx;
-      d2phidy2 = (phi[ly + 1][lx] + phi[ly - 1][lx] - R(2) * pc) * idx * idx;
-      d2phidxy = R(0.25) * (phi[ly + 1][lx + 1] - phi[ly + 1][lx - 1] - phi[ly - 1][lx + 1] + phi[ly - 1][lx - 1]) *
-                 idx * idx;
-    } else {
-      dphidx = hidx * (phi[ly][lx + 1] - phi[ly][lx - 1]);
-      dphidy = hidx * (phi[ly + 1][lx] - phi[ly - 1][lx]);
-      d2phidx2 = (phi[ly][lx + 1] + phi[ly][lx - 1] - R(2) * pc) * idx2;
-      d2phidy2 = (phi[ly + 1][lx] + phi[ly - 1][lx] - R(2) * pc) * idx2;
-      d2phidxy = qidx2 * (phi[ly + 1][lx + 1] - phi[ly + 1][lx - 1] - phi[ly - 1][lx + 1] + phi[ly - 1][lx - 1]);
-    }
-    R dphidx2 = dphidx * dphidx;
-    R dphidx3 = dphidx2 * dphidx;
-    R dphidy2 = dphidy * dphidy;
-    R dphidy3 = dphidy2 * dphidy;
-    R Tc = sT[ly][lx];
-    R dTdx = R(0.5) * idx * (sT[ly][lx + 1] - sT[ly][lx - 1]);
-    R dTdy = R(0.5) * idx * (sT[ly + 1][lx] - sT[ly - 1][lx]);
-    R lphi = d2phidx2 + d2phidy2;
-    R mag_grad_phi2 = dphidx * dphidx + dphidy * dphidy;
-    if (mag_grad_phi2 < R(1e-6)) mag_grad_phi2 = R(1e-6);
-    R mag_grad_phi4 = mag_grad_phi2 * mag_grad_phi2;
-    R mag_grad_phi8 = mag_grad_phi4 * mag_grad_phi4;
-    // orientation gradients (:109-125)
-    Nb5<R, W> Q1(q1, ly, lx), Q4(q4, ly, lx);
-    R dq1dx = R(0.5) * idx * (Q1.xp - Q1.xm);
-    R dq1dy = R(0.5) * idx * (Q1.yp - Q1.ym);
-    R dq4dx = R(0.5) * idx * (Q4.xp - Q4.xm);
-    R dq4dy = R(0.5) * idx * (Q4.yp - Q4.ym);
-    // face |grad q|; FP32 keeps them in grid units (r = dx |grad q|): the
-    // 1/dx goes into mag_grad_q's factor and the face weights' common factor
-    R mgq_xp = fsqrt((Q1.xp - Q1.c) * (Q1.xp - Q1.c) + (Q4.xp - Q4.c) * (Q4.xp - Q4.c));
-    R mgq_xm = fsqrt((Q1.xm - Q1.c) * (Q1.xm - Q1.c) + (Q4.xm - Q4.c) * (Q4.xm - Q4.c));
-    R mgq_yp = fsqrt((Q1.yp - Q1.c) * (Q1.yp - Q1.c) + (Q4.yp - Q4.c) * (Q4.yp - Q4.c));
-    R mgq_ym = fsqrt((Q1.ym - Q1.c) * (Q1.ym - Q1.c) + (Q4.ym - Q4.c) * (Q4.ym - Q4.c));
-    R mag_grad_q;
-    if constexpr (sizeof(R) == 8) {
-      mgq_xp = idx * mgq_xp;
-      mgq_xm = idx * mgq_xm;
-      mgq_yp = idx * mgq_yp;
-      mgq_ym = idx * mgq_ym;
-      mag_grad_q = R(0.5) * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
-    } else {
-      mag_grad_q = hidx * (mgq_xp + mgq_xm + mgq_yp + mgq_ym);
-    }
-    if (mgq_xp < beta_g) mgq_xp = beta_g;
-    if (mgq_xm < beta_g) mgq_xm = beta_g;
-    if (mgq_yp < beta_g) mgq_yp = beta_g;
-    if (mgq_ym < beta_g) mgq_ym = beta_g;
-    // psi terms (:127-146)
-    R q1c = Q1.c, q4c = Q4.c;
-    R q2q2 = (q1c * q1c - q4c * q4c);
-    R qq2 = R(2) * q1c * q4c;
-    R psix = q2q2 * dphidx - qq2 * dphidy;
-    R psiy = qq2 * dphidx + q2q2 * dphidy;
-    R psix2 = psix * psix;
-    R psix3 = psix2 * psix;
-    R psix4 = psix2 * psix2;
-    R psiy2 = psiy * psiy;
-    R psiy3 = psiy2 * psiy;
-    R psiy4 = psiy2 * psiy2;
-    R dq2q2dx = (R(2) * q1c * dq1dx - R(2) * q4c * dq4dx);
-    R dqq2dx = (R(2) * q4c * dq1dx + R(2) * q1c * dq4dx);
-    R dq2q2dy = (R(2) * q1c * dq1dy - R(2) * q4c * dq4dy);
-    R dqq2dy = (R(2) * q4c * dq1dy + R(2) * q1c * dq4dy);
-    R dpsixdx = dq2q2dx * dphidx + q2q2 * d2phidx2 - dqq2dx * dphidy - qq2 * d2phidxy;
-    R dpsixdy = dq2q2dy * dphidx + q2q2 * d2phidxy - dqq2dy * dphidy - qq2 * d2phidy2;
-    R dpsiydx = dq2q2dx * dphidy + q2q2 * d2phidxy + dqq2dx * dphidx + qq2 * d2phidx2;
-    R dpsiydy = dq2q2dy * dphidy + q2q2 * d2phidy2 + dqq2dy * dphidx + qq2 * d2phidxy;
-    R d_term_dx, d_term_dy;
-    if constexpr (sizeof(R) == 8) {
-      // reference form (oracle parity)
-      R psi4 = psix4 + psiy4;
-      // anisotropic terms (:148-160)
-      d_term_dx = dTdx * ((R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) / mag_grad_phi2 - dphidx * psi4 / mag_grad_phi4);
-      d_term_dx = d_term_dx + Tc *
-                                  (R(6) * psix2 * dpsixdx * q2q2 + R(2) * psix3 * dq2q2dx +
-                                   R(6) * psiy2 * dpsiydx * qq2 + R(2) * psiy3 * dqq2dx) /
-                                  mag_grad_phi2;
-      d_term_dx = d_term_dx - Tc * (R(2) * psix3 * q2q2 + R(2) * psiy3 * qq2) *
-                                  (R(2) * dphidx * d2phidx2 + R(2) * dphidy * d2phidxy) / mag_grad_phi4;
-      d_term_dx = d_term_dx - Tc * (d2phidx2 * psi4 + dphidx * (R(4) * psix3 * dpsixdx + R(4) * psiy3 * dpsiydx)) /
-                                  mag_grad_phi4;
-      d_term_dx = d_term_dx + Tc *
-                                  (dphidx * psi4 *
-                                   (R(4) * dphidx3 * d2phidx2 + R(4) * dphidx * d2phidx2 * dphidy2 +
-                                    R(4) * dphidy * d2phidxy * dphidx2 + R(4) * dphidy3 * d2phidxy)) /
-                                  mag_grad_phi8;
-      d_term_dx = d_term_dx * (R(4) * y_e * ebar2);
-      d_term_dy =
-          dTdy * ((R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) / mag_grad_phi2 - dphidy * psi4 / mag_grad_phi4);
-      d_term_dy = d_term_dy + Tc *
-                                  (R(-6) * psix2 * dpsixdy * qq2 - R(2) * psix3 * dqq2dy +
-                                   R(6) * psiy2 * dpsiydy * q2q2 + R(2) * psiy3 * dq2q2dy) /
-                                  mag_grad_phi2;
-      d_term_dy = d_term_dy - Tc * (R(-2) * psix3 * qq2 + R(2) * psiy3 * q2q2) *
-                                  (R(2) * dphidx * d2phidxy + R(2) * dphidy * d2phidy2) / mag_grad_phi4;
-      d_term_dy = d_term_dy - Tc * (d2phidy2 * psi4 + dphidy * (R(4) * psix3 * dpsixdy + R(4) * psiy3 * dpsiydy)) /
-                                  mag_grad_phi4;
-      d_term_dy = d_term_dy + Tc *
-                                  (dphidy * psi4 *
-                                   (R(4) * dphidx3 * d2phidxy + R(4) * dphidx * d2phidxy * dphidy2 +
-                                    R(4) * dphidy * d2phidy2 * dphidx2 + R(4) * dphidy3 * d2phidy2)) /
-                                  mag_grad_phi8;
-      d_term_dy = d_term_dy * (R(4) * y_e * ebar2);
-    } else {
-      // FP32: the same algebra with gradients and psi divided by m = sqrt of the
-      // clamped |grad phi|^2 (every term is homogeneous in that scale), so no
-      // |grad phi|^8 is formed; with dx ~ 5e-6 cm the reference form exceeds
-      // the FP32 range.
-      const R im = __builtin_amdgcn_rsqf(mag_grad_phi2);
-      const R m = mag_grad_phi2 * im;
-      // Common factors taken out: with s = X^2 + Y^2, u_x = X phi_xx + Y phi_xy
-      // and u_y = X phi_xy + Y phi_yy, the last term of each component,
-      // 4 X^3 phi_xx + 4 X phi_xx Y^2 + 4 Y phi_xy X^2 + 4 Y^3 phi_xy, is
-      // 4 s u_x (and 4 s u_y), and the third is 2 A u.
-      R X = dphidx * im, Yg = dphidy * im, Px = psix * im, Py = psiy * im;
-      R Px2 = Px * Px, Py2 = Py * Py, Px3 = Px2 * Px, Py3 = Py2 * Py;
-      R P4 = Px2 * Px2 + Py2 * Py2;
-      R Ax = R(2) * (Px3 * q2q2 + Py3 * qq2);
-      R Ay = R(2) * (Py3 * q2q2 - Px3 * qq2);
-      const R s4 = R(4) * P4 * (X * X + Yg * Yg);
-      const R ux = X * d2phidx2 + Yg * d2phidxy, uy = X * d2phidxy + Yg * d2phidy2;
-      R tx = dTdx * m * (Ax - X * P4);
-      tx += Tc * (R(6) * (Px2 * dpsixdx * q2q2 + Py2 * dpsiydx * qq2) + R(2) * m * (Px3 * dq2q2dx + Py3 * dqq2dx) -
-                  R(2) * Ax * ux - d2phidx2 * P4 - R(4) * X * (Px3 * dpsixdx + Py3 * dpsiydx) + X * s4 * ux);
-      R ty = dTdy * m * (Ay - Yg * P4);
-      ty += Tc * (R(6) * (Py2 * dpsiydy * q2q2 - Px2 * dpsixdy * qq2) + R(2) * m * (Py3 * dq2q2dy - Px3 * dqq2dy) -
-                  R(2) * Ay * uy - d2phidy2 * P4 - R(4) * Yg * (Px3 * dpsixdy + Py3 * dpsiydy) + Yg * s4 * uy);
-      d_term_dx = tx * (R(4) * y_e * ebar2);
-      d_term_dy = ty * (R(4) * y_e * ebar2);
-    }
-    // c_N, cW, M_phi (:162-173)
-    R cW = R(0), c_N = R(1), M_phi = R(0);
-#pragma unroll
-    for (int l = 0; l < N - 1; ++l) {
-      R ci = sf[3 + l][ly][lx];
-      cW = cW + ci * R(p.W[l]);
-      c_N = c_N - ci;
-      M_phi = M_phi + ci * R(p.M[l]);
-    }
-    cW = cW + c_N * R(p.W[N - 1]);
-    M_phi = M_phi + c_N * R(p.M[N - 1]);
-    R M_q = M_qmax + (R(1e-6) - M_qmax) * h;
-    // dphidt (:179-189)
-    R dG;  // G_S - G_L
-    if constexpr (FUSED) {
-      dG = sG[ly - 1][lx - 1];
-    } else {
-      dG = (tr + y * tsy)[x];
-    }
-    R dphidt = ebar2 * (R(1) - R(3) * y_e) * (Tc * lphi + dTdx * dphidx + dTdy * dphidy);
-    dphidt = dphidt + d_term_dx + d_term_dy;
-    if constexpr (sizeof(R) == 8) {
-      dphidt = dphidt - hprime * dG / v_m;
-    } else {
-      dphidt = dphidt - hprime * dG * inv_v_m;
-    }
-    dphidt = dphidt - gprime * Tc * cW;
-    dphidt = dphidt - R(4) * Hh * Tc * pc * mag_grad_q;
-    dphidt = dphidt * M_phi;
-    if (p.noise_phi != 0.0) {
-      const uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
-      R xi = normal01<R>(hash64(seed, rng_counter(step, S_NC_PHI, ncells, cell)));
-      dphidt = dphidt + sqrt(R(2.) * R(8.314) * Tc * M_phi / v_m) * xi * R(p.noise_phi);
-    }
+    const NcLds<R, N, W, HH, FUSED> v{sf, sT, st, &sG[0][0], tr, tsy, ly, lx, y, x};
+    R nph, nq1, nq4, ncs[N - 1];
+    nc_cell<R, N>(v, K, p, seed, step, ncells, uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x), nph, nq1, nq4, ncs);
     R* const orow = out + y * d.ay.stride;  // wave-uniform row base
-    // dc_i/dt (:191-203) and update (:243-249)
 #pragma unroll
-    for (int l = 0; l < N - 1; ++l) {
-      Nb5<R, W> Mc(st[l], ly, lx), dF(st[N - 1 + l], ly, lx);
-      R acc = divagradb(Mc.c, Mc.xp, Mc.xm, Mc.yp, Mc.ym, dF.c, dF.xp, dF.xm, dF.yp, dF.ym, idx);
-#pragma unroll
-      for (int m = 0; m < N - 1; ++m) {
-        Nb5<R, W> cj(sf[3 + m], ly, lx), dFj(st[N - 1 + m], ly, lx);
-        acc = acc - divagradb(Mc.c * cj.c, Mc.xp * cj.xp, Mc.xm * cj.xm, Mc.yp * cj.yp, Mc.ym * cj.ym, dFj.c,
-                              dFj.xp, dFj.xm, dFj.yp, dFj.ym, idx);
-      }
-      orow[x + (3 + l) * d.sf] = acc * dt + sf[3 + l][ly][lx];
-    }
-    // dq/dt (:205-232)
-    R f_ori_1, f_ori_4;
-    if constexpr (sizeof(R) == 8) {
-      R D_q = R(2) * Hh * Tc * (pc * pc);
-      R D_q_xp = R(2) * Hh * sT[ly][lx + 1] * (phi[ly][lx + 1] * phi[ly][lx + 1]);
-      R D_q_xm = R(2) * Hh * sT[ly][lx - 1] * (phi[ly][lx - 1] * phi[ly][lx - 1]);
-      R D_q_yp = R(2) * Hh * sT[ly + 1][lx] * (phi[ly + 1][lx] * phi[ly + 1][lx]);
-      R D_q_ym = R(2) * Hh * sT[ly - 1][lx] * (phi[ly - 1][lx] * phi[ly - 1][lx]);
-      f_ori_1 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q1.c, Q1.xp, Q1.xm,
-                           Q1.yp, Q1.ym, idx);
-      f_ori_4 = f_ori_term(D_q, D_q_xp, D_q_xm, D_q_yp, D_q_ym, mgq_xp, mgq_xm, mgq_yp, mgq_ym, Q4.c, Q4.xp, Q4.xm,
-                           Q4.yp, Q4.ym, idx);
-    } else {
-      // FP32: D_q = 2 H T phi^2 and |grad q|_face = r / dx, so the face weight
-      // (D_q + D_q,nb) / |grad q|_face is 2 H dx (e + e_nb) / r with e = T phi^2;
-      // 2 H dx goes into the common factor 0.5 / dx^2 (giving H / dx), and
-      // the four weights are shared by q1 and q4
-      const R e = Tc * (pc * pc);
-      const R wxp = (e + sT[ly][lx + 1] * (phi[ly][lx + 1] * phi[ly][lx + 1])) * __builtin_amdgcn_rcpf(mgq_xp);
-      const R wxm = (e + sT[ly][lx - 1] * (phi[ly][lx - 1] * phi[ly][lx - 1])) * __builtin_amdgcn_rcpf(mgq_xm);
-      const R wyp = (e + sT[ly + 1][lx] * (phi[ly + 1][lx] * phi[ly + 1][lx])) * __builtin_amdgcn_rcpf(mgq_yp);
-      const R wym = (e + sT[ly - 1][lx] * (phi[ly - 1][lx] * phi[ly - 1][lx])) * __builtin_amdgcn_rcpf(mgq_ym);
-      f_ori_1 = Hidx * (wxp * (Q1.xp - Q1.c) - wxm * (Q1.c - Q1.xm) + wyp * (Q1.yp - Q1.c) - wym * (Q1.c - Q1.ym));
-      f_ori_4 = Hidx * (wxp * (Q4.xp - Q4.c) - wxm * (Q4.c - Q4.xm) + wyp * (Q4.yp - Q4.c) - wym * (Q4.c - Q4.ym));
-    }
-    R lq1, lq4;
-    if constexpr (sizeof(R) == 8) {
-      lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx * idx;
-      lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx * idx;
-    } else {
-      lq1 = (Q1.xp + Q1.xm + Q1.yp + Q1.ym - R(4) * Q1.c) * idx2;
-      lq4 = (Q4.xp + Q4.xm + Q4.yp + Q4.ym - R(4) * Q4.c) * idx2;
-    }
-    R noise_q1 = R(0), noise_q4 = R(0);
-    if (p.noise_q != 0.0) {
-      const uint64_t cell = uint64_t(y) * uint64_t(d.ax.n + 2) + uint64_t(x);
-      R amp = R(p.noise_q) * sqrt(R(0.0000000001) * R(8.314) * Tc / v_m);
-      noise_q1 = amp * normal01<R>(hash64(seed, rng_counter(step, S_NC_Q1, ncells, cell)));
-      noise_q4 = amp * normal01<R>(hash64(seed, rng_counter(step, S_NC_Q4, ncells, cell)));
-    }
-    R t1 = f_ori_1 + lq1 * eqbar2 + noise_q1;
-    R t4 = f_ori_4 + lq4 * eqbar2 + noise_q4;
-    R dq1dt = M_q * ((R(1) - q1c * q1c) * t1 - q1c * q4c * t4);
-    R dq4dt = M_q * ((R(1) - q4c * q4c) * t4 - q1c * q4c * t1);
-    R phin = pc + dt * dphidt;
-    // NCGPU_new.py:234-237; the legacy engine's wider bounds, NCGPU.py:229-232
-    const bool wide = (p.flags & PPF_NC_CLAMP_WIDE) != 0;
-    const R plo = wide ? R(0.000001) : R(0.0001), phi_ = wide ? R(0.999999) : R(0.9999);
-    if (phin < plo) phin = plo;
-    if (phin > phi_) phin = phi_;
-    R q1n = q1c + dt * dq1dt;
-    R q4n = q4c + dt * dq4dt;
-    orow[x] = phin;
-    if constexpr (sizeof(R) == 8) {
-      R renorm = fsqrt(q1n * q1n + q4n * q4n);
-      orow[x + d.sf] = q1n / renorm;
-      orow[x + 2 * d.sf] = q4n / renorm;
-    } else {
-      const R irn = __builtin_amdgcn_rsqf(q1n * q1n + q4n * q4n);
-      orow[x + d.sf] = q1n * irn;
-      orow[x + 2 * d.sf] = q4n * irn;
-    }
+    for (int l = 0; l < N - 1; ++l) orow[x + (3 + l) * d.sf] = ncs[l];
+    orow[x] = nph;
+    orow[x + d.sf] = nq1;
+    orow[x + 2 * d.sf] = nq4;
   }
 }
 
@@ -1041,6 +1114,144 @@ __global__ __launch_bounds__(kThreads) PPF_NC_STEP_ATTR void k_ncomp_step_tdb(co
   ncomp_step_body<R, N, true, true>(in, out, b, d, dx, t, tr, tsy, tsf, k, seed, sr);
 }
 
+// ------------------------------------------------------------ wave tiles --
+// The step from the transfer stack (FP32): each wave of a 256-thread block
+// owns a 62 x kNwTY tile.  Lane l holds column x0 - 1 + l of the five fields,
+// T and the M_c / dF arrays on the rows y0 - 1 .. y0 + kNwTY in registers
+// (all loads issued together; fields through the halo remap when there are
+// no bc values and no Dirichlet side, else folded; the transfer stack and T
+// at their padded coordinates), and G_S - G_L on the output rows.  The cell
+// update is nc_cell with an accessor that takes x neighbours from DPP lane
+// shifts and y neighbours from the lane's rows: no LDS and no barrier.
+#ifndef PPF_NC_WTY
+#define PPF_NC_WTY 4
+#endif
+constexpr int kNwTX = 62, kNwTY = PPF_NC_WTY;
+
+template <int N, int NR, int R0>
+struct NcWaveAcc {
+  const float (&f)[N + 2][NR];       // phi, q1, q4, c_1..c_{N-1}
+  const float (&T)[NR];
+  const float (&tq)[2 * (N - 1)][NR];  // M_c, dF/dc
+  float dg;
+  __device__ __forceinline__ static Nb5v<float> five(const float (&a)[NR]) {
+    return Nb5v<float>{a[R0], lane_shift<kDppWaveShl1>(a[R0]), lane_shift<kDppWaveShr1>(a[R0]), a[R0 + 1],
+                       a[R0 - 1]};
+  }
+  __device__ __forceinline__ float ph(int i, int j) const {
+    const float v = f[0][R0 - 1 + i];
+    return j == 0 ? lane_shift<kDppWaveShr1>(v) : (j == 2 ? lane_shift<kDppWaveShl1>(v) : v);
+  }
+  __device__ __forceinline__ Nb5v<float> T5() const { return five(T); }
+  __device__ __forceinline__ Nb5v<float> q1() const { return five(f[1]); }
+  __device__ __forceinline__ Nb5v<float> q4() const { return five(f[2]); }
+  __device__ __forceinline__ Nb5v<float> c(int l) const { return five(f[3 + l]); }
+  __device__ __forceinline__ Nb5v<float> Mc(int l) const { return five(tq[l]); }
+  __device__ __forceinline__ Nb5v<float> dF(int l) const { return five(tq[N - 1 + l]); }
+  __device__ __forceinline__ float dG() const { return dg; }
+};
+
+// f(0), f(1), ... (compile-time row indices)
+template <typename F, int... P>
+__device__ __forceinline__ void nc_unrolled(F& f, std::integer_sequence<int, P...>) {
+  (f(std::integral_constant<int, P>{}), ...);
+}
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_ncomp_step_w(const float* __restrict__ in, float* __restrict__ out,
+                                                           const float* __restrict__ b, Dom d, float dx,
+                                                           ppf_temperature t, const float* __restrict__ tr,
+                                                           int64_t tsy, int64_t tsf, NcK k, uint64_t seed,
+                                                           StepRef sr, uint32_t nstrips, uint32_t ntiles) {
+  constexpr int NF = N + 2, NT = 2 * (N - 1), NR = kNwTY + 2;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
+  const int lane = int(threadIdx.x) % 64;
+  const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4u + uint32_t(wave);
+  if (tile >= ntiles) return;  // uniform per wave
+  const int32_t x0 = 1 + int32_t(tile % nstrips) * kNwTX, y0 = 1 + int32_t(tile / nstrips) * kNwTY;
+  const int32_t nx = int32_t(d.ax.n), ny = int32_t(d.ay.n);
+  const int32_t xr = x0 - 1 + lane;
+  const int64_t step = sr.get();
+  float f[NF][NR], T[NR], tq[NT][NR], dg[kNwTY];
+  // transfer stack and T at padded coordinates (lanes and rows past the
+  // padded domain read its last cell: never used)
+  const int32_t xp = xr <= nx + 1 ? xr : nx + 1;
+  const __amdgpu_buffer_rsrc_t rt = plane_rsrc(tr);
+  const uint32_t xto = uint32_t(xp) * 4u;
+  const int32_t tsyb = int32_t(tsy * 4), tsfb = int32_t(tsf * 4);
+  if (!b && d.ax.bc != PPF_DIRICHLET && d.ay.bc != PPF_DIRICHLET) {
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in);
+    const uint32_t xo = uint32_t(remap_coord32(xr, nx, d.ax.bc)) * 4u;
+    const int32_t sy = int32_t(d.ay.stride * 4), sfb = int32_t(d.sf * 4);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int32_t yo = remap_coord32(y0 - 1 + r, ny, d.ay.bc) * sy;
+#pragma unroll
+      for (int i = 0; i < NF; ++i) f[i][r] = buf_load<float>(rs, xo, yo + i * sfb);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int64_t y = y0 - 1 + r;
+      const bool ok = xr <= nx + 1 && y <= ny + 1;
+#pragma unroll
+      for (int i = 0; i < NF; ++i)
+        f[i][r] = ok ? fold(in + i * d.sf, b ? b + i * d.sf : nullptr, d, int64_t(xr), y, int64_t(0), dx) : 0.f;
+    }
+  }
+  const int64_t n3[3] = {d.ax.n, d.ay.n, 1};
+  const int bc3[3] = {d.ax.bc, d.ay.bc, PPF_PERIODIC};
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int32_t yp = y0 - 1 + r <= ny + 1 ? y0 - 1 + r : ny + 1;
+#pragma unroll
+    for (int i = 0; i < NT; ++i) tq[i][r] = buf_load<float>(rt, xto, (1 + i) * tsfb + yp * tsyb);
+    T[r] = temperature_at<float>(t, n3, bc3, xp, yp, 1, double(dx), step);
+  }
+#pragma unroll
+  for (int r = 0; r < kNwTY; ++r) {
+    const int32_t yp = y0 + r <= ny + 1 ? y0 + r : ny + 1;
+    dg[r] = buf_load<float>(rt, xto, yp * tsyb);
+  }
+  const ppf_ncomp_params& p = k.p;
+  const NcStepC<float> K(p, dx);
+  const uint64_t ncells = uint64_t(nx + 2) * uint64_t(ny + 2);
+  const bool out_lane = lane >= 1 && lane < 63 && xr <= nx;
+  const __amdgpu_buffer_rsrc_t ws = plane_rsrc(out);
+  const uint32_t oo = uint32_t(int64_t(y0) * d.ay.stride + xr) * 4u;
+  const int32_t osy = int32_t(d.ay.stride * 4), osf = int32_t(d.sf * 4);
+  auto row = [&](auto rc) {
+    constexpr int r = decltype(rc)::value;
+    const NcWaveAcc<N, NR, r + 1> acc{f, T, tq, dg[r]};
+    const int32_t y = y0 + r;
+    float nph, nq1, nq4, ncs[N - 1];
+    nc_cell<float, N>(acc, K, p, seed, step, ncells, uint64_t(y) * uint64_t(nx + 2) + uint64_t(xr), nph, nq1, nq4,
+                      ncs);
+    if (out_lane && y <= ny) {
+      buf_store<float>(nph, ws, oo, r * osy);
+      buf_store<float>(nq1, ws, oo, r * osy + osf);
+      buf_store<float>(nq4, ws, oo, r * osy + 2 * osf);
+#pragma unroll
+      for (int l = 0; l < N - 1; ++l) buf_store<float>(ncs[l], ws, oo, r * osy + (3 + l) * osf);
+    }
+  };
+  nc_unrolled(row, std::make_integer_sequence<int, kNwTY>{});
+}
+
+// whether the wave-tile step runs the grid: FP32 from a transfer stack, 2D,
+// enough wave tiles to fill the GPU, and the field and transfer stacks within
+// the 32-bit buffer offsets (PPF_NC_WAVE_TILES=0 keeps the LDS tiles)
+#ifndef PPF_NC_WAVE_TILES
+#define PPF_NC_WAVE_TILES 1
+#endif
+inline bool ncomp_wave_fits(const ppf_grid& g, const ppf_grid* tg, int ncomp) {
+  if (!PPF_NC_WAVE_TILES || !tg || g.dtype != PPF_F32 || g.ndim != 2) return false;
+  const int64_t tiles = ((g.n[0] + kNwTX - 1) / kNwTX) * ((g.n[1] + kNwTY - 1) / kNwTY);
+  const int64_t fbytes = (int64_t(ncomp + 1) * g.sf + (g.n[1] + 2) * g.sy) * 4;
+  const int64_t tbytes = (int64_t(2 * ncomp - 2) * tg->sf + (g.n[1] + 2) * tg->sy) * 4;
+  return tiles >= 1024 && fbytes < (int64_t(1) << 31) && tbytes < (int64_t(1) << 31);
+}
+
 template <typename R, bool FUSED>
 int ncomp_step_launch(const ppf_grid& g, const void* in, void* out, const void* bcvals, const ppf_temperature& t,
                       const ppf_grid* tg, const void* transfer, const ppf_ncomp_params& p, const ppf_thermo* th,
@@ -1053,6 +1264,30 @@ int ncomp_step_launch(const ppf_grid& g, const void* in, void* out, const void* 
   const R* bc = static_cast<const R*>(bcvals);
   const R* tr = static_cast<const R*>(transfer);
   const int64_t tsy = tg ? tg->sy : 0, tsf = tg ? tg->sf : 0;
+  if constexpr (sizeof(R) == 4 && !FUSED) {
+    if (ncomp_wave_fits(g, tg, p.ncomp)) {
+      const uint32_t nstrips = uint32_t((g.n[0] + kNwTX - 1) / kNwTX);
+      const uint32_t ntiles = nstrips * uint32_t((g.n[1] + kNwTY - 1) / kNwTY);
+      const dim3 wgrid((ntiles + 3) / 4);
+      switch (p.ncomp) {
+        case 2:
+          hipLaunchKernelGGL(k_ncomp_step_w<2>, wgrid, dim3(kThreads), 0, s, fin, fout, bc, d, float(g.dx), t, tr, tsy,
+                             tsf, k, seed, step, nstrips, ntiles);
+          break;
+        case 3:
+          hipLaunchKernelGGL(k_ncomp_step_w<3>, wgrid, dim3(kThreads), 0, s, fin, fout, bc, d, float(g.dx), t, tr, tsy,
+                             tsf, k, seed, step, nstrips, ntiles);
+          break;
+        case 4:
+          hipLaunchKernelGGL(k_ncomp_step_w<4>, wgrid, dim3(kThreads), 0, s, fin, fout, bc, d, float(g.dx), t, tr, tsy,
+                             tsf, k, seed, step, nstrips, ntiles);
+          break;
+        default:
+          return fail(PPF_EINVAL, "ncomp_step: ncomp must be 2..4");
+      }
+      return check_launch("ncomp_step");
+    }
+  }
   switch (p.ncomp) {
     case 2:
       hipLaunchKernelGGL((FUSED && th && th->tables ? k_ncomp_step_tdb<R, 2> : k_ncomp_step<R, 2, FUSED>),

@@ -122,3 +122,42 @@ def test_ncomp_fused_fp32_close_to_two_pass(gpu):
         assert np.max(np.abs(a - b)) <= 1e-5
     q1, q4 = sims[0].fields[1].get_cells(), sims[0].fields[2].get_cells()
     assert np.max(np.abs(q1 * q1 + q4 * q4 - 1.0)) <= 1e-5
+
+
+def test_ncomp_fp32_wave_step_close_to_fused(gpu):
+    """FP32 grids of 1024 or more 62 x 4 tiles run the step from the transfer
+    stack in the register-tile kernel (lane shifts for x, the lane's rows for
+    y; 1030 x 200: 4 strips, the last partial, x 258 tile rows).  It calls
+    the same cell update as the LDS tile of the fused step, so one step agrees
+    to a few FP32 ulp (contractions differ) and ten steps to 1e-5 in the
+    compositions; graph replays equal eager launches bit for bit with noise."""
+    import torch
+    E = _engines()
+    sims = []
+    for mode in ("fused", "two_pass"):
+        sim, _ = _nc_sim(E, 1030, 200, "FP32", 0.0, seeds=12)
+        sim.user_data["ncomp_kernel"] = mode
+        sims.append(sim)
+    for steps, tol in ((1, 1e-6), (10, 1e-5)):
+        for sim in sims:
+            sim.simulate(steps - sim.time_step_counter)
+        for f in range(5):
+            a, b = sims[0].fields[f].get_cells(), sims[1].fields[f].get_cells()
+            assert np.all(np.isfinite(b)), (steps, f)
+            assert np.max(np.abs(a - b)) <= tol * max(1.0, float(np.abs(a).max())), (steps, f, float(np.max(np.abs(a - b))))
+    q1, q4 = sims[1].fields[1].get_cells(), sims[1].fields[2].get_cells()
+    assert np.max(np.abs(q1 * q1 + q4 * q4 - 1.0)) <= 1e-5
+    # graph replay of the wave step with noise equals eager launches
+    runs = []
+    for graph in (True, False):
+        sim, _ = _nc_sim(E, 1030, 200, "FP32", 1.0, seeds=12)
+        sim.user_data["ncomp_kernel"] = "two_pass"
+        if not graph:
+            sim._GRAPH_CHUNK = 0
+        runs.append(sim)
+    for n in (25, 7):
+        for sim in runs:
+            sim.simulate(n)
+        torch.cuda.synchronize()
+        for fa, fb in zip(runs[0].fields, runs[1].fields):
+            assert np.array_equal(fa.get_cells(), fb.get_cells()), (n, fa.name)
