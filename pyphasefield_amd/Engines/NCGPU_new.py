@@ -213,12 +213,17 @@ class NCGPU_new(Simulation):
         t = "float" if self._precision == "FP32" else "double"
         n = len(self._tdb_components)
         mode = self._kernel_mode()
+        # the step from the transfer stack: FP32 2D grids of 1024 or more
+        # 62 x 4 tiles run the register-tile kernel (csrc/ppf_ncomp.inl)
+        ny, nx = self.dimensions[0], self.dimensions[-1]
+        wave = t == "float" and len(self.dimensions) == 2 and -(-nx // 62) * -(-ny // 4) >= 1024
+        step = "k_ncomp_step_w<%d>" % n if wave else "k_ncomp_step<%s,%d,false>" % (t, n)
         if mode == "surrogate":
-            return ["k_ncomp_thermo_mlp", "k_ncomp_step<float,3,false>"]
+            return ["k_ncomp_thermo_mlp", step]
         tdb = getattr(self._tdb, "device_tables", None) is not None and self._tdb.device_tables() is not None
         if mode == "fused":
             return ["k_ncomp_step_tdb<%s,%d>" % (t, n) if tdb else "k_ncomp_step<%s,%d,true>" % (t, n)]
-        return ["k_ncomp_thermo%s<%s,%d>" % ("_tdb" if tdb else "", t, n), "k_ncomp_step<%s,%d,false>" % (t, n)]
+        return ["k_ncomp_thermo%s<%s,%d>" % ("_tdb" if tdb else "", t, n), step]
 
     def thermo_pass(self):
         L = _lib.lib()
