@@ -75,6 +75,33 @@ def test_aniso_graph_replay_equals_eager(gpu):
     _compare_runs(sims)
 
 
+def test_warren_fp32_wave_tiles_graph_replay_equals_eager(gpu):
+    """512 x 512 FP32 has 1152 tiles of 62 x 4: the register-tile kernel.
+    With noise, graph replays equal eager launches bit for bit."""
+    E = _engines()
+    sims = [_warren_sim(E, 512, "FP32", 0.3, seed=9) for _ in range(2)]
+    sims[1]._GRAPH_CHUNK = 0
+    _compare_runs(sims)
+
+
+def test_aniso_fp32_wave_tiles_graph_replay_equals_eager(gpu):
+    """1030 x 200 FP32 AnisoDorr runs the register-tile kernel; its noise
+    reads the device step counter inside graph replays."""
+    E = _engines()
+    sims = []
+    for graph in (True, False):
+        sim = E.AnisoDorrGPU(dimensions=[1030, 200], seed=3)
+        sim.set_user_data({"sim_type": "seeds", "number_of_seeds": 12})
+        sim.set_framework("GPU_SERIAL")
+        sim.set_boundary_conditions("NEUMANN")
+        sim.set_precision("FP32")
+        sim.initialize_fields_and_imported_data()
+        if not graph:
+            sim._GRAPH_CHUNK = 0
+        sims.append(sim)
+    _compare_runs(sims)
+
+
 @pytest.mark.parametrize("noise,bc", [(1.0, "NEUMANN"), (0.0, "PERIODIC"), (1.0, ["PERIODIC", "DIRCHLET"])])
 def test_ncomp_fused_equals_two_pass_fp64(gpu, noise, bc):
     """The fused step evaluates the helper's quantities with the same device
