@@ -402,44 +402,30 @@ __global__ __launch_bounds__(kThreads) PPF_WARREN_W_ATTR void k_warren_w(const R
   const bool out_lane = lane >= 1 && lane < 63 && xr <= nx;
   const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in), ws = plane_rsrc(out);
   const uint32_t oo = uint32_t(int64_t(y0) * d.ay.stride + xr) * uint32_t(sizeof(R));
-  // the tile's march; REMAP: halo rows and columns are a coordinate remap and
-  // rows are fetched kWvAhead ahead of the update (unconditional loads: a
-  // conditional one makes the compiler wait for it at once), else every value
-  // is folded before the first update
-  auto march = [&](auto remap_c) {
-    constexpr bool REMAP = decltype(remap_c)::value;
+  // the tile's march: halo rows and columns are a coordinate remap (the
+  // dispatcher sends grids with bc values or a Dirichlet side to the LDS
+  // tiles, which fold them), and rows are fetched kWvAhead ahead of the
+  // update (unconditional loads: a conditional one makes the compiler wait
+  // for it at once)
+  auto march = [&]() {
     R ph[kWvTY + 2], cc[kWvTY + 2];  // rows y0 - 1 .. y0 + kWvTY
     const uint32_t xo = uint32_t(remap_coord32(xr, nx, d.ax.bc)) * uint32_t(sizeof(R));
     auto fetch = [&](auto kc) {
       constexpr int k = decltype(kc)::value;
-      if constexpr (REMAP && k < kWvTY + 2) {
+      if constexpr (k < kWvTY + 2) {
         const int32_t yo = remap_coord32(y0 - 1 + k, ny, d.ay.bc) * sy;
         ph[k] = buf_load<R>(rs, xo, yo);
         cc[k] = buf_load<R>(rs, xo, yo + sfb);
       }
     };
-    if constexpr (REMAP) {
-      warren_unrolled(fetch,
-                      std::make_integer_sequence<int, (kWvAhead + 3 < kWvTY + 2 ? kWvAhead + 3 : kWvTY + 2)>{});
-    } else {
-#pragma unroll
-      for (int k = 0; k < kWvTY + 2; ++k) {
-        const int64_t y = y0 - 1 + k;
-        ph[k] = R(1);
-        cc[k] = R(0);
-        if (xr <= nx + 1 && y <= ny + 1) {  // beyond: never used
-          ph[k] = fold(in, b, d, int64_t(xr), y, int64_t(0), dx);
-          cc[k] = fold(in + d.sf, b ? b + d.sf : nullptr, d, int64_t(xr), y, int64_t(0), dx);
-        }
-      }
-    }
+    warren_unrolled(fetch, std::make_integer_sequence<int, (kWvAhead + 3 < kWvTY + 2 ? kWvAhead + 3 : kWvTY + 2)>{});
     R Dm, tm, D1, t1;
     warren_point(K, ph[0], cc[0], Dm, tm);
     warren_point(K, ph[1], cc[1], D1, t1);
     auto row = [&](auto rc) {
       constexpr int r = decltype(rc)::value;
       // rows up to r + 2 are in flight; start row r + kWvAhead + 3
-      if constexpr (REMAP && kWvAhead + 3 < kWvTY + 2) {
+      if constexpr (kWvAhead + 3 < kWvTY + 2) {
         __builtin_amdgcn_sched_barrier(0);
         fetch(std::integral_constant<int, r + kWvAhead + 3>{});
       }
@@ -460,10 +446,7 @@ __global__ __launch_bounds__(kThreads) PPF_WARREN_W_ATTR void k_warren_w(const R
     };
     warren_unrolled(row, std::make_integer_sequence<int, kWvTY>{});
   };
-  if (!b && d.ax.bc != PPF_DIRICHLET && d.ay.bc != PPF_DIRICHLET)
-    march(std::true_type{});
-  else
-    march(std::false_type{});
+  march();
 }
 
 // whether the wave-tile kernel runs the grid: FP32 (FP64 keeps the LDS tiles,
@@ -475,8 +458,9 @@ __global__ __launch_bounds__(kThreads) PPF_WARREN_W_ATTR void k_warren_w(const R
 #define PPF_WARREN_WAVE_TILES 1
 #endif
 constexpr int64_t kWvMinTiles = 1024;
-inline bool warren_wave_fits(const ppf_grid& g, size_t elem) {
+inline bool warren_wave_fits(const ppf_grid& g, size_t elem, const void* bcvals) {
   if (!PPF_WARREN_WAVE_TILES || elem != sizeof(float) || g.ndim != 2) return false;
+  if (bcvals || g.bc[0] == PPF_DIRICHLET || g.bc[1] == PPF_DIRICHLET) return false;
   const int64_t tiles = ((g.n[0] + kWvTX - 1) / kWvTX) * ((g.n[1] + kWvTY - 1) / kWvTY);
   return tiles >= kWvMinTiles && (g.sf + (g.n[1] + 2) * g.sy) * int64_t(elem) < (int64_t(1) << 31);
 }
@@ -527,7 +511,7 @@ int warren_step(const ppf_grid& g, const void* in, void* out, const void* bcvals
   k.M_A = p.M_A;
   k.M_B = p.M_B;
   k.dt = p.dt;
-  if (sizeof(R) == 4 && warren_wave_fits(g, sizeof(R))) {
+  if (sizeof(R) == 4 && warren_wave_fits(g, sizeof(R), bcvals)) {
     const WarrenC<R> K(k, R(g.dx));
     const uint32_t nstrips = uint32_t((g.n[0] + kWvTX - 1) / kWvTX);
     const uint32_t ntiles = nstrips * uint32_t((g.n[1] + kWvTY - 1) / kWvTY);
