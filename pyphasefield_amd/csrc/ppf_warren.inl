@@ -332,8 +332,8 @@ __global__ __launch_bounds__(kThreads) void k_warren(const R* __restrict__ in, R
 // wave walks its rows computing D_C and temp one row ahead; x neighbours come
 // from the adjacent lanes (DPP lane shifts), y neighbours from the lane's own
 // rows.  No LDS and no barrier.  Halo rows and columns are a coordinate remap
-// (periodic wrap, nearest interior cell for zero gradient) when there are no
-// bc values and no Dirichlet side; otherwise each value is folded.  Stencil
+// (periodic wrap, nearest interior cell for zero gradient): the kernel runs
+// grids without bc values and without a Dirichlet side only.  Stencil
 // terms: the x differences of a row are differences of shifted copies, and
 // gxy = (pm0 - pp0) - (pm2 - pp2) is the difference of the x neighbours' gx,
 // with the same operands and operations as warren_cell.
