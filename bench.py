@@ -254,6 +254,17 @@ def cpu_diffusion(n, k):
             "sample": "%d steps of the %dx%d FP64 numpy restatement (np.roll laplacian)" % (k, n, n)}
 
 
+def _gpu_multistep_max(sim):
+    from pyphasefield_amd import _lib
+    mx = _lib.lib().ppf_multistep_max(sim._grid, _lib.ptr(sim._boundary_conditions_gpu_device))
+    chunk = sim._graph_chunk()
+    if mx < 2 or not chunk:
+        return 0
+    launches = -(-chunk // mx)
+    launches += launches % 2
+    return chunk // launches if chunk % launches == 0 else 0
+
+
 def bench_diffusion(args):
     import pyphasefield_amd.Engines as engines
     n = 256
@@ -269,7 +280,14 @@ def bench_diffusion(args):
     _warm(sim, args.warmup)
     t = _timed(sim.simulate, args.steps)
     cells = n * n
-    roof = _roofline(16.0 * cells, t / args.steps, "k_point2d<double,0,4>")
+    # each launch of the multi-step kernel reads and writes the field once
+    # for up to 8 steps (48-step graph chunks: 6 launches of 8 steps)
+    per_launch = _gpu_multistep_max(sim)
+    if per_launch >= 2:
+        roof = _roofline(16.0 * cells, t / args.steps * per_launch, "k_point2d_multi<double,0,32,8>")
+        roof["steps_per_launch"] = per_launch
+    else:
+        roof = _roofline(16.0 * cells, t / args.steps, "k_point2d<double,0,4>")
     cpu = None if args.no_cpu else cpu_diffusion(n, 1000)
     return _line("cell-updates/s", cells * args.steps / t, "cell-updates/s", args.steps, args.warmup, t, "f64",
                  {"workload": "Diffusion 2D 256x256 FP64 periodic, D=1 dx=1 dt=0.1", "engine": "Diffusion",
@@ -629,7 +647,7 @@ def main():
         print("bench.py measures one GPU: this engine path has no multi-GPU mode "
               "(the domain would shard by slabs with a 1-2 cell halo exchange; not built)", file=sys.stderr)
         sys.exit(2)
-    defaults = {"warren": (2000, 100), "diffusion": (1000, 50), "ch3d": (50, 5), "copy": (50, 5),
+    defaults = {"warren": (2000, 100), "diffusion": (4800, 96), "ch3d": (50, 5), "copy": (50, 5),
                 "ncomp": (500, 20), "ensemble": (5000, 100), "aniso": (500, 20),
                 "implicit": (50, 5)}
     st, wu = defaults[args.config]

@@ -202,6 +202,21 @@ int ppf_diffusion_step(const ppf_grid* g, const void* in, void* out, const void*
 int ppf_allencahn_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
                        double M, double W, double epsilon, double dt, ppf_stream_t s);
 
+/* nsteps (1..ppf_multistep_max) successive ppf_diffusion_step /
+   ppf_allencahn_step steps in one launch: out = step^nsteps(in), equal to the
+   one-step calls bit for bit (no counterpart in the reference: in place of
+   nsteps calls of its per-step kernel launch, Engines/Diffusion.py:198-208,
+   Engines/CahnAllen.py:128-135).  Each block stages its tile with an nsteps
+   halo and steps it in LDS.  ppf_multistep_max returns the largest nsteps
+   the grid allows: 0 unless the grid is 2D and every side is periodic or
+   Neumann without bc values (bcvals null), at most 8 and at most each side. */
+int ppf_multistep_max(const ppf_grid* g, const void* bcvals);
+int ppf_diffusion_multistep(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                            double D, double dt, int32_t nsteps, ppf_stream_t s);
+int ppf_allencahn_multistep(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                            double M, double W, double epsilon, double dt, int32_t nsteps,
+                            ppf_stream_t s);
+
 /* mu = -eps^2 lap(phi) + 4phi^3-6phi^2+2phi;  phi <- phi + dt*M*lap(mu)
    (Engines/CahnHilliard.py:134-146; mu on the +1 ring: periodic wrap or
    zero-flux copy of the adjacent interior cell). */

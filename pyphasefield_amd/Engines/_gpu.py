@@ -16,6 +16,31 @@ def stencil_step(sim, fn_name, *scalars, nactive=None):
     sim._swap_buffers()
 
 
+def multistep(sim, fn_name, nsteps, *scalars, nactive=None):
+    """nsteps steps with the engine's several-steps-per-launch kernel, in an
+    even number of launches (the two buffers end where they started) of at
+    most ppf_multistep_max steps each.  False (nothing launched) when the
+    grid does not allow it."""
+    L = _lib.lib()
+    mx = L.ppf_multistep_max(sim._grid, _lib.ptr(sim._boundary_conditions_gpu_device))
+    if mx < 2:
+        return False
+    launches = -(-nsteps // mx)
+    launches += launches % 2
+    if launches > nsteps:
+        return False
+    fn = getattr(L, fn_name)
+    base, extra = divmod(nsteps, launches)
+    for i in range(launches):
+        k = base + (1 if i < extra else 0)
+        rc = fn(sim._grid, _lib.ptr(sim._fields_gpu_device), _lib.ptr(sim._fields_out_gpu_device),
+                _lib.ptr(sim._boundary_conditions_gpu_device), *scalars, k, _lib.stream_handle())
+        _lib.check(rc, fn_name)
+        carry_passive(sim, nactive)
+        sim._swap_buffers()
+    return True
+
+
 def carry_passive(sim, nactive):
     """Fields an engine does not update are copied so they survive the swap."""
     F = sim._fields_gpu_device.shape[0]

@@ -122,6 +122,28 @@ int ppf_allencahn_step(const ppf_grid* g, const void* in, void* out, const void*
                              : allencahn_step<double>(*g, in, out, bcvals, M, W, epsilon, dt, stream_of(s));
 }
 
+int ppf_multistep_max(const ppf_grid* g, const void* bcvals) {
+  if (int rc = check_grid(g, 1, 3, "multistep_max")) return rc;
+  return multistep_max(*g, bcvals);
+}
+
+int ppf_diffusion_multistep(const ppf_grid* g, const void* in, void* out, const void* bcvals, double D, double dt,
+                            int32_t nsteps, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "diffusion_multistep")) return rc;
+  if (!in || !out || in == out) return fail(PPF_EINVAL, "diffusion_multistep: need distinct in/out buffers");
+  return g->dtype == PPF_F32 ? diffusion_multistep<float>(*g, in, out, bcvals, D, dt, nsteps, stream_of(s))
+                             : diffusion_multistep<double>(*g, in, out, bcvals, D, dt, nsteps, stream_of(s));
+}
+
+int ppf_allencahn_multistep(const ppf_grid* g, const void* in, void* out, const void* bcvals, double M, double W,
+                            double epsilon, double dt, int32_t nsteps, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "allencahn_multistep")) return rc;
+  if (!in || !out || in == out) return fail(PPF_EINVAL, "allencahn_multistep: need distinct in/out buffers");
+  return g->dtype == PPF_F32
+             ? allencahn_multistep<float>(*g, in, out, bcvals, M, W, epsilon, dt, nsteps, stream_of(s))
+             : allencahn_multistep<double>(*g, in, out, bcvals, M, W, epsilon, dt, nsteps, stream_of(s));
+}
+
 int ppf_cahnhilliard_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, double M,
                           double epsilon, double dt, ppf_stream_t s) {
   if (int rc = check_grid(g, 1, 3, "cahnhilliard_step")) return rc;

@@ -10,6 +10,10 @@ template int apply_bc<float>(const ppf_grid&, void*, const void*, hipStream_t);
 template int apply_bc_temperature<float>(const ppf_grid&, void*, hipStream_t);
 template int fill_temperature<float>(const ppf_grid&, void*, const ppf_temperature&, int64_t, hipStream_t);
 template int diffusion_step<float>(const ppf_grid&, const void*, void*, const void*, double, double, hipStream_t);
+template int diffusion_multistep<float>(const ppf_grid&, const void*, void*, const void*, double, double, int32_t,
+                                      hipStream_t);
+template int allencahn_multistep<float>(const ppf_grid&, const void*, void*, const void*, double, double, double,
+                                     double, int32_t, hipStream_t);
 template int allencahn_step<float>(const ppf_grid&, const void*, void*, const void*, double, double, double, double,
                                    hipStream_t);
 template int cahnhilliard_step<float>(const ppf_grid&, const void*, void*, const void*, double, double, double,

@@ -18,6 +18,22 @@ int diffusion_step(const ppf_grid& g, const void* in, void* out, const void* bcv
 template <typename R>
 int allencahn_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double W,
                    double eps, double dt, hipStream_t s);
+// several Diffusion / Allen-Cahn steps in one launch (2D grids whose halos
+// are a coordinate remap: every side periodic, or Neumann without bc values)
+constexpr int kMultiMax = 8;  // most steps in one launch
+// steps one launch can take on this grid (0: none, use the one-step kernels)
+inline int32_t multistep_max(const ppf_grid& g, const void* bcvals) {
+  if (g.ndim != 2 || bcvals || g.bc[0] == PPF_DIRICHLET || g.bc[1] == PPF_DIRICHLET) return 0;
+  if ((g.n[1] + 2) * g.sy >= (int64_t(1) << 31)) return 0;
+  const int64_t m = g.n[0] < g.n[1] ? g.n[0] : g.n[1];
+  return int32_t(m < kMultiMax ? m : kMultiMax);
+}
+template <typename R>
+int diffusion_multistep(const ppf_grid& g, const void* in, void* out, const void* bcvals, double D, double dt,
+                        int32_t nsteps, hipStream_t s);
+template <typename R>
+int allencahn_multistep(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double W,
+                        double eps, double dt, int32_t nsteps, hipStream_t s);
 template <typename R>
 int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M,
                       double eps, double dt, hipStream_t s);

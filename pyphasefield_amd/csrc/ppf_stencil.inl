@@ -389,6 +389,117 @@ int point_step(const ppf_grid& g, const void* in, void* out, const void* bcvals,
   return check_launch(what);
 }
 
+// ----------------------------------------- several steps in one launch --
+// Diffusion / Allen-Cahn on 2D grids whose halos are a coordinate remap
+// (every side periodic, or zero gradient without bc values): a block loads
+// its output tile plus a halo of ks cells once and advances ks steps in LDS,
+// the valid region shrinking by one cell a step (temporal blocking), then
+// writes the tile.  Periodic images in the halo evolve exactly as the cells
+// they copy; on zero-gradient sides the cells outside the domain are reset to
+// their nearest interior cell after every step, as the one-step kernels'
+// loads read them.  Each update is point_update on the same five values, so
+// the results equal ks one-step launches bit for bit.
+template <typename R, int KIND, int TX, int TY>
+__global__ __launch_bounds__(kThreads) void k_point2d_multi(const R* __restrict__ in, R* __restrict__ out, Dom d,
+                                                            R dx, PointParams p, int32_t ks) {
+  constexpr int H = kMultiMax, W = TX + 2 * H, HH = TY + 2 * H;
+  __shared__ R s[2][HH][W];
+  const int32_t nx = int32_t(d.ax.n), ny = int32_t(d.ay.n), sy = int32_t(d.ay.stride);
+  const uint32_t nbx = uint32_t((nx + TX - 1) / TX);
+  const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int32_t x0 = 1 + int32_t(b % nbx) * TX, y0 = 1 + int32_t(b / nbx) * TY;
+  // LDS (ly, lx) holds padded cell (x0 - H + lx, y0 - H + ly); the region
+  // of step s is [H - ks + s, H + T + ks - s) on both axes
+  const int lo0 = H - ks, wx = TX + 2 * ks, wy = TY + 2 * ks;
+  for (int t = threadIdx.x; t < wx * wy; t += kThreads) {
+    const int ry = t / wx, rx = t - ry * wx;
+    const int32_t gx = remap_coord32(x0 - ks + rx, nx, d.ax.bc), gy = remap_coord32(y0 - ks + ry, ny, d.ay.bc);
+    s[0][lo0 + ry][lo0 + rx] = in[gy * sy + gx];
+  }
+  const bool nx_edge = d.ax.bc != PPF_PERIODIC && (x0 - ks < 1 || x0 + TX - 1 + ks > nx);
+  const bool ny_edge = d.ay.bc != PPF_PERIODIC && (y0 - ks < 1 || y0 + TY - 1 + ks > ny);
+  const R idx = R(1) / dx;
+  const R idx2 = idx * idx;
+  for (int st = 1; st <= ks; ++st) {
+    __syncthreads();
+    const R(*src)[W] = s[(st - 1) & 1];
+    R(*dst)[W] = s[st & 1];
+    const int lo = lo0 + st, vx = TX + 2 * (ks - st), vy = TY + 2 * (ks - st);
+    for (int t = threadIdx.x; t < vx * vy; t += kThreads) {
+      const int ly = lo + t / vx, lx = lo + (t - (t / vx) * vx);
+      const R c = src[ly][lx];
+      // point_tile_body's order: y (c[y+1] + c[y-1]), then x (c[x+1] + c[x-1])
+      R lap = (src[ly + 1][lx] + src[ly - 1][lx] - R(2) * c) * idx2;
+      lap += (src[ly][lx + 1] + src[ly][lx - 1] - R(2) * c) * idx2;
+      dst[ly][lx] = point_update<R, KIND>(c, lap, p);
+    }
+    if (nx_edge || ny_edge) {  // uniform per block
+      __syncthreads();
+      for (int t = threadIdx.x; t < vx * vy; t += kThreads) {
+        const int ly = lo + t / vx, lx = lo + (t - (t / vx) * vx);
+        const int32_t gx = x0 - H + lx, gy = y0 - H + ly;
+        const bool ox = d.ax.bc != PPF_PERIODIC && (gx < 1 || gx > nx);
+        const bool oy = d.ay.bc != PPF_PERIODIC && (gy < 1 || gy > ny);
+        if (ox || oy) {
+          const int sx = ox ? (gx < 1 ? 1 : nx) - (x0 - H) : lx;
+          const int sy2 = oy ? (gy < 1 ? 1 : ny) - (y0 - H) : ly;
+          dst[ly][lx] = dst[sy2][sx];  // an interior cell: never rewritten here
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const R(*fin)[W] = s[ks & 1];
+  for (int t = threadIdx.x; t < TX * TY; t += kThreads) {
+    const int ty = t / TX, tx = t - ty * TX;
+    const int32_t gx = x0 + tx, gy = y0 + ty;
+    if (gx <= nx && gy <= ny) out[gy * sy + gx] = fin[H + ty][H + tx];
+  }
+}
+
+// grids with at least this many 64 x 16 tiles take them; smaller grids take
+// 32 x 8 tiles (more blocks)
+#ifndef PPF_MULTI_BIG_TILES
+#define PPF_MULTI_BIG_TILES 512
+#endif
+#ifndef PPF_MULTI_SMALL_TY
+#define PPF_MULTI_SMALL_TY 8
+#endif
+template <typename R, int KIND>
+int point_multistep(const ppf_grid& g, const void* in, void* out, const void* bcvals, const PointParams& p,
+                    int32_t nsteps, hipStream_t s, const char* what) {
+  const int32_t mx = multistep_max(g, bcvals);
+  if (nsteps < 1 || nsteps > mx)
+    return fail(PPF_EINVAL, std::string(what) + ": nsteps must be 1.." + std::to_string(mx) +
+                                " on this grid (2D, periodic or Neumann sides without bc values)");
+  Dom d = make_dom(g);
+  if (((g.n[0] + 63) / 64) * ((g.n[1] + 15) / 16) >= PPF_MULTI_BIG_TILES) {
+    const unsigned blocks = unsigned(((g.n[0] + 63) / 64) * ((g.n[1] + 15) / 16));
+    hipLaunchKernelGGL((k_point2d_multi<R, KIND, 64, 16>), dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const R*>(in), static_cast<R*>(out), d, R(g.dx), p, nsteps);
+  } else {
+    constexpr int STY = PPF_MULTI_SMALL_TY;
+    const unsigned blocks = unsigned(((g.n[0] + 31) / 32) * ((g.n[1] + STY - 1) / STY));
+    hipLaunchKernelGGL((k_point2d_multi<R, KIND, 32, STY>), dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const R*>(in), static_cast<R*>(out), d, R(g.dx), p, nsteps);
+  }
+  return check_launch(what);
+}
+
+template <typename R>
+int diffusion_multistep(const ppf_grid& g, const void* in, void* out, const void* bcvals, double D, double dt,
+                        int32_t nsteps, hipStream_t s) {
+  PointParams p{D, 0, 0, dt};
+  return point_multistep<R, 0>(g, in, out, bcvals, p, nsteps, s, "diffusion_multistep");
+}
+
+template <typename R>
+int allencahn_multistep(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double W,
+                        double eps, double dt, int32_t nsteps, hipStream_t s) {
+  PointParams p{M, W, eps, dt};
+  return point_multistep<R, 1>(g, in, out, bcvals, p, nsteps, s, "allencahn_multistep");
+}
+
 template <typename R>
 int diffusion_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double D, double dt,
                    hipStream_t s) {

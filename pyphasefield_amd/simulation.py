@@ -330,13 +330,7 @@ class Simulation:
         gc.disable()
         try:
             with torch.cuda.graph(graph):
-                for k in range(chunk):
-                    if self._DEVICE_STEP:
-                        self._graph_step_offset = k + 1
-                    self.simulation_loop()
-                if self._DEVICE_STEP:
-                    _lib.check(_lib.lib().ppf_step_counter_add(_lib.ptr(self._step_counter_device()), chunk,
-                                                              _lib.stream_handle()), "step_counter_add")
+                self._capture_chunk(chunk)
         finally:
             self._graph_step_offset = None
             if gc_was_on:
@@ -344,6 +338,18 @@ class Simulation:
         if key is not None:
             self._graph_cache = (key, graph)
         return graph
+
+    def _capture_chunk(self, chunk):
+        """The launches of `chunk` steps, recorded inside a graph capture
+        (an even number of buffer swaps).  Engines whose steps fuse into fewer
+        launches override this."""
+        for k in range(chunk):
+            if self._DEVICE_STEP:
+                self._graph_step_offset = k + 1
+            self.simulation_loop()
+        if self._DEVICE_STEP:
+            _lib.check(_lib.lib().ppf_step_counter_add(_lib.ptr(self._step_counter_device()), chunk,
+                                                      _lib.stream_handle()), "step_counter_add")
 
     def _graph_chunk(self):
         if not self._GRAPH_CHUNK or (self._autosave_flag and self._autosave_rate):

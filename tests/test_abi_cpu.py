@@ -177,3 +177,29 @@ def test_ncomp_flags_and_cahnhilliard_line_arguments_are_rejected():
     gn = _grid2(L, n=(16, 1), bc=(1, 0))
     gn.ndim = 1
     rejected(L.ppf_cahnhilliard_line_solve(gn, a, b, 0.1, 1.0, w, None), "PERIODIC")
+
+
+def test_multistep_limits_and_rejections():
+    """ppf_multistep_max: 0 for grids the several-steps kernel does not take
+    (3D, a Dirichlet side, bc values), else min(8, each side); the multistep
+    entry points refuse nsteps outside 1..max before any launch."""
+    from pyphasefield_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libppf_hip.so not built")
+    L = _lib.lib()
+    EINVAL = 1000
+    P, N, D = _lib.PPF_PERIODIC, _lib.PPF_NEUMANN, _lib.PPF_DIRICHLET
+    assert L.ppf_multistep_max(_grid2(L, (64, 40), (P, N)), None) == 8
+    assert L.ppf_multistep_max(_grid2(L, (5, 40), (P, P)), None) == 5
+    assert L.ppf_multistep_max(_grid2(L, (64, 40), (P, D)), None) == 0
+    assert L.ppf_multistep_max(_grid2(L, (64, 40), (N, N)), 16) == 0  # bc values
+    g3 = _grid2(L, (16, 16), (P, P))
+    g3.ndim, g3.n[2], g3.sz = 3, 16, 18 * 18
+    g3.sf = g3.sz * 18
+    assert L.ppf_multistep_max(g3, None) == 0
+    g = _grid2(L, (64, 40), (P, N))
+    for k in (0, 9):
+        assert L.ppf_diffusion_multistep(g, 16, 32, None, 1.0, 0.1, k, None) == EINVAL
+        assert b"nsteps" in L.ppf_last_error()
+        assert L.ppf_allencahn_multistep(g, 16, 32, None, 1.0, 1.0, 1.0, 0.01, k, None) == EINVAL
+    assert L.ppf_diffusion_multistep(g, 16, 16, None, 1.0, 0.1, 2, None) == EINVAL  # aliased buffers

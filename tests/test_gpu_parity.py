@@ -720,3 +720,60 @@ def test_ncgpu_legacy_differs_from_new_only_as_stated(gpu):
         s, _ = _legacy_sim(E, 16, 16, "FP32")
         s.user_data["thermo_eval"] = "surrogate"
         s.simulate(1)
+
+
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+@pytest.mark.parametrize("shape,bc", [((41, 70), ["PERIODIC", "PERIODIC"]), ((90, 37), ["NEUMANN", "PERIODIC"]),
+                                      ((13, 9), ["NEUMANN", "NEUMANN"]), ((520, 1100), ["PERIODIC", "NEUMANN"])])
+def test_multistep_equals_one_step_launches(gpu, precision, shape, bc):
+    """ppf_diffusion_multistep / ppf_allencahn_multistep with k steps equal k
+    one-step launches bit for bit, for every k the grid allows (tile edges,
+    periodic wrap and zero-gradient sides inside the fused halo; 520 x 1100
+    takes the 64 x 16 tiles, the others 32 x 8); the Diffusion and CahnAllen
+    engines' graph replays (several steps per launch) equal eager one-step
+    calls."""
+    import torch
+    from pyphasefield_amd import _lib
+    L = _lib.lib()
+    dt = torch.float64 if precision == "FP64" else torch.float32
+    ny, nx = shape
+    rng = np.random.default_rng(8)
+    init = rng.random((ny, nx))
+    a = _lib.alloc_stack(1, [ny, nx], dt, gpu)
+    b = _lib.alloc_stack(1, [ny, nx], dt, gpu)
+    g = _lib.make_grid(a, 2, bc, 1.0)
+    kmax = L.ppf_multistep_max(g, None)
+    assert kmax == min(8, nx, ny)
+    calls = [("ppf_diffusion_step", "ppf_diffusion_multistep", (1.0, 0.2)),
+             ("ppf_allencahn_step", "ppf_allencahn_multistep", (1.0, 1.0, 1.0, 0.01))]
+    for one, many, sc in calls:
+        for k in range(1, kmax + 1):
+            a[0, 1:-1, 1:-1].copy_(torch.from_numpy(init))
+            src, dst = a, b
+            for _ in range(k):
+                _lib.check(getattr(L, one)(g, _lib.ptr(src), _lib.ptr(dst), None, *sc, _lib.stream_handle()), one)
+                src, dst = dst, src
+            want = src[0, 1:-1, 1:-1].cpu().numpy()
+            a[0, 1:-1, 1:-1].copy_(torch.from_numpy(init))
+            _lib.check(getattr(L, many)(g, _lib.ptr(a), _lib.ptr(b), None, *sc, k, _lib.stream_handle()), many)
+            got = b[0, 1:-1, 1:-1].cpu().numpy()
+            assert np.array_equal(got, want), (one, k, float(np.abs(got - want).max()))
+    # engines: graph replays (fused launches) against eager one-step calls
+    E = _engines()
+    for cls, ud in ((E.Diffusion, {"D": 1.0}), (E.CahnAllen, {"M": 1.0, "W": 1.0, "epsilon": 1.0})):
+        runs = []
+        for graph in (True, False):
+            sim = cls(dimensions=[ny, nx])
+            sim.set_user_data(dict(ud))
+            sim.set_framework("GPU_SERIAL")
+            sim.set_dx(1.0)
+            sim.set_dt(0.01)
+            sim.set_boundary_conditions(bc)
+            sim.set_precision(precision)
+            sim.initialize_fields_and_imported_data()
+            sim.fields[0].data[1:-1, 1:-1] = init
+            if not graph:
+                sim._GRAPH_CHUNK = 0
+            sim.simulate(47)
+            runs.append(sim.fields[0].get_cells())
+        assert np.array_equal(runs[0], runs[1]), cls.__name__
