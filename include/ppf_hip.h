@@ -223,6 +223,17 @@ int ppf_allencahn_multistep(const ppf_grid* g, const void* in, void* out, const 
 int ppf_cahnhilliard_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,
                           double M, double epsilon, double dt, ppf_stream_t s);
 
+/* nsteps (1..ppf_cahnhilliard_multistep_max) successive ppf_cahnhilliard_step
+   steps in one launch, equal to the one-step calls bit for bit (no
+   counterpart in the reference: in place of nsteps calls of its per-step
+   kernel launch, Engines/CahnHilliard.py:134-146).  Two steps take one pass
+   of the 3D register march with a 4-cell halo.  ppf_cahnhilliard_multistep_max
+   returns 2 for FP32 3D grids with every side periodic and at least 8 cells
+   per axis, else 1. */
+int ppf_cahnhilliard_multistep_max(const ppf_grid* g);
+int ppf_cahnhilliard_multistep(const ppf_grid* g, const void* in, void* out, const void* bcvals,
+                               double M, double epsilon, double dt, int32_t nsteps, ppf_stream_t s);
+
 /* Warren1995 binary solidification step, fields (phi, c)
    (Engines/Warren1995.py:50-92).  Noise: counter RNG keyed by (seed, step). */
 int ppf_warren_step(const ppf_grid* g, const void* in, void* out, const void* bcvals,

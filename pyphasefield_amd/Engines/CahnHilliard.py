@@ -44,6 +44,16 @@ class CahnHilliard(Simulation):
             return 0
         return super()._graph_chunk()
 
+    def _capture_chunk(self, chunk):
+        # FP32 3D periodic grids: two explicit steps per launch
+        # (ppf_cahnhilliard_multistep, one pass of the register march)
+        from .. import _lib
+        u = self.user_data
+        mx = _lib.lib().ppf_cahnhilliard_multistep_max(self._grid)
+        if mx < 2 or not _gpu.multistep(self, "ppf_cahnhilliard_multistep", chunk, float(u["M"]),
+                                         float(u["epsilon"]), float(self.dt), nactive=1, mx=mx):
+            super()._capture_chunk(chunk)
+
     def simulation_loop(self):
         if self.user_data.get("solver", "explicit") != "explicit":
             _implicit.cahnhilliard_step(self)

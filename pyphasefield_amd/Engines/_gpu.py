@@ -16,13 +16,14 @@ def stencil_step(sim, fn_name, *scalars, nactive=None):
     sim._swap_buffers()
 
 
-def multistep(sim, fn_name, nsteps, *scalars, nactive=None):
+def multistep(sim, fn_name, nsteps, *scalars, nactive=None, mx=None):
     """nsteps steps with the engine's several-steps-per-launch kernel, in an
     even number of launches (the two buffers end where they started) of at
-    most ppf_multistep_max steps each.  False (nothing launched) when the
-    grid does not allow it."""
+    most mx steps each (default: ppf_multistep_max).  False (nothing
+    launched) when the grid does not allow it."""
     L = _lib.lib()
-    mx = L.ppf_multistep_max(sim._grid, _lib.ptr(sim._boundary_conditions_gpu_device))
+    if mx is None:
+        mx = L.ppf_multistep_max(sim._grid, _lib.ptr(sim._boundary_conditions_gpu_device))
     if mx < 2:
         return False
     launches = -(-nsteps // mx)

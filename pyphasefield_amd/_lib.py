@@ -107,6 +107,8 @@ def lib():
         "ppf_multistep_max": ([P(Grid), c_vp], c_i32),
         "ppf_diffusion_multistep": ([P(Grid), c_vp, c_vp, c_vp, c_f64, c_f64, c_i32, c_vp], c_i32),
         "ppf_allencahn_multistep": ([P(Grid), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, c_f64, c_i32, c_vp], c_i32),
+        "ppf_cahnhilliard_multistep_max": ([P(Grid)], c_i32),
+        "ppf_cahnhilliard_multistep": ([P(Grid), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, c_i32, c_vp], c_i32),
         "ppf_cahnhilliard_step": ([P(Grid), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, c_vp], c_i32),
         "ppf_warren_step": ([P(Grid), c_vp, c_vp, c_vp, P(WarrenParams), c_u64, c_i64, c_vp], c_i32),
         "ppf_ncomp_thermo": ([P(Grid), c_vp, c_vp, P(Temperature), P(Grid), c_vp, P(NcompParams), P(Thermo),

@@ -144,6 +144,20 @@ int ppf_allencahn_multistep(const ppf_grid* g, const void* in, void* out, const 
              : allencahn_multistep<double>(*g, in, out, bcvals, M, W, epsilon, dt, nsteps, stream_of(s));
 }
 
+int ppf_cahnhilliard_multistep_max(const ppf_grid* g) {
+  if (int rc = check_grid(g, 1, 3, "cahnhilliard_multistep_max")) return rc;
+  return cahnhilliard_multistep_max(*g);
+}
+
+int ppf_cahnhilliard_multistep(const ppf_grid* g, const void* in, void* out, const void* bcvals, double M,
+                               double epsilon, double dt, int32_t nsteps, ppf_stream_t s) {
+  if (int rc = check_grid(g, 1, 3, "cahnhilliard_multistep")) return rc;
+  if (!in || !out || in == out) return fail(PPF_EINVAL, "cahnhilliard_multistep: need distinct in/out buffers");
+  return g->dtype == PPF_F32
+             ? cahnhilliard_multistep<float>(*g, in, out, bcvals, M, epsilon, dt, nsteps, stream_of(s))
+             : cahnhilliard_multistep<double>(*g, in, out, bcvals, M, epsilon, dt, nsteps, stream_of(s));
+}
+
 int ppf_cahnhilliard_step(const ppf_grid* g, const void* in, void* out, const void* bcvals, double M,
                           double epsilon, double dt, ppf_stream_t s) {
   if (int rc = check_grid(g, 1, 3, "cahnhilliard_step")) return rc;

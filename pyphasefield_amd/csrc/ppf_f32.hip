@@ -12,6 +12,8 @@ template int fill_temperature<float>(const ppf_grid&, void*, const ppf_temperatu
 template int diffusion_step<float>(const ppf_grid&, const void*, void*, const void*, double, double, hipStream_t);
 template int diffusion_multistep<float>(const ppf_grid&, const void*, void*, const void*, double, double, int32_t,
                                       hipStream_t);
+template int cahnhilliard_multistep<float>(const ppf_grid&, const void*, void*, const void*, double, double, double,
+                                        int32_t, hipStream_t);
 template int allencahn_multistep<float>(const ppf_grid&, const void*, void*, const void*, double, double, double,
                                      double, int32_t, hipStream_t);
 template int allencahn_step<float>(const ppf_grid&, const void*, void*, const void*, double, double, double, double,

@@ -12,6 +12,8 @@ template int fill_temperature<double>(const ppf_grid&, void*, const ppf_temperat
 template int diffusion_step<double>(const ppf_grid&, const void*, void*, const void*, double, double, hipStream_t);
 template int diffusion_multistep<double>(const ppf_grid&, const void*, void*, const void*, double, double, int32_t,
                                       hipStream_t);
+template int cahnhilliard_multistep<double>(const ppf_grid&, const void*, void*, const void*, double, double, double,
+                                        int32_t, hipStream_t);
 template int allencahn_multistep<double>(const ppf_grid&, const void*, void*, const void*, double, double, double,
                                      double, int32_t, hipStream_t);
 template int allencahn_step<double>(const ppf_grid&, const void*, void*, const void*, double, double, double, double,

@@ -37,6 +37,19 @@ int allencahn_multistep(const ppf_grid& g, const void* in, void* out, const void
 template <typename R>
 int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M,
                       double eps, double dt, hipStream_t s);
+// Cahn-Hilliard steps one launch can take: 2 on FP32 3D grids with every side
+// periodic, at least 8 cells per axis (the 4-cell halo of two steps wraps
+// once at most) and 32-bit plane offsets (the two-step register march); 1
+// elsewhere
+inline int32_t cahnhilliard_multistep_max(const ppf_grid& g) {
+  if (g.ndim != 3 || g.dtype != PPF_F32) return 1;
+  for (int a = 0; a < 3; ++a)
+    if (g.bc[a] != PPF_PERIODIC || g.n[a] < 8) return 1;
+  return g.sz * int64_t(sizeof(float)) < (int64_t(1) << 31) ? 2 : 1;
+}
+template <typename R>
+int cahnhilliard_multistep(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double eps,
+                           double dt, int32_t nsteps, hipStream_t s);
 template <typename R>
 int reduce(const ppf_grid& g, const void* in, const void* bcvals, int kind, double eps, double* out,
            void* work, hipStream_t s);

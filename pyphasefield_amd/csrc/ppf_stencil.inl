@@ -1127,6 +1127,150 @@ __global__ __launch_bounds__(64) void k_ch3w(const R* __restrict__ in, R* __rest
   }
 }
 
+// Two explicit Cahn-Hilliard steps per pass of the register march (FP32,
+// every side periodic).  Each wave owns 56 x (RPW - 8) output cells of a z
+// chunk: a 4-cell ring of x and y halo (2 per step) in its lanes and rows.
+// Plane k of the march computes mu(k), then c1(k-1) (the first step), then
+// mu2(k-2) from c1, then c2(k-3) (the second step), which is written.  The
+// halo cells are periodic images, which evolve exactly as the cells they
+// copy, and every value is the same FP32 expression as in k_ch3w, so the
+// result equals two k_ch3w launches bit for bit.
+constexpr int kCh3w2Tx = 56;
+#ifndef PPF_CH3W2_AHEAD
+#define PPF_CH3W2_AHEAD 2  // planes of c fetched ahead of the march
+#endif
+template <int RPW>
+__global__ __launch_bounds__(64) void k_ch3w2(const float* __restrict__ in, float* __restrict__ out, Dom d,
+                                              float dx, float eps, float dtM, int32_t zchunk) {
+  using R = float;
+  constexpr int TYO = RPW - 8;
+  const R idx = R(1) / dx;
+  const R idx2 = idx * idx;
+  const R eps2 = eps * eps;
+  const R k_mu = -eps2 * idx2;  // as k_ch3w
+  const R k_out = dtM * idx2;
+  const uint32_t nbx = uint32_t((d.ax.n + kCh3w2Tx - 1) / kCh3w2Tx), nby = uint32_t((d.ay.n + TYO - 1) / TYO);
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t bxy = blk % (nbx * nby), bz = blk / (nbx * nby);
+  const int64_t x0 = 1 + int64_t(bxy % nbx) * kCh3w2Tx, y0 = 1 + int64_t(bxy / nbx) * TYO;
+  const int64_t zb = 1 + int64_t(bz) * zchunk;
+  const int64_t ze = zb + zchunk - 1 < d.az.n ? zb + zchunk - 1 : d.az.n;
+  const int lane = threadIdx.x;
+  const int64_t xr = x0 - 4 + lane;
+  const uint32_t xoff = uint32_t(remap_halo(xr, d.ax)) * uint32_t(sizeof(R));
+  const uint32_t ooff = uint32_t(xr) * uint32_t(sizeof(R));  // used on output lanes only (xr >= 1 there)
+  int32_t yoff[RPW];  // byte offset of row r in its plane (periodic image)
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) yoff[r] = int32_t(remap_halo(y0 - 4 + r, d.ay) * d.ay.stride * int64_t(sizeof(R)));
+  const bool out_lane = lane >= 4 && lane < 4 + kCh3w2Tx && xr <= d.ax.n;
+  const int32_t zb32 = int32_t(zb), ze32 = int32_t(ze), k0 = zb32 - 3;
+  const int32_t ny_out = int32_t(d.ay.n - (y0 - 4));  // rows r <= ny_out are inside the domain
+  const int32_t orow_bytes = int32_t(d.ay.stride * int64_t(sizeof(R)));
+  constexpr int NC = 2 + PPF_CH3W2_AHEAD;  // c sets: c(k-1), c(k), then AHEAD planes
+  R C[NC + 1][RPW];  // c(k) lives in set (k - k0) mod (NC + 1)
+  R M[3][RPW];       // mu(k): set (k - k0) mod 3, rows 1 .. RPW - 2
+  R C1[3][RPW];      // c1(k): set (k - k0) mod 3, rows 2 .. RPW - 3
+  R M2[3][RPW];      // mu2(k): set (k - k0) mod 3, rows 3 .. RPW - 4
+  ch3w_fetch<R, RPW>(C[NC], in, d, yoff, xoff, k0 - 1);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) ch3w_fetch<R, RPW>(C[j], in, d, yoff, xoff, k0 + j);
+  // mu on rows lo .. hi - 1 of plane c0 (neighbour planes cm, cp)
+  auto mu_rows = [&](R(&dst)[RPW], const R(&cm)[RPW], const R(&c0)[RPW], const R(&cp)[RPW], auto lo_c,
+                     auto hi_c) {
+    constexpr int lo = decltype(lo_c)::value, hi = decltype(hi_c)::value;
+#pragma unroll
+    for (int r = lo; r < hi; ++r) {
+      const R c = c0[r];
+      const R cxp = lane_shift<kDppWaveShl1>(c), cxm = lane_shift<kDppWaveShr1>(c);
+      const R sum = ((cp[r] + cm[r]) + (c0[r + 1] + c0[r - 1])) + (cxp + cxm);
+      const R well = c * __builtin_fmaf(c, __builtin_fmaf(c, R(4), R(-6)), R(2));
+      dst[r] = __builtin_fmaf(k_mu, __builtin_fmaf(R(-6), c, sum), well);
+    }
+  };
+  // c + dt M lap(mu) on rows lo .. hi - 1 (mu planes mm, m0, mp; c plane cc)
+  auto step_rows = [&](R(&dst)[RPW], const R(&mm)[RPW], const R(&m0)[RPW], const R(&mp)[RPW],
+                       const R(&cc)[RPW], auto lo_c, auto hi_c) {
+    constexpr int lo = decltype(lo_c)::value, hi = decltype(hi_c)::value;
+#pragma unroll
+    for (int r = lo; r < hi; ++r) {
+      const R m = m0[r];
+      const R mxm = lane_shift<kDppWaveShr1>(m), mxp = lane_shift<kDppWaveShl1>(m);
+      const R sum = ((mm[r] + mp[r]) + (m0[r - 1] + m0[r + 1])) + (mxm + mxp);
+      dst[r] = __builtin_fmaf(k_out, __builtin_fmaf(R(-6), m, sum), cc[r]);
+    }
+  };
+  auto iteration = [&](auto phase, int32_t k) {
+    constexpr int p = decltype(phase)::value;
+    __builtin_amdgcn_sched_barrier(0);  // one plane at a time (register pressure)
+    R(&cm)[RPW] = C[(p + NC) % (NC + 1)];
+    // mu(k) from c(k-1), c(k), c(k+1)
+    mu_rows(M[p % 3], cm, C[p % (NC + 1)], C[(p + 1) % (NC + 1)], std::integral_constant<int, 1>{},
+            std::integral_constant<int, RPW - 1>{});
+    // c1(k-1) from mu(k-2), mu(k-1), mu(k) and c(k-1)
+    step_rows(C1[(p + 2) % 3], M[(p + 1) % 3], M[(p + 2) % 3], M[p % 3], cm, std::integral_constant<int, 2>{},
+              std::integral_constant<int, RPW - 2>{});
+    // mu2(k-2) from c1(k-3), c1(k-2), c1(k-1)
+    mu_rows(M2[(p + 1) % 3], C1[p % 3], C1[(p + 1) % 3], C1[(p + 2) % 3], std::integral_constant<int, 3>{},
+            std::integral_constant<int, RPW - 3>{});
+    // c2(k-3) from mu2(k-4), mu2(k-3), mu2(k-2) and c1(k-3): the output plane
+    const int32_t z = k - 3;
+    if (z >= zb32) {
+      R o[RPW];
+      step_rows(o, M2[(p + 2) % 3], M2[p % 3], M2[(p + 1) % 3], C1[p % 3], std::integral_constant<int, 4>{},
+                std::integral_constant<int, RPW - 4>{});
+      const __amdgpu_buffer_rsrc_t ws = plane_rsrc(out + int64_t(z) * d.az.stride + (y0 - 4) * d.ay.stride);
+      if (out_lane) {
+#pragma unroll
+        for (int r = 4; r < RPW - 4; ++r)
+          if (r <= ny_out) buf_store<R>(o[r], ws, ooff, r * orow_bytes);
+      }
+    }
+    // unconditional (the last iterations re-read plane ze + 4, whose values
+    // are never used): a conditional fetch makes the compiler wait for it
+    __builtin_amdgcn_sched_barrier(0);
+    ch3w_fetch<R, RPW>(cm, in, d, yoff, xoff, int64_t(k + NC <= ze32 + 4 ? k + NC : ze32 + 4));
+  };
+  constexpr int kPeriod = (NC + 1) % 3 == 0 ? NC + 1 : 3 * (NC + 1);
+  for (int32_t k = k0;; k += kPeriod) {
+    auto step = [&](auto phase) {
+      constexpr int q = decltype(phase)::value;
+      if (k + q > ze32 + 3) return false;
+      iteration(phase, k + q);
+      return true;
+    };
+    if (!ch3w_unrolled(step, std::make_integer_sequence<int, kPeriod>{})) break;
+  }
+}
+
+#ifndef PPF_CH3W2_RPW
+#define PPF_CH3W2_RPW 12
+#endif
+#ifndef PPF_CH3W2_ROUNDS
+#define PPF_CH3W2_ROUNDS 4
+#endif
+inline int ch3w2_launch(const ppf_grid& g, const void* in, void* out, double M, double eps, double dt,
+                        hipStream_t s) {
+  // the step constants are formed in the kernel from these, as k_ch3w forms them
+  constexpr int RPW = PPF_CH3W2_RPW;
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_ch3w2<RPW>, 64, 0) != hipSuccess || v < 1) v = 1;
+    per_cu = v;
+  }
+  Dom d = make_dom(g);
+  const int64_t xy = ((g.n[0] + kCh3w2Tx - 1) / kCh3w2Tx) * ((g.n[1] + RPW - 9) / (RPW - 8));
+  const int64_t resident = int64_t(device_cus()) * per_cu;
+  const int64_t chunks = (PPF_CH3W2_ROUNDS * resident + xy - 1) / xy;
+  int64_t zc = (g.n[2] + chunks - 1) / chunks;
+  if (zc < 8) zc = 8;
+  if (zc > g.n[2]) zc = g.n[2];
+  const int64_t blocks = xy * ((g.n[2] + zc - 1) / zc);
+  hipLaunchKernelGGL((k_ch3w2<RPW>), dim3(unsigned(blocks)), dim3(64), 0, s, static_cast<const float*>(in),
+                     static_cast<float*>(out), d, float(g.dx), float(eps), float(dt * M), int32_t(zc));
+  return check_launch("cahnhilliard_multistep");
+}
+
 #ifndef PPF_CH3D_WAVE
 #define PPF_CH3D_WAVE 1
 #endif
@@ -1211,6 +1355,21 @@ int cahnhilliard_step(const ppf_grid& g, const void* in, void* out, const void* 
                        R(g.dx), R(eps), dtM);
   }
   return check_launch("cahnhilliard_step");
+}
+
+// nsteps (1 .. cahnhilliard_multistep_max) successive explicit steps
+template <typename R>
+int cahnhilliard_multistep(const ppf_grid& g, const void* in, void* out, const void* bcvals, double M, double eps,
+                           double dt, int32_t nsteps, hipStream_t s) {
+  const int32_t mx = cahnhilliard_multistep_max(g);
+  if (nsteps < 1 || nsteps > mx)
+    return fail(PPF_EINVAL, "cahnhilliard_multistep: nsteps " + std::to_string(nsteps) + " outside 1.." +
+                                std::to_string(mx) + " for this grid");
+  if (nsteps == 1) return cahnhilliard_step<R>(g, in, out, bcvals, M, eps, dt, s);
+  if constexpr (sizeof(R) == 4)
+    return ch3w2_launch(g, in, out, M, eps, dt, s);
+  else
+    return fail(PPF_EINVAL, "cahnhilliard_multistep: FP64 grids take one step per launch");
 }
 
 // ------------------------------------------------------------ reductions --

@@ -203,3 +203,36 @@ def test_multistep_limits_and_rejections():
         assert b"nsteps" in L.ppf_last_error()
         assert L.ppf_allencahn_multistep(g, 16, 32, None, 1.0, 1.0, 1.0, 0.01, k, None) == EINVAL
     assert L.ppf_diffusion_multistep(g, 16, 16, None, 1.0, 0.1, 2, None) == EINVAL  # aliased buffers
+
+
+def _grid3(L, n, bc, dtype):
+    g = _grid2(L, (n[0], n[1]), (bc[0], bc[1]))
+    g.ndim, g.n[2], g.bc[2], g.dtype = 3, n[2], bc[2], dtype
+    g.sz = (n[0] + 2) * (n[1] + 2)
+    g.sf = g.sz * (n[2] + 2)
+    return g
+
+
+def test_cahnhilliard_multistep_limits_and_rejections():
+    """ppf_cahnhilliard_multistep_max: 2 only for FP32 3D grids with every
+    side periodic and at least 8 cells a side, else 1; nsteps outside 1..max
+    is refused before any launch."""
+    from pyphasefield_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libppf_hip.so not built")
+    L = _lib.lib()
+    EINVAL = 1000
+    P, N = _lib.PPF_PERIODIC, _lib.PPF_NEUMANN
+    F32, F64 = _lib.PPF_F32, _lib.PPF_F64
+    assert L.ppf_cahnhilliard_multistep_max(_grid3(L, (16, 9, 8), (P, P, P), F32)) == 2
+    assert L.ppf_cahnhilliard_multistep_max(_grid3(L, (16, 7, 8), (P, P, P), F32)) == 1  # a side below 8
+    assert L.ppf_cahnhilliard_multistep_max(_grid3(L, (16, 16, 16), (P, N, P), F32)) == 1
+    assert L.ppf_cahnhilliard_multistep_max(_grid3(L, (16, 16, 16), (P, P, P), F64)) == 1
+    assert L.ppf_cahnhilliard_multistep_max(_grid2(L, (64, 64), (P, P))) == 1
+    g = _grid3(L, (16, 16, 16), (P, P, P), F32)
+    for k in (0, 3):
+        assert L.ppf_cahnhilliard_multistep(g, 16, 32, None, 1.0, 1.0, 0.01, k, None) == EINVAL
+        assert b"nsteps" in L.ppf_last_error()
+    gn = _grid3(L, (16, 16, 16), (P, N, P), F32)
+    assert L.ppf_cahnhilliard_multistep(gn, 16, 32, None, 1.0, 1.0, 0.01, 2, None) == EINVAL
+    assert L.ppf_cahnhilliard_multistep(g, 16, 16, None, 1.0, 1.0, 0.01, 2, None) == EINVAL  # aliased buffers

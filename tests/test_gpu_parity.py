@@ -777,3 +777,53 @@ def test_multistep_equals_one_step_launches(gpu, precision, shape, bc):
             sim.simulate(47)
             runs.append(sim.fields[0].get_cells())
         assert np.array_equal(runs[0], runs[1]), cls.__name__
+
+
+@pytest.mark.parametrize("shape", [(18, 20, 24), (23, 29, 37), (8, 8, 8), (64, 9, 130), (40, 50, 61)])
+def test_cahnhilliard_two_step_march_equals_two_launches(gpu, shape):
+    """ppf_cahnhilliard_multistep with 2 steps (FP32, periodic: one pass of
+    the two-step register march) equals two ppf_cahnhilliard_step launches
+    bit for bit (x tiles of 56 lanes, 4-row y tiles and z chunks cut the
+    grids above at odd places); 1 step is the one-step launch; the engine's
+    graph replays (two steps per launch) equal eager one-step calls."""
+    import torch
+    from pyphasefield_amd import _lib
+    L = _lib.lib()
+    nz, ny, nx = shape
+    init = 0.5 + 0.2 * (np.random.default_rng(9).random(shape) - 0.5)
+    a = _lib.alloc_stack(1, list(shape), torch.float32, gpu)
+    b = _lib.alloc_stack(1, list(shape), torch.float32, gpu)
+    g = _lib.make_grid(a, 3, ["PERIODIC"] * 3, 1.0)
+    assert L.ppf_cahnhilliard_multistep_max(g) == 2
+    sc = (1.0, 1.0, 0.01)
+    for k in (1, 2):
+        a[0, 1:-1, 1:-1, 1:-1].copy_(torch.from_numpy(init))
+        src, dst = a, b
+        for _ in range(k):
+            _lib.check(L.ppf_cahnhilliard_step(g, _lib.ptr(src), _lib.ptr(dst), None, *sc, _lib.stream_handle()),
+                       "step")
+            src, dst = dst, src
+        want = src[0, 1:-1, 1:-1, 1:-1].cpu().numpy()
+        a[0, 1:-1, 1:-1, 1:-1].copy_(torch.from_numpy(init))
+        _lib.check(L.ppf_cahnhilliard_multistep(g, _lib.ptr(a), _lib.ptr(b), None, *sc, k, _lib.stream_handle()),
+                   "multistep")
+        got = b[0, 1:-1, 1:-1, 1:-1].cpu().numpy()
+        assert np.array_equal(got, want), (k, float(np.abs(got - want).max()))
+    assert L.ppf_cahnhilliard_multistep(g, _lib.ptr(a), _lib.ptr(b), None, *sc, 3, _lib.stream_handle()) != 0
+    E = _engines()
+    runs = []
+    for graph in (True, False):
+        sim = E.CahnHilliard(dimensions=list(shape))
+        sim.set_user_data({"M": 1.0, "epsilon": 1.0})
+        sim.set_framework("GPU_SERIAL")
+        sim.set_dx(1.0)
+        sim.set_dt(0.01)
+        sim.set_boundary_conditions("PERIODIC")
+        sim.set_precision("FP32")
+        sim.initialize_fields_and_imported_data()
+        sim.fields[0].data[1:-1, 1:-1, 1:-1] = init
+        if not graph:
+            sim._GRAPH_CHUNK = 0
+        sim.simulate(47)
+        runs.append(sim.fields[0].get_cells())
+    assert np.array_equal(runs[0], runs[1])
