@@ -284,7 +284,7 @@ def bench_diffusion(args):
     # for up to 8 steps (48-step graph chunks: 6 launches of 8 steps)
     per_launch = _gpu_multistep_max(sim)
     if per_launch >= 2:
-        roof = _roofline(16.0 * cells, t / args.steps * per_launch, "k_point2d_multi<double,0,32,8>")
+        roof = _roofline(16.0 * cells, t / args.steps * per_launch, "k_point2d_multi<double,0,16,8>")
         roof["steps_per_launch"] = per_launch
     else:
         roof = _roofline(16.0 * cells, t / args.steps, "k_point2d<double,0,4>")

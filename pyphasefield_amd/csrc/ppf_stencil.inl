@@ -458,12 +458,15 @@ __global__ __launch_bounds__(kThreads) void k_point2d_multi(const R* __restrict_
 }
 
 // grids with at least this many 64 x 16 tiles take them; smaller grids take
-// 32 x 8 tiles (more blocks)
+// 16 x 8 tiles (more blocks: two per CU at 256^2)
 #ifndef PPF_MULTI_BIG_TILES
 #define PPF_MULTI_BIG_TILES 512
 #endif
 #ifndef PPF_MULTI_SMALL_TY
 #define PPF_MULTI_SMALL_TY 8
+#endif
+#ifndef PPF_MULTI_SMALL_TX
+#define PPF_MULTI_SMALL_TX 16
 #endif
 template <typename R, int KIND>
 int point_multistep(const ppf_grid& g, const void* in, void* out, const void* bcvals, const PointParams& p,
@@ -478,9 +481,9 @@ int point_multistep(const ppf_grid& g, const void* in, void* out, const void* bc
     hipLaunchKernelGGL((k_point2d_multi<R, KIND, 64, 16>), dim3(blocks), dim3(kThreads), 0, s,
                        static_cast<const R*>(in), static_cast<R*>(out), d, R(g.dx), p, nsteps);
   } else {
-    constexpr int STY = PPF_MULTI_SMALL_TY;
-    const unsigned blocks = unsigned(((g.n[0] + 31) / 32) * ((g.n[1] + STY - 1) / STY));
-    hipLaunchKernelGGL((k_point2d_multi<R, KIND, 32, STY>), dim3(blocks), dim3(kThreads), 0, s,
+    constexpr int STY = PPF_MULTI_SMALL_TY, STX = PPF_MULTI_SMALL_TX;
+    const unsigned blocks = unsigned(((g.n[0] + STX - 1) / STX) * ((g.n[1] + STY - 1) / STY));
+    hipLaunchKernelGGL((k_point2d_multi<R, KIND, STX, STY>), dim3(blocks), dim3(kThreads), 0, s,
                        static_cast<const R*>(in), static_cast<R*>(out), d, R(g.dx), p, nsteps);
   }
   return check_launch(what);

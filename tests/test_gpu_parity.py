@@ -729,7 +729,7 @@ def test_multistep_equals_one_step_launches(gpu, precision, shape, bc):
     """ppf_diffusion_multistep / ppf_allencahn_multistep with k steps equal k
     one-step launches bit for bit, for every k the grid allows (tile edges,
     periodic wrap and zero-gradient sides inside the fused halo; 520 x 1100
-    takes the 64 x 16 tiles, the others 32 x 8); the Diffusion and CahnAllen
+    takes the 64 x 16 tiles, the others 16 x 8); the Diffusion and CahnAllen
     engines' graph replays (several steps per launch) equal eager one-step
     calls."""
     import torch
