@@ -335,7 +335,7 @@ def bench_ch3d(args):
     per_launch = _lib.lib().ppf_cahnhilliard_multistep_max(sim._grid) if sim._graph_chunk() else 1
     if per_launch == 2 and args.steps % sim._graph_chunk() == 0:
         # two steps per launch: one read and one write of the field per launch
-        roof = _roofline(8.0 * cells, t / args.steps * 2, "k_ch3w2<12>")
+        roof = _roofline(8.0 * cells, t / args.steps * 2, "k_ch3w2<13>")
         roof["steps_per_launch"] = 2
         roof["frac_per_step_equivalent"] = 8.0 * cells / (t / args.steps) / 1e9 / HBM_PEAK_GBS
     else:

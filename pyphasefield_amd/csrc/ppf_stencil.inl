@@ -1246,7 +1246,7 @@ __global__ __launch_bounds__(64) void k_ch3w2(const float* __restrict__ in, floa
 }
 
 #ifndef PPF_CH3W2_RPW
-#define PPF_CH3W2_RPW 12
+#define PPF_CH3W2_RPW 13
 #endif
 #ifndef PPF_CH3W2_ROUNDS
 #define PPF_CH3W2_ROUNDS 4

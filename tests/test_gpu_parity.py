@@ -783,7 +783,7 @@ def test_multistep_equals_one_step_launches(gpu, precision, shape, bc):
 def test_cahnhilliard_two_step_march_equals_two_launches(gpu, shape):
     """ppf_cahnhilliard_multistep with 2 steps (FP32, periodic: one pass of
     the two-step register march) equals two ppf_cahnhilliard_step launches
-    bit for bit (x tiles of 56 lanes, 4-row y tiles and z chunks cut the
+    bit for bit (x tiles of 56 lanes, 5-row y tiles and z chunks cut the
     grids above at odd places); 1 step is the one-step launch; the engine's
     graph replays (two steps per launch) equal eager one-step calls."""
     import torch
