@@ -20,7 +20,10 @@ int allencahn_step(const ppf_grid& g, const void* in, void* out, const void* bcv
                    double eps, double dt, hipStream_t s);
 // several Diffusion / Allen-Cahn steps in one launch (2D grids whose halos
 // are a coordinate remap: every side periodic, or Neumann without bc values)
-constexpr int kMultiMax = 8;  // most steps in one launch
+#ifndef PPF_MULTI_MAX
+#define PPF_MULTI_MAX 8
+#endif
+constexpr int kMultiMax = PPF_MULTI_MAX;  // most steps in one launch
 // steps one launch can take on this grid (0: none, use the one-step kernels)
 inline int32_t multistep_max(const ppf_grid& g, const void* bcvals) {
   if (g.ndim != 2 || bcvals || g.bc[0] == PPF_DIRICHLET || g.bc[1] == PPF_DIRICHLET) return 0;
